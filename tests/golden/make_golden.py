@@ -1,0 +1,261 @@
+"""Write the golden fixtures in this folder by running the reference solver.
+
+Run in the build container only (the reference is not present where the GPU
+tests run):  ``python tests/golden/make_golden.py``.  It imports DiffNano's
+``fdtd2d.py`` / ``fdtd3d.py`` from /root/reference (read-only; bytecode
+writing disabled) and records inputs and outputs as float64 ``.npz``:
+
+* ``tables.npz``   CPML b/c profiles (fdtd3d.py:51-85, fdtd2d.py:52-88) and
+                   source waveform tables (fdtd3d.py:319-338, fdtd2d.py:178-205)
+* ``step3d.npz``   one ``FDTDSolver3D._time_step`` on random fields with CPML
+                   (fdtd3d.py:378-465) — pins the axis-mixed profile broadcast
+* ``fwd3d.npz``    public ``forward`` results for several source kinds
+                   (fdtd3d.py:564-650) and ``time_series`` (:652-700)
+* ``grad3d.npz``   ``backward="time_reversal"`` gradients (fdtd3d.py:88-219)
+* ``dft3d.npz``    harness loop around ``_time_step`` with an Ey Gaussian
+                   sheet source and DFT monitors at 3 wavelengths + gradient
+* ``fdtd2d.npz``   TM/TE forward, TM/TE autograd gradients, ``time_series``
+* ``c0.npz``       the full 200x200 / 2000-step TM case: final Ez, the 1x60
+                   DFT line at 1550 nm and d(sum |DFT|^2)/d eps
+The DFT convention (shared with the product and the oracle): after step n
+(0-based) the fields are E^{n+1}; F = sum_{n} E^{n+1} exp(-i w (n+1) dt).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+
+sys.dont_write_bytecode = True
+os.environ.setdefault("PYTHONDONTWRITEBYTECODE", "1")
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+import torch.utils.checkpoint as cp  # noqa: E402
+from scipy.ndimage import gaussian_filter  # noqa: E402
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REF)
+
+from diffnano.solvers.fdtd2d import FDTDSolver2D, _CPMLRegion  # noqa: E402
+from diffnano.solvers.fdtd3d import FDTDSolver3D, _CPMLRegion3D  # noqa: E402
+
+torch.set_default_dtype(torch.float32)
+
+
+def blob_eps(seed, H=200, W=200, lo=60, size=80, sigma=3.0):
+    """Seeded Si/SiO2 blob design (uniform noise, Gaussian filter, threshold)."""
+    g = torch.Generator().manual_seed(seed)
+    noise = torch.rand(size, size, dtype=torch.float64, generator=g).numpy()
+    blobs = gaussian_filter(noise, sigma, mode="reflect", truncate=4.0) > 0.5
+    eps = np.full((H, W), 2.085)
+    eps[lo:lo + size, lo:lo + size] = np.where(blobs, 12.1, 2.085)
+    return eps
+
+
+def tables():
+    out = {}
+    cases = [(200, 10, 20.0, 0.5, 2), (128, 12, 25.0, 0.4, 3), (256, 16, 25.0, 0.4, 3),
+             (96, 12, 25.0, 0.4, 3), (160, 12, 25.0, 0.4, 3), (192, 12, 25.0, 0.4, 3),
+             (12, 3, 20.0, 0.4, 3), (8, 0, 20.0, 0.35, 3), (7, 4, 20.0, 0.5, 2)]
+    for (n, L, dl, cour, dim) in cases:
+        dt = cour * dl / math.sqrt(float(dim))
+        r = (_CPMLRegion if dim == 2 else _CPMLRegion3D)(n, L, dl, dt)
+        key = f"cpml_{dim}d_{n}_{L}_{dl:g}_{cour:g}"
+        out[key + "_b"] = r.b.numpy()
+        out[key + "_c"] = r.c.numpy()
+    lam0 = 1550.0
+    spread = lam0**2 / (2 * math.pi * 200.0)
+    srcs = {
+        "default": {"type": "gaussian_pulse"},
+        "c0pulse": {"type": "gaussian_pulse", "spread": spread, "t0": 3 * spread},
+        "amp": {"type": "gaussian_pulse", "amplitude": 0.3},
+        "cw": {"type": "continuous", "amplitude": 2.0},
+        "other": {"type": "ricker"},
+    }
+    s3 = FDTDSolver3D(grid_shape=(4, 4, 4), dl=25.0, pml_layers=0)
+    s2 = FDTDSolver2D(grid_shape=(4, 4), dl=20.0, pml_layers=0)
+    for name, src in srcs.items():
+        out[f"wave3d_{name}"] = np.array([s3._source_waveform(n, src).item() for n in range(4000)])
+        out[f"wave2d_{name}"] = np.array([s2._source_waveform(n, src).item() for n in range(2000)])
+    out["wave_meta"] = np.array(json.dumps({k: v for k, v in srcs.items()}))
+    np.savez_compressed(os.path.join(HERE, "tables.npz"), **out)
+
+
+def step3d():
+    torch.manual_seed(7)
+    N = 10
+    s = FDTDSolver3D(grid_shape=(N, N, N), dl=20.0, pml_layers=3, courant=0.4)
+    f = [torch.randn(N, N, N, dtype=torch.float64) for _ in range(6)]
+    eps = 1.5 + torch.rand(N, N, N, dtype=torch.float64)
+    mu = torch.ones_like(eps)
+    out = s._time_step(*f, eps, mu)
+    np.savez_compressed(os.path.join(HERE, "step3d.npz"),
+                        fin=torch.stack(f).numpy(), eps=eps.numpy(),
+                        fout=torch.stack(out).numpy())
+
+
+def _eps3d(N, seed=42):
+    torch.manual_seed(seed)
+    e = 1.5 + 1.0 * torch.rand(N, N, N, dtype=torch.float64)
+    d = N // 4
+    e[N // 2 - d:N // 2 + d, N // 2 - d:N // 2 + d, N // 2 - d:N // 2 + d] = 4.0
+    return e
+
+
+def fwd3d():
+    N = 12
+    eps = _eps3d(N)
+    out = {"eps": eps.numpy()}
+    cfg = dict(grid_shape=(N, N, N), dl=20.0, pml_layers=3, n_steps=60, courant=0.4)
+    srcs = {
+        "line": None,
+        "point": {"type": "gaussian_pulse", "pos": [6, 5, 4], "amplitude": 2.0},
+        "xy": {"type": "gaussian_pulse", "plane": "xy", "z": 3},
+        "xz": {"type": "gaussian_pulse", "plane": "xz"},
+        "yz": {"type": "continuous", "plane": "yz", "x": 2},
+        "oob": {"type": "gaussian_pulse", "pos": [12, 0, 0]},
+    }
+    for name, src in srcs.items():
+        s = FDTDSolver3D(**cfg)
+        out[f"field_{name}"] = s.forward(eps, source=src).field.numpy()
+    s = FDTDSolver3D(**cfg)
+    out["ts"] = s.time_series(eps, probe=(6, 6, 9), n_steps=40).numpy()
+    np.savez_compressed(os.path.join(HERE, "fwd3d.npz"), **out)
+
+
+def grad3d():
+    out = {}
+    # Fixture of the reference's own TR tests: 8^3, no CPML, 6 steps, courant 0.35.
+    for tag, N, L, T, cour in (("small", 8, 0, 6, 0.35), ("pml", 10, 2, 24, 0.4)):
+        eps0 = _eps3d(N)
+        s = FDTDSolver3D(grid_shape=(N, N, N), dl=20.0, pml_layers=L, n_steps=T, courant=cour,
+                         backward="time_reversal")
+        eps = eps0.clone().requires_grad_(True)
+        field = s.forward(eps).field
+        field.sum().backward()
+        out[f"{tag}_eps"] = eps0.numpy()
+        out[f"{tag}_field"] = field.detach().numpy()
+        out[f"{tag}_grad_sum"] = eps.grad.numpy()
+        eps = eps0.clone().requires_grad_(True)
+        field = s.forward(eps).field
+        (field**2).sum().backward()
+        out[f"{tag}_grad_energy"] = eps.grad.numpy()
+        eps = eps0.clone().requires_grad_(True)
+        field = s.forward(eps, source={"type": "gaussian_pulse", "pos": [N // 2, N // 2, N // 2]}).field
+        (field[0, N // 2, N // 2, N // 2] ** 2).backward()
+        out[f"{tag}_grad_point"] = eps.grad.numpy()
+    np.savez_compressed(os.path.join(HERE, "grad3d.npz"), **out)
+
+
+def dft3d():
+    """Harness loop: Ey Gaussian-profile yz sheet, DFT of Ex/Ey/Ez on a yz plane."""
+    D, H, W = 16, 16, 16
+    L, T = 4, 260
+    s = FDTDSolver3D(grid_shape=(D, H, W), dl=25.0, pml_layers=L, courant=0.4)
+    rng = np.random.default_rng(3)
+    eps0 = np.full((D, H, W), 2.085)
+    eps0[6:10, 5:11, 4:12] = 2.085 + 10.015 * rng.random((4, 6, 8))
+    zz, yy = np.meshgrid(np.arange(D), np.arange(H), indexing="ij")
+    prof = np.exp(-(((zz - 7.5) / 3.0) ** 2 + ((yy - 7.5) / 3.0) ** 2) / 2)
+    mask = torch.zeros(D, H, W, dtype=torch.float64)
+    mask[:, :, L + 1] = torch.from_numpy(prof)
+    src = {"type": "gaussian_pulse"}
+    omegas = [2 * math.pi / lam for lam in (1500.0, 1550.0, 1600.0)]
+    xm = W - L - 2
+    eps = torch.from_numpy(eps0).clone().requires_grad_(True)
+    mu = torch.ones_like(eps)
+    z = lambda: torch.zeros(D, H, W, dtype=torch.float64)  # noqa: E731
+    Ex, Ey, Ez, Hx, Hy, Hz = z(), z(), z(), z(), z(), z()
+    Fr = torch.zeros(len(omegas), 3, D, H, dtype=torch.float64)
+    Fi = torch.zeros_like(Fr)
+    for n in range(T):
+        Ex, Ey, Ez, Hx, Hy, Hz = s._time_step(Ex, Ey, Ez, Hx, Hy, Hz, eps, mu)
+        Ey = Ey + mask * s._source_waveform(n, src)
+        ph = [w * (n + 1) * s.dt for w in omegas]
+        planes = torch.stack([Ex[:, :, xm], Ey[:, :, xm], Ez[:, :, xm]])
+        Fr = Fr + torch.stack([planes * math.cos(p) for p in ph])
+        Fi = Fi + torch.stack([planes * -math.sin(p) for p in ph])
+    loss = (Fr[:, 1] ** 2 + Fi[:, 1] ** 2).sum()
+    loss.backward()
+    np.savez_compressed(os.path.join(HERE, "dft3d.npz"), eps=eps0, profile=prof,
+                        Fr=Fr.detach().numpy(), Fi=Fi.detach().numpy(),
+                        E=torch.stack([Ex, Ey, Ez]).detach().numpy(),
+                        loss=loss.item(), grad=eps.grad.numpy(),
+                        meta=np.array(json.dumps({"shape": [D, H, W], "L": L, "T": T, "dl": 25.0,
+                                                  "courant": 0.4, "src_x": L + 1, "mon_x": xm,
+                                                  "lams": [1500.0, 1550.0, 1600.0]})))
+
+
+def fdtd2d():
+    out = {}
+    torch.manual_seed(11)
+    H, W = 24, 20
+    eps0 = 1.5 + 2.0 * torch.rand(H, W, dtype=torch.float64)
+    out["eps"] = eps0.numpy()
+    for pol in ("TM", "TE"):
+        for name, src in (("point", {"type": "gaussian_pulse", "pos": [9, 7]}),
+                          ("row", {"type": "gaussian_pulse", "row": 15, "amplitude": 0.5}),
+                          ("cw", {"type": "continuous"})):
+            s = FDTDSolver2D(grid_shape=(H, W), dl=20.0, polarization=pol, pml_layers=4, n_steps=80)
+            eps = eps0.clone().requires_grad_(True)
+            res = s.forward(eps, source=src)
+            out[f"{pol}_{name}_field"] = res.field.detach().numpy()
+            (res.field**2).sum().backward()
+            out[f"{pol}_{name}_grad"] = eps.grad.numpy()
+    s = FDTDSolver2D(grid_shape=(H, W), dl=20.0, pml_layers=4, n_steps=80)
+    out["ts"] = s.time_series(eps0, probe=(12, 10), n_steps=50).numpy()
+    np.savez_compressed(os.path.join(HERE, "fdtd2d.npz"), **out)
+
+
+def c0(seed=0):
+    """Full 200x200 TM, 2000 steps, 10-cell CPML, courant 0.5, point Jz source."""
+    H = W = 200
+    T = 2000
+    lam0 = 1550.0
+    spread = lam0**2 / (2 * math.pi * 200.0)
+    src = {"type": "gaussian_pulse", "pos": [40, 100], "spread": spread, "t0": 3 * spread}
+    s = FDTDSolver2D(grid_shape=(H, W), dl=20.0, wavelength_nm=lam0, pml_layers=10, n_steps=T,
+                     courant=0.5)
+    eps0 = blob_eps(seed)
+    mask = s._build_source_mask(src)
+    eps = torch.from_numpy(eps0).clone().requires_grad_(True)
+    mu = torch.ones_like(eps)
+    w = 2 * math.pi / lam0
+    waves = [s._source_waveform(n, src) for n in range(T)]
+
+    def seg(Ez, Hx, Hy, Fr, Fi, e, n0, n1):
+        for n in range(n0, n1):
+            Ez, Hx, Hy = s._time_step_tm(Ez, Hx, Hy, e, mu)
+            Ez = Ez + mask * waves[n]
+            ph = w * (n + 1) * s.dt
+            line = Ez[160, 70:130]
+            Fr = Fr + line * math.cos(ph)
+            Fi = Fi + line * -math.sin(ph)
+        return Ez, Hx, Hy, Fr, Fi
+
+    z = torch.zeros(H, W, dtype=torch.float64)
+    Ez, Hx, Hy = z.clone(), z.clone(), z.clone()
+    Fr = torch.zeros(60, dtype=torch.float64)
+    Fi = torch.zeros(60, dtype=torch.float64)
+    for n0 in range(0, T, 100):
+        Ez, Hx, Hy, Fr, Fi = cp.checkpoint(seg, Ez, Hx, Hy, Fr, Fi, eps, n0, n0 + 100,
+                                           use_reentrant=False)
+    loss = (Fr**2 + Fi**2).sum()
+    loss.backward()
+    # Public-API forward of the same case (final Ez only).
+    ref_field = s.forward(torch.from_numpy(eps0), source=src).field.numpy()
+    np.savez_compressed(os.path.join(HERE, "c0.npz"), eps=eps0, Ez=Ez.detach().numpy(),
+                        field=ref_field, Fr=Fr.detach().numpy(), Fi=Fi.detach().numpy(),
+                        loss=loss.item(), grad=eps.grad.numpy(),
+                        meta=np.array(json.dumps({"seed": seed, "src": src, "T": T})))
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["tables", "step3d", "fwd3d", "grad3d", "dft3d", "fdtd2d", "c0"]
+    for name in which:
+        globals()[name]()
+        print("wrote", name, flush=True)
