@@ -1,0 +1,141 @@
+"""ctypes binding of libdnx_fdtd.so (the C ABI declared in include/dnx_fdtd.h).
+
+The library is required: there is no CPU fallback.  ``lib()`` raises a
+RuntimeError naming the missing file when it has not been built
+(``python -m diffnano_amd._build``).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdnx_fdtd.so")
+ABI_VERSION = 1
+
+DNX_F32 = 0
+DNX_F64 = 1
+
+_lock = threading.Lock()
+_lib = None
+
+
+class Box(C.Structure):
+    _fields_ = [("z0", C.c_int32), ("z1", C.c_int32), ("y0", C.c_int32), ("y1", C.c_int32),
+                ("x0", C.c_int32), ("x1", C.c_int32)]
+
+
+class Grid3D(C.Structure):
+    _fields_ = [("nz", C.c_int32), ("ny", C.c_int32), ("nx", C.c_int32), ("dtype", C.c_int32),
+                ("kh", C.c_double), ("ce", C.c_void_p), ("by", C.c_void_p), ("ay", C.c_void_p),
+                ("bx", C.c_void_p), ("ax", C.c_void_p)]
+
+
+class Grid2D(C.Structure):
+    _fields_ = [("ny", C.c_int32), ("nx", C.c_int32), ("dtype", C.c_int32), ("te", C.c_int32),
+                ("kh", C.c_double), ("ce", C.c_void_p), ("by", C.c_void_p), ("ay", C.c_void_p),
+                ("bx", C.c_void_p), ("ax", C.c_void_p)]
+
+
+class Source(C.Structure):
+    _fields_ = [("comp", C.c_int32), ("box", Box), ("profile", C.c_void_p), ("wave", C.c_void_p)]
+
+
+class Monitor(C.Structure):
+    _fields_ = [("box", Box), ("comp_mask", C.c_int32), ("n_freq", C.c_int32),
+                ("phase", C.c_void_p), ("acc", C.c_void_p), ("grad", C.c_void_p)]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_L = C.c_int64
+
+_SIGS = {
+    "dnx_abi_version": ([], _I),
+    "dnx_last_error": ([], C.c_char_p),
+    "dnx_prepare_coef": ([_P, _P, _L, C.c_double, _I, _P], _I),
+    "dnx_finalize_grad": ([_P, _P, _P, _L, _I, _P], _I),
+    "dnx_fdtd3d_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, C.POINTER(Source), _I,
+                        C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),
+    "dnx_fdtd3d_adjoint_init": ([C.POINTER(Grid3D), _P, _P, C.POINTER(Monitor), _I, _I, _P, _P],
+                                _I),
+    "dnx_fdtd3d_adjoint_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, _P, _P, C.POINTER(Source), _I,
+                                C.POINTER(Monitor), _I, _P, _P, C.POINTER(_I)], _I),
+    "dnx_fdtd2d_run": ([C.POINTER(Grid2D), _P, _P, _I, _I, C.POINTER(Source), _I,
+                        C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),
+    "dnx_fdtd2d_adjoint_init": ([C.POINTER(Grid2D), _P, _P, C.POINTER(Monitor), _I, _I, _P, _P],
+                                _I),
+    "dnx_fdtd2d_adjoint_run": ([C.POINTER(Grid2D), _P, _P, _I, _I, _P, _P, C.POINTER(Source), _I,
+                                C.POINTER(Monitor), _I, _P, _P, C.POINTER(_I)], _I),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def load(path=LIB_PATH):
+    """Load the library and declare every signature (no GPU needed)."""
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"libdnx_fdtd.so not found at {path}: build it with `python -m diffnano_amd._build` "
+            "(the HIP library is required; there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    v = lib.dnx_abi_version()
+    if v != ABI_VERSION:
+        raise RuntimeError(f"libdnx_fdtd ABI version {v}, expected {ABI_VERSION}")
+    return lib
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                _lib = load()
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().dnx_last_error()
+        raise RuntimeError(f"libdnx_fdtd error {rc}: {msg.decode() if msg else '?'}")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_handle(device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dtype_code(dtype):
+    if dtype == torch.float32:
+        return DNX_F32
+    if dtype == torch.float64:
+        return DNX_F64
+    raise ValueError(f"compute dtype must be float32 or float64, got {dtype}")
+
+
+def compute_device(device=None):
+    """The HIP device the kernels run on.
+
+    ``device`` is the caller's result device; when it is a GPU it is used,
+    otherwise the current GPU is.  Raises when no GPU is visible: the
+    product has no CPU path.
+    """
+    if not torch.cuda.is_available():
+        raise RuntimeError("diffnano_amd runs its FDTD kernels on an AMD GPU (gfx950) and none is "
+                           "visible; there is no CPU fallback")
+    if device is not None:
+        d = torch.device(device)
+        if d.type == "cuda":
+            return d if d.index is not None else torch.device("cuda", torch.cuda.current_device())
+    return torch.device("cuda", torch.cuda.current_device())

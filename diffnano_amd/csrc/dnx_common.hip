@@ -1,0 +1,272 @@
+// libdnx_fdtd: error state, coefficient preparation, and the per-step side
+// kernels (soft sources, running DFT monitors and their adjoints).
+#include "dnx_aux.h"
+
+namespace dnx {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+int check_sources(const dnx_source* s, int n, int nz, int ny, int nx, int max_comp) {
+  DNX_REQUIRE(n >= 0 && (n == 0 || s != nullptr), "sources: null array with n_src=%d", n);
+  for (int i = 0; i < n; ++i) {
+    DNX_REQUIRE(s[i].comp >= 0 && s[i].comp <= max_comp, "source %d: bad component %d", i,
+                s[i].comp);
+    DNX_REQUIRE(box_ok(s[i].box, nz, ny, nx), "source %d: box outside the grid", i);
+    DNX_REQUIRE(s[i].profile && s[i].wave, "source %d: null profile or waveform", i);
+  }
+  return DNX_OK;
+}
+
+int check_monitors(const dnx_monitor* m, int n, int nz, int ny, int nx, int comp_bits,
+                   bool need_grad) {
+  DNX_REQUIRE(n >= 0 && (n == 0 || m != nullptr), "monitors: null array with n_mon=%d", n);
+  for (int i = 0; i < n; ++i) {
+    DNX_REQUIRE(box_ok(m[i].box, nz, ny, nx), "monitor %d: box outside the grid", i);
+    DNX_REQUIRE(m[i].comp_mask > 0 && (m[i].comp_mask & ~comp_bits) == 0,
+                "monitor %d: bad component mask %d", i, m[i].comp_mask);
+    DNX_REQUIRE(m[i].n_freq > 0 && m[i].phase, "monitor %d: no frequencies", i);
+    if (need_grad) {
+      DNX_REQUIRE(m[i].grad != nullptr, "monitor %d: null gradient", i);
+    } else {
+      DNX_REQUIRE(m[i].acc != nullptr, "monitor %d: null accumulator", i);
+    }
+  }
+  return DNX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_prepare_coef(const double* __restrict__ eps, T* __restrict__ ce, int64_t n,
+                               double k) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) ce[i] = T(k / eps[i]);
+}
+
+template <typename T>
+__global__ void k_finalize_grad(const T* __restrict__ S, const double* __restrict__ eps,
+                                double* __restrict__ g, int64_t n) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) g[i] = -double(S[i]) / eps[i];
+}
+
+template <typename T>
+__global__ void k_zero(T* __restrict__ p, int64_t n) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_dot_init(T* __restrict__ S, Comps<T> a, Comps<T> b, int ncomp, int64_t n) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T s = T(0);
+  for (int c = 0; c < ncomp; ++c) s += a.c[c][i] * b.c[c][i];
+  S[i] = s;
+}
+
+__device__ inline int64_t box_index(const Box& b, int64_t i, int ny, int nx) {
+  int bx = int(i % b.nx);
+  int64_t r = i / b.nx;
+  int by = int(r % b.ny);
+  int bz = int(r / b.ny);
+  return (int64_t(b.z0 + bz) * ny + (b.y0 + by)) * nx + (b.x0 + bx);
+}
+
+// f[cell] += wave[n] * prof[cell in box]
+template <typename T>
+__global__ void k_source(T* __restrict__ f, int ny, int nx, Box b, const T* __restrict__ prof,
+                         const T* __restrict__ wave_n) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.cells()) return;
+  f[box_index(b, i, ny, nx)] += wave_n[0] * prof[i];
+}
+
+template <typename T>
+__global__ void k_source_adj(T* __restrict__ S, const T* __restrict__ lam, int ny, int nx, Box b,
+                             const T* __restrict__ prof, const T* __restrict__ wave_m) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.cells()) return;
+  int64_t idx = box_index(b, i, ny, nx);
+  S[idx] -= wave_m[0] * prof[i] * lam[idx];
+}
+
+template <typename T>
+__global__ void k_monitor(Comps<T> E, int ny, int nx, Box b, int comp_mask, int nf,
+                          const double* __restrict__ ph, double* __restrict__ acc) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t cells = b.cells();
+  if (i >= cells) return;
+  int64_t idx = box_index(b, i, ny, nx);
+  int ncomp = ((comp_mask >> 0) & 1) + ((comp_mask >> 1) & 1) + ((comp_mask >> 2) & 1);
+  int ci = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (!((comp_mask >> c) & 1)) continue;
+    double v = double(E.c[c][idx]);
+    for (int f = 0; f < nf; ++f) {
+      double* a = acc + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+      a[0] += v * ph[2 * f];
+      a[1] -= v * ph[2 * f + 1];
+    }
+    ++ci;
+  }
+}
+
+template <typename T>
+__global__ void k_monitor_adj(Comps<T> lam, Comps<T> E, T* __restrict__ S, int use_s, int ny,
+                              int nx, Box b, int comp_mask, int nf,
+                              const double* __restrict__ ph, const double* __restrict__ g) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t cells = b.cells();
+  if (i >= cells) return;
+  int64_t idx = box_index(b, i, ny, nx);
+  int ncomp = ((comp_mask >> 0) & 1) + ((comp_mask >> 1) & 1) + ((comp_mask >> 2) & 1);
+  int ci = 0;
+  T sacc = T(0);
+  for (int c = 0; c < 3; ++c) {
+    if (!((comp_mask >> c) & 1)) continue;
+    double d = 0.0;
+    for (int f = 0; f < nf; ++f) {
+      const double* gg = g + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+      d += gg[0] * ph[2 * f] - gg[1] * ph[2 * f + 1];
+    }
+    T dv = T(d);
+    lam.c[c][idx] += dv;
+    if (use_s) sacc += E.c[c][idx] * dv;
+    ++ci;
+  }
+  if (use_s) S[idx] += sacc;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+int launch_source(T* comp_base, int ny, int nx, const dnx_source& s, int n, hipStream_t st) {
+  Box b = make_box(s.box);
+  hipLaunchKernelGGL(k_source<T>, dim3(cdiv(b.cells(), 256)), dim3(256), 0, st, comp_base, ny,
+                     nx, b, static_cast<const T*>(s.profile), static_cast<const T*>(s.wave) + n);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_source_adj(T* S, const T* lam_comp, int ny, int nx, const dnx_source& s, int m,
+                      hipStream_t st) {
+  Box b = make_box(s.box);
+  hipLaunchKernelGGL(k_source_adj<T>, dim3(cdiv(b.cells(), 256)), dim3(256), 0, st, S, lam_comp,
+                     ny, nx, b, static_cast<const T*>(s.profile),
+                     static_cast<const T*>(s.wave) + m);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_monitor(Comps<T> E, int ny, int nx, const dnx_monitor& m, int n, hipStream_t st) {
+  Box b = make_box(m.box);
+  hipLaunchKernelGGL(k_monitor<T>, dim3(cdiv(b.cells(), 256)), dim3(256), 0, st, E, ny, nx, b,
+                     m.comp_mask, m.n_freq, m.phase + int64_t(n) * m.n_freq * 2, m.acc);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_monitor_adj(Comps<T> lam, Comps<T> E, T* S, int ny, int nx, const dnx_monitor& m,
+                       int sample_row, hipStream_t st) {
+  Box b = make_box(m.box);
+  int use_s = (E.c[0] != nullptr || E.c[1] != nullptr || E.c[2] != nullptr) && S != nullptr;
+  hipLaunchKernelGGL(k_monitor_adj<T>, dim3(cdiv(b.cells(), 256)), dim3(256), 0, st, lam, E, S,
+                     use_s, ny, nx, b, m.comp_mask, m.n_freq,
+                     m.phase + int64_t(sample_row) * m.n_freq * 2, m.grad);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_zero(T* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return DNX_OK;
+  hipLaunchKernelGGL(k_zero<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p, n);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_dot_init(T* S, Comps<T> a, Comps<T> b, int ncomp, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_init<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, S, a, b, ncomp, n);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+#define DNX_INST(T)                                                                        \
+  template int launch_source<T>(T*, int, int, const dnx_source&, int, hipStream_t);      \
+  template int launch_source_adj<T>(T*, const T*, int, int, const dnx_source&, int,      \
+                                    hipStream_t);                                        \
+  template int launch_monitor<T>(Comps<T>, int, int, const dnx_monitor&, int, hipStream_t); \
+  template int launch_monitor_adj<T>(Comps<T>, Comps<T>, T*, int, int, const dnx_monitor&, \
+                                     int, hipStream_t);                                  \
+  template int launch_zero<T>(T*, int64_t, hipStream_t);                                 \
+  template int launch_dot_init<T>(T*, Comps<T>, Comps<T>, int, int64_t, hipStream_t);
+DNX_INST(float)
+DNX_INST(double)
+#undef DNX_INST
+
+}  // namespace dnx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t dnx_abi_version(void) { return DNX_ABI_VERSION; }
+
+extern "C" const char* dnx_last_error(void) { return dnx::last_error(); }
+
+extern "C" int32_t dnx_prepare_coef(const double* eps, void* ce, int64_t n, double dt_over_dl,
+                                    int32_t dtype, void* stream) {
+  using namespace dnx;
+  DNX_REQUIRE(eps && ce && n > 0, "prepare_coef: null pointer or n=%lld", (long long)n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == DNX_F32) {
+    hipLaunchKernelGGL(k_prepare_coef<float>, dim3(cdiv(n, 256)), dim3(256), 0, st, eps,
+                       static_cast<float*>(ce), n, dt_over_dl);
+  } else if (dtype == DNX_F64) {
+    hipLaunchKernelGGL(k_prepare_coef<double>, dim3(cdiv(n, 256)), dim3(256), 0, st, eps,
+                       static_cast<double*>(ce), n, dt_over_dl);
+  } else {
+    DNX_REQUIRE(false, "prepare_coef: bad dtype %d", dtype);
+  }
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_finalize_grad(const void* S, const double* eps, double* grad_eps,
+                                     int64_t n, int32_t dtype, void* stream) {
+  using namespace dnx;
+  DNX_REQUIRE(S && eps && grad_eps && n > 0, "finalize_grad: null pointer or n=%lld",
+              (long long)n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == DNX_F32) {
+    hipLaunchKernelGGL(k_finalize_grad<float>, dim3(cdiv(n, 256)), dim3(256), 0, st,
+                       static_cast<const float*>(S), eps, grad_eps, n);
+  } else if (dtype == DNX_F64) {
+    hipLaunchKernelGGL(k_finalize_grad<double>, dim3(cdiv(n, 256)), dim3(256), 0, st,
+                       static_cast<const double*>(S), eps, grad_eps, n);
+  } else {
+    DNX_REQUIRE(false, "finalize_grad: bad dtype %d", dtype);
+  }
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}

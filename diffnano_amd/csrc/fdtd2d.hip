@@ -1,0 +1,378 @@
+// 2D Yee FDTD on gfx950 (DiffNano FDTDSolver2D): TM and TE leapfrog steps
+// and their exact discrete adjoints.
+//
+// Rows (axis 0, ny) carry the profile (by, ay = 1 + cy); columns (axis 1, nx)
+// carry (bx, ax).  kh = dt/dl, ce = dt/(eps dl).  D+r / D-r differ along
+// rows (axis 0), D+c / D-c along columns (axis 1); zero at the truncated end.
+// TM (fdtd2d.py:233-267), state [Ez, Hx, Hy]:
+//   Hx' = by Hx - kh ay D+c Ez
+//   Hy' = bx Hy + kh ax D+r Ez
+//   Ez' = bxby Ez + ce (ax D-r Hy' - ay D-c Hx')
+// TE (fdtd2d.py:269-303), state [Hz, Ex, Ey], E first, reference Hz sign:
+//   Ex' = bx Ex + ce ax D-c Hz
+//   Ey' = by Ey - ce ay D-r Hz
+//   Hz' = bxby Hz + kh (ax D+r Ey' - ay D+c Ex')
+#include "dnx_aux.h"
+
+namespace dnx {
+
+template <typename T>
+struct G2 {
+  int ny, nx;
+  int64_t n;
+  T kh;
+  const T* __restrict__ ce;
+  const T* __restrict__ by;
+  const T* __restrict__ ay;
+  const T* __restrict__ bx;
+  const T* __restrict__ ax;
+};
+
+constexpr int BX2 = 64;
+constexpr int BY2 = 4;
+
+template <typename T>
+static G2<T> make_g2(const dnx_grid2d* g) {
+  G2<T> r;
+  r.ny = g->ny;
+  r.nx = g->nx;
+  r.n = int64_t(g->ny) * g->nx;
+  r.kh = T(g->kh);
+  r.ce = static_cast<const T*>(g->ce);
+  r.by = static_cast<const T*>(g->by);
+  r.ay = static_cast<const T*>(g->ay);
+  r.bx = static_cast<const T*>(g->bx);
+  r.ax = static_cast<const T*>(g->ax);
+  return r;
+}
+
+#define DNX_2D_PROLOGUE                                  \
+  const int c = blockIdx.x * BX2 + threadIdx.x;         \
+  const int r = blockIdx.y * BY2 + threadIdx.y;         \
+  if (c >= g.nx || r >= g.ny) return;                    \
+  const int64_t N = g.n, nx = g.nx;                      \
+  const int64_t i = int64_t(r) * nx + c;
+
+// ---- TM forward ----------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2tm_h(G2<T> g, T* __restrict__ st) {
+  DNX_2D_PROLOGUE
+  const T* Ez = st;
+  T* Hx = st + N;
+  T* Hy = st + 2 * N;
+  const T ez = Ez[i];
+  const T dc = (c + 1 < g.nx) ? Ez[i + 1] - ez : T(0);
+  const T dr = (r + 1 < g.ny) ? Ez[i + nx] - ez : T(0);
+  Hx[i] = g.by[r] * Hx[i] - g.kh * (g.ay[r] * dc);
+  Hy[i] = g.bx[c] * Hy[i] + g.kh * (g.ax[c] * dr);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2tm_e(G2<T> g, T* __restrict__ st,
+                                                   T* __restrict__ es) {
+  DNX_2D_PROLOGUE
+  T* Ez = st;
+  const T* Hx = st + N;
+  const T* Hy = st + 2 * N;
+  const T dHy = r > 0 ? Hy[i] - Hy[i - nx] : T(0);
+  const T dHx = c > 0 ? Hx[i] - Hx[i - 1] : T(0);
+  const T ez = (g.bx[c] * g.by[r]) * Ez[i] + g.ce[i] * (g.ax[c] * dHy - g.ay[r] * dHx);
+  Ez[i] = ez;
+  if (es) es[i] = ez;
+}
+
+// ---- TE forward ----------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2te_e(G2<T> g, T* __restrict__ st) {
+  DNX_2D_PROLOGUE
+  const T* Hz = st;
+  T* Ex = st + N;
+  T* Ey = st + 2 * N;
+  const T hz = Hz[i];
+  const T dc = c > 0 ? hz - Hz[i - 1] : T(0);
+  const T dr = r > 0 ? hz - Hz[i - nx] : T(0);
+  const T ce = g.ce[i];
+  Ex[i] = g.bx[c] * Ex[i] + ce * (g.ax[c] * dc);
+  Ey[i] = g.by[r] * Ey[i] - ce * (g.ay[r] * dr);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2te_h(G2<T> g, T* __restrict__ st,
+                                                   T* __restrict__ es) {
+  DNX_2D_PROLOGUE
+  T* Hz = st;
+  const T* Ex = st + N;
+  const T* Ey = st + 2 * N;
+  const T dEy = (r + 1 < g.ny) ? Ey[i + nx] - Ey[i] : T(0);
+  const T dEx = (c + 1 < g.nx) ? Ex[i + 1] - Ex[i] : T(0);
+  const T hz = (g.bx[c] * g.by[r]) * Hz[i] + g.kh * (g.ax[c] * dEy - g.ay[r] * dEx);
+  Hz[i] = hz;
+  if (es) es[i] = hz;
+}
+
+// ---- TM adjoint: lam = [lamEz, muHx, muHy] -------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2tm_adj_h(G2<T> g, T* __restrict__ lam) {
+  DNX_2D_PROLOGUE
+  const T* L = lam;
+  T* Mx = lam + N;
+  T* My = lam + 2 * N;
+  const T u = g.ce[i] * L[i];
+  // muHx = by muHx - ay D-c^T(u);  muHy = bx muHy + D-r^T(ax u)
+  const T tc = (c >= 1 ? u : T(0)) - (c + 1 < g.nx ? g.ce[i + 1] * L[i + 1] : T(0));
+  const T tr = (r >= 1 ? u : T(0)) - (r + 1 < g.ny ? g.ce[i + nx] * L[i + nx] : T(0));
+  Mx[i] = g.by[r] * Mx[i] - g.ay[r] * tc;
+  My[i] = g.bx[c] * My[i] + g.ax[c] * tr;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2tm_adj_e(G2<T> g, T* __restrict__ lam,
+                                                       const T* __restrict__ em,
+                                                       T* __restrict__ S) {
+  DNX_2D_PROLOGUE
+  T* L = lam;
+  const T* Mx = lam + N;
+  const T* My = lam + 2 * N;
+  // t = kh (ax D+r^T(muHy) - ay D+c^T(muHx))
+  const T r_my = (r >= 1 ? My[i - nx] : T(0)) - (r <= g.ny - 2 ? My[i] : T(0));
+  const T c_mx = (c >= 1 ? Mx[i - 1] : T(0)) - (c <= g.nx - 2 ? Mx[i] : T(0));
+  const T t = g.kh * (g.ax[c] * r_my - g.ay[r] * c_mx);
+  L[i] = (g.bx[c] * g.by[r]) * L[i] + t;
+  S[i] += em[i] * t;
+}
+
+// ---- TE adjoint: lam = [lamHz, muEx, muEy] -------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2te_adj_e(G2<T> g, T* __restrict__ lam,
+                                                       const T* __restrict__ hzm,
+                                                       T* __restrict__ S) {
+  DNX_2D_PROLOGUE
+  const T* L = lam;
+  T* Mx = lam + N;
+  T* My = lam + 2 * N;
+  // muEx = bx muEx - kh ay D+c^T(lamHz); muEy = by muEy + kh ax D+r^T(lamHz)
+  const T l = L[i];
+  const T tc = (c >= 1 ? L[i - 1] : T(0)) - (c <= g.nx - 2 ? l : T(0));
+  const T tr = (r >= 1 ? L[i - nx] : T(0)) - (r <= g.ny - 2 ? l : T(0));
+  const T mx = g.bx[c] * Mx[i] - g.kh * g.ay[r] * tc;
+  const T my = g.by[r] * My[i] + g.kh * g.ax[c] * tr;
+  Mx[i] = mx;
+  My[i] = my;
+  if (hzm) {
+    // S += ce (muEx ax D-c Hz^m - muEy ay D-r Hz^m)
+    const T h = hzm[i];
+    const T dc = c > 0 ? h - hzm[i - 1] : T(0);
+    const T dr = r > 0 ? h - hzm[i - nx] : T(0);
+    S[i] += g.ce[i] * (mx * g.ax[c] * dc - my * g.ay[r] * dr);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2te_adj_h(G2<T> g, T* __restrict__ lam) {
+  DNX_2D_PROLOGUE
+  T* L = lam;
+  const T* Mx = lam + N;
+  const T* My = lam + 2 * N;
+  // lamHz = bxby lamHz + D-c^T(ax ce muEx) - D-r^T(ay ce muEy)
+  const T vx = g.ce[i] * Mx[i];
+  const T vy = g.ce[i] * My[i];
+  const T tc = (c >= 1 ? g.ax[c] * vx : T(0)) -
+               (c + 1 < g.nx ? g.ax[c + 1] * g.ce[i + 1] * Mx[i + 1] : T(0));
+  const T tr = (r >= 1 ? g.ay[r] * vy : T(0)) -
+               (r + 1 < g.ny ? g.ay[r + 1] * g.ce[i + nx] * My[i + nx] : T(0));
+  L[i] = (g.bx[c] * g.by[r]) * L[i] + tc - tr;
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers
+// ---------------------------------------------------------------------------
+
+static int check_grid2(const dnx_grid2d* g) {
+  DNX_REQUIRE(g != nullptr, "grid2d: null");
+  DNX_REQUIRE(g->ny > 0 && g->nx > 0, "grid2d: bad shape %d x %d", g->ny, g->nx);
+  DNX_REQUIRE(g->dtype == DNX_F32 || g->dtype == DNX_F64, "grid2d: bad dtype %d", g->dtype);
+  DNX_REQUIRE(g->te == 0 || g->te == 1, "grid2d: te must be 0 or 1");
+  DNX_REQUIRE(g->ce && g->by && g->ay && g->bx && g->ax, "grid2d: null coefficient array");
+  return DNX_OK;
+}
+
+static dim3 grid2_launch(const dnx_grid2d* g) { return dim3(cdiv(g->nx, BX2), cdiv(g->ny, BY2)); }
+
+template <typename T>
+static int run2(const dnx_grid2d* gd, T* st, int n0, int n_steps, const dnx_source* srcs,
+                int n_src, const dnx_monitor* mons, int n_mon, T* ckpt, int ckpt_every, T* es,
+                int64_t probe, T* probe_out, hipStream_t s) {
+  G2<T> g = make_g2<T>(gd);
+  const dim3 grid = grid2_launch(gd), block(BX2, BY2);
+  const int64_t N = g.n;
+  Comps<T> F{{st, nullptr, nullptr}};
+  for (int k = 0; k < n_steps; ++k) {
+    const int n = n0 + k;
+    if (ckpt && k % ckpt_every == 0) {
+      DNX_CHECK_HIP(hipMemcpyAsync(ckpt + int64_t(k / ckpt_every) * 3 * N, st,
+                                   sizeof(T) * 3 * N, hipMemcpyDeviceToDevice, s));
+    }
+    T* slot = es ? es + int64_t(k) * N : nullptr;
+    if (gd->te) {
+      hipLaunchKernelGGL(k2te_e<T>, grid, block, 0, s, g, st);
+      hipLaunchKernelGGL(k2te_h<T>, grid, block, 0, s, g, st, slot);
+    } else {
+      hipLaunchKernelGGL(k2tm_h<T>, grid, block, 0, s, g, st);
+      hipLaunchKernelGGL(k2tm_e<T>, grid, block, 0, s, g, st, slot);
+    }
+    DNX_CHECK_HIP(hipGetLastError());
+    for (int j = 0; j < n_src; ++j) {
+      int rc = launch_source<T>(st, g.ny, g.nx, srcs[j], n, s);
+      if (rc) return rc;
+      if (slot && (rc = launch_source<T>(slot, g.ny, g.nx, srcs[j], n, s))) return rc;
+    }
+    if (probe >= 0) {
+      DNX_CHECK_HIP(hipMemcpyAsync(probe_out + k, st + probe, sizeof(T), hipMemcpyDeviceToDevice,
+                                   s));
+    }
+    for (int j = 0; j < n_mon; ++j) {
+      int rc = launch_monitor<T>(F, g.ny, g.nx, mons[j], n, s);
+      if (rc) return rc;
+    }
+  }
+  return DNX_OK;
+}
+
+template <typename T>
+static int adj_init2(const dnx_grid2d* gd, T* lam, const T* eT, const dnx_monitor* mons,
+                     int n_mon, int n_total, T* S, hipStream_t s) {
+  G2<T> g = make_g2<T>(gd);
+  const int64_t N = g.n;
+  Comps<T> L{{lam, nullptr, nullptr}};
+  Comps<T> none{{nullptr, nullptr, nullptr}};
+  for (int j = 0; j < n_mon; ++j) {
+    int rc = launch_monitor_adj<T>(L, none, nullptr, g.ny, g.nx, mons[j], n_total - 1, s);
+    if (rc) return rc;
+  }
+  int rc = launch_zero<T>(lam + N, 2 * N, s);
+  if (rc) return rc;
+  if (gd->te) return launch_zero<T>(S, N, s);
+  Comps<T> Ev{{const_cast<T*>(eT), nullptr, nullptr}};
+  return launch_dot_init<T>(S, Ev, L, 1, N, s);
+}
+
+template <typename T>
+static int adj_run2(const dnx_grid2d* gd, T* lam, int m_hi, int m_lo, const T* e_lo,
+                    const T* es, const dnx_source* srcs, int n_src, const dnx_monitor* mons,
+                    int n_mon, T* S, hipStream_t s) {
+  G2<T> g = make_g2<T>(gd);
+  const dim3 grid = grid2_launch(gd), block(BX2, BY2);
+  const int64_t N = g.n;
+  Comps<T> L{{lam, nullptr, nullptr}};
+  for (int m = m_hi - 1; m >= m_lo && m >= 1; --m) {
+    const T* em = (m == m_lo) ? e_lo : es + int64_t(m - m_lo - 1) * N;
+    Comps<T> Em{{const_cast<T*>(em), nullptr, nullptr}};
+    if (gd->te) {
+      hipLaunchKernelGGL(k2te_adj_e<T>, grid, block, 0, s, g, lam, em, S);
+      hipLaunchKernelGGL(k2te_adj_h<T>, grid, block, 0, s, g, lam);
+      DNX_CHECK_HIP(hipGetLastError());
+      Comps<T> none{{nullptr, nullptr, nullptr}};
+      for (int j = 0; j < n_mon; ++j) {
+        int rc = launch_monitor_adj<T>(L, none, nullptr, g.ny, g.nx, mons[j], m - 1, s);
+        if (rc) return rc;
+      }
+    } else {
+      for (int j = 0; j < n_src; ++j) {
+        int rc = launch_source_adj<T>(S, lam, g.ny, g.nx, srcs[j], m, s);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL(k2tm_adj_h<T>, grid, block, 0, s, g, lam);
+      hipLaunchKernelGGL(k2tm_adj_e<T>, grid, block, 0, s, g, lam, em, S);
+      DNX_CHECK_HIP(hipGetLastError());
+      for (int j = 0; j < n_mon; ++j) {
+        int rc = launch_monitor_adj<T>(L, Em, S, g.ny, g.nx, mons[j], m - 1, s);
+        if (rc) return rc;
+      }
+    }
+  }
+  if (m_lo == 0 && !gd->te) {
+    for (int j = 0; j < n_src; ++j) {
+      int rc = launch_source_adj<T>(S, lam, g.ny, g.nx, srcs[j], 0, s);
+      if (rc) return rc;
+    }
+  }
+  return DNX_OK;
+}
+
+}  // namespace dnx
+
+using namespace dnx;
+
+extern "C" int32_t dnx_fdtd2d_run(const dnx_grid2d* g, void* state, void* work, int32_t n0,
+                                  int32_t n_steps, const dnx_source* srcs, int32_t n_src,
+                                  const dnx_monitor* mons, int32_t n_mon, void* ckpt,
+                                  int32_t ckpt_every, void* estore, int64_t probe_index,
+                                  void* probe_out, void* stream, int32_t* final_in_work) {
+  (void)work;
+  int rc = check_grid2(g);
+  if (rc) return rc;
+  DNX_REQUIRE(state != nullptr, "fdtd2d_run: null state");
+  DNX_REQUIRE(n0 >= 0 && n_steps >= 0, "fdtd2d_run: bad step range");
+  DNX_REQUIRE(ckpt == nullptr || ckpt_every > 0, "fdtd2d_run: ckpt_every must be > 0");
+  if ((rc = check_sources(srcs, n_src, 1, g->ny, g->nx, 0))) return rc;
+  if ((rc = check_monitors(mons, n_mon, 1, g->ny, g->nx, 1, false))) return rc;
+  DNX_REQUIRE(probe_index < 0 || (probe_out != nullptr &&
+                                   probe_index < int64_t(3) * g->ny * g->nx),
+              "run: bad probe index %lld", (long long)probe_index);
+  if (final_in_work) *final_in_work = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->dtype == DNX_F32)
+    return run2<float>(g, static_cast<float*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
+                       static_cast<float*>(ckpt), ckpt_every, static_cast<float*>(estore),
+                       probe_index, static_cast<float*>(probe_out), s);
+  return run2<double>(g, static_cast<double*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
+                      static_cast<double*>(ckpt), ckpt_every, static_cast<double*>(estore),
+                      probe_index, static_cast<double*>(probe_out), s);
+}
+
+extern "C" int32_t dnx_fdtd2d_adjoint_init(const dnx_grid2d* g, void* lam, const void* e_T,
+                                           const dnx_monitor* mons, int32_t n_mon,
+                                           int32_t n_total, void* S, void* stream) {
+  int rc = check_grid2(g);
+  if (rc) return rc;
+  DNX_REQUIRE(lam && S && (g->te || e_T), "fdtd2d_adjoint_init: null pointer");
+  DNX_REQUIRE(n_total >= 1, "fdtd2d_adjoint_init: n_total must be >= 1");
+  if ((rc = check_monitors(mons, n_mon, 1, g->ny, g->nx, 1, true))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->dtype == DNX_F32)
+    return adj_init2<float>(g, static_cast<float*>(lam), static_cast<const float*>(e_T), mons,
+                            n_mon, n_total, static_cast<float*>(S), s);
+  return adj_init2<double>(g, static_cast<double*>(lam), static_cast<const double*>(e_T), mons,
+                           n_mon, n_total, static_cast<double*>(S), s);
+}
+
+extern "C" int32_t dnx_fdtd2d_adjoint_run(const dnx_grid2d* g, void* lam, void* work,
+                                          int32_t m_hi, int32_t m_lo, const void* e_lo,
+                                          const void* estore, const dnx_source* srcs,
+                                          int32_t n_src, const dnx_monitor* mons, int32_t n_mon,
+                                          void* S, void* stream, int32_t* final_in_work) {
+  (void)work;
+  int rc = check_grid2(g);
+  if (rc) return rc;
+  DNX_REQUIRE(lam && S, "fdtd2d_adjoint_run: null pointer");
+  DNX_REQUIRE(m_lo >= 0 && m_hi >= m_lo, "fdtd2d_adjoint_run: bad range [%d, %d)", m_lo, m_hi);
+  DNX_REQUIRE(!(m_hi - 1 > m_lo && m_hi - 1 >= 1) || estore != nullptr,
+              "fdtd2d_adjoint_run: estore required");
+  DNX_REQUIRE(!(m_lo >= 1 && m_hi > m_lo) || e_lo != nullptr,
+              "fdtd2d_adjoint_run: e_lo required");
+  if ((rc = check_sources(srcs, n_src, 1, g->ny, g->nx, 0))) return rc;
+  if ((rc = check_monitors(mons, n_mon, 1, g->ny, g->nx, 1, true))) return rc;
+  if (final_in_work) *final_in_work = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->dtype == DNX_F32)
+    return adj_run2<float>(g, static_cast<float*>(lam), m_hi, m_lo,
+                           static_cast<const float*>(e_lo), static_cast<const float*>(estore),
+                           srcs, n_src, mons, n_mon, static_cast<float*>(S), s);
+  return adj_run2<double>(g, static_cast<double*>(lam), m_hi, m_lo,
+                          static_cast<const double*>(e_lo), static_cast<const double*>(estore),
+                          srcs, n_src, mons, n_mon, static_cast<double*>(S), s);
+}

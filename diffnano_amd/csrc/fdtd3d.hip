@@ -1,0 +1,432 @@
+// 3D Yee FDTD on gfx950: forward leapfrog, replay with E storage, and the
+// exact discrete adjoint (reverse sweep) of DiffNano's FDTDSolver3D.
+//
+// Semantics (fdtd3d.py:378-465), with kh = dt/dl, ce = dt/(eps dl), the
+// axis-1 profile (by, ay = 1 + cy) and the axis-2 profile (bx, ax = 1 + cx):
+//   Hx' = by  Hx - kh (ay D+y Ez - ax D+z Ey)
+//   Hy' = bx  Hy - kh (ax D+z Ex - ay D+x Ez)
+//   Hz' = bx  Hz - kh (ax D+x Ey - ay D+y Ex)
+//   Ex' = bxby Ex + ce (ay D-y Hz' - ax D-z Hy')
+//   Ey' = bxby Ey + ce (ax D-z Hx' - ay D-x Hz')
+//   Ez' = bxby Ez + ce (ax D-x Hy' - ay D-y Hx')
+// D+ is the forward difference, zero at the last index; D- the backward
+// difference, zero at index 0.  The reference applies its z-named profile
+// along axis 2 and its x-named profile (in two terms) along axis 1, so no
+// profile varies along axis 0; on cubic grids all named profiles are equal.
+//
+// Kernel layout: a 64x4 thread tile in (x, y), each thread marching a chunk
+// of z so that the z-neighbour comes from a register; x is the fastest axis
+// so every wave's loads are 256-byte coalesced rows.
+#include "dnx_aux.h"
+
+namespace dnx {
+
+template <typename T>
+struct G3 {
+  int nz, ny, nx;
+  int64_t plane, n;
+  T kh;
+  const T* __restrict__ ce;
+  const T* __restrict__ by;
+  const T* __restrict__ ay;
+  const T* __restrict__ bx;
+  const T* __restrict__ ax;
+};
+
+constexpr int TX = 64;
+constexpr int TY = 4;
+constexpr int ZC = 16;  // z planes per thread
+
+template <typename T>
+static G3<T> make_g3(const dnx_grid3d* g) {
+  G3<T> r;
+  r.nz = g->nz;
+  r.ny = g->ny;
+  r.nx = g->nx;
+  r.plane = int64_t(g->ny) * g->nx;
+  r.n = r.plane * g->nz;
+  r.kh = T(g->kh);
+  r.ce = static_cast<const T*>(g->ce);
+  r.by = static_cast<const T*>(g->by);
+  r.ay = static_cast<const T*>(g->ay);
+  r.bx = static_cast<const T*>(g->bx);
+  r.ax = static_cast<const T*>(g->ax);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(TX* TY) void k3_update_h(G3<T> g, T* __restrict__ st) {
+  const int x = blockIdx.x * TX + threadIdx.x;
+  const int y = blockIdx.y * TY + threadIdx.y;
+  if (x >= g.nx || y >= g.ny) return;
+  const int z0 = blockIdx.z * ZC;
+  const int z1 = min(g.nz, z0 + ZC);
+  const int64_t N = g.n, P = g.plane, nx = g.nx;
+  const T* Ex = st;
+  const T* Ey = st + N;
+  const T* Ez = st + 2 * N;
+  T* Hx = st + 3 * N;
+  T* Hy = st + 4 * N;
+  T* Hz = st + 5 * N;
+  const T by = g.by[y], ay = g.ay[y], bx = g.bx[x], ax = g.ax[x], kh = g.kh;
+  const bool yl = y + 1 < g.ny, xl = x + 1 < g.nx;
+  int64_t i = z0 * P + int64_t(y) * nx + x;
+  T ex = Ex[i], ey = Ey[i];
+  for (int z = z0; z < z1; ++z, i += P) {
+    const bool zl = z + 1 < g.nz;
+    const T exn = zl ? Ex[i + P] : T(0);
+    const T eyn = zl ? Ey[i + P] : T(0);
+    const T ez = Ez[i];
+    const T dEz_dy = yl ? Ez[i + nx] - ez : T(0);
+    const T dEy_dz = zl ? eyn - ey : T(0);
+    const T dEx_dz = zl ? exn - ex : T(0);
+    const T dEz_dx = xl ? Ez[i + 1] - ez : T(0);
+    const T dEy_dx = xl ? Ey[i + 1] - ey : T(0);
+    const T dEx_dy = yl ? Ex[i + nx] - ex : T(0);
+    Hx[i] = by * Hx[i] - kh * (ay * dEz_dy - ax * dEy_dz);
+    Hy[i] = bx * Hy[i] - kh * (ax * dEx_dz - ay * dEz_dx);
+    Hz[i] = bx * Hz[i] - kh * (ax * dEy_dx - ay * dEx_dy);
+    ex = exn;
+    ey = eyn;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TX* TY) void k3_update_e(G3<T> g, T* __restrict__ st,
+                                                      T* __restrict__ es) {
+  const int x = blockIdx.x * TX + threadIdx.x;
+  const int y = blockIdx.y * TY + threadIdx.y;
+  if (x >= g.nx || y >= g.ny) return;
+  const int z0 = blockIdx.z * ZC;
+  const int z1 = min(g.nz, z0 + ZC);
+  const int64_t N = g.n, P = g.plane, nx = g.nx;
+  T* Ex = st;
+  T* Ey = st + N;
+  T* Ez = st + 2 * N;
+  const T* Hx = st + 3 * N;
+  const T* Hy = st + 4 * N;
+  const T* Hz = st + 5 * N;
+  const T by = g.by[y], ay = g.ay[y], bx = g.bx[x], ax = g.ax[x];
+  const T bxy = bx * by;
+  const bool yf = y > 0, xf = x > 0;
+  int64_t i = z0 * P + int64_t(y) * nx + x;
+  T hxp = z0 > 0 ? Hx[i - P] : T(0);
+  T hyp = z0 > 0 ? Hy[i - P] : T(0);
+  for (int z = z0; z < z1; ++z, i += P) {
+    const bool zf = z > 0;
+    const T hx = Hx[i], hy = Hy[i], hz = Hz[i];
+    const T dHz_dy = yf ? hz - Hz[i - nx] : T(0);
+    const T dHy_dz = zf ? hy - hyp : T(0);
+    const T dHx_dz = zf ? hx - hxp : T(0);
+    const T dHz_dx = xf ? hz - Hz[i - 1] : T(0);
+    const T dHy_dx = xf ? hy - Hy[i - 1] : T(0);
+    const T dHx_dy = yf ? hx - Hx[i - nx] : T(0);
+    const T c = g.ce[i];
+    const T ex = bxy * Ex[i] + c * (ay * dHz_dy - ax * dHy_dz);
+    const T ey = bxy * Ey[i] + c * (ax * dHx_dz - ay * dHz_dx);
+    const T ez = bxy * Ez[i] + c * (ax * dHy_dx - ay * dHx_dy);
+    Ex[i] = ex;
+    Ey[i] = ey;
+    Ez[i] = ez;
+    if (es) {
+      es[i] = ex;
+      es[N + i] = ey;
+      es[2 * N + i] = ez;
+    }
+    hxp = hx;
+    hyp = hy;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint.  lam = [lamEx, lamEy, lamEz, muHx, muHy, muHz].
+//   kernel A:  muH <- B_H muH + K_E^T (ce lamE)       (in place on muH)
+//   kernel B:  t = kh K_H^T-part(muH);  lamE <- bxby lamE + t;  S += E^m . t
+// with the exact transposes of the truncated differences:
+//   (D-^T g)[j] = [j>=1] g[j] - [j+1<n] g[j+1]
+//   (D+^T g)[j] = [j>=1] g[j-1] - [j<=n-2] g[j]
+// and each profile factor applied at the index where g lives.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(TX* TY) void k3_adj_h(G3<T> g, T* __restrict__ lam) {
+  const int x = blockIdx.x * TX + threadIdx.x;
+  const int y = blockIdx.y * TY + threadIdx.y;
+  if (x >= g.nx || y >= g.ny) return;
+  const int z0 = blockIdx.z * ZC;
+  const int z1 = min(g.nz, z0 + ZC);
+  const int64_t N = g.n, P = g.plane, nx = g.nx;
+  const T* Lx = lam;
+  const T* Ly = lam + N;
+  const T* Lz = lam + 2 * N;
+  T* Mx = lam + 3 * N;
+  T* My = lam + 4 * N;
+  T* Mz = lam + 5 * N;
+  const T by = g.by[y], ay = g.ay[y], bx = g.bx[x], ax = g.ax[x];
+  const bool my = y >= 1, myp = y + 1 < g.ny, mx = x >= 1, mxp = x + 1 < g.nx;
+  const T ayp = myp ? g.ay[y + 1] : T(0);
+  const T axp = mxp ? g.ax[x + 1] : T(0);
+  int64_t i = z0 * P + int64_t(y) * nx + x;
+  // u = ce * lamE at the current plane (x, y) for the z-direction terms
+  T cez = g.ce[i];
+  T ux = cez * Lx[i], uy = cez * Ly[i];
+  for (int z = z0; z < z1; ++z, i += P) {
+    const bool mz = z >= 1, mzp = z + 1 < g.nz;
+    T uxn = T(0), uyn = T(0);
+    if (mzp) {
+      const T cn = g.ce[i + P];
+      uxn = cn * Lx[i + P];
+      uyn = cn * Ly[i + P];
+    }
+    const T c = g.ce[i];
+    const T uz = c * Lz[i];
+    const T uz_y1 = myp ? g.ce[i + nx] * Lz[i + nx] : T(0);
+    const T uz_x1 = mxp ? g.ce[i + 1] * Lz[i + 1] : T(0);
+    const T ux_y1 = myp ? g.ce[i + nx] * Lx[i + nx] : T(0);
+    const T uy_x1 = mxp ? g.ce[i + 1] * Ly[i + 1] : T(0);
+    const T tz_y = (mz ? uy : T(0)) - (mzp ? uyn : T(0));  // D-z^T (uEy)
+    const T tz_x = (mz ? ux : T(0)) - (mzp ? uxn : T(0));  // D-z^T (uEx)
+    const T ty_z = (my ? ay * uz : T(0)) - ayp * uz_y1;    // D-y^T (ay uEz)
+    const T tx_z = (mx ? ax * uz : T(0)) - axp * uz_x1;    // D-x^T (ax uEz)
+    const T ty_x = (my ? ay * ux : T(0)) - ayp * ux_y1;    // D-y^T (ay uEx)
+    const T tx_y = (mx ? uy : T(0)) - uy_x1;               // D-x^T (uEy)
+    Mx[i] = by * Mx[i] + ax * tz_y - ty_z;
+    My[i] = bx * My[i] + tx_z - ax * tz_x;
+    Mz[i] = bx * Mz[i] + ty_x - ay * tx_y;
+    ux = uxn;
+    uy = uyn;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TX* TY) void k3_adj_e(G3<T> g, T* __restrict__ lam,
+                                                   const T* __restrict__ em, T* __restrict__ S) {
+  const int x = blockIdx.x * TX + threadIdx.x;
+  const int y = blockIdx.y * TY + threadIdx.y;
+  if (x >= g.nx || y >= g.ny) return;
+  const int z0 = blockIdx.z * ZC;
+  const int z1 = min(g.nz, z0 + ZC);
+  const int64_t N = g.n, P = g.plane, nx = g.nx;
+  T* Lx = lam;
+  T* Ly = lam + N;
+  T* Lz = lam + 2 * N;
+  const T* Mx = lam + 3 * N;
+  const T* My = lam + 4 * N;
+  const T* Mz = lam + 5 * N;
+  const T by = g.by[y], ay = g.ay[y], bx = g.bx[x], ax = g.ax[x], kh = g.kh;
+  const T bxy = bx * by;
+  const bool yf = y >= 1, yl = y <= g.ny - 2, xf = x >= 1, xl = x <= g.nx - 2;
+  const T aym = yf ? g.ay[y - 1] : T(0);
+  const T axm = xf ? g.ax[x - 1] : T(0);
+  int64_t i = z0 * P + int64_t(y) * nx + x;
+  T mxp = z0 > 0 ? Mx[i - P] : T(0);
+  T myp = z0 > 0 ? My[i - P] : T(0);
+  for (int z = z0; z < z1; ++z, i += P) {
+    const bool zf = z >= 1, zl = z <= g.nz - 2;
+    const T mx = Mx[i], my = My[i], mz = Mz[i];
+    // D+^T terms
+    const T y_mz = (yf ? aym * Mz[i - nx] : T(0)) - (yl ? ay * mz : T(0));  // D+y^T(ay muHz)
+    const T z_my = (zf ? myp : T(0)) - (zl ? my : T(0));                     // D+z^T(muHy)
+    const T z_mx = (zf ? mxp : T(0)) - (zl ? mx : T(0));                     // D+z^T(muHx)
+    const T x_mz = (xf ? axm * Mz[i - 1] : T(0)) - (xl ? ax * mz : T(0));   // D+x^T(ax muHz)
+    const T x_my = (xf ? My[i - 1] : T(0)) - (xl ? my : T(0));               // D+x^T(muHy)
+    const T y_mx = (yf ? aym * Mx[i - nx] : T(0)) - (yl ? ay * mx : T(0));  // D+y^T(ay muHx)
+    const T tx = kh * (y_mz - ax * z_my);
+    const T ty = kh * (ax * z_mx - x_mz);
+    const T tz = kh * (ay * x_my - y_mx);
+    Lx[i] = bxy * Lx[i] + tx;
+    Ly[i] = bxy * Ly[i] + ty;
+    Lz[i] = bxy * Lz[i] + tz;
+    S[i] += em[i] * tx + em[N + i] * ty + em[2 * N + i] * tz;
+    mxp = mx;
+    myp = my;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers
+// ---------------------------------------------------------------------------
+
+static int check_grid3(const dnx_grid3d* g) {
+  DNX_REQUIRE(g != nullptr, "grid3d: null");
+  DNX_REQUIRE(g->nz > 0 && g->ny > 0 && g->nx > 0, "grid3d: bad shape %d x %d x %d", g->nz,
+              g->ny, g->nx);
+  DNX_REQUIRE(g->dtype == DNX_F32 || g->dtype == DNX_F64, "grid3d: bad dtype %d", g->dtype);
+  DNX_REQUIRE(g->ce && g->by && g->ay && g->bx && g->ax, "grid3d: null coefficient array");
+  return DNX_OK;
+}
+
+static dim3 grid3_launch(const dnx_grid3d* g) {
+  return dim3(cdiv(g->nx, TX), cdiv(g->ny, TY), cdiv(g->nz, ZC));
+}
+
+template <typename T>
+static int run3(const dnx_grid3d* gd, T* st, int n0, int n_steps, const dnx_source* srcs,
+                int n_src, const dnx_monitor* mons, int n_mon, T* ckpt, int ckpt_every, T* es,
+                int64_t probe, T* probe_out, hipStream_t s) {
+  G3<T> g = make_g3<T>(gd);
+  const dim3 grid = grid3_launch(gd), block(TX, TY);
+  const int64_t N = g.n;
+  Comps<T> E{{st, st + N, st + 2 * N}};
+  for (int k = 0; k < n_steps; ++k) {
+    const int n = n0 + k;
+    if (ckpt && k % ckpt_every == 0) {
+      DNX_CHECK_HIP(hipMemcpyAsync(ckpt + int64_t(k / ckpt_every) * 6 * N, st,
+                                   sizeof(T) * 6 * N, hipMemcpyDeviceToDevice, s));
+    }
+    hipLaunchKernelGGL(k3_update_h<T>, grid, block, 0, s, g, st);
+    hipLaunchKernelGGL(k3_update_e<T>, grid, block, 0, s, g, st,
+                       es ? es + int64_t(k) * 3 * N : nullptr);
+    DNX_CHECK_HIP(hipGetLastError());
+    for (int j = 0; j < n_src; ++j) {
+      int rc = launch_source<T>(st + srcs[j].comp * N, g.ny, g.nx, srcs[j], n, s);
+      if (rc) return rc;
+      if (es) {
+        rc = launch_source<T>(es + int64_t(k) * 3 * N + srcs[j].comp * N, g.ny, g.nx, srcs[j], n,
+                              s);
+        if (rc) return rc;
+      }
+    }
+    if (probe >= 0) {
+      DNX_CHECK_HIP(hipMemcpyAsync(probe_out + k, st + probe, sizeof(T), hipMemcpyDeviceToDevice,
+                                   s));
+    }
+    for (int j = 0; j < n_mon; ++j) {
+      int rc = launch_monitor<T>(E, g.ny, g.nx, mons[j], n, s);
+      if (rc) return rc;
+    }
+  }
+  return DNX_OK;
+}
+
+template <typename T>
+static int adj_init3(const dnx_grid3d* gd, T* lam, const T* eT, const dnx_monitor* mons,
+                     int n_mon, int n_total, T* S, hipStream_t s) {
+  G3<T> g = make_g3<T>(gd);
+  const int64_t N = g.n;
+  Comps<T> L{{lam, lam + N, lam + 2 * N}};
+  Comps<T> none{{nullptr, nullptr, nullptr}};
+  for (int j = 0; j < n_mon; ++j) {
+    int rc = launch_monitor_adj<T>(L, none, nullptr, g.ny, g.nx, mons[j], n_total - 1, s);
+    if (rc) return rc;
+  }
+  int rc = launch_zero<T>(lam + 3 * N, 3 * N, s);
+  if (rc) return rc;
+  Comps<T> Ev{{const_cast<T*>(eT), const_cast<T*>(eT) + N, const_cast<T*>(eT) + 2 * N}};
+  return launch_dot_init<T>(S, Ev, L, 3, N, s);
+}
+
+template <typename T>
+static int adj_run3(const dnx_grid3d* gd, T* lam, int m_hi, int m_lo, const T* e_lo,
+                    const T* es, const dnx_source* srcs, int n_src, const dnx_monitor* mons,
+                    int n_mon, T* S, hipStream_t s) {
+  G3<T> g = make_g3<T>(gd);
+  const dim3 grid = grid3_launch(gd), block(TX, TY);
+  const int64_t N = g.n;
+  Comps<T> L{{lam, lam + N, lam + 2 * N}};
+  for (int m = m_hi - 1; m >= m_lo && m >= 1; --m) {
+    const T* em = (m == m_lo) ? e_lo : es + int64_t(m - m_lo - 1) * 3 * N;
+    for (int j = 0; j < n_src; ++j) {
+      int rc = launch_source_adj<T>(S, lam + srcs[j].comp * N, g.ny, g.nx, srcs[j], m, s);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k3_adj_h<T>, grid, block, 0, s, g, lam);
+    hipLaunchKernelGGL(k3_adj_e<T>, grid, block, 0, s, g, lam, em, S);
+    DNX_CHECK_HIP(hipGetLastError());
+    Comps<T> Em{{const_cast<T*>(em), const_cast<T*>(em) + N, const_cast<T*>(em) + 2 * N}};
+    for (int j = 0; j < n_mon; ++j) {
+      int rc = launch_monitor_adj<T>(L, Em, S, g.ny, g.nx, mons[j], m - 1, s);
+      if (rc) return rc;
+    }
+  }
+  if (m_lo == 0) {
+    for (int j = 0; j < n_src; ++j) {
+      int rc = launch_source_adj<T>(S, lam + srcs[j].comp * N, g.ny, g.nx, srcs[j], 0, s);
+      if (rc) return rc;
+    }
+  }
+  return DNX_OK;
+}
+
+}  // namespace dnx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+using namespace dnx;
+
+extern "C" int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, int32_t n0,
+                                  int32_t n_steps, const dnx_source* srcs, int32_t n_src,
+                                  const dnx_monitor* mons, int32_t n_mon, void* ckpt,
+                                  int32_t ckpt_every, void* estore, int64_t probe_index,
+                                  void* probe_out, void* stream, int32_t* final_in_work) {
+  (void)work;
+  int rc = check_grid3(g);
+  if (rc) return rc;
+  DNX_REQUIRE(state != nullptr, "fdtd3d_run: null state");
+  DNX_REQUIRE(n0 >= 0 && n_steps >= 0, "fdtd3d_run: bad step range n0=%d n_steps=%d", n0,
+              n_steps);
+  DNX_REQUIRE(ckpt == nullptr || ckpt_every > 0, "fdtd3d_run: ckpt_every must be > 0");
+  if ((rc = check_sources(srcs, n_src, g->nz, g->ny, g->nx, 2))) return rc;
+  if ((rc = check_monitors(mons, n_mon, g->nz, g->ny, g->nx, 7, false))) return rc;
+  DNX_REQUIRE(probe_index < 0 || (probe_out != nullptr &&
+                                   probe_index < int64_t(6) * g->nz * g->ny * g->nx),
+              "run: bad probe index %lld", (long long)probe_index);
+  if (final_in_work) *final_in_work = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->dtype == DNX_F32)
+    return run3<float>(g, static_cast<float*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
+                       static_cast<float*>(ckpt), ckpt_every, static_cast<float*>(estore),
+                       probe_index, static_cast<float*>(probe_out), s);
+  return run3<double>(g, static_cast<double*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
+                      static_cast<double*>(ckpt), ckpt_every, static_cast<double*>(estore),
+                      probe_index, static_cast<double*>(probe_out), s);
+}
+
+extern "C" int32_t dnx_fdtd3d_adjoint_init(const dnx_grid3d* g, void* lam, const void* e_T,
+                                           const dnx_monitor* mons, int32_t n_mon,
+                                           int32_t n_total, void* S, void* stream) {
+  int rc = check_grid3(g);
+  if (rc) return rc;
+  DNX_REQUIRE(lam && e_T && S, "fdtd3d_adjoint_init: null pointer");
+  DNX_REQUIRE(n_total >= 1, "fdtd3d_adjoint_init: n_total must be >= 1");
+  if ((rc = check_monitors(mons, n_mon, g->nz, g->ny, g->nx, 7, true))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->dtype == DNX_F32)
+    return adj_init3<float>(g, static_cast<float*>(lam), static_cast<const float*>(e_T), mons,
+                            n_mon, n_total, static_cast<float*>(S), s);
+  return adj_init3<double>(g, static_cast<double*>(lam), static_cast<const double*>(e_T), mons,
+                           n_mon, n_total, static_cast<double*>(S), s);
+}
+
+extern "C" int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work,
+                                          int32_t m_hi, int32_t m_lo, const void* e_lo,
+                                          const void* estore, const dnx_source* srcs,
+                                          int32_t n_src, const dnx_monitor* mons, int32_t n_mon,
+                                          void* S, void* stream, int32_t* final_in_work) {
+  (void)work;
+  int rc = check_grid3(g);
+  if (rc) return rc;
+  DNX_REQUIRE(lam && S, "fdtd3d_adjoint_run: null pointer");
+  DNX_REQUIRE(m_lo >= 0 && m_hi >= m_lo, "fdtd3d_adjoint_run: bad range [%d, %d)", m_lo, m_hi);
+  DNX_REQUIRE(!(m_hi - 1 > m_lo && m_hi - 1 >= 1) || estore != nullptr,
+              "fdtd3d_adjoint_run: estore required");
+  DNX_REQUIRE(!(m_lo >= 1 && m_hi > m_lo) || e_lo != nullptr,
+              "fdtd3d_adjoint_run: e_lo required");
+  if ((rc = check_sources(srcs, n_src, g->nz, g->ny, g->nx, 2))) return rc;
+  if ((rc = check_monitors(mons, n_mon, g->nz, g->ny, g->nx, 7, true))) return rc;
+  if (final_in_work) *final_in_work = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->dtype == DNX_F32)
+    return adj_run3<float>(g, static_cast<float*>(lam), m_hi, m_lo,
+                           static_cast<const float*>(e_lo), static_cast<const float*>(estore),
+                           srcs, n_src, mons, n_mon, static_cast<float*>(S), s);
+  return adj_run3<double>(g, static_cast<double*>(lam), m_hi, m_lo,
+                          static_cast<const double*>(e_lo), static_cast<const double*>(estore),
+                          srcs, n_src, mons, n_mon, static_cast<double*>(S), s);
+}

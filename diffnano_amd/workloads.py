@@ -1,0 +1,119 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+* ``blob_eps_2d``      2D Si/SiO2 blob design: uniform noise (torch
+                       generator), Gaussian filter sigma=3 cells (truncate 4,
+                       reflect), threshold 0.5 -> eps 12.1 / 2.085.
+* ``waveguide_3d``     3D Si waveguide(s) along x in SiO2 with a seeded
+                       continuous design slab eps = 2.085 + 10.015 rho.
+* ``config_*``         the solver kwargs, source, monitors and loss of each
+                       benchmark configuration.
+
+Lengths are in nm; grids are (D, H, W) = (z, y, x), x fastest.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+from scipy.ndimage import gaussian_filter
+
+EPS_SI = 12.1
+EPS_SIO2 = 2.085
+
+
+def blob_eps_2d(seed, H=200, W=200, lo=60, size=80, sigma=3.0):
+    """200x200 field with an 80x80 blob design window at rows/cols 60..139."""
+    g = torch.Generator().manual_seed(int(seed))
+    noise = torch.rand(size, size, dtype=torch.float64, generator=g).numpy()
+    blobs = gaussian_filter(noise, sigma, mode="reflect", truncate=4.0) > 0.5
+    eps = np.full((H, W), EPS_SIO2)
+    eps[lo:lo + size, lo:lo + size] = np.where(blobs, EPS_SI, EPS_SIO2)
+    return eps
+
+
+def waveguide_3d(shape, dl, design=(8, 64, 64), seed=0, split=False):
+    """Si waveguide along x (400 nm wide in y, 220 nm tall in z), centred in
+    SiO2, with a centred design block (dz, dy, dx) of eps = 2.085 + 10.015 rho,
+    rho ~ U[0, 1] (numpy default_rng(seed)).  ``split``: the waveguide exists
+    only outside the design block's x extent (input and output guides)."""
+    D, H, W = shape
+    eps = np.full(shape, EPS_SIO2)
+    wy = int(round(400.0 / dl))
+    wz = int(round(220.0 / dl))
+    y0, z0 = (H - wy) // 2, (D - wz) // 2
+    eps[z0:z0 + wz, y0:y0 + wy, :] = EPS_SI
+    dz, dy, dx = design
+    cz, cy, cx = (D - dz) // 2, (H - dy) // 2, (W - dx) // 2
+    if split:
+        eps[z0:z0 + wz, y0:y0 + wy, cx:cx + dx] = EPS_SIO2
+    rho = np.random.default_rng(seed).random((dz, dy, dx))
+    eps[cz:cz + dz, cy:cy + dy, cx:cx + dx] = EPS_SIO2 + (EPS_SI - EPS_SIO2) * rho
+    return eps
+
+
+def gaussian_sheet(shape, dl, sigma_nm=200.0):
+    """Gaussian yz profile (D, H, 1) centred on the waveguide axis."""
+    D, H, _ = shape
+    s = sigma_nm / dl
+    zz, yy = np.meshgrid(np.arange(D) - (D - 1) / 2, np.arange(H) - (H - 1) / 2, indexing="ij")
+    return np.exp(-(zz**2 + yy**2) / (2 * s**2))[:, :, None]
+
+
+def config_c2(seed=0, n_steps=8000):
+    """256^3, dl 25 nm, 16-cell CPML, 8000 steps: 128x128x16 random-rho slab
+    between input and output Si waveguides; Ey Gaussian sheet with the
+    default broadband pulse at 1550 nm; DFT of Ey on the output yz plane at
+    1450/1500/1550/1600/1650 nm; loss = sum |DFT Ey|^2."""
+    shape = (256, 256, 256)
+    dl, L = 25.0, 16
+    eps = waveguide_3d(shape, dl, design=(16, 128, 128), seed=seed, split=True)
+    solver = dict(grid_shape=shape, dl=dl, wavelength_nm=1550.0, pml_layers=L, n_steps=n_steps,
+                  courant=0.4, backward="time_reversal", checkpoint_every=64)
+    src = {"type": "gaussian_pulse", "plane": "yz", "x": L + 4, "component": "Ey",
+           "profile": torch.from_numpy(gaussian_sheet(shape, dl))}
+    xm = shape[2] - L - 4
+    mons = [{"box": (0, shape[0], 0, shape[1], xm, xm + 1), "components": ["Ey"],
+             "wavelengths": [1450.0, 1500.0, 1550.0, 1600.0, 1650.0]}]
+    return eps, solver, src, mons
+
+
+def config_c1(seed=0, n_steps=4000):
+    """128^3, dl 25 nm, 12-cell CPML, 4000 steps: straight Si waveguide with a
+    64x64x8 random-rho design region; Ey Gaussian sheet at x = L+4; DFT of
+    Ey at 1500/1550/1600 nm on the output plane x = W-L-4."""
+    shape = (128, 128, 128)
+    dl, L = 25.0, 12
+    eps = waveguide_3d(shape, dl, design=(8, 64, 64), seed=seed)
+    solver = dict(grid_shape=shape, dl=dl, wavelength_nm=1550.0, pml_layers=L, n_steps=n_steps,
+                  courant=0.4, backward="time_reversal", checkpoint_every=64)
+    src = {"type": "gaussian_pulse", "plane": "yz", "x": L + 4, "component": "Ey",
+           "profile": torch.from_numpy(gaussian_sheet(shape, dl))}
+    xm = shape[2] - L - 4
+    mons = [{"box": (0, shape[0], 0, shape[1], xm, xm + 1), "components": ["Ey"],
+             "wavelengths": [1500.0, 1550.0, 1600.0]}]
+    return eps, solver, src, mons
+
+
+def config_c0(seed=0):
+    """200x200 TM, dl 20 nm, 10-cell CPML, courant 0.5, 2000 steps; point Ez
+    source at (40, 100) with a 200 nm-bandwidth pulse at 1550 nm; loss =
+    sum |DFT Ez(1550 nm)|^2 over the 1x60 line at row 160, cols 70..129."""
+    lam0 = 1550.0
+    spread = lam0**2 / (2 * math.pi * 200.0)
+    eps = blob_eps_2d(seed)
+    solver = dict(grid_shape=(200, 200), dl=20.0, wavelength_nm=lam0, pml_layers=10,
+                  n_steps=2000, courant=0.5)
+    src = {"type": "gaussian_pulse", "pos": [40, 100], "spread": spread, "t0": 3 * spread}
+    mons = [{"box": (160, 161, 70, 130), "wavelengths": [lam0]}]
+    return eps, solver, src, mons
+
+
+def monitor_loss(monitors_out, comp_index=0):
+    """sum over monitors and wavelengths of |F|^2 of one recorded component."""
+    tot = 0.0
+    for F in monitors_out:
+        f = F[:, comp_index] if F.dim() >= 5 else F
+        tot = tot + (f.abs() ** 2).sum()
+    return tot

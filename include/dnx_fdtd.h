@@ -1,0 +1,194 @@
+/*
+ * dnx_fdtd.h -- C ABI of the MI355X (gfx950) Yee-FDTD library libdnx_fdtd.so.
+ *
+ * The library replaces the arithmetic of DiffNano's FDTD solvers
+ * (diffnano/solvers/fdtd3d.py, fdtd2d.py).  Those are pure PyTorch, so the
+ * reference has no FFI of its own; each entry point below names the
+ * reference function whose semantics it implements.  The Python host
+ * (diffnano_amd) binds them with ctypes; INTEGRATION.md shows the binding.
+ *
+ * Conventions
+ *  - Arrays are row-major [axis0][axis1][axis2] = [z][y][x] (3D) or [y][x]
+ *    (2D), x fastest, exactly the reference's (D, H, W) / (H, W) tensors.
+ *  - Every pointer inside a struct or argument is a DEVICE pointer unless
+ *    the comment says otherwise; the structs themselves live in host memory.
+ *  - dtype: DNX_F32 or DNX_F64 selects the element type of every field,
+ *    coefficient, profile, waveform and gradient array of the call.
+ *    DFT accumulators are always float64.
+ *  - Calls are asynchronous on the HIP stream passed in (NULL = the null
+ *    stream); the library never synchronises and allocates no device memory.
+ *  - Return 0 on success, a negative DNX_E* code on error; the message is in
+ *    dnx_last_error() (thread-local).  No C++ exception crosses the ABI.
+ */
+#ifndef DNX_FDTD_H
+#define DNX_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The library is built with hidden visibility; only these symbols export. */
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
+#define DNX_ABI_VERSION 1
+
+enum { DNX_F32 = 0, DNX_F64 = 1 };
+enum {
+  DNX_OK = 0,
+  DNX_EINVAL = -1,  /* bad argument (shape, dtype, box, null pointer) */
+  DNX_EHIP = -2,    /* HIP runtime error (launch, copy) */
+  DNX_ERANGE = -3   /* size beyond a documented limit */
+};
+
+/* Axis-aligned box [z0,z1) x [y0,y1) x [x0,x1) (2D: z0=0, z1=1). */
+typedef struct dnx_box {
+  int32_t z0, z1, y0, y1, x0, x1;
+} dnx_box;
+
+/* 3D grid and coefficients.  Profiles: b and a = 1 + c of the reference's
+ * memoryless CPML (fdtd3d.py:75-85); (by, ay) vary along axis 1 (length ny),
+ * (bx, ax) along axis 2 (length nx); axis 0 is undamped (fdtd3d.py:402-463). */
+typedef struct dnx_grid3d {
+  int32_t nz, ny, nx;
+  int32_t dtype;       /* DNX_F32 / DNX_F64 */
+  double kh;           /* dt / dl */
+  const void* ce;      /* [nz*ny*nx]  dt / (eps * dl) */
+  const void* by;      /* [ny] */
+  const void* ay;      /* [ny] */
+  const void* bx;      /* [nx] */
+  const void* ax;      /* [nx] */
+} dnx_grid3d;
+
+/* 2D grid (fdtd2d.py): axis 0 (ny rows) carries (by, ay), axis 1 (nx
+ * columns) carries (bx, ax).  te = 0: TM state [Ez, Hx, Hy]; te = 1: TE
+ * state [Hz, Ex, Ey] with the reference's Hz sign (fdtd2d.py:299-301). */
+typedef struct dnx_grid2d {
+  int32_t ny, nx;
+  int32_t dtype;
+  int32_t te;
+  double kh;
+  const void* ce;      /* [ny*nx] dt / (eps * dl) */
+  const void* by;
+  const void* ay;
+  const void* bx;
+  const void* ax;
+} dnx_grid2d;
+
+/* Soft source: after step n, field[comp] += wave[n] * profile over box
+ * (fdtd3d.py:145,502 with a 0/1 mask and comp = Ez; here any component
+ * and any amplitude profile).  3D comp: 0 Ex, 1 Ey, 2 Ez; 2D comp must be 0
+ * (Ez in TM, Hz in TE). */
+typedef struct dnx_source {
+  int32_t comp;
+  dnx_box box;
+  const void* profile; /* [box cells], x fastest */
+  const void* wave;    /* [total steps] */
+} dnx_source;
+
+/* Running DFT monitor (no reference counterpart; the nearest is the probe
+ * series of fdtd3d.py:652-700).  After step n it adds
+ *   acc[f][c][cell] += E_c(cell) * (cos(phi), -sin(phi)),  phi = phase[n][f]
+ * for every component c in comp_mask (3D bit c = Ex/Ey/Ez; 2D bit 0).
+ * phase holds (cos, sin) pairs: phase[(n*n_freq + f)*2 + {0,1}].
+ * acc layout: [n_freq][n_comp][box cells][2] (re, im), float64.
+ * grad (adjoint only): dL/dRe, dL/dIm in the same layout. */
+typedef struct dnx_monitor {
+  dnx_box box;
+  int32_t comp_mask;
+  int32_t n_freq;
+  const double* phase;
+  double* acc;
+  const double* grad;
+} dnx_monitor;
+
+int32_t dnx_abi_version(void);
+const char* dnx_last_error(void);
+
+/* ce[i] = dt / (eps[i] * dl) computed in float64, stored in dtype.
+ * eps is float64.  Replaces the reference's per-step (dt / eps_r)
+ * (fdtd3d.py:435; fdtd2d.py:262). */
+int32_t dnx_prepare_coef(const double* eps, void* ce, int64_t n, double dt_over_dl,
+                         int32_t dtype, void* stream);
+
+/* grad_eps[i] = -S[i] / eps[i] (float64 out); S from the adjoint. */
+int32_t dnx_finalize_grad(const void* S, const double* eps, double* grad_eps, int64_t n,
+                          int32_t dtype, void* stream);
+
+/* ---- 3D -------------------------------------------------------------- */
+
+/* Advance `state` ([Ex,Ey,Ez,Hx,Hy,Hz], 6*N) by n_steps leapfrog steps
+ * starting at absolute step n0: FDTDSolver3D._time_step (fdtd3d.py:378-465)
+ * then source injection (fdtd3d.py:502) then monitors, per step.
+ * work: second 6*N state buffer used for ping-pong (may equal NULL only if
+ *   the implementation selected needs none; always pass one).
+ * ckpt: NULL or (ceil(n_steps/ckpt_every)) slots of 6*N: slot k receives the
+ *   state at step n0 + k*ckpt_every before that step runs.
+ * estore: NULL or n_steps slots of 3*N: slot i receives (Ex,Ey,Ez) after
+ *   step n0 + i.
+ * probe_index >= 0: after step n0 + i, probe_out[i] = state[probe_index]
+ *   (flat index into the 6*N state; FDTDSolver3D.time_series, fdtd3d.py:652-700).
+ * *final_in_work (host int): 1 if the final state is in `work`, else 0. */
+int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, int32_t n0,
+                       int32_t n_steps, const dnx_source* srcs, int32_t n_src,
+                       const dnx_monitor* mons, int32_t n_mon, void* ckpt,
+                       int32_t ckpt_every, void* estore, int64_t probe_index,
+                       void* probe_out, void* stream, int32_t* final_in_work);
+
+/* Adjoint initialisation at m = T: lam[0:3N] holds dL/dE^T on entry;
+ * adds the monitor terms of sample T (n_total = T), clears lam[3N:6N]
+ * (the H adjoint) and sets S = sum_c E^T_c * lam_c.  e_T: E^T (3*N). */
+int32_t dnx_fdtd3d_adjoint_init(const dnx_grid3d* g, void* lam, const void* e_T,
+                                const dnx_monitor* mons, int32_t n_mon, int32_t n_total,
+                                void* S, void* stream);
+
+/* Reverse sweep over adjoint steps m = m_hi-1 ... max(m_lo, 1), each
+ * mapping (lam_E^{m+1}, mu_H^{m+1}) in `lam` to (lam_E^m, mu_H^m) and
+ * accumulating S += sum_c E^m_c (lam^m_c - b lam^{m+1}_c) - s_m lam^{m+1};
+ * if m_lo == 0 it finally adds the step-0 source term.  This is the exact
+ * discrete transpose of dnx_fdtd3d_run, i.e. the gradient the reference
+ * obtains by autograd (fdtd3d.py:166-219).  dL/deps = -S / eps.
+ * e_lo: E^{m_lo} (3*N); estore: E^{m_lo+1..m_hi-1} (slot j = E^{m_lo+1+j}). */
+int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work, int32_t m_hi,
+                               int32_t m_lo, const void* e_lo, const void* estore,
+                               const dnx_source* srcs, int32_t n_src,
+                               const dnx_monitor* mons, int32_t n_mon, void* S,
+                               void* stream, int32_t* final_in_work);
+
+/* ---- 2D -------------------------------------------------------------- */
+
+/* As dnx_fdtd3d_run for the 2D state (3*N): TM FDTDSolver2D._time_step_tm
+ * (fdtd2d.py:233-267), TE _time_step_te (fdtd2d.py:269-303), source into
+ * Ez/Hz (fdtd2d.py:333,351).  estore slots hold the first component (Ez or
+ * Hz), N each.  probe: as 3D (fdtd2d.py:335-337, :486-524). */
+int32_t dnx_fdtd2d_run(const dnx_grid2d* g, void* state, void* work, int32_t n0,
+                       int32_t n_steps, const dnx_source* srcs, int32_t n_src,
+                       const dnx_monitor* mons, int32_t n_mon, void* ckpt,
+                       int32_t ckpt_every, void* estore, int64_t probe_index,
+                       void* probe_out, void* stream, int32_t* final_in_work);
+
+/* 2D adjoint, same contract as the 3D pair; e_T / e_lo / estore hold the
+ * first state component (N each).  TE accumulates S from the stored Hz
+ * (the E update reads the old Hz). */
+int32_t dnx_fdtd2d_adjoint_init(const dnx_grid2d* g, void* lam, const void* e_T,
+                                const dnx_monitor* mons, int32_t n_mon, int32_t n_total,
+                                void* S, void* stream);
+int32_t dnx_fdtd2d_adjoint_run(const dnx_grid2d* g, void* lam, void* work, int32_t m_hi,
+                               int32_t m_lo, const void* e_lo, const void* estore,
+                               const dnx_source* srcs, int32_t n_src,
+                               const dnx_monitor* mons, int32_t n_mon, void* S,
+                               void* stream, int32_t* final_in_work);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNX_FDTD_H */
