@@ -186,26 +186,31 @@ class Engine:
 
     # -- kernels ------------------------------------------------------------
     def run(self, state, n0, n_steps, srcs, mons, ckpt=None, ckpt_every=0, estore=None,
-            probe=-1, probe_out=None):
+            probe=-1, probe_out=None, work=None):
+        """Advance ``state``; returns the buffer (``state`` or ``work``) that
+        holds the final state (the fused kernels ping-pong between them)."""
+        if work is None:
+            work = torch.empty_like(state)
         fin = C.c_int32(0)
         fn = L.lib().dnx_fdtd3d_run if self.dim == 3 else L.lib().dnx_fdtd2d_run
-        L.check(fn(self.grid(), L.ptr(state), None, int(n0), int(n_steps), srcs[0], srcs[1],
-                   mons[0], mons[1], L.ptr(ckpt), int(ckpt_every), L.ptr(estore), int(probe),
-                   L.ptr(probe_out), self.stream(), C.byref(fin)))
-        assert fin.value == 0
+        L.check(fn(self.grid(), L.ptr(state), L.ptr(work), int(n0), int(n_steps), srcs[0],
+                   srcs[1], mons[0], mons[1], L.ptr(ckpt), int(ckpt_every), L.ptr(estore),
+                   int(probe), L.ptr(probe_out), self.stream(), C.byref(fin)))
+        return work if fin.value else state
 
     def adjoint_init(self, lam, e_T, mons, n_total, S):
         fn = L.lib().dnx_fdtd3d_adjoint_init if self.dim == 3 else L.lib().dnx_fdtd2d_adjoint_init
         L.check(fn(self.grid(), L.ptr(lam), L.ptr(e_T), mons[0], mons[1], int(n_total), L.ptr(S),
                    self.stream()))
 
-    def adjoint_run(self, lam, m_hi, m_lo, e_lo, estore, srcs, mons, S):
+    def adjoint_run(self, lam, m_hi, m_lo, e_lo, estore, srcs, mons, S, work):
+        """Reverse sweep; returns (final, spare) adjoint buffers."""
         fin = C.c_int32(0)
         fn = L.lib().dnx_fdtd3d_adjoint_run if self.dim == 3 else L.lib().dnx_fdtd2d_adjoint_run
-        L.check(fn(self.grid(), L.ptr(lam), None, int(m_hi), int(m_lo), L.ptr(e_lo),
+        L.check(fn(self.grid(), L.ptr(lam), L.ptr(work), int(m_hi), int(m_lo), L.ptr(e_lo),
                    L.ptr(estore), srcs[0], srcs[1], mons[0], mons[1], L.ptr(S), self.stream(),
                    C.byref(fin)))
-        assert fin.value == 0
+        return (work, lam) if fin.value else (lam, work)
 
     def finalize_grad(self, S):
         g = torch.empty(self.N, dtype=torch.float64, device=self.dev)
@@ -254,7 +259,7 @@ def forward_only(plan, eps64):
     probe_out = None
     if plan.probe >= 0:
         probe_out = torch.zeros(max(T, 1), dtype=eng.dtype, device=eng.dev)
-    eng.run(state, 0, T, srcs[:2], mons[:2], probe=plan.probe, probe_out=probe_out)
+    state = eng.run(state, 0, T, srcs[:2], mons[:2], probe=plan.probe, probe_out=probe_out)
     plan.probe_values = probe_out[:T] if probe_out is not None else None
     return _field_out(plan, state), [m.to(plan.out_device) for m in mons[2]], state
 
@@ -289,7 +294,8 @@ class _TimeReversalFDTD(torch.autograd.Function):
         if plan.probe >= 0:
             probe_out = torch.zeros(max(T, 1), dtype=eng.dtype, device=eng.dev)
         _begin("forward")
-        eng.run(state, 0, T, srcs[:2], mons[:2], ckpt, K, probe=plan.probe, probe_out=probe_out)
+        state = eng.run(state, 0, T, srcs[:2], mons[:2], ckpt, K, probe=plan.probe,
+                        probe_out=probe_out)
         _end("forward")
         plan.probe_values = probe_out[:T] if probe_out is not None else None
         ctx.plan = plan
@@ -322,16 +328,19 @@ class _TimeReversalFDTD(torch.autograd.Function):
         nseg = math.ceil(T / K)
         estore = torch.empty(max(K - 1, 1) * ne * N, dtype=dt, device=dev)
         work = torch.empty(ns * N, dtype=dt, device=dev)
+        work2 = torch.empty(ns * N, dtype=dt, device=dev)
+        lam_spare = torch.empty(ns * N, dtype=dt, device=dev)
         for k in range(nseg - 1, -1, -1):
             a, b = k * K, min(T, (k + 1) * K)
             slot = ctx.ckpt[k * ns * N:(k + 1) * ns * N]
             if b - a > 1:
                 _begin("replay")
                 work.copy_(slot)
-                eng.run(work, a, b - a - 1, srcs[:2], (None, 0), estore=estore)
+                eng.run(work, a, b - a - 1, srcs[:2], (None, 0), estore=estore, work=work2)
                 _end("replay")
             _begin("adjoint")
-            eng.adjoint_run(lam, b, a, slot[:ne * N], estore, srcs[:2], mons[:2], S)
+            lam, lam_spare = eng.adjoint_run(lam, b, a, slot[:ne * N], estore, srcs[:2], mons[:2],
+                                             S, lam_spare)
             _end("adjoint")
         grad = eng.finalize_grad(S)
         return grad.to(ctx.eps_device), None

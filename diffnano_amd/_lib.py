@@ -64,6 +64,8 @@ _SIGS = {
                                 _I),
     "dnx_fdtd3d_adjoint_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, _P, _P, C.POINTER(Source), _I,
                                 C.POINTER(Monitor), _I, _P, _P, C.POINTER(_I)], _I),
+    "dnx_fdtd3d_set_tiles": ([_I, _I, _I, _I], _I),
+    "dnx_fdtd3d_get_tiles": ([_I, _I, C.POINTER(_I), C.POINTER(_I)], _I),
     "dnx_fdtd2d_run": ([C.POINTER(Grid2D), _P, _P, _I, _I, C.POINTER(Source), _I,
                         C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),
     "dnx_fdtd2d_adjoint_init": ([C.POINTER(Grid2D), _P, _P, C.POINTER(Monitor), _I, _I, _P, _P],

@@ -17,7 +17,10 @@
 // Kernel layout: a 64x4 thread tile in (x, y), each thread marching a chunk
 // of z so that the z-neighbour comes from a register; x is the fastest axis
 // so every wave's loads are 256-byte coalesced rows.
-#include "dnx_aux.h"
+#include <cstdlib>
+#include <cstring>
+
+#include "fdtd3d_fused.h"
 
 namespace dnx {
 
@@ -264,42 +267,97 @@ static dim3 grid3_launch(const dnx_grid3d* g) {
   return dim3(cdiv(g->nx, TX), cdiv(g->ny, TY), cdiv(g->nz, ZC));
 }
 
+// Kernel variant: the fused single-pass kernels for nx <= 256 (default), or
+// the two-kernel in-place variant ("simple") when DNX_FDTD3D_VARIANT=simple,
+// no ping-pong buffer is given, or nx > 256.
+static bool use_fused(const dnx_grid3d* g, const void* work) {
+  if (work == nullptr || g->nx > FUSED_MAX_W) return false;
+  const char* v = getenv("DNX_FDTD3D_VARIANT");
+  return !(v && strcmp(v, "simple") == 0);
+}
+
 template <typename T>
-static int run3(const dnx_grid3d* gd, T* st, int n0, int n_steps, const dnx_source* srcs,
-                int n_src, const dnx_monitor* mons, int n_mon, T* ckpt, int ckpt_every, T* es,
-                int64_t probe, T* probe_out, hipStream_t s) {
+static Fused3Args<T> fused_args(const dnx_grid3d* gd, const Fused3Tiles& t) {
+  Fused3Args<T> a{};
+  a.nz = gd->nz;
+  a.ny = gd->ny;
+  a.nx = gd->nx;
+  a.ty_tiles = int(cdiv(gd->ny, t.ty));
+  a.zc = t.zc;
+  a.nblocks = a.ty_tiles * int(cdiv(gd->nz, t.zc));
+  a.P = int64_t(gd->ny) * gd->nx;
+  a.N = a.P * gd->nz;
+  a.kh = T(gd->kh);
+  a.ce = static_cast<const T*>(gd->ce);
+  a.by = static_cast<const T*>(gd->by);
+  a.ay = static_cast<const T*>(gd->ay);
+  a.bx = static_cast<const T*>(gd->bx);
+  a.ax = static_cast<const T*>(gd->ax);
+  a.src_comp = -1;
+  return a;
+}
+
+template <typename T>
+static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
+                const dnx_source* srcs, int n_src, const dnx_monitor* mons, int n_mon, T* ckpt,
+                int ckpt_every, T* es, int64_t probe, T* probe_out, hipStream_t s, int* fin) {
   G3<T> g = make_g3<T>(gd);
-  const dim3 grid = grid3_launch(gd), block(TX, TY);
   const int64_t N = g.n;
-  Comps<T> E{{st, st + N, st + 2 * N}};
+  const bool fused = use_fused(gd, work);
+  const Fused3Tiles tiles = fused3_tiles(gd->dtype, 0);
+  Fused3Args<T> fa = fused ? fused_args<T>(gd, tiles) : Fused3Args<T>{};
+  const int ty = tiles.ty;
+  int j0 = 0;  // sources handled by the side kernel start at j0
+  if (fused && n_src > 0) {
+    fa.src_comp = srcs[0].comp;
+    fa.src_box = make_box(srcs[0].box);
+    fa.src_prof = static_cast<const T*>(srcs[0].profile);
+    j0 = 1;
+  }
+  T* cur = st;
+  T* nxt = fused ? work : st;
+  const dim3 grid = grid3_launch(gd), block(TX, TY);
   for (int k = 0; k < n_steps; ++k) {
     const int n = n0 + k;
+    T* slot = es ? es + int64_t(k) * 3 * N : nullptr;
     if (ckpt && k % ckpt_every == 0) {
-      DNX_CHECK_HIP(hipMemcpyAsync(ckpt + int64_t(k / ckpt_every) * 6 * N, st,
+      DNX_CHECK_HIP(hipMemcpyAsync(ckpt + int64_t(k / ckpt_every) * 6 * N, cur,
                                    sizeof(T) * 6 * N, hipMemcpyDeviceToDevice, s));
     }
-    hipLaunchKernelGGL(k3_update_h<T>, grid, block, 0, s, g, st);
-    hipLaunchKernelGGL(k3_update_e<T>, grid, block, 0, s, g, st,
-                       es ? es + int64_t(k) * 3 * N : nullptr);
-    DNX_CHECK_HIP(hipGetLastError());
-    for (int j = 0; j < n_src; ++j) {
-      int rc = launch_source<T>(st + srcs[j].comp * N, g.ny, g.nx, srcs[j], n, s);
+    if (fused) {
+      fa.in = cur;
+      fa.out = nxt;
+      fa.es = slot;
+      if (fa.src_comp >= 0) fa.src_wave = static_cast<const T*>(srcs[0].wave) + n;
+      int rc = launch_fused3_fwd<T>(fa, ty, s);
       if (rc) return rc;
-      if (es) {
-        rc = launch_source<T>(es + int64_t(k) * 3 * N + srcs[j].comp * N, g.ny, g.nx, srcs[j], n,
-                              s);
-        if (rc) return rc;
-      }
+    } else {
+      hipLaunchKernelGGL(k3_update_h<T>, grid, block, 0, s, g, cur);
+      hipLaunchKernelGGL(k3_update_e<T>, grid, block, 0, s, g, cur, slot);
+      DNX_CHECK_HIP(hipGetLastError());
+    }
+    for (int j = j0; j < n_src; ++j) {
+      int rc = launch_source<T>(nxt + srcs[j].comp * N, g.ny, g.nx, srcs[j], n, s);
+      if (rc) return rc;
+      if (slot && (rc = launch_source<T>(slot + srcs[j].comp * N, g.ny, g.nx, srcs[j], n, s)))
+        return rc;
     }
     if (probe >= 0) {
-      DNX_CHECK_HIP(hipMemcpyAsync(probe_out + k, st + probe, sizeof(T), hipMemcpyDeviceToDevice,
-                                   s));
+      DNX_CHECK_HIP(hipMemcpyAsync(probe_out + k, nxt + probe, sizeof(T),
+                                   hipMemcpyDeviceToDevice, s));
     }
+    Comps<T> E{{nxt, nxt + N, nxt + 2 * N}};
     for (int j = 0; j < n_mon; ++j) {
       int rc = launch_monitor<T>(E, g.ny, g.nx, mons[j], n, s);
       if (rc) return rc;
     }
+    if (fused) {
+      T* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
   }
+  *fin = (cur != st) ? 1 : 0;
   return DNX_OK;
 }
 
@@ -321,34 +379,63 @@ static int adj_init3(const dnx_grid3d* gd, T* lam, const T* eT, const dnx_monito
 }
 
 template <typename T>
-static int adj_run3(const dnx_grid3d* gd, T* lam, int m_hi, int m_lo, const T* e_lo,
+static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, const T* e_lo,
                     const T* es, const dnx_source* srcs, int n_src, const dnx_monitor* mons,
-                    int n_mon, T* S, hipStream_t s) {
+                    int n_mon, T* S, hipStream_t s, int* fin) {
   G3<T> g = make_g3<T>(gd);
-  const dim3 grid = grid3_launch(gd), block(TX, TY);
   const int64_t N = g.n;
-  Comps<T> L{{lam, lam + N, lam + 2 * N}};
+  const bool fused = use_fused(gd, work);
+  const Fused3Tiles tiles = fused3_tiles(gd->dtype, 1);
+  Fused3Args<T> fa = fused ? fused_args<T>(gd, tiles) : Fused3Args<T>{};
+  const int ty = tiles.ty;
+  int j0 = 0;
+  if (fused && n_src > 0) {
+    fa.src_comp = srcs[0].comp;
+    fa.src_box = make_box(srcs[0].box);
+    fa.src_prof = static_cast<const T*>(srcs[0].profile);
+    j0 = 1;
+  }
+  fa.S = S;
+  T* cur = lam;
+  T* nxt = fused ? work : lam;
+  const dim3 grid = grid3_launch(gd), block(TX, TY);
   for (int m = m_hi - 1; m >= m_lo && m >= 1; --m) {
     const T* em = (m == m_lo) ? e_lo : es + int64_t(m - m_lo - 1) * 3 * N;
-    for (int j = 0; j < n_src; ++j) {
-      int rc = launch_source_adj<T>(S, lam + srcs[j].comp * N, g.ny, g.nx, srcs[j], m, s);
+    for (int j = j0; j < n_src; ++j) {
+      int rc = launch_source_adj<T>(S, cur + srcs[j].comp * N, g.ny, g.nx, srcs[j], m, s);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(k3_adj_h<T>, grid, block, 0, s, g, lam);
-    hipLaunchKernelGGL(k3_adj_e<T>, grid, block, 0, s, g, lam, em, S);
-    DNX_CHECK_HIP(hipGetLastError());
+    if (fused) {
+      fa.in = cur;
+      fa.out = nxt;
+      fa.em = em;
+      if (fa.src_comp >= 0) fa.src_wave = static_cast<const T*>(srcs[0].wave) + m;
+      int rc = launch_fused3_adj<T>(fa, ty, s);
+      if (rc) return rc;
+    } else {
+      hipLaunchKernelGGL(k3_adj_h<T>, grid, block, 0, s, g, cur);
+      hipLaunchKernelGGL(k3_adj_e<T>, grid, block, 0, s, g, cur, em, S);
+      DNX_CHECK_HIP(hipGetLastError());
+    }
+    Comps<T> L{{nxt, nxt + N, nxt + 2 * N}};
     Comps<T> Em{{const_cast<T*>(em), const_cast<T*>(em) + N, const_cast<T*>(em) + 2 * N}};
     for (int j = 0; j < n_mon; ++j) {
       int rc = launch_monitor_adj<T>(L, Em, S, g.ny, g.nx, mons[j], m - 1, s);
       if (rc) return rc;
     }
+    if (fused) {
+      T* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
   }
   if (m_lo == 0) {
     for (int j = 0; j < n_src; ++j) {
-      int rc = launch_source_adj<T>(S, lam + srcs[j].comp * N, g.ny, g.nx, srcs[j], 0, s);
+      int rc = launch_source_adj<T>(S, cur + srcs[j].comp * N, g.ny, g.nx, srcs[j], 0, s);
       if (rc) return rc;
     }
   }
+  *fin = (cur != lam) ? 1 : 0;
   return DNX_OK;
 }
 
@@ -365,7 +452,6 @@ extern "C" int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, 
                                   const dnx_monitor* mons, int32_t n_mon, void* ckpt,
                                   int32_t ckpt_every, void* estore, int64_t probe_index,
                                   void* probe_out, void* stream, int32_t* final_in_work) {
-  (void)work;
   int rc = check_grid3(g);
   if (rc) return rc;
   DNX_REQUIRE(state != nullptr, "fdtd3d_run: null state");
@@ -377,15 +463,21 @@ extern "C" int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, 
   DNX_REQUIRE(probe_index < 0 || (probe_out != nullptr &&
                                    probe_index < int64_t(6) * g->nz * g->ny * g->nx),
               "run: bad probe index %lld", (long long)probe_index);
-  if (final_in_work) *final_in_work = 0;
+  DNX_REQUIRE(work != state, "fdtd3d_run: work must differ from state");
+  int fin = 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (g->dtype == DNX_F32)
-    return run3<float>(g, static_cast<float*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
-                       static_cast<float*>(ckpt), ckpt_every, static_cast<float*>(estore),
-                       probe_index, static_cast<float*>(probe_out), s);
-  return run3<double>(g, static_cast<double*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
-                      static_cast<double*>(ckpt), ckpt_every, static_cast<double*>(estore),
-                      probe_index, static_cast<double*>(probe_out), s);
+    rc = run3<float>(g, static_cast<float*>(state), static_cast<float*>(work), n0, n_steps, srcs,
+                     n_src, mons, n_mon, static_cast<float*>(ckpt), ckpt_every,
+                     static_cast<float*>(estore), probe_index, static_cast<float*>(probe_out), s,
+                     &fin);
+  else
+    rc = run3<double>(g, static_cast<double*>(state), static_cast<double*>(work), n0, n_steps,
+                      srcs, n_src, mons, n_mon, static_cast<double*>(ckpt), ckpt_every,
+                      static_cast<double*>(estore), probe_index, static_cast<double*>(probe_out),
+                      s, &fin);
+  if (final_in_work) *final_in_work = fin;
+  return rc;
 }
 
 extern "C" int32_t dnx_fdtd3d_adjoint_init(const dnx_grid3d* g, void* lam, const void* e_T,
@@ -409,10 +501,10 @@ extern "C" int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* 
                                           const void* estore, const dnx_source* srcs,
                                           int32_t n_src, const dnx_monitor* mons, int32_t n_mon,
                                           void* S, void* stream, int32_t* final_in_work) {
-  (void)work;
   int rc = check_grid3(g);
   if (rc) return rc;
   DNX_REQUIRE(lam && S, "fdtd3d_adjoint_run: null pointer");
+  DNX_REQUIRE(work != lam, "fdtd3d_adjoint_run: work must differ from lam");
   DNX_REQUIRE(m_lo >= 0 && m_hi >= m_lo, "fdtd3d_adjoint_run: bad range [%d, %d)", m_lo, m_hi);
   DNX_REQUIRE(!(m_hi - 1 > m_lo && m_hi - 1 >= 1) || estore != nullptr,
               "fdtd3d_adjoint_run: estore required");
@@ -420,13 +512,16 @@ extern "C" int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* 
               "fdtd3d_adjoint_run: e_lo required");
   if ((rc = check_sources(srcs, n_src, g->nz, g->ny, g->nx, 2))) return rc;
   if ((rc = check_monitors(mons, n_mon, g->nz, g->ny, g->nx, 7, true))) return rc;
-  if (final_in_work) *final_in_work = 0;
+  int fin = 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (g->dtype == DNX_F32)
-    return adj_run3<float>(g, static_cast<float*>(lam), m_hi, m_lo,
-                           static_cast<const float*>(e_lo), static_cast<const float*>(estore),
-                           srcs, n_src, mons, n_mon, static_cast<float*>(S), s);
-  return adj_run3<double>(g, static_cast<double*>(lam), m_hi, m_lo,
+    rc = adj_run3<float>(g, static_cast<float*>(lam), static_cast<float*>(work), m_hi, m_lo,
+                         static_cast<const float*>(e_lo), static_cast<const float*>(estore), srcs,
+                         n_src, mons, n_mon, static_cast<float*>(S), s, &fin);
+  else
+    rc = adj_run3<double>(g, static_cast<double*>(lam), static_cast<double*>(work), m_hi, m_lo,
                           static_cast<const double*>(e_lo), static_cast<const double*>(estore),
-                          srcs, n_src, mons, n_mon, static_cast<double*>(S), s);
+                          srcs, n_src, mons, n_mon, static_cast<double*>(S), s, &fin);
+  if (final_in_work) *final_in_work = fin;
+  return rc;
 }

@@ -1,0 +1,52 @@
+// Fused single-pass 3D step kernels (forward leapfrog and adjoint), shared
+// declarations.  One launch advances the whole grid by one time step:
+// reads (E, H, ce) once and writes (E', H') once.
+#pragma once
+
+#include "dnx_aux.h"
+
+namespace dnx {
+
+constexpr int FUSED_MAX_W = 256;  // full-row tiles: one thread per x column
+
+template <typename T>
+struct Fused3Args {
+  int nz, ny, nx;
+  int ty_tiles;   // ceil(ny / TY)
+  int zc;         // z planes per work-group
+  int nblocks;    // ty_tiles * z-chunks (logical work-groups)
+  int64_t P, N;   // plane and grid sizes
+  T kh;
+  const T* __restrict__ ce;
+  const T* __restrict__ by;
+  const T* __restrict__ ay;
+  const T* __restrict__ bx;
+  const T* __restrict__ ax;
+  const T* __restrict__ in;  // forward: state at n; adjoint: (lamE^{m+1}, muH^{m+1})
+  T* __restrict__ out;       // forward: state at n+1; adjoint: (lamE^m, muH^m)
+  T* __restrict__ es;        // forward: E store slot (3N) or null
+  const T* __restrict__ em;  // adjoint: E^m (3N)
+  T* __restrict__ S;         // adjoint: gradient accumulator (N)
+  // one fused soft source (comp < 0: none)
+  int src_comp;
+  Box src_box;
+  const T* __restrict__ src_prof;
+  const T* __restrict__ src_wave;  // points at wave[n] (forward) / wave[m] (adjoint)
+};
+
+struct Fused3Tiles {
+  int ty;  // rows per work-group (4 or 8)
+  int zc;  // z planes per work-group
+};
+
+// Tile configuration of the fused kernels (kernel 0 forward, 1 adjoint),
+// set by dnx_fdtd3d_set_tiles.
+Fused3Tiles fused3_tiles(int dtype, int kernel);
+
+template <typename T>
+int launch_fused3_fwd(const Fused3Args<T>& a, int ty, hipStream_t s);
+
+template <typename T>
+int launch_fused3_adj(const Fused3Args<T>& a, int ty, hipStream_t s);
+
+}  // namespace dnx

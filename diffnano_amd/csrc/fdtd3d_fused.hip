@@ -1,0 +1,553 @@
+// Fused single-pass 3D Yee step on gfx950, forward and adjoint.
+//
+// Work decomposition.  A work-group owns TY full rows (all nx <= 256 x
+// columns, one thread per column, waves of 64 consecutive x) of a chunk of
+// zc z-planes and marches z upward.  Per plane it loads the "E-like" input
+// of plane z+1 (the z+1 neighbour of this plane and the own value of the
+// next), the "H-like" input and ce of plane z, computes the H-like update
+// for rows y0-1 .. y0+TY-1 (the row y0-1 is a halo recomputed locally, so
+// no work-group reads another's new values), exchanges x neighbours through
+// LDS (double-buffered by plane parity: one barrier per plane), then
+// computes the E-like update of rows y0 .. y0+TY-1 and stores both.  y and
+// z neighbours live in registers.  Reads and writes are whole 256-byte rows
+// per wave; input and output are separate buffers (ping-pong), so adjacent
+// work-groups never race.  A chunk starting at z0 > 0 recomputes the
+// H-like plane z0-1 first.
+//
+// Work-group ids are remapped so that consecutive tiles (y-neighbours that
+// share halo rows) run on the same XCD and share its L2: hardware assigns
+// block b to XCD b % 8.
+//
+// Forward (FDTDSolver3D._time_step, fdtd3d.py:378-465; source fdtd3d.py:502):
+//   H' = bH H - kh (..D+ E..);   E' = bxby E + ce (..D- H'..);   E'_c += w s
+// Adjoint (transpose of one forward step; lam = dL/dE, mu = dL/dH'):
+//   mu' = bH mu + K_E^T (ce lam)  ... uses +1 neighbours of u = ce lam
+//   t   = kh (D+^T terms of mu')  ... uses -1 neighbours of mu'
+//   lam' = bxby lam + t;  S += E^m . t  - w_m s lam_src
+#include "fdtd3d_fused.h"
+
+#include <atomic>
+
+namespace dnx {
+
+// [dtype][kernel: 0 forward, 1 adjoint] -> (ty, zc).  Defaults from a sweep
+// at 256^3 on MI355X (float32: forward 4 x 32 ~5.0 TB/s algorithmic,
+// adjoint 2 x 16 ~4.6 TB/s); float64 uses ty = 2 to stay within registers.
+static std::atomic<int> g_ty[2][2] = {{{4}, {2}}, {{2}, {2}}};
+static std::atomic<int> g_zc[2][2] = {{{32}, {16}}, {{32}, {16}}};
+
+Fused3Tiles fused3_tiles(int dtype, int kernel) {
+  const int d = dtype == DNX_F64 ? 1 : 0;
+  const int k = kernel ? 1 : 0;
+  return {g_ty[d][k].load(), g_zc[d][k].load()};
+}
+
+template <int TY>
+__device__ inline void block_tile(int nblocks, int ty_tiles, int zc, int& y0, int& z0,
+                                  bool& live) {
+  // XCD-aware remap: blocks b, b+8, b+16 ... run on one XCD; give that XCD a
+  // contiguous run of logical tiles.
+  const int nb = gridDim.x;  // multiple of 8
+  const int b = blockIdx.x;
+  const int logical = (b & 7) * (nb >> 3) + (b >> 3);
+  live = logical < nblocks;
+  const int yt = logical % ty_tiles;
+  const int zt = logical / ty_tiles;
+  y0 = yt * TY;
+  z0 = zt * zc;
+}
+
+// ---------------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------------
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k3_fused_fwd(Fused3Args<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int W = a.nx;
+  T* sE = sm;                          // [2][2: Ey, Ez][TY+1][W]
+  T* sH = sm + 2 * 2 * (TY + 1) * W;   // [2][2: Hy, Hz][TY][W]
+#define SE(buf, c, r) sE[(((buf) * 2 + (c)) * (TY + 1) + (r)) * W]
+#define SH(buf, c, r) sH[(((buf) * 2 + (c)) * TY + (r)) * W]
+
+  int y0, z0;
+  bool live;
+  block_tile<TY>(a.nblocks, a.ty_tiles, a.zc, y0, z0, live);
+  if (!live) return;  // uniform per work-group
+  const int z1 = min(a.nz, z0 + a.zc);
+  const int x = threadIdx.x;
+  const bool xv = x < W;
+  const int64_t N = a.N, P = a.P;
+  const T* __restrict__ Ein = a.in;
+  const T* __restrict__ Hin = a.in + 3 * N;
+  T* __restrict__ Eout = a.out;
+  T* __restrict__ Hout = a.out + 3 * N;
+
+  const T kh = a.kh;
+  const T bxv = xv ? a.bx[x] : T(0);
+  const T axv = xv ? a.ax[x] : T(0);
+  const bool xf = xv && x >= 1, xl = x + 1 < W;
+
+  // Row r <-> y = y0 - 1 + r, r = 0 .. TY+1.
+  T byr[TY + 2], ayr[TY + 2];
+  bool rv[TY + 2];
+#pragma unroll
+  for (int r = 0; r < TY + 2; ++r) {
+    const int y = y0 - 1 + r;
+    rv[r] = y >= 0 && y < a.ny;
+    byr[r] = rv[r] ? a.by[y] : T(0);
+    ayr[r] = rv[r] ? a.ay[y] : T(0);
+  }
+  const int64_t colbase = int64_t(y0 - 1) * W + x;  // row r at colbase + r*W
+
+  T eC[3][TY + 2], eN[3][TY + 2];
+  T hp[2][TY];
+  const int zs = z0 > 0 ? z0 - 1 : z0;
+  {
+    const int64_t pb = int64_t(zs) * P + colbase;
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        eC[c][r] = (rv[r] && xv) ? Ein[c * N + pb + r * W] : T(0);
+    }
+    if (xv) {
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        (&SE(zs & 1, 0, r))[x] = eC[1][r];
+        (&SE(zs & 1, 1, r))[x] = eC[2][r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < TY; ++r) hp[0][r] = hp[1][r] = T(0);
+  }
+  __syncthreads();
+
+  const bool has_src = a.src_comp >= 0;
+  const T wsrc = has_src ? a.src_wave[0] : T(0);
+
+  for (int z = zs; z < z1; ++z) {
+    const int buf = z & 1;
+    const bool zl = z + 1 < a.nz;
+    const bool zf = z >= 1;
+    const bool doB = z >= z0;
+    // ---- loads: E(z+1) rows 0..TY+1, H(z) rows 0..TY, ce(z) rows 1..TY
+    T h[3][TY + 1];
+    T cez[TY];
+    {
+      const int64_t pn = int64_t(z + 1) * P + colbase;
+      const int64_t pz = int64_t(z) * P + colbase;
+#pragma unroll
+      for (int r = 0; r < TY + 2; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          eN[c][r] = (zl && rv[r] && xv) ? Ein[c * N + pn + r * W] : T(0);
+      }
+#pragma unroll
+      for (int r = 0; r < TY + 1; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          h[c][r] = (rv[r] && xv) ? Hin[c * N + pz + r * W] : T(0);
+      }
+#pragma unroll
+      for (int r = 1; r <= TY; ++r)
+        cez[r - 1] = (doB && rv[r] && xv) ? a.ce[pz + r * W] : T(0);
+    }
+    // ---- H-like update, rows 0..TY
+    T hn[3][TY + 1];
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) {
+      const int y = y0 - 1 + r;
+      const bool yl = y + 1 < a.ny;
+      const T ex = eC[0][r], ey = eC[1][r], ez = eC[2][r];
+      const T ey_x1 = xl ? (&SE(buf, 0, r))[x + 1] : T(0);
+      const T ez_x1 = xl ? (&SE(buf, 1, r))[x + 1] : T(0);
+      const T dEz_dy = yl ? eC[2][r + 1] - ez : T(0);
+      const T dEx_dy = yl ? eC[0][r + 1] - ex : T(0);
+      const T dEy_dz = zl ? eN[1][r] - ey : T(0);
+      const T dEx_dz = zl ? eN[0][r] - ex : T(0);
+      const T dEz_dx = xl ? ez_x1 - ez : T(0);
+      const T dEy_dx = xl ? ey_x1 - ey : T(0);
+      hn[0][r] = byr[r] * h[0][r] - kh * (ayr[r] * dEz_dy - axv * dEy_dz);
+      hn[1][r] = bxv * h[1][r] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
+      hn[2][r] = bxv * h[2][r] - kh * (axv * dEy_dx - ayr[r] * dEx_dy);
+    }
+    if (xv) {
+#pragma unroll
+      for (int r = 1; r <= TY; ++r) {
+        (&SH(buf, 0, r - 1))[x] = hn[1][r];
+        (&SH(buf, 1, r - 1))[x] = hn[2][r];
+      }
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        (&SE(buf ^ 1, 0, r))[x] = eN[1][r];
+        (&SE(buf ^ 1, 1, r))[x] = eN[2][r];
+      }
+    }
+    __syncthreads();
+    // ---- E-like update, rows 1..TY
+    if (doB) {
+      const int64_t pz = int64_t(z) * P + colbase;
+#pragma unroll
+      for (int r = 1; r <= TY; ++r) {
+        const int y = y0 - 1 + r;
+        const bool yf = y >= 1;
+        const T hx = hn[0][r], hy = hn[1][r], hz = hn[2][r];
+        const T hy_x0 = xf ? (&SH(buf, 0, r - 1))[x - 1] : T(0);
+        const T hz_x0 = xf ? (&SH(buf, 1, r - 1))[x - 1] : T(0);
+        const T dHz_dy = yf ? hz - hn[2][r - 1] : T(0);
+        const T dHx_dy = yf ? hx - hn[0][r - 1] : T(0);
+        const T dHy_dz = zf ? hy - hp[1][r - 1] : T(0);
+        const T dHx_dz = zf ? hx - hp[0][r - 1] : T(0);
+        const T dHz_dx = xf ? hz - hz_x0 : T(0);
+        const T dHy_dx = xf ? hy - hy_x0 : T(0);
+        const T c = cez[r - 1];
+        const T bxy = bxv * byr[r];
+        T e[3];
+        e[0] = bxy * eC[0][r] + c * (ayr[r] * dHz_dy - axv * dHy_dz);
+        e[1] = bxy * eC[1][r] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
+        e[2] = bxy * eC[2][r] + c * (axv * dHy_dx - ayr[r] * dHx_dy);
+        if (has_src) {
+          const Box& b = a.src_box;
+          const int dz = z - b.z0, dy = y - b.y0, dx = x - b.x0;
+          if (dz >= 0 && dz < b.nz && dy >= 0 && dy < b.ny && dx >= 0 && dx < b.nx) {
+            const T s = wsrc * a.src_prof[(int64_t(dz) * b.ny + dy) * b.nx + dx];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+              if (cc == a.src_comp) e[cc] += s;
+          }
+        }
+        if (rv[r] && xv) {
+          const int64_t i = pz + r * W;
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            Eout[cc * N + i] = e[cc];
+            Hout[cc * N + i] = hn[cc][r];
+          }
+          if (a.es) {
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) a.es[cc * N + i] = e[cc];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 1; r <= TY; ++r) {
+      hp[0][r - 1] = hn[0][r];
+      hp[1][r - 1] = hn[1][r];
+    }
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) eC[c][r] = eN[c][r];
+    }
+  }
+#undef SE
+#undef SH
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint.  Input lam = (lamE^{m+1}, muH^{m+1}); output (lamE^m, muH^m).
+//   u = ce lamE (E-like input scaled by its own ce)
+//   muHx' = by muHx + ax ([z>=1] uy - [z+1<nz] uy(z+1))
+//                   - ([y>=1] ay uz - [y+1<ny] ay(y+1) uz(y+1))
+//   muHy' = bx muHy + ([x>=1] ax uz - [x+1<nx] ax(x+1) uz(x+1))
+//                   - ax ([z>=1] ux - [z+1<nz] ux(z+1))
+//   muHz' = bx muHz + ([y>=1] ay ux - [y+1<ny] ay(y+1) ux(y+1))
+//                   - ay ([x>=1] uy - [x+1<nx] uy(x+1))
+//   tx = kh (D+y^T(ay muHz') - ax D+z^T(muHy'))
+//   ty = kh (ax D+z^T(muHx') - D+x^T(ax muHz'))
+//   tz = kh (ay D+x^T(muHy') - D+y^T(ay muHx'))
+//   with (D+^T g)[j] = [j>=1] g[j-1] - [j<=n-2] g[j].
+// ---------------------------------------------------------------------------
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k3_fused_adj(Fused3Args<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int W = a.nx;
+  T* sE = sm;                          // [2][2: uy, uz][TY+1][W]
+  T* sH = sm + 2 * 2 * (TY + 1) * W;   // [2][2: muHy, muHz][TY][W]
+#define SE(buf, c, r) sE[(((buf) * 2 + (c)) * (TY + 1) + (r)) * W]
+#define SH(buf, c, r) sH[(((buf) * 2 + (c)) * TY + (r)) * W]
+
+  int y0, z0;
+  bool live;
+  block_tile<TY>(a.nblocks, a.ty_tiles, a.zc, y0, z0, live);
+  if (!live) return;
+  const int z1 = min(a.nz, z0 + a.zc);
+  const int x = threadIdx.x;
+  const bool xv = x < W;
+  const int64_t N = a.N, P = a.P;
+  const T* __restrict__ Lin = a.in;
+  const T* __restrict__ Min = a.in + 3 * N;
+  T* __restrict__ Lout = a.out;
+  T* __restrict__ Mout = a.out + 3 * N;
+
+  const T kh = a.kh;
+  const T bxv = xv ? a.bx[x] : T(0);
+  const T axv = xv ? a.ax[x] : T(0);
+  const T axp = (x + 1 < W) ? a.ax[x + 1] : T(0);
+  const T axm = (xv && x >= 1) ? a.ax[x - 1] : T(0);
+  const bool xf = xv && x >= 1, xl = x + 1 < W, xin = x <= W - 2;
+
+  T byr[TY + 2], ayr[TY + 2];
+  bool rv[TY + 2];
+#pragma unroll
+  for (int r = 0; r < TY + 2; ++r) {
+    const int y = y0 - 1 + r;
+    rv[r] = y >= 0 && y < a.ny;
+    byr[r] = rv[r] ? a.by[y] : T(0);
+    ayr[r] = rv[r] ? a.ay[y] : T(0);
+  }
+  const int64_t colbase = int64_t(y0 - 1) * W + x;
+
+  T uC[3][TY + 2], uN[3][TY + 2];
+  T lC[3][TY], lN[3][TY];  // lamE own rows 1..TY at planes z, z+1
+  T mp[2][TY];             // muHx', muHy' of plane z-1, rows 1..TY
+  const int zs = z0 > 0 ? z0 - 1 : z0;
+  {
+    const int64_t pb = int64_t(zs) * P + colbase;
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+      const T c = (rv[r] && xv) ? a.ce[pb + r * W] : T(0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T l = (rv[r] && xv) ? Lin[k * N + pb + r * W] : T(0);
+        uC[k][r] = c * l;
+        if (r >= 1 && r <= TY) lC[k][r - 1] = l;
+      }
+    }
+    if (xv) {
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        (&SE(zs & 1, 0, r))[x] = uC[1][r];
+        (&SE(zs & 1, 1, r))[x] = uC[2][r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < TY; ++r) mp[0][r] = mp[1][r] = T(0);
+  }
+  __syncthreads();
+
+  const bool has_src = a.src_comp >= 0;
+  const T wsrc = has_src ? a.src_wave[0] : T(0);
+
+  for (int z = zs; z < z1; ++z) {
+    const int buf = z & 1;
+    const bool zl = z + 1 < a.nz;
+    const bool zf = z >= 1;
+    const bool zin = z <= a.nz - 2;
+    const bool doB = z >= z0;
+    T m[3][TY + 1];
+    T em[3][TY];
+    {
+      const int64_t pn = int64_t(z + 1) * P + colbase;
+      const int64_t pz = int64_t(z) * P + colbase;
+#pragma unroll
+      for (int r = 0; r < TY + 2; ++r) {
+        const bool ok = zl && rv[r] && xv;
+        const T c = ok ? a.ce[pn + r * W] : T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const T l = ok ? Lin[k * N + pn + r * W] : T(0);
+          uN[k][r] = c * l;
+          if (r >= 1 && r <= TY) lN[k][r - 1] = l;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < TY + 1; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          m[k][r] = (rv[r] && xv) ? Min[k * N + pz + r * W] : T(0);
+      }
+#pragma unroll
+      for (int r = 1; r <= TY; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          em[k][r - 1] = (doB && rv[r] && xv) ? a.em[k * N + pz + r * W] : T(0);
+      }
+    }
+    // ---- mu update, rows 0..TY
+    T mn[3][TY + 1];
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) {
+      const int y = y0 - 1 + r;
+      const bool yf = y >= 1, yl = y + 1 < a.ny;
+      const T ux = uC[0][r], uy = uC[1][r], uz = uC[2][r];
+      const T uy_x1 = xl ? (&SE(buf, 0, r))[x + 1] : T(0);
+      const T uz_x1 = xl ? (&SE(buf, 1, r))[x + 1] : T(0);
+      const T tz_y = (zf ? uy : T(0)) - (zl ? uN[1][r] : T(0));
+      const T tz_x = (zf ? ux : T(0)) - (zl ? uN[0][r] : T(0));
+      const T ty_z = (yf ? ayr[r] * uz : T(0)) - (yl ? ayr[r + 1] * uC[2][r + 1] : T(0));
+      const T ty_x = (yf ? ayr[r] * ux : T(0)) - (yl ? ayr[r + 1] * uC[0][r + 1] : T(0));
+      const T tx_z = (xf ? axv * uz : T(0)) - (xl ? axp * uz_x1 : T(0));
+      const T tx_y = (xf ? uy : T(0)) - (xl ? uy_x1 : T(0));
+      mn[0][r] = byr[r] * m[0][r] + axv * tz_y - ty_z;
+      mn[1][r] = bxv * m[1][r] + tx_z - axv * tz_x;
+      mn[2][r] = bxv * m[2][r] + ty_x - ayr[r] * tx_y;
+    }
+    if (xv) {
+#pragma unroll
+      for (int r = 1; r <= TY; ++r) {
+        (&SH(buf, 0, r - 1))[x] = mn[1][r];
+        (&SH(buf, 1, r - 1))[x] = mn[2][r];
+      }
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        (&SE(buf ^ 1, 0, r))[x] = uN[1][r];
+        (&SE(buf ^ 1, 1, r))[x] = uN[2][r];
+      }
+    }
+    __syncthreads();
+    // ---- lam update + gradient, rows 1..TY
+    if (doB) {
+      const int64_t pz = int64_t(z) * P + colbase;
+#pragma unroll
+      for (int r = 1; r <= TY; ++r) {
+        const int y = y0 - 1 + r;
+        const bool yf = y >= 1, yin = y <= a.ny - 2;
+        const T mx = mn[0][r], my = mn[1][r], mz = mn[2][r];
+        const T my_x0 = xf ? (&SH(buf, 0, r - 1))[x - 1] : T(0);
+        const T mz_x0 = xf ? (&SH(buf, 1, r - 1))[x - 1] : T(0);
+        const T y_mz = (yf ? ayr[r - 1] * mn[2][r - 1] : T(0)) - (yin ? ayr[r] * mz : T(0));
+        const T y_mx = (yf ? ayr[r - 1] * mn[0][r - 1] : T(0)) - (yin ? ayr[r] * mx : T(0));
+        const T z_my = (zf ? mp[1][r - 1] : T(0)) - (zin ? my : T(0));
+        const T z_mx = (zf ? mp[0][r - 1] : T(0)) - (zin ? mx : T(0));
+        const T x_mz = (xf ? axm * mz_x0 : T(0)) - (xin ? axv * mz : T(0));
+        const T x_my = (xf ? my_x0 : T(0)) - (xin ? my : T(0));
+        T t[3];
+        t[0] = kh * (y_mz - axv * z_my);
+        t[1] = kh * (axv * z_mx - x_mz);
+        t[2] = kh * (ayr[r] * x_my - y_mx);
+        const T bxy = bxv * byr[r];
+        T s = em[0][r - 1] * t[0] + em[1][r - 1] * t[1] + em[2][r - 1] * t[2];
+        if (has_src) {
+          const Box& b = a.src_box;
+          const int dz = z - b.z0, dy = y - b.y0, dx = x - b.x0;
+          if (dz >= 0 && dz < b.nz && dy >= 0 && dy < b.ny && dx >= 0 && dx < b.nx) {
+            const T w = wsrc * a.src_prof[(int64_t(dz) * b.ny + dy) * b.nx + dx];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+              if (cc == a.src_comp) s -= w * lC[cc][r - 1];
+          }
+        }
+        if (rv[r] && xv) {
+          const int64_t i = pz + r * W;
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            Lout[cc * N + i] = bxy * lC[cc][r - 1] + t[cc];
+            Mout[cc * N + i] = mn[cc][r];
+          }
+          a.S[i] += s;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 1; r <= TY; ++r) {
+      mp[0][r - 1] = mn[0][r];
+      mp[1][r - 1] = mn[1][r];
+    }
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) uC[k][r] = uN[k][r];
+    }
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) lC[k][r] = lN[k][r];
+    }
+  }
+#undef SE
+#undef SH
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+template <typename T, int TY>
+static int launch_fwd_ty(const Fused3Args<T>& a, hipStream_t s) {
+  const int threads = ((a.nx + 63) / 64) * 64;
+  const int nb = ((a.nblocks + 7) / 8) * 8;
+  const size_t smem = sizeof(T) * size_t(2 * 2 * (TY + 1) + 2 * 2 * TY) * a.nx;
+  static bool attr_set = false;
+  if (!attr_set) {
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused_fwd<T, TY>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(sizeof(T) * (2 * 2 * (TY + 1) + 2 * 2 * TY) *
+                                          FUSED_MAX_W)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k3_fused_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, a);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T, int TY>
+static int launch_adj_ty(const Fused3Args<T>& a, hipStream_t s) {
+  const int threads = ((a.nx + 63) / 64) * 64;
+  const int nb = ((a.nblocks + 7) / 8) * 8;
+  const size_t smem = sizeof(T) * size_t(2 * 2 * (TY + 1) + 2 * 2 * TY) * a.nx;
+  static bool attr_set = false;
+  if (!attr_set) {
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused_adj<T, TY>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(sizeof(T) * (2 * 2 * (TY + 1) + 2 * 2 * TY) *
+                                          FUSED_MAX_W)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k3_fused_adj<T, TY>), dim3(nb), dim3(threads), smem, s, a);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_fused3_fwd(const Fused3Args<T>& a, int ty, hipStream_t s) {
+  if (ty == 8) return launch_fwd_ty<T, 8>(a, s);
+  if (ty == 4) return launch_fwd_ty<T, 4>(a, s);
+  if (ty == 2) return launch_fwd_ty<T, 2>(a, s);
+  set_error("fused3: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template <typename T>
+int launch_fused3_adj(const Fused3Args<T>& a, int ty, hipStream_t s) {
+  if (ty == 8) return launch_adj_ty<T, 8>(a, s);
+  if (ty == 4) return launch_adj_ty<T, 4>(a, s);
+  if (ty == 2) return launch_adj_ty<T, 2>(a, s);
+  set_error("fused3: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template int launch_fused3_fwd<float>(const Fused3Args<float>&, int, hipStream_t);
+template int launch_fused3_fwd<double>(const Fused3Args<double>&, int, hipStream_t);
+template int launch_fused3_adj<float>(const Fused3Args<float>&, int, hipStream_t);
+template int launch_fused3_adj<double>(const Fused3Args<double>&, int, hipStream_t);
+
+}  // namespace dnx
+
+extern "C" int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t ty,
+                                        int32_t zc) {
+  using namespace dnx;
+  DNX_REQUIRE(dtype == DNX_F32 || dtype == DNX_F64, "set_tiles: bad dtype %d", dtype);
+  DNX_REQUIRE(kernel == 0 || kernel == 1, "set_tiles: kernel must be 0 (forward) or 1 (adjoint)");
+  DNX_REQUIRE(ty == 2 || ty == 4 || ty == 8, "set_tiles: ty must be 2, 4 or 8 (got %d)", ty);
+  DNX_REQUIRE(zc >= 1 && zc <= 4096, "set_tiles: zc must be in [1, 4096] (got %d)", zc);
+  const int d = dtype == DNX_F64 ? 1 : 0;
+  g_ty[d][kernel] = ty;
+  g_zc[d][kernel] = zc;
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_fdtd3d_get_tiles(int32_t dtype, int32_t kernel, int32_t* ty,
+                                        int32_t* zc) {
+  using namespace dnx;
+  DNX_REQUIRE(ty && zc, "get_tiles: null output");
+  const Fused3Tiles t = fused3_tiles(dtype, kernel);
+  *ty = t.ty;
+  *zc = t.zc;
+  return DNX_OK;
+}

@@ -98,7 +98,7 @@ class FDTDSolver2D:
         eng = self._engine(dev, te=te)
         eng.set_eps(eps_r.to(dev, torch.float64))
         state = torch.cat([t.reshape(-1).to(dev, self.dtype) for t in (a, b, c)])
-        eng.run(state, 0, 1, (None, 0), (None, 0))
+        state = eng.run(state, 0, 1, (None, 0), (None, 0))
         N = eng.N
         return tuple(state[k * N:(k + 1) * N].view(eng.shape).to(a.device, torch.float64)
                      for k in range(3))

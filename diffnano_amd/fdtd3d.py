@@ -116,7 +116,7 @@ class FDTDSolver3D:
         eng = self._engine(dev)
         eng.set_eps(eps_r.to(dev, torch.float64))
         state = torch.cat([t.reshape(-1).to(dev, self.dtype) for t in (Ex, Ey, Ez, Hx, Hy, Hz)])
-        eng.run(state, 0, 1, (None, 0), (None, 0))
+        state = eng.run(state, 0, 1, (None, 0), (None, 0))
         N = eng.N
         out = [state[k * N:(k + 1) * N].view(eng.shape).to(Ex.device, torch.float64)
                for k in range(6)]

@@ -159,6 +159,14 @@ int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work, int32
                                const dnx_monitor* mons, int32_t n_mon, void* S,
                                void* stream, int32_t* final_in_work);
 
+/* Tile shape of the fused single-pass 3D kernels for one dtype and kernel
+ * (0 forward/replay, 1 adjoint): ty rows (2, 4 or 8) and zc z-planes per
+ * work-group.  Process-wide; set by the autotuner.  No reference
+ * counterpart (the reference has no kernels); it changes performance only,
+ * never results. */
+int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t ty, int32_t zc);
+int32_t dnx_fdtd3d_get_tiles(int32_t dtype, int32_t kernel, int32_t* ty, int32_t* zc);
+
 /* ---- 2D -------------------------------------------------------------- */
 
 /* As dnx_fdtd3d_run for the 2D state (3*N): TM FDTDSolver2D._time_step_tm

@@ -305,6 +305,54 @@ def test_solver_protocol_and_metadata():
     assert s3c.forward(torch.ones(10, 10, 10) * 2.0).field.device.type == "cuda"
 
 
+def _run_fwd_grad(shape, dtype, T=30, src=None, L=3):
+    rng = np.random.default_rng(11)
+    eps = torch.tensor(1.5 + 3.0 * rng.random(shape), requires_grad=True)
+    s = _solver3d(grid_shape=shape, dl=25.0, pml_layers=L, n_steps=T, courant=0.45, dtype=dtype,
+                  checkpoint_every=7)
+    src = src or {"type": "gaussian_pulse", "pos": [shape[0] // 2, shape[1] // 3, shape[2] // 2]}
+    f = s.forward(eps, source=src).field
+    w = torch.from_numpy(rng.standard_normal(f.shape))
+    (f * w).sum().backward()
+    return f.detach().clone(), eps.grad.clone()
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-13), (torch.float32, 1e-6)])
+def test_fused_kernels_match_simple_variant(dtype, tol, monkeypatch):
+    """The fused single-pass kernels against the two-kernel variant, on a
+    ragged grid (partial y tiles, partial waves, several z chunks)."""
+    shape = (37, 21, 70)
+    f1, g1 = _run_fwd_grad(shape, dtype)
+    monkeypatch.setenv("DNX_FDTD3D_VARIANT", "simple")
+    f2, g2 = _run_fwd_grad(shape, dtype)
+    assert rel_l2(f1.numpy(), f2.numpy()) < tol
+    assert rel_l2(g1.numpy(), g2.numpy()) < tol
+
+
+def test_tile_shapes_give_identical_results():
+    import diffnano_amd._lib as L
+    import ctypes as C
+    shape = (33, 19, 96)
+    outs = []
+    saved = []
+    for k in (0, 1):
+        ty, zc = C.c_int32(), C.c_int32()
+        L.check(L.lib().dnx_fdtd3d_get_tiles(L.DNX_F32, k, C.byref(ty), C.byref(zc)))
+        saved.append((ty.value, zc.value))
+    try:
+        for ty, zc in ((8, 16), (4, 5), (2, 3), (8, 1)):
+            for k in (0, 1):
+                L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, ty, zc))
+            outs.append(_run_fwd_grad(shape, torch.float32, src={"type": "gaussian_pulse",
+                                                                 "plane": "xz", "y": 7,
+                                                                 "component": "Ex"}))
+    finally:
+        for k, (ty, zc) in enumerate(saved):
+            L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, ty, zc))
+    for f, g in outs[1:]:
+        assert torch.equal(f, outs[0][0]) and torch.equal(g, outs[0][1])
+
+
 def test_determinism_bitwise():
     g = golden("grad3d.npz")
     outs = []
