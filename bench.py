@@ -9,9 +9,15 @@ One bench "step" = forward + loss.backward() (dL/d eps over the full grid).
 Prints one JSON line.  value = Mcell-updates/s of fwd+adj = N_cells * T /
 (t_fwd + t_adj).  roofline: the dominant phase's algorithmic HBM bytes per
 cell-update (forward 52 B, replay 64 B, adjoint 72 B, FP32) times the cell
-updates it performs, over its HIP-event time, against 8 TB/s.
-cpu_baseline: the FP64 NumPy oracle (a port of the reference's CPU path),
-fwd+adj over a few steps of the same 256^3 grid on the host.
+updates it performs, over its HIP-event time (events on the stream the
+kernels run on), against 8 TB/s; ``traffic`` is the measured HBM bytes per
+launch of that kernel from rocprofv3 counters (profiles/, scaled by a copy
+kernel of known size).  cpu_baseline: the FP64 NumPy oracle (a port of the
+reference's CPU path), fwd+adj over a few steps of the same grid.
+
+``--pmc-probe``: run a short fixed sequence (a 256 MiB device copy three
+times, then 4 fused forward and 4 fused adjoint steps of the workload grid)
+for hardware-counter collection; prints what it ran and exits.
 """
 
 from __future__ import annotations
@@ -19,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,7 +37,12 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 BYTES = {"forward": 52, "replay": 64, "adjoint": 72}
+# library timing name -> (phase, kernel symbol, algorithmic bytes per cell-update)
+TIMED = {"fdtd3d_fwd": ("forward", "k3_fused_fwd", 52),
+         "fdtd3d_fwd_store": ("replay", "k3_fused_fwd", 64),
+         "fdtd3d_adj": ("adjoint", "k3_fused_adj", 72)}
 METRIC = "Mcell-updates/s fwd+adjoint, %HBM roofline; designs/s batched vs plain @1e-4"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fdtd3d_256_hbm_traffic.md")
 
 
 def _cpu_baseline(eps, solver_kw, steps):
@@ -54,6 +66,62 @@ def _cpu_baseline(eps, solver_kw, steps):
                       f"{t:.1f} s"}
 
 
+def _copy_bandwidth(torch, dev, mib=1024):
+    """Device-to-device copy bandwidth (read + write bytes / time)."""
+    a = torch.empty(mib * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2 * a.numel() * 4 / (e0.elapsed_time(e1) / 10) / 1e6
+
+
+def _read_traffic():
+    """Measured HBM bytes per launch per kernel from the committed counter
+    summary (lines `kernel_name: <bytes> bytes per launch`), or {}."""
+    out = {}
+    if os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            m = re.match(r"\s*[-*]?\s*`?(\w+)`?\s*:\s*([0-9.eE+]+)\s*bytes per launch", line)
+            if m:
+                out[m.group(1)] = float(m.group(2))
+    return out
+
+
+def _pmc_probe(torch, dev, solver_kw):
+    from diffnano_amd import FDTDSolver3D
+
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    shape = solver_kw["grid_shape"]
+    N = int(np.prod(shape))
+    s = FDTDSolver3D(**solver_kw, device=dev)
+    eng = s._engine(dev)
+    rng = np.random.default_rng(0)
+    eng.set_eps(torch.tensor(2.0 + 10 * rng.random(shape), device=dev))
+    state = torch.randn(6 * N, device=dev) * 1e-3
+    work = torch.empty_like(state)
+    eng.run(state, 0, 4, (None, 0), (None, 0), work=work)
+    estore = torch.empty(3 * 3 * N, device=dev)
+    lam = torch.randn(6 * N, device=dev) * 1e-3
+    S = torch.zeros(N, device=dev)
+    eng.adjoint_run(lam, 4, 0, None, estore, (None, 0), (None, 0), S, torch.empty_like(lam))
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_probe": {"copy_bytes_read": a.numel() * 4,
+                                    "copy_bytes_written": a.numel() * 4,
+                                    "grid": list(shape), "fwd_launches": 4, "adj_launches": 3,
+                                    "fwd_alg_bytes": 52 * N, "adj_alg_bytes": 72 * N}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +130,7 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--n-steps", type=int, default=None, help="time steps (default: config's)")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the FDTD "
@@ -78,11 +147,15 @@ def main():
     cfg = workloads.config_c2 if args.config == "c2" else workloads.config_c1
     kw = {} if args.n_steps is None else {"n_steps": args.n_steps}
     eps_np, solver_kw, src, mons = cfg(seed=0, **kw)
+    if args.pmc_probe:
+        _pmc_probe(torch, dev, solver_kw)
+        return
     solver = FDTDSolver3D(**solver_kw, device=dev)
     eps0 = torch.tensor(eps_np, dtype=torch.float64, device=dev)
     D, H, W = solver_kw["grid_shape"]
     N = D * H * W
     T = solver_kw["n_steps"]
+    K = solver_kw["checkpoint_every"]
 
     def one_step():
         eps = eps0.clone().requires_grad_(True)
@@ -91,11 +164,19 @@ def main():
         loss.backward()
         return loss, eps.grad
 
+    import ctypes
+
+    from diffnano_amd import _lib as L
+
     for _ in range(args.warmup):
         one_step()
     torch.cuda.synchronize()
     timer = PhaseTimer()
     set_phase_timer(timer)
+    L.lib().dnx_timing_enable(1)
+    for name in TIMED:  # drop anything recorded before the timed region
+        L.lib().dnx_timing_read(name.encode(), ctypes.byref(ctypes.c_double()),
+                                ctypes.byref(ctypes.c_int64()))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -103,17 +184,32 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     set_phase_timer(None)
+    L.lib().dnx_timing_enable(0)
     phases = timer.totals_ms()
+    kern = {}
+    for name, (phase, sym, byt) in TIMED.items():
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        L.check(L.lib().dnx_timing_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        if cnt.value:
+            avg = ms.value / cnt.value
+            kern[name] = {"kernel": sym, "launches": cnt.value, "total_ms": round(ms.value, 2),
+                          "avg_us": round(avg * 1e3, 2),
+                          "gbs": round(byt * N / (avg / 1e3) / 1e9, 1)}
     ms_per_step = wall / args.steps * 1e3
     value = N * T / (wall / args.steps) / 1e6
 
-    # Per-phase algorithmic bandwidth; the dominant phase is the roofline row.
-    updates = {"forward": N * T * args.steps, "replay": N * (T - (T + 63) // 64) * args.steps,
+    # Per-phase algorithmic bandwidth (phases include the small side kernels).
+    nseg = (T + K - 1) // K
+    updates = {"forward": N * T * args.steps, "replay": N * (T - nseg) * args.steps,
                "adjoint": N * (T - 1) * args.steps}
     gbs = {k: BYTES[k] * updates[k] / (phases[k] / 1e3) / 1e9 for k in phases if phases[k] > 0}
-    dom = max(phases, key=lambda k: phases[k])
     total_bytes = sum(BYTES[k] * updates[k] for k in BYTES)
     overall_gbs = total_bytes / (sum(phases.values()) / 1e3) / 1e9
+    copy_gbs = _copy_bandwidth(torch, dev)
+    # Dominant kernel: the timed launch class with the largest total time.
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    dom_k = kern[dom]
+    traffic = _read_traffic().get(dom_k["kernel"])
 
     cpu = _cpu_baseline(eps_np, solver_kw, args.cpu_steps)
 
@@ -132,14 +228,20 @@ def main():
         "data": "synthetic (seeded random-rho design slab, waveguides)",
         "config": {"workload": f"3D Yee FDTD {D}x{H}x{W}, {T} steps, "
                                f"{solver_kw['pml_layers']}-cell CPML, fwd+time-reversal adjoint, "
-                               f"checkpoint every {solver_kw['checkpoint_every']}",
+                               f"checkpoint every {K}",
                    "grid": [D, H, W], "n_steps": T, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": f"{dom} phase", "achieved": round(gbs[dom], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs[dom] / HBM_PEAK_GBS, 4), "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": f"{dom_k['kernel']} ({TIMED[dom][0]} launches)",
+                     "achieved": dom_k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom_k["gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "alg_bytes_per_launch": TIMED[dom][2] * N,
+                     "avg_launch_us": dom_k["avg_us"]},
+        "kernels": kern,
         "phases_ms_per_step": {k: round(v / args.steps, 2) for k, v in phases.items()},
         "phase_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "fwd_adj_roofline_frac": round(overall_gbs / HBM_PEAK_GBS, 4),
+        "copy_gbs": round(copy_gbs, 1),
+        "fwd_adj_frac_of_copy": round(overall_gbs / copy_gbs, 4),
         "fwd_mcells": round(N * T / (phases["forward"] / args.steps / 1e3) / 1e6, 2),
         "loss": float(loss.item()),
         "grad_l2": float(grad.norm().item()),

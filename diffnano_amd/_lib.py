@@ -56,6 +56,8 @@ _L = C.c_int64
 _SIGS = {
     "dnx_abi_version": ([], _I),
     "dnx_last_error": ([], C.c_char_p),
+    "dnx_timing_enable": ([_I], _I),
+    "dnx_timing_read": ([C.c_char_p, C.POINTER(C.c_double), C.POINTER(_L)], _I),
     "dnx_prepare_coef": ([_P, _P, _L, C.c_double, _I, _P], _I),
     "dnx_finalize_grad": ([_P, _P, _P, _L, _I, _P], _I),
     "dnx_fdtd3d_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, C.POINTER(Source), _I,

@@ -57,6 +57,15 @@ inline Box make_box(const dnx_box& b) {
   return Box{b.z0, b.y0, b.x0, b.z1 - b.z0, b.y1 - b.y0, b.x1 - b.x0};
 }
 
+// Optional per-launch timing of the hot kernels (dnx_timing_*): when enabled,
+// timed_begin records a start event on the launch stream and timed_end a stop
+// event filed under `name`.  Disabled: both are no-ops.
+struct TimedLaunch {
+  hipEvent_t start = nullptr;
+};
+TimedLaunch timed_begin(hipStream_t s);
+void timed_end(const char* name, hipStream_t s, TimedLaunch& t);
+
 // Validate sources / monitors against a grid (2D callers pass nz = 1).
 int check_sources(const dnx_source* s, int n, int nz, int ny, int nx, int max_comp);
 int check_monitors(const dnx_monitor* m, int n, int nz, int ny, int nx, int comp_bits,

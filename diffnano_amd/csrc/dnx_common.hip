@@ -1,5 +1,10 @@
-// libdnx_fdtd: error state, coefficient preparation, and the per-step side
-// kernels (soft sources, running DFT monitors and their adjoints).
+// libdnx_fdtd: error state, launch timing, coefficient preparation, and the
+// per-step side kernels (soft sources, running DFT monitors and adjoints).
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "dnx_aux.h"
 
 namespace dnx {
@@ -16,6 +21,42 @@ void set_error(const char* fmt, ...) {
 }
 
 const char* last_error() { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Per-launch timing registry
+// ---------------------------------------------------------------------------
+
+static std::atomic<bool> g_timing_on{false};
+static std::mutex g_timing_mu;
+static std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> g_timing;
+
+TimedLaunch timed_begin(hipStream_t s) {
+  TimedLaunch t;
+  if (!g_timing_on.load(std::memory_order_relaxed)) return t;
+  if (hipEventCreate(&t.start) != hipSuccess) {
+    t.start = nullptr;
+    return t;
+  }
+  if (hipEventRecord(t.start, s) != hipSuccess) {
+    (void)hipEventDestroy(t.start);
+    t.start = nullptr;
+  }
+  return t;
+}
+
+void timed_end(const char* name, hipStream_t s, TimedLaunch& t) {
+  if (t.start == nullptr) return;
+  hipEvent_t stop = nullptr;
+  if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, s) != hipSuccess) {
+    if (stop) (void)hipEventDestroy(stop);
+    (void)hipEventDestroy(t.start);
+    t.start = nullptr;
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing[name].emplace_back(t.start, stop);
+  t.start = nullptr;
+}
 
 int check_sources(const dnx_source* s, int n, int nz, int ny, int nx, int max_comp) {
   DNX_REQUIRE(n >= 0 && (n == 0 || s != nullptr), "sources: null array with n_src=%d", n);
@@ -233,6 +274,41 @@ DNX_INST(double)
 extern "C" int32_t dnx_abi_version(void) { return DNX_ABI_VERSION; }
 
 extern "C" const char* dnx_last_error(void) { return dnx::last_error(); }
+
+extern "C" int32_t dnx_timing_enable(int32_t on) {
+  dnx::g_timing_on = on != 0;
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_timing_read(const char* name, double* total_ms, int64_t* launches) {
+  using namespace dnx;
+  DNX_REQUIRE(name && total_ms && launches, "timing_read: null argument");
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    auto it = g_timing.find(name);
+    if (it != g_timing.end()) evs.swap(it->second);
+  }
+  double tot = 0.0;
+  int rc = DNX_OK;
+  for (auto& p : evs) {
+    float ms = 0.f;
+    if (rc == DNX_OK) {
+      hipError_t e = hipEventSynchronize(p.second);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.first, p.second);
+      if (e != hipSuccess) {
+        set_error("timing_read: %s", hipGetErrorString(e));
+        rc = DNX_EHIP;
+      }
+    }
+    tot += ms;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  *total_ms = tot;
+  *launches = int64_t(evs.size());
+  return rc;
+}
 
 extern "C" int32_t dnx_prepare_coef(const double* eps, void* ce, int64_t n, double dt_over_dl,
                                     int32_t dtype, void* stream) {

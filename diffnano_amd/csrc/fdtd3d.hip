@@ -329,7 +329,9 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
       fa.out = nxt;
       fa.es = slot;
       if (fa.src_comp >= 0) fa.src_wave = static_cast<const T*>(srcs[0].wave) + n;
+      TimedLaunch tl = timed_begin(s);
       int rc = launch_fused3_fwd<T>(fa, ty, s);
+      timed_end(slot ? "fdtd3d_fwd_store" : "fdtd3d_fwd", s, tl);
       if (rc) return rc;
     } else {
       hipLaunchKernelGGL(k3_update_h<T>, grid, block, 0, s, g, cur);
@@ -410,7 +412,9 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
       fa.out = nxt;
       fa.em = em;
       if (fa.src_comp >= 0) fa.src_wave = static_cast<const T*>(srcs[0].wave) + m;
+      TimedLaunch tl = timed_begin(s);
       int rc = launch_fused3_adj<T>(fa, ty, s);
+      timed_end("fdtd3d_adj", s, tl);
       if (rc) return rc;
     } else {
       hipLaunchKernelGGL(k3_adj_h<T>, grid, block, 0, s, g, cur);

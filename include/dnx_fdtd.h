@@ -109,6 +109,14 @@ typedef struct dnx_monitor {
 int32_t dnx_abi_version(void);
 const char* dnx_last_error(void);
 
+/* Per-launch timing of the hot kernels with HIP events on the launch
+ * stream (no reference counterpart; measurement only).  Names:
+ * "fdtd3d_fwd", "fdtd3d_fwd_store" (replay, also stores E), "fdtd3d_adj".
+ * dnx_timing_read waits for the recorded events, returns the summed
+ * duration and the launch count, and clears that name. */
+int32_t dnx_timing_enable(int32_t on);
+int32_t dnx_timing_read(const char* name, double* total_ms, int64_t* launches);
+
 /* ce[i] = dt / (eps[i] * dl) computed in float64, stored in dtype.
  * eps is float64.  Replaces the reference's per-step (dt / eps_r)
  * (fdtd3d.py:435; fdtd2d.py:262). */
