@@ -49,6 +49,15 @@ class Monitor(C.Structure):
                 ("phase", C.c_void_p), ("acc", C.c_void_p), ("grad", C.c_void_p)]
 
 
+class Design3D(C.Structure):
+    _fields_ = [("grid", Grid3D), ("src", Source), ("mon", Monitor)]
+
+
+class Pools3D(C.Structure):
+    _fields_ = [("state", C.c_void_p * 2), ("lam", C.c_void_p * 2), ("ckpt", C.c_void_p),
+                ("estore", C.c_void_p), ("S", C.c_void_p)]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _L = C.c_int64
@@ -68,6 +77,13 @@ _SIGS = {
                                 C.POINTER(Monitor), _I, _P, _P, C.POINTER(_I)], _I),
     "dnx_fdtd3d_set_tiles": ([_I, _I, _I, _I], _I),
     "dnx_fdtd3d_get_tiles": ([_I, _I, C.POINTER(_I), C.POINTER(_I)], _I),
+    "dnx_batch3d_workspace_bytes": ([_I], _L),
+    "dnx_batch3d_create": ([C.POINTER(Design3D), _I, _I, _I, C.POINTER(Pools3D), _P, _L, _I, _P,
+                            C.POINTER(_P)], _I),
+    "dnx_batch3d_forward": ([_P, _I, _P], _I),
+    "dnx_batch3d_backward": ([_P, _P], _I),
+    "dnx_batch3d_uses_graph": ([_P], _I),
+    "dnx_batch3d_destroy": ([_P], _I),
     "dnx_fdtd2d_run": ([C.POINTER(Grid2D), _P, _P, _I, _I, C.POINTER(Source), _I,
                         C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),
     "dnx_fdtd2d_adjoint_init": ([C.POINTER(Grid2D), _P, _P, C.POINTER(Monitor), _I, _I, _P, _P],

@@ -1,0 +1,358 @@
+// Batched 3D FDTD over many designs with different grids: one persistent
+// launch per time step covers every design's tiles; the step index lives in
+// a device counter, so a segment of K steps is captured once into a HIP
+// graph and replayed for every checkpoint segment of the forward, the
+// replay and the adjoint.  Semantics per design are exactly those of
+// dnx_fdtd3d_run / dnx_fdtd3d_adjoint_run (same tile bodies).
+#include <algorithm>
+#include <vector>
+
+#include "fdtd3d_fused.h"
+
+struct dnx_batch3d_plan {
+  int dtype = 0, n = 0, T = 0, K = 0;
+  int ty_f = 0, ty_a = 0, max_w = 0, grid_f = 0, grid_a = 0;
+  int64_t max_mon_cells = 0, sum6 = 0, sum3 = 0, sumN = 0;
+  std::vector<unsigned char> host_tables;  // staging of the two descriptor tables
+  void* d_fwd = nullptr;
+  void* d_adj = nullptr;
+  int* d_ctr = nullptr;
+  dnx::Batch3Launch Lf{}, La{};
+  dnx_batch3d_pools pools{};
+  std::vector<dnx_design3d> designs;
+  std::vector<int64_t> off;  // sum_{e<d} N_e
+  bool use_graph = false;
+  hipGraphExec_t g_fwd = nullptr, g_rep = nullptr, g_adj = nullptr;
+};
+
+namespace dnx {
+
+template <typename T>
+static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, bool adjoint,
+                       Batch3Design<T>* out, int* total_tiles) {
+  const Fused3Tiles t = fused3_tiles(p->dtype, adjoint ? 1 : 0);
+  int start = 0;
+  for (int d = 0; d < p->n; ++d) {
+    const dnx_design3d& de = ds[d];
+    Batch3Design<T> b{};
+    const dnx_grid3d& g = de.grid;
+    b.a.nz = g.nz;
+    b.a.ny = g.ny;
+    b.a.nx = g.nx;
+    b.a.ty_tiles = int(cdiv(g.ny, t.ty));
+    b.a.zc = t.zc;
+    b.a.nblocks = b.a.ty_tiles * int(cdiv(g.nz, t.zc));
+    b.a.P = int64_t(g.ny) * g.nx;
+    b.a.N = b.a.P * g.nz;
+    b.a.kh = T(g.kh);
+    b.a.ce = static_cast<const T*>(g.ce);
+    b.a.by = static_cast<const T*>(g.by);
+    b.a.ay = static_cast<const T*>(g.ay);
+    b.a.bx = static_cast<const T*>(g.bx);
+    b.a.ax = static_cast<const T*>(g.ax);
+    b.a.S = static_cast<T*>(p->pools.S) + p->off[d];
+    b.a.src_comp = de.src.comp;
+    if (de.src.comp >= 0) {
+      b.a.src_box = make_box(de.src.box);
+      b.a.src_prof = static_cast<const T*>(de.src.profile);
+      b.wave = static_cast<const T*>(de.src.wave);
+    }
+    void* const* bufs = adjoint ? p->pools.lam : p->pools.state;
+    b.buf[0] = static_cast<T*>(bufs[0]) + 6 * p->off[d];
+    b.buf[1] = static_cast<T*>(bufs[1]) + 6 * p->off[d];
+    b.ckpt = static_cast<T*>(p->pools.ckpt) + 6 * p->off[d];
+    b.estore = p->pools.estore ? static_cast<T*>(p->pools.estore) + 3 * p->off[d] : nullptr;
+    b.ckpt_stride = p->sum6;
+    b.estore_stride = p->sum3;
+    b.tile_start = start;
+    start += b.a.nblocks;
+    b.m_nf = de.mon.n_freq;
+    if (de.mon.n_freq > 0) {
+      b.m_mask = de.mon.comp_mask;
+      b.mbox = make_box(de.mon.box);
+      b.m_phase = de.mon.phase;
+      b.m_acc = de.mon.acc;
+      b.m_grad = de.mon.grad;
+    }
+    out[d] = b;
+  }
+  *total_tiles = start;
+}
+
+template <typename T>
+static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hipStream_t s) {
+  const size_t tb = sizeof(Batch3Design<T>) * p->n;
+  p->host_tables.resize(2 * tb);
+  auto* tf = reinterpret_cast<Batch3Design<T>*>(p->host_tables.data());
+  auto* ta = reinterpret_cast<Batch3Design<T>*>(p->host_tables.data() + tb);
+  int tot_f = 0, tot_a = 0;
+  fill_table<T>(p, ds, false, tf, &tot_f);
+  fill_table<T>(p, ds, true, ta, &tot_a);
+  auto* base = static_cast<unsigned char*>(ws);
+  p->d_fwd = base;
+  p->d_adj = base + ((tb + 255) / 256) * 256;
+  p->d_ctr = reinterpret_cast<int*>(base + 2 * ((tb + 255) / 256) * 256);
+  DNX_CHECK_HIP(hipMemcpyAsync(p->d_fwd, tf, tb, hipMemcpyHostToDevice, s));
+  DNX_CHECK_HIP(hipMemcpyAsync(p->d_adj, ta, tb, hipMemcpyHostToDevice, s));
+  int dev = 0, cus = 256;
+  DNX_CHECK_HIP(hipGetDevice(&dev));
+  DNX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  p->ty_f = fused3_tiles(p->dtype, 0).ty;
+  p->ty_a = fused3_tiles(p->dtype, 1).ty;
+  // Kernel attributes are set here, never while a stream is being captured.
+  int rc = prepare_batch3<T>(p->ty_f, false);
+  if (rc == DNX_OK) rc = prepare_batch3<T>(p->ty_a, true);
+  if (rc) return rc;
+  p->grid_f = std::min(tot_f, 4 * cus);
+  p->grid_a = std::min(tot_a, 4 * cus);
+  p->Lf = Batch3Launch{p->d_fwd, p->n, tot_f, p->K, 0, p->d_ctr};
+  p->La = Batch3Launch{p->d_adj, p->n, tot_a, p->K, 0, p->d_ctr};
+  return DNX_OK;
+}
+
+// ---- step sequences (each reads the device counter) ----------------------
+
+template <typename T>
+static int seq_forward(dnx_batch3d_plan* p, int steps, bool store, hipStream_t s,
+                       bool timed = true) {
+  Batch3Launch L = p->Lf;
+  L.store = store ? 1 : 0;
+  for (int k = 0; k < steps; ++k) {
+    TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+    int rc = launch_batch3<T>(L, false, p->ty_f, p->max_w, p->grid_f, s);
+    timed_end(store ? "batch3d_fwd_store" : "batch3d_fwd", s, tl);
+    if (rc) return rc;
+    if (!store && (rc = launch_batch_monitor<T>(p->d_fwd, p->n, p->max_mon_cells, p->K,
+                                                 p->d_ctr, false, s)))
+      return rc;
+    if ((rc = launch_counter_add(p->d_ctr, 1, s))) return rc;
+  }
+  return DNX_OK;
+}
+
+template <typename T>
+static int seq_adjoint(dnx_batch3d_plan* p, int steps, hipStream_t s, bool timed = true) {
+  for (int k = 0; k < steps; ++k) {
+    TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+    int rc = launch_batch3<T>(p->La, true, p->ty_a, p->max_w, p->grid_a, s);
+    timed_end("batch3d_adj", s, tl);
+    if (rc) return rc;
+    if ((rc = launch_batch_monitor<T>(p->d_adj, p->n, p->max_mon_cells, p->K, p->d_ctr, true,
+                                      s)))
+      return rc;
+    if ((rc = launch_counter_add(p->d_ctr, -1, s))) return rc;
+  }
+  return DNX_OK;
+}
+
+template <typename T>
+static int capture(dnx_batch3d_plan* p, int which, int steps, hipGraphExec_t* out) {
+  hipStream_t cs = nullptr;
+  DNX_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  int rc = DNX_OK;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = which == 2 ? seq_adjoint<T>(p, steps, cs, false)
+                    : seq_forward<T>(p, steps, which == 1, cs, false);
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    if (rc == DNX_OK && e2 != hipSuccess) {
+      set_error("batch3d: graph capture failed: %s", hipGetErrorString(e2));
+      rc = DNX_EHIP;
+    }
+  } else {
+    set_error("batch3d: begin capture failed: %s", hipGetErrorString(e));
+    rc = DNX_EHIP;
+  }
+  if (rc == DNX_OK) {
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      set_error("batch3d: graph instantiate failed: %s", hipGetErrorString(e));
+      rc = DNX_EHIP;
+      *out = nullptr;
+    }
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  return rc;
+}
+
+// Run `steps` steps of a sequence: from the graph when the count matches a
+// captured full segment, else by direct launches.
+template <typename T>
+static int run_seq(dnx_batch3d_plan* p, int which, int steps, hipStream_t s) {
+  if (steps <= 0) return DNX_OK;
+  hipGraphExec_t* slot = which == 0 ? &p->g_fwd : which == 1 ? &p->g_rep : &p->g_adj;
+  const int full = which == 1 ? p->K - 1 : p->K;
+  if (p->use_graph && steps == full) {
+    if (*slot == nullptr) {
+      int rc = capture<T>(p, which, steps, slot);
+      if (rc) {
+        // Capture failure: fall back to eager launches for this plan.
+        p->use_graph = false;
+      }
+    }
+    if (*slot) {
+      DNX_CHECK_HIP(hipGraphLaunch(*slot, s));
+      return DNX_OK;
+    }
+  }
+  return which == 2 ? seq_adjoint<T>(p, steps, s) : seq_forward<T>(p, steps, which == 1, s);
+}
+
+template <typename T>
+static int batch_forward(dnx_batch3d_plan* p, bool record, hipStream_t s) {
+  int rc = launch_counter_set(p->d_ctr, 0, s);
+  if (rc) return rc;
+  for (int k0 = 0; k0 < p->T; k0 += p->K) {
+    const int len = std::min(p->K, p->T - k0);
+    if (record) {
+      DNX_CHECK_HIP(hipMemcpyAsync(static_cast<T*>(p->pools.ckpt) + int64_t(k0 / p->K) * p->sum6,
+                                   p->pools.state[0], sizeof(T) * p->sum6,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    if ((rc = run_seq<T>(p, 0, len, s))) return rc;
+  }
+  return DNX_OK;
+}
+
+template <typename T>
+static int batch_backward(dnx_batch3d_plan* p, hipStream_t s) {
+  const int T_ = p->T;
+  // lam^T: zero both adjoint pools, then per design the monitor terms of
+  // sample T and S = E^T . lam^T (E^T in state buffer T & 1).
+  int rc = launch_zero<T>(static_cast<T*>(p->pools.lam[0]), p->sum6, s);
+  if (rc) return rc;
+  if ((rc = launch_zero<T>(static_cast<T*>(p->pools.lam[1]), p->sum6, s))) return rc;
+  for (int d = 0; d < p->n; ++d) {
+    const dnx_design3d& de = p->designs[d];
+    T* lam = static_cast<T*>(p->pools.lam[T_ & 1]) + 6 * p->off[d];
+    const T* eT = static_cast<const T*>(p->pools.state[T_ & 1]) + 6 * p->off[d];
+    T* S = static_cast<T*>(p->pools.S) + p->off[d];
+    rc = dnx_fdtd3d_adjoint_init(&de.grid, lam, eT, de.mon.n_freq > 0 ? &de.mon : nullptr,
+                                 de.mon.n_freq > 0 ? 1 : 0, T_, S, s);
+    if (rc) return rc;
+  }
+  const int nseg = (T_ + p->K - 1) / p->K;
+  for (int k = nseg - 1; k >= 0; --k) {
+    const int a = k * p->K;
+    const int len = std::min(p->K, T_ - a);
+    if (len > 1) {
+      DNX_CHECK_HIP(hipMemcpyAsync(p->pools.state[0],
+                                   static_cast<const T*>(p->pools.ckpt) + int64_t(k) * p->sum6,
+                                   sizeof(T) * p->sum6, hipMemcpyDeviceToDevice, s));
+      if ((rc = launch_counter_set(p->d_ctr, a, s))) return rc;
+      if ((rc = run_seq<T>(p, 1, len - 1, s))) return rc;
+    }
+    if ((rc = launch_counter_set(p->d_ctr, a + len - 1, s))) return rc;
+    const int steps = k == 0 ? len - 1 : len;  // m runs down to max(a, 1)
+    if ((rc = run_seq<T>(p, 2, steps, s))) return rc;
+  }
+  // step-0 source term with lam^1 (in adjoint buffer 1)
+  for (int d = 0; d < p->n; ++d) {
+    const dnx_design3d& de = p->designs[d];
+    if (de.src.comp < 0) continue;
+    T* lam1 = static_cast<T*>(p->pools.lam[1]) + 6 * p->off[d];
+    rc = launch_source_adj<T>(static_cast<T*>(p->pools.S) + p->off[d],
+                              lam1 + int64_t(de.src.comp) * (int64_t(de.grid.nz) * de.grid.ny *
+                                                             de.grid.nx),
+                              de.grid.ny, de.grid.nx, de.src, 0, s);
+    if (rc) return rc;
+  }
+  return DNX_OK;
+}
+
+}  // namespace dnx
+
+using namespace dnx;
+
+extern "C" int64_t dnx_batch3d_workspace_bytes(int32_t n_designs) {
+  const size_t tb = sizeof(Batch3Design<double>) * size_t(std::max(n_designs, 1));
+  return int64_t(2 * ((tb + 255) / 256) * 256 + 256);
+}
+
+extern "C" int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, int32_t n_steps,
+                                      int32_t ckpt_every, const dnx_batch3d_pools* pools,
+                                      void* workspace, int64_t ws_bytes, int32_t use_graph,
+                                      void* stream, dnx_batch3d_plan** out) {
+  DNX_REQUIRE(out != nullptr, "batch3d_create: null output");
+  *out = nullptr;
+  DNX_REQUIRE(designs && n > 0 && pools && workspace, "batch3d_create: null argument");
+  DNX_REQUIRE(n_steps >= 1, "batch3d_create: n_steps must be >= 1");
+  DNX_REQUIRE(ckpt_every >= 2 && ckpt_every % 2 == 0,
+              "batch3d_create: ckpt_every must be even and >= 2 (got %d)", ckpt_every);
+  DNX_REQUIRE(ws_bytes >= dnx_batch3d_workspace_bytes(n), "batch3d_create: workspace too small");
+  DNX_REQUIRE(pools->state[0] && pools->state[1] && pools->lam[0] && pools->lam[1] &&
+                  pools->ckpt && pools->S && (ckpt_every < 2 || pools->estore),
+              "batch3d_create: null pool");
+  const int dtype = designs[0].grid.dtype;
+  auto* p = new dnx_batch3d_plan();
+  p->dtype = dtype;
+  p->n = n;
+  p->T = n_steps;
+  p->K = ckpt_every;
+  p->pools = *pools;
+  p->use_graph = use_graph != 0;
+  p->designs.assign(designs, designs + n);
+  int64_t acc = 0;
+  for (int d = 0; d < n; ++d) {
+    const dnx_design3d& de = designs[d];
+    const dnx_grid3d& g = de.grid;
+    bool ok = g.dtype == dtype && g.nz > 0 && g.ny > 0 && g.nx > 0 && g.nx <= FUSED_MAX_W &&
+              g.ce && g.by && g.ay && g.bx && g.ax;
+    if (ok && de.src.comp >= 0)
+      ok = de.src.comp <= 2 && box_ok(de.src.box, g.nz, g.ny, g.nx) && de.src.profile &&
+           de.src.wave;
+    if (ok && de.mon.n_freq > 0)
+      ok = box_ok(de.mon.box, g.nz, g.ny, g.nx) && de.mon.comp_mask > 0 &&
+           (de.mon.comp_mask & ~7) == 0 && de.mon.phase && de.mon.acc && de.mon.grad;
+    if (!ok) {
+      delete p;
+      set_error("batch3d_create: design %d invalid (dtype, shape, nx <= %d, source or monitor)",
+                d, FUSED_MAX_W);
+      return DNX_EINVAL;
+    }
+    p->off.push_back(acc);
+    acc += int64_t(g.nz) * g.ny * g.nx;
+    p->max_w = std::max(p->max_w, int(g.nx));
+    if (de.mon.n_freq > 0) p->max_mon_cells = std::max(p->max_mon_cells, box_cells(de.mon.box));
+  }
+  p->sumN = acc;
+  p->sum6 = 6 * acc;
+  p->sum3 = 3 * acc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = dtype == DNX_F32 ? batch_setup<float>(p, designs, workspace, s)
+                            : batch_setup<double>(p, designs, workspace, s);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_batch3d_forward(dnx_batch3d_plan* p, int32_t record_ckpt, void* stream) {
+  DNX_REQUIRE(p != nullptr, "batch3d_forward: null plan");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return p->dtype == DNX_F32 ? batch_forward<float>(p, record_ckpt != 0, s)
+                             : batch_forward<double>(p, record_ckpt != 0, s);
+}
+
+extern "C" int32_t dnx_batch3d_backward(dnx_batch3d_plan* p, void* stream) {
+  DNX_REQUIRE(p != nullptr, "batch3d_backward: null plan");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return p->dtype == DNX_F32 ? batch_backward<float>(p, s) : batch_backward<double>(p, s);
+}
+
+extern "C" int32_t dnx_batch3d_uses_graph(const dnx_batch3d_plan* p) {
+  return (p && p->use_graph) ? 1 : 0;
+}
+
+extern "C" int32_t dnx_batch3d_destroy(dnx_batch3d_plan* p) {
+  if (!p) return DNX_OK;
+  if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
+  if (p->g_rep) (void)hipGraphExecDestroy(p->g_rep);
+  if (p->g_adj) (void)hipGraphExecDestroy(p->g_adj);
+  delete p;
+  return DNX_OK;
+}

@@ -34,6 +34,46 @@ struct Fused3Args {
   const T* __restrict__ src_wave;  // points at wave[n] (forward) / wave[m] (adjoint)
 };
 
+// One design of a batched launch (device-resident table entry).
+template <typename T>
+struct Batch3Design {
+  Fused3Args<T> a;        // geometry, coefficients, fused source, tiles; a.S set
+  T* buf[2];              // ping-pong buffers (forward state or adjoint), 6N each
+  T* ckpt;                // checkpoint of step k*K at ckpt + k*ckpt_stride (6N)
+  T* estore;              // replay E of step kK+1+j at estore + j*estore_stride (3N)
+  int64_t ckpt_stride;
+  int64_t estore_stride;
+  const T* wave;          // source waveform table
+  int tile_start;         // first global tile of this design
+  // one DFT monitor (m_nf == 0: none)
+  int m_mask;
+  int m_nf;
+  Box mbox;
+  const double* m_phase;
+  double* m_acc;
+  const double* m_grad;
+};
+
+struct Batch3Launch {
+  const void* designs;  // device array of Batch3Design<T>
+  int n_designs;
+  int total_tiles;
+  int K;                // checkpoint interval
+  int store;            // forward: also store E (replay)
+  const int* step;      // device step counter
+};
+
+template <typename T>
+int prepare_batch3(int ty, bool adjoint);  // sets kernel attributes (not during capture)
+template <typename T>
+int launch_batch3(const Batch3Launch& L, bool adjoint, int ty, int max_w, int grid,
+                  hipStream_t s);
+template <typename T>
+int launch_batch_monitor(const void* designs, int n_designs, int64_t max_cells, int K,
+                         const int* step, bool adjoint, hipStream_t s);
+int launch_counter_add(int* c, int delta, hipStream_t s);
+int launch_counter_set(int* c, int v, hipStream_t s);
+
 struct Fused3Tiles {
   int ty;  // rows per work-group (4 or 8)
   int zc;  // z planes per work-group

@@ -61,20 +61,17 @@ __device__ inline void block_tile(int nblocks, int ty_tiles, int zc, int& y0, in
 // Forward
 // ---------------------------------------------------------------------------
 
+// One work-group tile of the forward step: rows y0 .. y0+TY-1, planes
+// z0 .. z0+zc-1.  Must be called by the whole work-group (it synchronises).
 template <typename T, int TY>
-__global__ __launch_bounds__(256) void k3_fused_fwd(Fused3Args<T> a) {
-  extern __shared__ unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
+__device__ __forceinline__ void fwd_tile(const Fused3Args<T>& a, const int y0, const int z0,
+                                         T* sm) {
   const int W = a.nx;
   T* sE = sm;                          // [2][2: Ey, Ez][TY+1][W]
   T* sH = sm + 2 * 2 * (TY + 1) * W;   // [2][2: Hy, Hz][TY][W]
 #define SE(buf, c, r) sE[(((buf) * 2 + (c)) * (TY + 1) + (r)) * W]
 #define SH(buf, c, r) sH[(((buf) * 2 + (c)) * TY + (r)) * W]
 
-  int y0, z0;
-  bool live;
-  block_tile<TY>(a.nblocks, a.ty_tiles, a.zc, y0, z0, live);
-  if (!live) return;  // uniform per work-group
   const int z1 = min(a.nz, z0 + a.zc);
   const int x = threadIdx.x;
   const bool xv = x < W;
@@ -247,6 +244,16 @@ __global__ __launch_bounds__(256) void k3_fused_fwd(Fused3Args<T> a) {
 #undef SH
 }
 
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k3_fused_fwd(Fused3Args<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  int y0, z0;
+  bool live;
+  block_tile<TY>(a.nblocks, a.ty_tiles, a.zc, y0, z0, live);
+  if (!live) return;  // uniform per work-group
+  fwd_tile<T, TY>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint.  Input lam = (lamE^{m+1}, muH^{m+1}); output (lamE^m, muH^m).
 //   u = ce lamE (E-like input scaled by its own ce)
@@ -263,19 +270,14 @@ __global__ __launch_bounds__(256) void k3_fused_fwd(Fused3Args<T> a) {
 // ---------------------------------------------------------------------------
 
 template <typename T, int TY>
-__global__ __launch_bounds__(256) void k3_fused_adj(Fused3Args<T> a) {
-  extern __shared__ unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
+__device__ __forceinline__ void adj_tile(const Fused3Args<T>& a, const int y0, const int z0,
+                                         T* sm) {
   const int W = a.nx;
   T* sE = sm;                          // [2][2: uy, uz][TY+1][W]
   T* sH = sm + 2 * 2 * (TY + 1) * W;   // [2][2: muHy, muHz][TY][W]
 #define SE(buf, c, r) sE[(((buf) * 2 + (c)) * (TY + 1) + (r)) * W]
 #define SH(buf, c, r) sH[(((buf) * 2 + (c)) * TY + (r)) * W]
 
-  int y0, z0;
-  bool live;
-  block_tile<TY>(a.nblocks, a.ty_tiles, a.zc, y0, z0, live);
-  if (!live) return;
   const int z1 = min(a.nz, z0 + a.zc);
   const int x = threadIdx.x;
   const bool xv = x < W;
@@ -464,6 +466,147 @@ __global__ __launch_bounds__(256) void k3_fused_adj(Fused3Args<T> a) {
 #undef SH
 }
 
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k3_fused_adj(Fused3Args<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  int y0, z0;
+  bool live;
+  block_tile<TY>(a.nblocks, a.ty_tiles, a.zc, y0, z0, live);
+  if (!live) return;
+  adj_tile<T, TY>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+}
+
+// ---------------------------------------------------------------------------
+// Batched step over many designs of different grids (one launch per step).
+// Each work-group walks the global tile list with stride gridDim.x (tiles of
+// design d are [tile_start[d], tile_start[d+1])), XCD-remapped like the
+// single-grid kernels.  The step index comes from a device counter, so the
+// same launch can be replayed from a captured graph for every segment.
+// Buffer of the state at step n: buf[n & 1] (forward); adjoint input
+// lam^{m+1} in buf[(m+1) & 1], output lam^m in buf[m & 1].
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ inline int batch_design(const Batch3Design<T>* D, int n_designs, int t) {
+  int d = 0;
+  while (d + 1 < n_designs && D[d + 1].tile_start <= t) ++d;
+  return d;
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k3_batch_fwd(Batch3Launch L) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
+  const int n = *L.step;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int lw = (b & 7) * (G >> 3) + (b >> 3);
+  for (int t = lw; t < L.total_tiles; t += G) {
+    const int d = batch_design(D, L.n_designs, t);
+    Fused3Args<T> a = D[d].a;
+    a.in = D[d].buf[n & 1];
+    a.out = D[d].buf[(n & 1) ^ 1];
+    a.es = L.store ? D[d].estore + int64_t(n % L.K) * D[d].estore_stride : nullptr;
+    if (a.src_comp >= 0) a.src_wave = D[d].wave + n;
+    const int lt = t - D[d].tile_start;
+    fwd_tile<T, TY>(a, (lt % a.ty_tiles) * TY, (lt / a.ty_tiles) * a.zc, sm);
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k3_batch_adj(Batch3Launch L) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
+  const int m = *L.step;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int lw = (b & 7) * (G >> 3) + (b >> 3);
+  for (int t = lw; t < L.total_tiles; t += G) {
+    const int d = batch_design(D, L.n_designs, t);
+    Fused3Args<T> a = D[d].a;
+    a.in = D[d].buf[(m + 1) & 1];
+    a.out = D[d].buf[m & 1];
+    const int r = m % L.K;
+    a.em = r == 0 ? D[d].ckpt + int64_t(m / L.K) * D[d].ckpt_stride
+                  : D[d].estore + int64_t(r - 1) * D[d].estore_stride;
+    if (a.src_comp >= 0) a.src_wave = D[d].wave + m;
+    const int lt = t - D[d].tile_start;
+    adj_tile<T, TY>(a, (lt % a.ty_tiles) * TY, (lt / a.ty_tiles) * a.zc, sm);
+    __syncthreads();
+  }
+}
+
+__global__ void k_counter_add(int* c, int delta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c += delta;
+}
+
+__global__ void k_counter_set(int* c, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
+}
+
+// Batched DFT monitors: blockIdx.y = design.  adjoint = 0: after forward step
+// n, acc += E^{n+1} (cos, -sin)(phase[n]).  adjoint = 1: after adjoint step
+// m, d = Re(g e^{i phase[m-1]}); lam^m_c += d; S += E^m_c d.
+template <typename T>
+__global__ void k_batch_monitor(const Batch3Design<T>* D, int K, const int* step, int adjoint) {
+  const Batch3Design<T>& dz = D[blockIdx.y];
+  const Box b = dz.mbox;
+  if (dz.m_nf <= 0) return;
+  const int64_t cells = b.cells();
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  const int k = *step;
+  const int bx = int(i % b.nx);
+  const int64_t rr = i / b.nx;
+  const int by = int(rr % b.ny);
+  const int bz = int(rr / b.ny);
+  const int ny = dz.a.ny, nx = dz.a.nx;
+  const int64_t N = dz.a.N;
+  const int64_t idx = (int64_t(b.z0 + bz) * ny + (b.y0 + by)) * nx + (b.x0 + bx);
+  const int mask = dz.m_mask;
+  const int ncomp = (mask & 1) + ((mask >> 1) & 1) + ((mask >> 2) & 1);
+  const int nf = dz.m_nf;
+  if (!adjoint) {
+    const T* E = dz.buf[(k + 1) & 1];
+    const double* ph = dz.m_phase + int64_t(k) * nf * 2;
+    int ci = 0;
+    for (int c = 0; c < 3; ++c) {
+      if (!((mask >> c) & 1)) continue;
+      const double v = double(E[c * N + idx]);
+      for (int f = 0; f < nf; ++f) {
+        double* acc = dz.m_acc + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+        acc[0] += v * ph[2 * f];
+        acc[1] -= v * ph[2 * f + 1];
+      }
+      ++ci;
+    }
+  } else {
+    T* Lm = dz.buf[k & 1];
+    const int r = k % K;
+    const T* Em = r == 0 ? dz.ckpt + int64_t(k / K) * dz.ckpt_stride
+                         : dz.estore + int64_t(r - 1) * dz.estore_stride;
+    const double* ph = dz.m_phase + int64_t(k - 1) * nf * 2;
+    int ci = 0;
+    T s = T(0);
+    for (int c = 0; c < 3; ++c) {
+      if (!((mask >> c) & 1)) continue;
+      double dv = 0.0;
+      for (int f = 0; f < nf; ++f) {
+        const double* g = dz.m_grad + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+        dv += g[0] * ph[2 * f] - g[1] * ph[2 * f + 1];
+      }
+      const T dt = T(dv);
+      Lm[c * N + idx] += dt;
+      s += Em[c * N + idx] * dt;
+      ++ci;
+    }
+    dz.a.S[idx] += s;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -522,10 +665,84 @@ int launch_fused3_adj(const Fused3Args<T>& a, int ty, hipStream_t s) {
   return DNX_EINVAL;
 }
 
+template <typename T, int TY>
+static int prepare_batch_ty(bool adjoint) {
+  const int smem_max = int(sizeof(T) * (2 * 2 * (TY + 1) + 2 * 2 * TY) * FUSED_MAX_W);
+  const void* f = adjoint ? reinterpret_cast<const void*>(&k3_batch_adj<T, TY>)
+                          : reinterpret_cast<const void*>(&k3_batch_fwd<T, TY>);
+  DNX_CHECK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, smem_max));
+  return DNX_OK;
+}
+
+template <typename T>
+int prepare_batch3(int ty, bool adjoint) {
+  if (ty == 8) return prepare_batch_ty<T, 8>(adjoint);
+  if (ty == 4) return prepare_batch_ty<T, 4>(adjoint);
+  if (ty == 2) return prepare_batch_ty<T, 2>(adjoint);
+  set_error("batch3: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template <typename T, int TY>
+static int launch_batch_ty(const Batch3Launch& L, bool adjoint, int max_w, int grid,
+                           hipStream_t s) {
+  const int threads = ((max_w + 63) / 64) * 64;
+  const size_t smem = sizeof(T) * size_t(2 * 2 * (TY + 1) + 2 * 2 * TY) * max_w;
+  const int nb = ((grid + 7) / 8) * 8;
+  if (adjoint) {
+    hipLaunchKernelGGL((k3_batch_adj<T, TY>), dim3(nb), dim3(threads), smem, s, L);
+  } else {
+    hipLaunchKernelGGL((k3_batch_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, L);
+  }
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+// Launch one batched step; prepare_batch3 must have been called for (ty, kind).
+template <typename T>
+int launch_batch3(const Batch3Launch& L, bool adjoint, int ty, int max_w, int grid,
+                  hipStream_t s) {
+  if (ty == 8) return launch_batch_ty<T, 8>(L, adjoint, max_w, grid, s);
+  if (ty == 4) return launch_batch_ty<T, 4>(L, adjoint, max_w, grid, s);
+  if (ty == 2) return launch_batch_ty<T, 2>(L, adjoint, max_w, grid, s);
+  set_error("batch3: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template <typename T>
+int launch_batch_monitor(const void* designs, int n_designs, int64_t max_cells, int K,
+                         const int* step, bool adjoint, hipStream_t s) {
+  if (max_cells <= 0) return DNX_OK;
+  hipLaunchKernelGGL(k_batch_monitor<T>, dim3(cdiv(max_cells, 256), n_designs), dim3(256), 0, s,
+                     static_cast<const Batch3Design<T>*>(designs), K, step, adjoint ? 1 : 0);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+int launch_counter_add(int* c, int delta, hipStream_t s) {
+  hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, s, c, delta);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+int launch_counter_set(int* c, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_counter_set, dim3(1), dim3(64), 0, s, c, v);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
 template int launch_fused3_fwd<float>(const Fused3Args<float>&, int, hipStream_t);
 template int launch_fused3_fwd<double>(const Fused3Args<double>&, int, hipStream_t);
 template int launch_fused3_adj<float>(const Fused3Args<float>&, int, hipStream_t);
 template int launch_fused3_adj<double>(const Fused3Args<double>&, int, hipStream_t);
+template int prepare_batch3<float>(int, bool);
+template int prepare_batch3<double>(int, bool);
+template int launch_batch3<float>(const Batch3Launch&, bool, int, int, int, hipStream_t);
+template int launch_batch3<double>(const Batch3Launch&, bool, int, int, int, hipStream_t);
+template int launch_batch_monitor<float>(const void*, int, int64_t, int, const int*, bool,
+                                         hipStream_t);
+template int launch_batch_monitor<double>(const void*, int, int64_t, int, const int*, bool,
+                                          hipStream_t);
 
 }  // namespace dnx
 

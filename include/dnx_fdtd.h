@@ -175,6 +175,52 @@ int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work, int32
 int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t ty, int32_t zc);
 int32_t dnx_fdtd3d_get_tiles(int32_t dtype, int32_t kernel, int32_t* ty, int32_t* zc);
 
+/* ---- batched 3D (many designs, uneven grids, one launch per step) ------ */
+
+/* One design of a batch: its grid (dtype shared by the batch, nx <= 256),
+ * at most one soft source (src.comp < 0: none) and one DFT monitor
+ * (mon.n_freq == 0: none; acc and grad both required).  No reference
+ * counterpart: the reference solves designs one at a time
+ * (fdtd3d.py:471-504 per design). */
+typedef struct dnx_design3d {
+  dnx_grid3d grid;
+  dnx_source src;
+  dnx_monitor mon;
+} dnx_design3d;
+
+/* Device pools shared by all designs; design d starts at element offset
+ * 6*Nd (state, lam), 3*Nd (estore slot), Nd (S), Nd = sum_{e<d} N_e.
+ * ckpt: ceil(T/K) slots of sum 6N; estore: K-1 slots of sum 3N. */
+typedef struct dnx_batch3d_pools {
+  void* state[2];
+  void* lam[2];
+  void* ckpt;
+  void* estore;
+  void* S;
+} dnx_batch3d_pools;
+
+typedef struct dnx_batch3d_plan dnx_batch3d_plan;  /* opaque host object */
+
+/* Device workspace the plan needs (descriptor tables + step counter). */
+int64_t dnx_batch3d_workspace_bytes(int32_t n_designs);
+/* Build a plan (uploads descriptor tables into `workspace` on `stream`).
+ * ckpt_every must be even.  use_graph: capture one K-step segment of each
+ * sequence (forward, replay, adjoint) into a HIP graph on first use and
+ * replay it per segment (falls back to direct launches if capture fails). */
+int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, int32_t n_steps,
+                           int32_t ckpt_every, const dnx_batch3d_pools* pools,
+                           void* workspace, int64_t ws_bytes, int32_t use_graph,
+                           void* stream, dnx_batch3d_plan** out);
+/* Forward T steps from the state in pools.state[0] (caller zeroes it and
+ * the monitor accumulators); record_ckpt: store checkpoints.  Final state
+ * of every design in pools.state[T & 1]. */
+int32_t dnx_batch3d_forward(dnx_batch3d_plan* plan, int32_t record_ckpt, void* stream);
+/* Gradient of a loss on the monitors (grads in each design's mon.grad):
+ * replay + adjoint for every design; S in pools.S (dL/deps = -S/eps). */
+int32_t dnx_batch3d_backward(dnx_batch3d_plan* plan, void* stream);
+int32_t dnx_batch3d_uses_graph(const dnx_batch3d_plan* plan);
+int32_t dnx_batch3d_destroy(dnx_batch3d_plan* plan);
+
 /* ---- 2D -------------------------------------------------------------- */
 
 /* As dnx_fdtd3d_run for the 2D state (3*N): TM FDTDSolver2D._time_step_tm
