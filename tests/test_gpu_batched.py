@@ -58,3 +58,20 @@ def test_batched_equals_single_design(use_graph):
     F1, g1 = _plain(shapes[1], eps[1] * 1.01, srcs[1], mons[1], T, K)
     assert rel_l2(outs2[1].detach().cpu().numpy(), F1.cpu().numpy()) < 1e-6
     assert rel_l2(es2[1].grad.cpu().numpy(), g1.cpu().numpy()) < 1e-5
+
+
+def test_autotuner_measures_caches_and_applies(tmp_path, monkeypatch):
+    from diffnano_amd import autotune
+    monkeypatch.setenv("DNX_AUTOTUNE_CACHE", str(tmp_path / "tiles.json"))
+    saved = [autotune.get_tiles(torch.float32, k) for k in (0, 1)]
+    try:
+        out = autotune.tune((24, 40, 64), steps=4)
+        assert out["forward"][0] in (2, 4, 8) and out["adjoint"][0] in (2, 4, 8)
+        assert autotune.get_tiles(torch.float32, 0) == out["forward"]
+        (tmp_path / "tiles.json").write_text("garbage")  # corrupt cache -> re-tune
+        out2 = autotune.tune((24, 40, 64), steps=4)
+        assert set(out2) == {"forward", "adjoint"}
+        assert autotune._load_cache(str(tmp_path / "tiles.json"))
+    finally:
+        for k, (ty, zc) in enumerate(saved):
+            autotune.set_tiles(torch.float32, k, ty, zc)

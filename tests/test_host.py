@@ -147,3 +147,15 @@ def test_monitor_box_validation():
     with pytest.raises(ValueError):
         H.monitor_box((5, 6), {"box": (0, 1, 0, 1, 0, 1)})
     assert H.comp_mask_3d("xz") == 5 and H.comp_mask_3d(["Ey"]) == 2
+
+
+def test_autotune_cache_corrupt_file_is_discarded(tmp_path):
+    from diffnano_amd import autotune
+    p = tmp_path / "tiles.json"
+    p.write_text("{not json")
+    assert autotune._load_cache(str(p)) == {}
+    p.write_text("[1, 2]")
+    assert autotune._load_cache(str(p)) == {}
+    autotune._save_cache(str(p), {"k": {"ty": 4, "zc": 16}})
+    assert autotune._load_cache(str(p)) == {"k": {"ty": 4, "zc": 16}}
+    assert autotune._valid({"ty": 4, "zc": 16}) and not autotune._valid({"ty": 3, "zc": 16})
