@@ -131,12 +131,15 @@ def _bench_batched(args):
     dev = torch.device("cuda", 0)
     T = args.n_steps or 3000
     probs_b = inverse_design.batched_problems(args.designs, seed=0)
-    tiles = autotune.tune((128, 128, 128))
+    classes = sorted({p.shape for p in probs_b})
+    tiles_plain = {s: autotune.tune(s, apply=False) for s in classes}
+    tiles_batch = autotune.tune_batch(classes * 2, apply=False)
     res_b = inverse_design.adam_loop(probs_b, iterations=args.iters, n_steps=T, mode="batched",
-                                     device=dev)
+                                     device=dev, tiles=tiles_batch)
     probs_p = inverse_design.batched_problems(args.designs, seed=0)
     res_p = inverse_design.adam_loop(probs_p, iterations=args.iters, n_steps=T, mode="plain",
-                                     device=dev)
+                                     device=dev, tiles=tiles_plain)
+    tiles = {"batched": tiles_batch, "plain": {str(k): v for k, v in tiles_plain.items()}}
     fb, fp = res_b["fom"][-1], res_p["fom"][-1]
     cells = sum(int(np.prod(p.shape)) for p in probs_b)
     out = {

@@ -138,6 +138,69 @@ def tune(shape, dtype=torch.float32, device=None, steps=8, force=False, apply=Tr
     return out
 
 
+def tune_batch(shapes, dtype=torch.float32, device=None, steps=8, force=False, apply=True):
+    """Tile shapes for a batched plan over these grids (measured on the
+    batched kernels themselves, cached under the sorted shape list)."""
+    from .batched import BatchedFDTD3D
+
+    dev = L.compute_device(device)
+    shapes = [tuple(int(v) for v in s) for s in shapes]
+    tag = "batch:" + "+".join("x".join(map(str, s)) for s in sorted(shapes))
+    path = cache_path()
+    with _lock:
+        cache = _load_cache(path)
+        keys = {k: _key(dev, dtype, k, ()) + "|" + tag for k in ("forward", "adjoint")}
+        hit = all(k in cache and _valid(cache[k]) for k in keys.values())
+        if force or not hit:
+            saved = [get_tiles(dtype, k) for k in (0, 1)]
+            srcs = [{"type": "gaussian_pulse", "pos": [s[0] // 2, s[1] // 2, s[2] // 2]}
+                    for s in shapes]
+            mons = [{"box": (0, s[0], 0, s[1], s[2] - 2, s[2] - 1)} for s in shapes]
+            rng = np.random.default_rng(0)
+            eps = [torch.tensor(2.0 + rng.random(s), device=dev) for s in shapes]
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            times = {0: {}, 1: {}}
+            try:
+                for ty in TY_CHOICES:
+                    if dtype == torch.float64 and ty == 8:
+                        continue
+                    for zc in ZC_CHOICES:
+                        for k in (0, 1):
+                            set_tiles(dtype, k, ty, zc)
+                        b = BatchedFDTD3D(shapes, srcs, mons, n_steps=steps, checkpoint_every=steps,
+                                          pml_layers=2, device=dev, dtype=dtype)
+                        b._set_eps(eps)
+                        best = [float("inf"), float("inf")]
+                        for rep in range(3):
+                            e0.record()
+                            b._run_forward(record=True)
+                            e1.record()
+                            torch.cuda.synchronize()
+                            tf = e0.elapsed_time(e1)
+                            e0.record()
+                            b._run_backward([None] * len(shapes))
+                            e1.record()
+                            torch.cuda.synchronize()
+                            if rep:
+                                best = [min(best[0], tf), min(best[1], e0.elapsed_time(e1))]
+                        times[0][(ty, zc)] = best[0]
+                        times[1][(ty, zc)] = best[1]
+                        del b
+            finally:
+                for k, (ty, zc) in enumerate(saved):
+                    set_tiles(dtype, k, ty, zc)
+            for kernel, name in ((0, "forward"), (1, "adjoint")):
+                (ty, zc), ms = min(times[kernel].items(), key=lambda kv: kv[1])
+                cache[keys[name]] = {"ty": ty, "zc": zc, "ms_per_step": ms / steps}
+            _save_cache(path, cache)
+        out = {name: (cache[keys[name]]["ty"], cache[keys[name]]["zc"])
+               for name in ("forward", "adjoint")}
+    if apply:
+        set_tiles(dtype, 0, *out["forward"])
+        set_tiles(dtype, 1, *out["adjoint"])
+    return out
+
+
 def _valid(entry):
     return (isinstance(entry, dict) and entry.get("ty") in TY_CHOICES
             and isinstance(entry.get("zc"), int) and 1 <= entry["zc"] <= 4096)

@@ -99,12 +99,14 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
   DNX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   p->ty_f = fused3_tiles(p->dtype, 0).ty;
   p->ty_a = fused3_tiles(p->dtype, 1).ty;
-  // Kernel attributes are set here, never while a stream is being captured.
-  int rc = prepare_batch3<T>(p->ty_f, false);
-  if (rc == DNX_OK) rc = prepare_batch3<T>(p->ty_a, true);
+  // Kernel attributes are set here, never while a stream is being captured;
+  // the persistent grid is as many work-groups as fit on the device at once.
+  int bf = 1, ba = 1;
+  int rc = prepare_batch3<T>(p->ty_f, false, p->max_w, &bf);
+  if (rc == DNX_OK) rc = prepare_batch3<T>(p->ty_a, true, p->max_w, &ba);
   if (rc) return rc;
-  p->grid_f = std::min(tot_f, 4 * cus);
-  p->grid_a = std::min(tot_a, 4 * cus);
+  p->grid_f = std::min(tot_f, bf * cus);
+  p->grid_a = std::min(tot_a, ba * cus);
   p->Lf = Batch3Launch{p->d_fwd, p->n, tot_f, p->K, 0, p->d_ctr};
   p->La = Batch3Launch{p->d_adj, p->n, tot_a, p->K, 0, p->d_ctr};
   return DNX_OK;

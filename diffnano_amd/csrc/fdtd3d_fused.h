@@ -64,7 +64,7 @@ struct Batch3Launch {
 };
 
 template <typename T>
-int prepare_batch3(int ty, bool adjoint);  // sets kernel attributes (not during capture)
+int prepare_batch3(int ty, bool adjoint, int max_w, int* blocks_per_cu);
 template <typename T>
 int launch_batch3(const Batch3Launch& L, bool adjoint, int ty, int max_w, int grid,
                   hipStream_t s);

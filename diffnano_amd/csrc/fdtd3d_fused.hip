@@ -666,19 +666,27 @@ int launch_fused3_adj(const Fused3Args<T>& a, int ty, hipStream_t s) {
 }
 
 template <typename T, int TY>
-static int prepare_batch_ty(bool adjoint) {
+static int prepare_batch_ty(bool adjoint, int max_w, int* blocks_per_cu) {
   const int smem_max = int(sizeof(T) * (2 * 2 * (TY + 1) + 2 * 2 * TY) * FUSED_MAX_W);
   const void* f = adjoint ? reinterpret_cast<const void*>(&k3_batch_adj<T, TY>)
                           : reinterpret_cast<const void*>(&k3_batch_fwd<T, TY>);
   DNX_CHECK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, smem_max));
+  const int threads = ((max_w + 63) / 64) * 64;
+  const size_t smem = sizeof(T) * size_t(2 * 2 * (TY + 1) + 2 * 2 * TY) * max_w;
+  int nb = 0;
+  DNX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, smem));
+  *blocks_per_cu = nb > 0 ? nb : 1;
   return DNX_OK;
 }
 
+// Sets the kernel attributes (never called while a stream is captured) and
+// reports how many work-groups of this kernel fit on one CU, which sizes
+// the persistent grid.
 template <typename T>
-int prepare_batch3(int ty, bool adjoint) {
-  if (ty == 8) return prepare_batch_ty<T, 8>(adjoint);
-  if (ty == 4) return prepare_batch_ty<T, 4>(adjoint);
-  if (ty == 2) return prepare_batch_ty<T, 2>(adjoint);
+int prepare_batch3(int ty, bool adjoint, int max_w, int* blocks_per_cu) {
+  if (ty == 8) return prepare_batch_ty<T, 8>(adjoint, max_w, blocks_per_cu);
+  if (ty == 4) return prepare_batch_ty<T, 4>(adjoint, max_w, blocks_per_cu);
+  if (ty == 2) return prepare_batch_ty<T, 2>(adjoint, max_w, blocks_per_cu);
   set_error("batch3: unsupported tile height %d", ty);
   return DNX_EINVAL;
 }
@@ -735,8 +743,8 @@ template int launch_fused3_fwd<float>(const Fused3Args<float>&, int, hipStream_t
 template int launch_fused3_fwd<double>(const Fused3Args<double>&, int, hipStream_t);
 template int launch_fused3_adj<float>(const Fused3Args<float>&, int, hipStream_t);
 template int launch_fused3_adj<double>(const Fused3Args<double>&, int, hipStream_t);
-template int prepare_batch3<float>(int, bool);
-template int prepare_batch3<double>(int, bool);
+template int prepare_batch3<float>(int, bool, int, int*);
+template int prepare_batch3<double>(int, bool, int, int*);
 template int launch_batch3<float>(const Batch3Launch&, bool, int, int, int, hipStream_t);
 template int launch_batch3<double>(const Batch3Launch&, bool, int, int, int, hipStream_t);
 template int launch_batch_monitor<float>(const void*, int, int64_t, int, const int*, bool,

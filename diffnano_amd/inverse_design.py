@@ -59,11 +59,20 @@ def batched_problems(n_designs=48, seed=0, dl=25.0, pml_layers=12, classes=GRID_
 
 
 def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layers=12,
-              checkpoint_every=64, mode="batched", device="cuda", use_graph=True, log=None):
+              checkpoint_every=64, mode="batched", device="cuda", use_graph=True, log=None,
+              tiles=None):
     """Run `iterations` Adam steps on every design.  Returns dict with the
     FoM history (iterations x designs), final rho, wall time (s) and
-    design-iterations per second."""
+    design-iterations per second.  ``tiles``: batched mode, one
+    {"forward": (ty, zc), "adjoint": (ty, zc)} applied before the plan is
+    built; plain mode, a dict shape -> such a config applied before each
+    design's solve."""
+    from . import autotune
+
     dev = torch.device(device)
+    if mode == "batched" and tiles:
+        autotune.set_tiles(torch.float32, 0, *tiles["forward"])
+        autotune.set_tiles(torch.float32, 1, *tiles["adjoint"])
     rhos = [torch.tensor(p.rho0, dtype=torch.float64, device=dev, requires_grad=True)
             for p in problems]
     bgs = [torch.tensor(p.eps_bg, dtype=torch.float64, device=dev) for p in problems]
@@ -93,6 +102,9 @@ def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layer
         else:
             foms = []
             for p, r, bg in zip(problems, rhos, bgs):
+                if tiles and p.shape in tiles:
+                    autotune.set_tiles(torch.float32, 0, *tiles[p.shape]["forward"])
+                    autotune.set_tiles(torch.float32, 1, *tiles[p.shape]["adjoint"])
                 F = solvers[p.shape].forward(p.eps(r, bg), source=p.source,
                                              monitors=[p.monitor]).metadata["monitors"][0]
                 fom = (F.abs() ** 2).sum()
