@@ -58,6 +58,18 @@ class Pools3D(C.Structure):
                 ("estore", C.c_void_p), ("S", C.c_void_p)]
 
 
+class GemmArgs(C.Structure):
+    """dnx_gemm_args (include/dnx_fno.h)."""
+    _fields_ = [("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32), ("batch", C.c_int32),
+                ("A", C.c_void_p), ("sAm", C.c_int64), ("sAk", C.c_int64), ("sAb", C.c_int64),
+                ("B", C.c_void_p), ("sBk", C.c_int64), ("sBn", C.c_int64), ("sBb", C.c_int64),
+                ("C", C.c_void_p), ("sCm", C.c_int64), ("sCn", C.c_int64), ("sCb", C.c_int64),
+                ("Cin", C.c_void_p), ("sIm", C.c_int64), ("sIn", C.c_int64), ("sIb", C.c_int64),
+                ("alpha", C.c_float), ("beta", C.c_float), ("bias", C.c_void_p),
+                ("bias_mode", C.c_int32), ("act", C.c_int32), ("pre", C.c_void_p),
+                ("a_bf16", C.c_int32), ("b_bf16", C.c_int32)]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _L = C.c_int64
@@ -77,6 +89,8 @@ _SIGS = {
                                 C.POINTER(Monitor), _I, _P, _P, C.POINTER(_I)], _I),
     "dnx_fdtd3d_set_tiles": ([_I, _I, _I, _I], _I),
     "dnx_fdtd3d_get_tiles": ([_I, _I, C.POINTER(_I), C.POINTER(_I)], _I),
+    "dnx_gemm_bf16": ([C.POINTER(GemmArgs), _P], _I),
+    "dnx_gelu_bwd": ([_P, _P, _P, _L, _P], _I),
     "dnx_batch3d_workspace_bytes": ([_I], _L),
     "dnx_batch3d_create": ([C.POINTER(Design3D), _I, _I, _I, C.POINTER(Pools3D), _P, _L, _I, _P,
                             C.POINTER(_P)], _I),

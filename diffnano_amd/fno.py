@@ -1,0 +1,312 @@
+"""Fourier neural operator (2D) on hand-written BF16 MFMA kernels.
+
+Surrogate for the 2D TM FDTD: maps a permittivity map to complex DFT
+fields.  Every dense operation runs through ``dnx_gemm_bf16`` (BF16
+operands, FP32 accumulation, fused epilogues) and the GELU backward through
+``dnx_gelu_bwd``; forward and backward are written out by hand (no torch
+autograd inside), wrapped in one ``torch.autograd.Function`` so a torch
+optimizer can train the FP32 master weights.
+
+Architecture (Li et al., "Fourier Neural Operator", 2021): lift (pointwise
+linear, in_ch -> width), L Fourier layers h <- act(W h + b + K h), projection
+(width -> hidden, GELU, hidden -> out_ch).  K h keeps modes ky in
+{0..m1-1} U {-m1..-1} and kx in {0..m2-1} of the 2D DFT, mixes channels per
+mode with complex weights, and returns to space (real part, Hermitian
+weights over kx, 1/(H W) normalisation, as irfft2 of the truncated
+spectrum).  Here the truncated 2D DFT and its inverse are dense matrices
+(H W x 2 m1 m2 complex, stored BF16), so each spectral convolution is two
+large-K GEMMs on MFMA plus 4 small per-mode GEMMs.  The activation after
+the last Fourier layer is omitted, as in the original FNO.
+
+Layout: activations are [C][B][H*W] FP32 (channel-major over the batch), so
+pointwise layers are single GEMMs over N = B*H*W and the DFT GEMMs see rows
+(c, b).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+GemmArgs = L.GemmArgs
+
+
+def _off(t, elems):
+    return t.data_ptr() + int(elems) * t.element_size()
+
+
+def gemm(M, N, K, A, sA, B, sB, Cout, sC, *, batch=1, a_off=0, b_off=0, c_off=0, alpha=1.0,
+         beta=0.0, Cin=None, sI=None, i_off=0, bias=None, bias_mode=0, act=0, pre=None):
+    """C = act(alpha A B + beta Cin + bias); sA = (sAm, sAk, sAb) etc. (elements)."""
+    g = GemmArgs()
+    g.M, g.N, g.K, g.batch = int(M), int(N), int(K), int(batch)
+    g.A, (g.sAm, g.sAk, g.sAb) = _off(A, a_off), sA
+    g.B, (g.sBk, g.sBn, g.sBb) = _off(B, b_off), sB
+    g.C, (g.sCm, g.sCn, g.sCb) = _off(Cout, c_off), sC
+    if Cin is Cout and i_off == 0:
+        i_off = c_off  # accumulating in place: read C where it is written
+    if Cin is not None:
+        g.Cin = _off(Cin, i_off)
+        g.sIm, g.sIn, g.sIb = sI if sI is not None else sC
+    g.alpha, g.beta = float(alpha), float(beta)
+    g.bias = bias.data_ptr() if bias is not None else None
+    g.bias_mode, g.act = int(bias_mode), int(act)
+    g.pre = _off(pre, c_off) if pre is not None else None
+    g.a_bf16 = 1 if A.dtype == torch.bfloat16 else 0
+    g.b_bf16 = 1 if B.dtype == torch.bfloat16 else 0
+    for t in (A, B):
+        if t.dtype not in (torch.float32, torch.bfloat16):
+            raise TypeError("gemm operands must be float32 or bfloat16")
+    L.check(L.lib().dnx_gemm_bf16(C.byref(g), L.stream_handle(Cout.device)))
+    return Cout
+
+
+def gelu_bwd(dy, pre, dx):
+    L.check(L.lib().dnx_gelu_bwd(dy.data_ptr(), pre.data_ptr(), dx.data_ptr(), dy.numel(),
+                                 L.stream_handle(dy.device)))
+    return dx
+
+
+def dft_matrices(H, W, m1, m2, device):
+    """Forward F (H W x 2 M) and inverse G (2 M x H W), M = 2 m1 m2 modes,
+    column/row n = ri * M + mu, mu = kyi * m2 + kx, ky = kyi (kyi < m1) or
+    kyi - 2 m1 (negative half).  Built in float64, stored BF16."""
+    kyi = np.arange(2 * m1)
+    ky = np.where(kyi < m1, kyi, kyi - 2 * m1).astype(np.float64)
+    kx = np.arange(m2, dtype=np.float64)
+    y = np.arange(H, dtype=np.float64)
+    x = np.arange(W, dtype=np.float64)
+    theta = 2 * np.pi * (ky[None, None, :, None] * y[:, None, None, None] / H
+                         + kx[None, None, None, :] * x[None, :, None, None] / W)
+    theta = theta.reshape(H * W, 2 * m1 * m2)
+    F = np.concatenate([np.cos(theta), -np.sin(theta)], axis=1)
+    ckx = np.where(kx == 0, 1.0, 2.0)
+    cw = np.tile(ckx, 2 * m1)[None, :] / (H * W)
+    G = np.concatenate([np.cos(theta) * cw, -np.sin(theta) * cw], axis=1).T
+    return (torch.tensor(F, device=device).to(torch.bfloat16).contiguous(),
+            torch.tensor(G, device=device).to(torch.bfloat16).contiguous())
+
+
+class FNO2d:
+    """Parameters and the hand-written forward/backward of a 2D FNO."""
+
+    def __init__(self, H, W, in_ch=3, out_ch=6, width=32, modes=(16, 16), layers=4, hidden=128,
+                 device="cuda", seed=0):
+        self.H, self.W = int(H), int(W)
+        self.in_ch, self.out_ch, self.width = in_ch, out_ch, width
+        self.m1, self.m2 = modes
+        self.layers, self.hidden = layers, hidden
+        self.dev = torch.device(device)
+        self.nmodes = 2 * self.m1 * self.m2
+        self.F, self.G = dft_matrices(self.H, self.W, self.m1, self.m2, self.dev)
+        g = torch.Generator().manual_seed(seed)
+
+        def u(shape, fan_in):
+            bound = 1.0 / math.sqrt(fan_in)
+            return ((torch.rand(shape, generator=g, dtype=torch.float64) * 2 - 1) * bound).to(
+                device=self.dev, dtype=torch.float32)
+
+        c = width
+        scale = 1.0 / (c * c)
+        p = {"w_lift": u((c, in_ch), in_ch), "b_lift": u((c,), in_ch)}
+        for l in range(layers):
+            p[f"w{l}"] = u((c, c), c)
+            p[f"b{l}"] = u((c,), c)
+            p[f"sr{l}"] = (scale * torch.rand((self.nmodes, c, c), generator=g,
+                                              dtype=torch.float64)).to(self.dev, torch.float32)
+            p[f"si{l}"] = (scale * torch.rand((self.nmodes, c, c), generator=g,
+                                              dtype=torch.float64)).to(self.dev, torch.float32)
+        p["w_p1"] = u((hidden, c), c)
+        p["b_p1"] = u((hidden,), c)
+        p["w_p2"] = u((out_ch, hidden), hidden)
+        p["b_p2"] = u((out_ch,), hidden)
+        self.params = {k: v.contiguous().requires_grad_(True) for k, v in p.items()}
+        self.names = list(self.params)
+
+    # -- building blocks ------------------------------------------------------
+    def _pointwise(self, Wt, x, out, bias=None, act=0, pre=None, Cin=None, beta=0.0):
+        """out[o][n] = act(sum_i Wt[o][i] x[i][n] + bias[o] + beta Cin[o][n])."""
+        O, I = Wt.shape
+        n = x.shape[1]
+        return gemm(O, n, I, Wt, (I, 1, 0), x, (n, 1, 0), out, (n, 1, 0), bias=bias,
+                    bias_mode=2 if bias is not None else 0, act=act, pre=pre, Cin=Cin, beta=beta)
+
+    def _mix(self, Z, sr, si, Y, B):
+        """Y[(o,b)][ri*M+mu] = sum_i Z[(i,b)][.] (Wr + i Wi)[mu][i][o] per mode."""
+        c, M = self.width, self.nmodes
+        ld = 2 * M
+        sA = (ld, B * ld, 1)
+        sB = (c, 1, c * c)
+        sC = (ld, B * ld, 1)
+        kw = dict(batch=M)
+        gemm(B, c, c, Z, sA, sr, sB, Y, sC, a_off=0, c_off=0, **kw)                 # Yr = Zr Wr
+        gemm(B, c, c, Z, sA, si, sB, Y, sC, a_off=M, c_off=0, alpha=-1.0, beta=1.0,  # - Zi Wi
+             Cin=Y, **kw)
+        gemm(B, c, c, Z, sA, si, sB, Y, sC, a_off=0, c_off=M, **kw)                 # Yi = Zr Wi
+        gemm(B, c, c, Z, sA, sr, sB, Y, sC, a_off=M, c_off=M, beta=1.0, Cin=Y, **kw)  # + Zi Wr
+
+    # -- forward / backward ----------------------------------------------------
+    def forward_raw(self, x, save=True):
+        """x: [in_ch][B][H*W] FP32 -> ([out_ch][B][H*W] FP32, saved tensors)."""
+        p = self.params
+        c, B = self.width, x.shape[1]
+        P = self.H * self.W
+        n = B * P
+        M2 = 2 * self.nmodes
+        dev = self.dev
+        x2 = x.reshape(self.in_ch, n)
+        h = torch.empty(c, n, device=dev)
+        self._pointwise(p["w_lift"], x2, h, bias=p["b_lift"])
+        saved = {"x": x2, "h": [], "Z": [], "pre": []}
+        for l in range(self.layers):
+            Z = torch.empty(c * B, M2, device=dev)
+            gemm(c * B, M2, P, h, (P, 1, 0), self.F, (M2, 1, 0), Z, (M2, 1, 0))
+            Y = torch.empty(c * B, M2, device=dev)
+            self._mix(Z, p[f"sr{l}"], p[f"si{l}"], Y, B)
+            S = torch.empty(c, n, device=dev)
+            gemm(c * B, P, M2, Y, (M2, 1, 0), self.G, (P, 1, 0), S, (P, 1, 0))
+            last = l == self.layers - 1
+            hn = torch.empty(c, n, device=dev)
+            pre = None if last else torch.empty(c, n, device=dev)
+            self._pointwise(p[f"w{l}"], h, hn, bias=p[f"b{l}"], act=0 if last else 1, pre=pre,
+                            Cin=S, beta=1.0)
+            if save:
+                saved["h"].append(h)
+                saved["Z"].append(Z)
+                saved["pre"].append(pre)
+            h = hn
+        q = torch.empty(self.hidden, n, device=dev)
+        qpre = torch.empty(self.hidden, n, device=dev)
+        self._pointwise(p["w_p1"], h, q, bias=p["b_p1"], act=1, pre=qpre)
+        out = torch.empty(self.out_ch, n, device=dev)
+        self._pointwise(p["w_p2"], q, out, bias=p["b_p2"])
+        saved.update({"hL": h, "q": q, "qpre": qpre, "B": B})
+        return out.view(self.out_ch, B, P), (saved if save else None)
+
+    SPLIT = 256  # split-K factor for reductions over the B*H*W axis
+
+    def _split(self, n):
+        s = self.SPLIT
+        while n % s:
+            s //= 2
+        return s, n // s
+
+    def _wgrad(self, dY, X, out):
+        """out[o][i] = sum_n dY[o][n] X[i][n] over n = B*H*W: split-K partial
+        GEMMs across the batch grid axis, then a GEMM with a ones vector sums
+        the partials."""
+        O, n = dY.shape
+        I = X.shape[0]
+        s, chunk = self._split(n)
+        part = torch.empty(s, O * I, device=self.dev)
+        gemm(O, I, chunk, dY, (n, 1, chunk), X, (1, n, chunk), part, (I, 1, O * I), batch=s)
+        ones = self._ones(s)
+        gemm(1, O * I, s, ones, (0, 1, 0), part, (O * I, 1, 0), out.view(1, O * I),
+             (O * I, 1, 0))
+
+    def _rowsum(self, dY, out):
+        O, n = dY.shape
+        s, chunk = self._split(n)
+        ones = self._ones(n)
+        part = torch.empty(s, O, device=self.dev)
+        gemm(O, 1, chunk, dY, (n, 1, chunk), ones, (1, 0, chunk), part, (1, 1, O), batch=s)
+        gemm(1, O, s, self._ones(s), (0, 1, 0), part, (O, 1, 0), out.view(1, O), (O, 1, 0))
+
+    def _ones(self, n):
+        o = getattr(self, "_ones_buf", None)
+        if o is None or o.numel() < n:
+            o = torch.ones(n, device=self.dev)
+            self._ones_buf = o
+        return o
+
+    def backward_raw(self, dout, saved):
+        """Gradients of all parameters given d(out) [out_ch][B][H*W]."""
+        p = self.params
+        c, B = self.width, saved["B"]
+        P = self.H * self.W
+        n = B * P
+        M, M2 = self.nmodes, 2 * self.nmodes
+        dev = self.dev
+        g = {k: torch.zeros_like(v) for k, v in p.items()}
+        d2 = dout.reshape(self.out_ch, n).contiguous()
+        # projection
+        self._wgrad(d2, saved["q"], g["w_p2"])
+        self._rowsum(d2, g["b_p2"])
+        dq = torch.empty(self.hidden, n, device=dev)
+        gemm(self.hidden, n, self.out_ch, p["w_p2"], (1, self.hidden, 0), d2, (n, 1, 0), dq,
+             (n, 1, 0))
+        gelu_bwd(dq, saved["qpre"], dq)
+        self._wgrad(dq, saved["hL"], g["w_p1"])
+        self._rowsum(dq, g["b_p1"])
+        dh = torch.empty(c, n, device=dev)
+        gemm(c, n, self.hidden, p["w_p1"], (1, c, 0), dq, (n, 1, 0), dh, (n, 1, 0))
+        del dq
+        for l in range(self.layers - 1, -1, -1):
+            h, Z, pre = saved["h"][l], saved["Z"][l], saved["pre"][l]
+            dpre = dh if pre is None else gelu_bwd(dh, pre, torch.empty_like(dh))
+            self._wgrad(dpre, h, g[f"w{l}"])
+            self._rowsum(dpre, g[f"b{l}"])
+            dhn = torch.empty(c, n, device=dev)
+            gemm(c, n, c, p[f"w{l}"], (1, c, 0), dpre, (n, 1, 0), dhn, (n, 1, 0))
+            # spectral branch: dY = dS G^T
+            dY = torch.empty(c * B, M2, device=dev)
+            gemm(c * B, M2, P, dpre, (P, 1, 0), self.G, (1, P, 0), dY, (M2, 1, 0))
+            sr, si = p[f"sr{l}"], p[f"si{l}"]
+            ld = M2
+            sZ = (ld, B * ld, 1)       # Z/dZ/dY as [b][i] per mode
+            sWt = (1, c, c * c)        # W^T: B[k=o][n=i] = W[mu][i][o]
+            sWc = (c, 1, c * c)        # dW[mu][i][o]
+            dZ = torch.empty(c * B, M2, device=dev)
+            kw = dict(batch=M)
+            # dZr = dYr Wr^T + dYi Wi^T ; dZi = -dYr Wi^T + dYi Wr^T
+            gemm(B, c, c, dY, sZ, sr, sWt, dZ, sZ, a_off=0, c_off=0, **kw)
+            gemm(B, c, c, dY, sZ, si, sWt, dZ, sZ, a_off=M, c_off=0, beta=1.0, Cin=dZ, **kw)
+            gemm(B, c, c, dY, sZ, si, sWt, dZ, sZ, a_off=0, c_off=M, alpha=-1.0, **kw)
+            gemm(B, c, c, dY, sZ, sr, sWt, dZ, sZ, a_off=M, c_off=M, beta=1.0, Cin=dZ, **kw)
+            # dWr = Zr^T dYr + Zi^T dYi ; dWi = Zr^T dYi - Zi^T dYr   (K = batch)
+            sZt = (B * ld, ld, 1)      # A[m=i][k=b]
+            sdY = (ld, B * ld, 1)      # B[k=b][n=o]
+            gemm(c, c, B, Z, sZt, dY, sdY, g[f"sr{l}"], sWc, a_off=0, b_off=0, **kw)
+            gemm(c, c, B, Z, sZt, dY, sdY, g[f"sr{l}"], sWc, a_off=M, b_off=M, beta=1.0,
+                 Cin=g[f"sr{l}"], **kw)
+            gemm(c, c, B, Z, sZt, dY, sdY, g[f"si{l}"], sWc, a_off=0, b_off=M, **kw)
+            gemm(c, c, B, Z, sZt, dY, sdY, g[f"si{l}"], sWc, a_off=M, b_off=0, alpha=-1.0,
+                 beta=1.0, Cin=g[f"si{l}"], **kw)
+            # dh = W^T dpre + dZ F^T
+            gemm(c * B, P, M2, dZ, (M2, 1, 0), self.F, (1, M2, 0), dhn, (P, 1, 0), beta=1.0,
+                 Cin=dhn)
+            dh = dhn
+        self._wgrad(dh, saved["x"], g["w_lift"])
+        self._rowsum(dh, g["b_lift"])
+        return g
+
+    def __call__(self, x):
+        """x: [B, in_ch, H, W] -> [B, out_ch, H, W] (differentiable w.r.t. the
+        parameters; the input is not differentiated)."""
+        B = x.shape[0]
+        xt = x.to(self.dev, torch.float32).permute(1, 0, 2, 3).reshape(
+            self.in_ch, B, self.H * self.W).contiguous()
+        out = _FNOFn.apply(self, xt, *[self.params[k] for k in self.names])
+        return out.view(self.out_ch, B, self.H, self.W).permute(1, 0, 2, 3)
+
+
+class _FNOFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, model, xt, *params):
+        need = any(ctx.needs_input_grad[2:])
+        out, saved = model.forward_raw(xt, save=need)
+        ctx.model, ctx.saved_fno = model, saved
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        model = ctx.model
+        g = model.backward_raw(dout.contiguous(), ctx.saved_fno)
+        ctx.saved_fno = None
+        return (None, None) + tuple(g[k] for k in model.names)

@@ -1,0 +1,62 @@
+/*
+ * dnx_fno.h -- C ABI of the BF16 MFMA kernels behind the Fourier-neural-
+ * operator surrogate (libdnx_fdtd.so).  No reference counterpart: DiffNano's
+ * surrogate (diffnano/solvers/surrogate.py:23-75) is a PyTorch CNN for RCWA;
+ * its prescreen-then-verify pattern (proxy_prescreen.py:180-210) is what the
+ * FNO prescreen of the FDTD path follows.
+ *
+ * Same conventions as dnx_fdtd.h: device pointers, asynchronous on the given
+ * HIP stream, 0 on success, negative code + dnx_last_error() on failure.
+ */
+#ifndef DNX_FNO_H
+#define DNX_FNO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
+/* C[b] = act(alpha * A[b] B[b] + beta * Cin[b] + bias), b < batch.
+ * A[m][k] at A + b*sAb + m*sAm + k*sAk (element strides; any layout, so
+ * transposes are free); likewise B[k][n], C[m][n], Cin[m][n].  A and B are
+ * FP32 (a_bf16/b_bf16 = 0) or BF16 (= 1) in memory and are multiplied in
+ * BF16 on MFMA with FP32 accumulation; C, Cin, bias, pre are FP32.
+ * bias_mode: 0 none, 1 per column, 2 per row.  act: 0 none, 1 GELU (erf).
+ * pre: if not NULL, receives the value before the activation. */
+typedef struct dnx_gemm_args {
+  int32_t M, N, K, batch;
+  const void* A;
+  int64_t sAm, sAk, sAb;
+  const void* B;
+  int64_t sBk, sBn, sBb;
+  float* C;
+  int64_t sCm, sCn, sCb;
+  const float* Cin;
+  int64_t sIm, sIn, sIb;
+  float alpha, beta;
+  const float* bias;
+  int32_t bias_mode;
+  int32_t act;
+  float* pre;
+  int32_t a_bf16, b_bf16;
+} dnx_gemm_args;
+
+int32_t dnx_gemm_bf16(const dnx_gemm_args* args, void* stream);
+
+/* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32. */
+int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, int64_t n, void* stream);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNX_FNO_H */

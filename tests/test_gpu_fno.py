@@ -1,0 +1,93 @@
+"""GPU: the BF16 MFMA GEMM against FP32 products of the same BF16-rounded
+inputs (pins the MFMA lane layout, strides, transposes, epilogues), and the
+FNO forward / hand-written backward against the FP64 torch.fft oracle."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fno_oracle
+from tests._helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+@pytest.mark.parametrize("M,N,K,batch", [(128, 128, 64, 1), (77, 45, 100, 3), (1, 300, 513, 2),
+                                         (256, 32, 32, 4)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_matches_fp32_product_of_bf16_inputs(M, N, K, batch, ta, tb):
+    from diffnano_amd.fno import gemm
+    g = torch.Generator(device="cuda").manual_seed(M * 7 + N + K)
+    A = torch.randn(batch, K, M, device="cuda", generator=g) if ta else \
+        torch.randn(batch, M, K, device="cuda", generator=g)
+    B = torch.randn(batch, N, K, device="cuda", generator=g) if tb else \
+        torch.randn(batch, K, N, device="cuda", generator=g)
+    Cin = torch.randn(batch, M, N, device="cuda", generator=g)
+    bias = torch.randn(N, device="cuda", generator=g)
+    C = Cin.clone()
+    sA = (1, M, M * K) if ta else (K, 1, M * K)
+    sB = (1, K, N * K) if tb else (N, 1, N * K)
+    gemm(M, N, K, A, sA, B, sB, C, (N, 1, M * N), batch=batch, alpha=0.5, beta=-1.25, Cin=C,
+         bias=bias, bias_mode=1)
+    Am = _bf(A).transpose(1, 2) if ta else _bf(A)
+    Bm = _bf(B).transpose(1, 2) if tb else _bf(B)
+    ref = 0.5 * torch.bmm(Am.contiguous(), Bm.contiguous()) - 1.25 * Cin + bias
+    torch.cuda.synchronize()
+    assert rel_l2(C.cpu().numpy(), ref.cpu().numpy()) < 2e-6
+
+
+def test_gemm_bf16_operands_gelu_and_pre():
+    from diffnano_amd.fno import gemm
+    torch.manual_seed(0)
+    A = torch.randn(96, 40, device="cuda").to(torch.bfloat16)
+    B = torch.randn(40, 200, device="cuda").to(torch.bfloat16)
+    bias = torch.randn(96, device="cuda")
+    C = torch.empty(96, 200, device="cuda")
+    pre = torch.empty_like(C)
+    gemm(96, 200, 40, A, (40, 1, 0), B, (200, 1, 0), C, (200, 1, 0), bias=bias, bias_mode=2,
+         act=1, pre=pre)
+    z = A.float() @ B.float() + bias[:, None]
+    assert rel_l2(pre.cpu().numpy(), z.cpu().numpy()) < 2e-6
+    gel = torch.nn.functional.gelu(z)
+    assert rel_l2(C.cpu().numpy(), gel.cpu().numpy()) < 2e-6
+
+
+def _small_model(layers=2):
+    from diffnano_amd.fno import FNO2d
+    return FNO2d(32, 32, in_ch=3, out_ch=6, width=8, modes=(4, 4), layers=layers, hidden=16,
+                 device="cuda", seed=1)
+
+
+def test_fno_forward_matches_fft_oracle():
+    m = _small_model(layers=3)
+    torch.manual_seed(2)
+    x = torch.randn(4, 3, 32, 32)
+    out = m(x.cuda()).detach().double().cpu()
+    p64 = {k: v.detach().double().cpu() for k, v in m.params.items()}
+    ref = fno_oracle.fno_forward(p64, x.double(), 3, 4, 4)
+    err = rel_l2(out.numpy(), ref.numpy())
+    print("fno fwd rel L2", err)
+    assert err < 2e-2
+
+
+def test_fno_backward_matches_fft_oracle():
+    m = _small_model(layers=2)
+    torch.manual_seed(3)
+    x = torch.randn(4, 3, 32, 32)
+    w = torch.randn(4, 6, 32, 32)
+    out = m(x.cuda())
+    (out * w.cuda()).sum().backward()
+    p64 = {k: v.detach().double().cpu().requires_grad_(True) for k, v in m.params.items()}
+    ref = fno_oracle.fno_forward(p64, x.double(), 2, 4, 4)
+    (ref * w.double()).sum().backward()
+    worst = 0.0
+    for k in m.names:
+        e = rel_l2(m.params[k].grad.double().cpu().numpy(), p64[k].grad.numpy())
+        worst = max(worst, e)
+        print(f"grad {k}: {e:.3e}")
+        assert e < 5e-2, k
+    assert worst < 5e-2

@@ -25,7 +25,9 @@ HEADER = os.path.join(ROOT, "include", "dnx_fdtd.h")
 
 
 def _declared():
-    text = open(HEADER).read()
+    inc = os.path.dirname(HEADER)
+    text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
+                   if f.endswith(".h"))
     return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(dnx_\w+)\(", text, flags=re.M)))
 
 
