@@ -38,9 +38,13 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 BYTES = {"forward": 52, "replay": 64, "adjoint": 72}
 # library timing name -> (phase, kernel symbol, algorithmic bytes per cell-update)
+# (the two-step kernels advance two time steps per launch)
 TIMED = {"fdtd3d_fwd": ("forward", "k3_fused_fwd", 52),
          "fdtd3d_fwd_store": ("replay", "k3_fused_fwd", 64),
-         "fdtd3d_adj": ("adjoint", "k3_fused_adj", 72)}
+         "fdtd3d_adj": ("adjoint", "k3_fused_adj", 72),
+         "fdtd3d_fwd2": ("forward", "k3_fused2_fwd", 104),
+         "fdtd3d_fwd2_store": ("replay", "k3_fused2_fwd", 128),
+         "fdtd3d_adj2": ("adjoint", "k3_fused2_adj", 144)}
 METRIC = "Mcell-updates/s fwd+adjoint, %HBM roofline; designs/s batched vs plain @1e-4"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fdtd3d_256_hbm_traffic.md")
 
@@ -117,9 +121,11 @@ def _pmc_probe(torch, dev, solver_kw):
     eng.adjoint_run(lam, 4, 0, None, estore, (None, 0), (None, 0), S, torch.empty_like(lam))
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": {"copy_bytes_read": a.numel() * 4,
-                                    "copy_bytes_written": a.numel() * 4,
-                                    "grid": list(shape), "fwd_launches": 4, "adj_launches": 3,
-                                    "fwd_alg_bytes": 52 * N, "adj_alg_bytes": 72 * N}}))
+                                    "copy_bytes_written": a.numel() * 4, "grid": list(shape),
+                                    "launches": "4 forward steps (2 two-step launches), 3 "
+                                                "adjoint steps (1 two-step, 1 one-step launch)",
+                                    "fwd2_alg_bytes": 104 * N, "adj2_alg_bytes": 144 * N,
+                                    "adj_alg_bytes": 72 * N}}))
 
 
 def _bench_batched(args):
@@ -160,17 +166,134 @@ def _bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def fno_flops_per_sample(m):
+    """Multiply-add FLOPs of one FNO2d forward sample (every product runs on
+    the BF16 MFMA GEMM kernels)."""
+    P, c, M2 = m.H * m.W, m.width, 2 * m.nmodes
+    per_layer = 2 * c * P * M2 + 8 * c * c * m.nmodes + 2 * c * M2 * P + 2 * c * c * P
+    return (2 * m.in_ch * c * P + m.layers * per_layer + 2 * c * m.hidden * P
+            + 2 * m.hidden * m.out_ch * P)
+
+
+def _bench_surrogate(args):
+    """Surrogate-prescreened design loop (128x128 TM, 2000 steps) against the
+    same loop judged exhaustively by the batched FDTD and by the plain
+    one-design-at-a-time solver; plus the surrogate's data generation,
+    training and MFMA throughput."""
+    import torch
+
+    from diffnano_amd import surrogate as SG
+    from diffnano_amd.workloads import SurrogateConfig
+
+    dev = torch.device("cuda", 0)
+    cfg = SurrogateConfig()
+    sync = torch.cuda.synchronize
+    out = {"metric": "design-iterations/s, batched FDTD + FNO prescreen vs plain loop @ equal FoM",
+           "unit": "design-iterations/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f32 FDTD, bf16 MFMA (fp32 accumulate) FNO", "data": "synthetic (seeded blobs)",
+           "config": {"workload": "2D TM 128x128, dl 20 nm, 10-cell CPML, 2000 steps; FNO 4 "
+                                  "layers, width 32, 16 modes; 256 candidates, top 8",
+                      "samples": args.samples, "epochs": args.epochs}}
+    sync()
+    t0 = time.perf_counter()
+    noise, eps, fields = SG.generate_dataset(cfg, range(args.samples), device=dev)
+    sync()
+    t_data = time.perf_counter() - t0
+    out["datagen_s"] = round(t_data, 2)
+    out["datagen_mcells"] = round(args.samples * cfg.H * cfg.W * 2000 / t_data / 1e6, 1)
+    nval = max(256, args.samples // 20)
+    ntr = args.samples - nval
+    sur = SG.FieldSurrogate(cfg, device=dev)
+    t0 = time.perf_counter()
+    hist = sur.fit(eps[:ntr], fields[:ntr], epochs=args.epochs, batch=128, lr=3e-3)
+    sync()
+    out["train_s"] = round(time.perf_counter() - t0, 1)
+    out["train_loss"] = [round(h, 5) for h in hist[-3:]]
+    ev, fv = eps[ntr:], fields[ntr:]
+    pf, tf = cfg.fom(sur.predict(ev)), cfg.fom(fv)
+    out["val_field_rel_err"] = round(float(sur.rel_error(ev, fv).mean()), 4)
+    out["val_fom_rel_err"] = round(float(((pf - tf).abs() / tf).mean()), 4)
+    rk = lambda a: np.argsort(np.argsort(a.cpu().numpy()))  # noqa: E731
+    out["val_fom_spearman"] = round(float(np.corrcoef(rk(pf), rk(tf))[0, 1]), 3)
+    del fields
+    # inference throughput / MFMA utilisation (HIP events on the launch stream)
+    xb = ev[:256]
+    sur.predict(xb)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        sur.predict(xb)
+    e1.record()
+    sync()
+    ms = e0.elapsed_time(e1) / 5
+    fl = fno_flops_per_sample(sur.fno) * 256
+    out["fno_predict_256_ms"] = round(ms, 2)
+    out["fno_tflops"] = round(fl / ms / 1e9, 1)
+    out["fno_mfma_frac"] = round(fl / ms / 1e9 / 2500.0, 4)
+
+    # design loops from the same start and candidate stream
+    iters = args.iters
+    ex = SG.PrescreenDesigner(cfg, mode="exhaustive", seed=7, device=dev)
+    sync()
+    t0 = time.perf_counter()
+    ex.run(iters)
+    sync()
+    t_ex = time.perf_counter() - t0
+    target = ex.parent_fom
+    sl = SG.PrescreenDesigner(cfg, sur, mode="surrogate", seed=7, device=dev,
+                              finetune_steps=args.finetune)
+    sync()
+    t0 = time.perf_counter()
+    while sl.parent_fom < 0.99 * target and sl.stats["steps"] < 3 * iters:
+        sl.step()
+    sync()
+    t_sl = time.perf_counter() - t0
+    rnd = SG.PrescreenDesigner(cfg, mode="random", seed=7, device=dev)
+    rnd.run(sl.stats["steps"])
+    pl = SG.PrescreenDesigner(cfg, mode="plain", seed=7, device=dev)
+    sync()
+    t0 = time.perf_counter()
+    pl.step()
+    sync()
+    t_pl_iter = time.perf_counter() - t0
+    t_pl = t_pl_iter * iters  # plain makes the exhaustive loop's decisions
+    reached = sl.parent_fom >= 0.99 * target
+    out.update({
+        "value": round(sl.stats["steps"] / t_sl, 3),
+        "iterations_target": iters,
+        "fom_start": ex.history[0], "fom_target": target,
+        "surrogate": {"iterations": sl.stats["steps"], "wall_s": round(t_sl, 2),
+                      "fom": sl.parent_fom, "reached_99pct": bool(reached),
+                      "fallbacks": sl.stats["fallbacks"], "fdtd_solves": sl.stats["fdtd_solves"],
+                      "mean_rel_err": round(float(np.mean(sl.stats["rel_err"])), 4),
+                      "time_s": {k: round(v, 3) for k, v in sl.stats["time_s"].items()}},
+        "exhaustive_batched": {"iterations": iters, "wall_s": round(t_ex, 2),
+                               "iters_per_s": round(iters / t_ex, 3)},
+        "plain": {"iter_s": round(t_pl_iter, 2), "iters_per_s": round(1 / t_pl_iter, 4),
+                  "fom_after_1": pl.parent_fom, "matches_exhaustive": pl.history[1] == ex.history[1]},
+        "random_same_budget": {"iterations": rnd.stats["steps"], "fom": rnd.parent_fom},
+        "speedup_vs_plain_at_equal_fom": round(t_pl / t_sl, 1) if reached else None,
+        "speedup_vs_exhaustive_batched_at_equal_fom": round(t_ex / t_sl, 2) if reached else None,
+    })
+    out["vs_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--n-steps", type=int, default=None, help="time steps (default: config's)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--designs", type=int, default=48, help="c3: number of designs")
-    ap.add_argument("--iters", type=int, default=2, help="c3: Adam iterations per mode")
+    ap.add_argument("--iters", type=int, default=2,
+                    help="c3: Adam iterations per mode; c4: design iterations of the reference loop")
+    ap.add_argument("--samples", type=int, default=20000, help="c4: surrogate training samples")
+    ap.add_argument("--epochs", type=int, default=30, help="c4: surrogate training epochs")
+    ap.add_argument("--finetune", type=int, default=4, help="c4: fine-tune steps per iteration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the FDTD "
@@ -178,6 +301,9 @@ def main():
         sys.exit(2)
     if args.config == "c3":
         _bench_batched(args)
+        return
+    if args.config == "c4":
+        _bench_surrogate(args)
         return
 
     import torch

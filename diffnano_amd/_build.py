@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-fPIC",
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast",
+    "-ffp-contract=fast-honor-pragmas",  # FDTD sources turn contraction off
     "-fvisibility=hidden",
     "-Wall", "-Wno-unused-function",
 ]

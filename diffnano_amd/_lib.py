@@ -89,8 +89,11 @@ _SIGS = {
                                 C.POINTER(Monitor), _I, _P, _P, C.POINTER(_I)], _I),
     "dnx_fdtd3d_set_tiles": ([_I, _I, _I, _I], _I),
     "dnx_fdtd3d_get_tiles": ([_I, _I, C.POINTER(_I), C.POINTER(_I)], _I),
+    "dnx_fdtd2d_tm_batch": ([C.POINTER(Grid2D), _I, _P, _I, C.POINTER(Source), C.POINTER(Box),
+                             _I, _P, _P, _P], _I),
     "dnx_gemm_bf16": ([C.POINTER(GemmArgs), _P], _I),
     "dnx_gelu_bwd": ([_P, _P, _P, _L, _P], _I),
+    "dnx_sum_partials": ([_P, _I, _L, _P, C.c_float, _P], _I),
     "dnx_batch3d_workspace_bytes": ([_I], _L),
     "dnx_batch3d_create": ([C.POINTER(Design3D), _I, _I, _I, C.POINTER(Pools3D), _P, _L, _I, _P,
                             C.POINTER(_P)], _I),

@@ -32,6 +32,16 @@ template <typename T>
 int launch_monitor_adj(Comps<T> lam, Comps<T> E, T* S, int ny, int nx, const dnx_monitor& m,
                        int sample_row, hipStream_t st);
 
+// Two consecutive samples (rows n, n+1) staged by the two-step kernels in
+// side = [2][ncomp][cells] (mask order): acc += both, in step order.
+template <typename T>
+int launch_monitor_pair(const T* side, const dnx_monitor& m, int n, hipStream_t st);
+
+// d = [ncomp][cells]: the adjoint source of sample `sample_row` (what
+// launch_monitor_adj adds to lam), for the two-step adjoint kernel.
+template <typename T>
+int launch_monitor_d(T* d, const dnx_monitor& m, int sample_row, hipStream_t st);
+
 // Fill n elements with zero (a kernel, so it is safe inside captured graphs).
 template <typename T>
 int launch_zero(T* p, int64_t n, hipStream_t st);

@@ -7,6 +7,9 @@
 
 #include "dnx_aux.h"
 
+// No multiply-add contraction (same rounding in every FDTD kernel variant).
+#pragma clang fp contract(off)
+
 namespace dnx {
 
 static thread_local std::string g_err;
@@ -252,7 +255,69 @@ int launch_dot_init(T* S, Comps<T> a, Comps<T> b, int ncomp, int64_t n, hipStrea
   return DNX_OK;
 }
 
+// Two samples (steps n, n+1) staged compactly by the two-step kernels:
+// side = [2][ncomp][cells] in mask order.  Per accumulator element the
+// order of additions is that of two k_monitor launches.
+template <typename T>
+__global__ void k_monitor_pair(const T* __restrict__ side, int64_t cells, int comp_mask, int nf,
+                               const double* __restrict__ ph, double* __restrict__ acc) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  int ncomp = ((comp_mask >> 0) & 1) + ((comp_mask >> 1) & 1) + ((comp_mask >> 2) & 1);
+  for (int ci = 0; ci < ncomp; ++ci) {
+    const double v1 = double(side[int64_t(ci) * cells + i]);
+    const double v2 = double(side[(int64_t(ncomp) + ci) * cells + i]);
+    for (int f = 0; f < nf; ++f) {
+      double* a = acc + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+      a[0] += v1 * ph[2 * f];
+      a[1] -= v1 * ph[2 * f + 1];
+      a[0] += v2 * ph[2 * nf + 2 * f];
+      a[1] -= v2 * ph[2 * nf + 2 * f + 1];
+    }
+  }
+}
+
+// d[ci][i] = T(sum_f Re(g e^{i phi})) at one sample (the adjoint source
+// k_monitor_adj adds), for the two-step adjoint kernel to apply.
+template <typename T>
+__global__ void k_monitor_d(T* __restrict__ d, int64_t cells, int comp_mask, int nf,
+                            const double* __restrict__ ph, const double* __restrict__ g) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  int ncomp = ((comp_mask >> 0) & 1) + ((comp_mask >> 1) & 1) + ((comp_mask >> 2) & 1);
+  for (int ci = 0; ci < ncomp; ++ci) {
+    double s = 0.0;
+    for (int f = 0; f < nf; ++f) {
+      const double* gg = g + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+      s += gg[0] * ph[2 * f] - gg[1] * ph[2 * f + 1];
+    }
+    d[int64_t(ci) * cells + i] = T(s);
+  }
+}
+
+template <typename T>
+int launch_monitor_pair(const T* side, const dnx_monitor& m, int n, hipStream_t st) {
+  Box b = make_box(m.box);
+  hipLaunchKernelGGL(k_monitor_pair<T>, dim3(cdiv(b.cells(), 256)), dim3(256), 0, st, side,
+                     b.cells(), m.comp_mask, m.n_freq, m.phase + int64_t(n) * m.n_freq * 2,
+                     m.acc);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_monitor_d(T* d, const dnx_monitor& m, int sample_row, hipStream_t st) {
+  Box b = make_box(m.box);
+  hipLaunchKernelGGL(k_monitor_d<T>, dim3(cdiv(b.cells(), 256)), dim3(256), 0, st, d, b.cells(),
+                     m.comp_mask, m.n_freq, m.phase + int64_t(sample_row) * m.n_freq * 2,
+                     m.grad);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
 #define DNX_INST(T)                                                                        \
+  template int launch_monitor_pair<T>(const T*, const dnx_monitor&, int, hipStream_t);   \
+  template int launch_monitor_d<T>(T*, const dnx_monitor&, int, hipStream_t);            \
   template int launch_source<T>(T*, int, int, const dnx_source&, int, hipStream_t);      \
   template int launch_source_adj<T>(T*, const T*, int, int, const dnx_source&, int,      \
                                     hipStream_t);                                        \

@@ -14,6 +14,10 @@
 //   Hz' = bxby Hz + kh (ax D+r Ey' - ay D+c Ex')
 #include "dnx_aux.h"
 
+// No multiply-add contraction: the batched and single-design kernels round
+// identically.
+#pragma clang fp contract(off)
+
 namespace dnx {
 
 template <typename T>
@@ -303,6 +307,81 @@ static int adj_run2(const dnx_grid2d* gd, T* lam, int m_hi, int m_lo, const T* e
   return DNX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched TM forward over many designs of one shape (blockIdx.z = design),
+// source shared, with a running DFT over a box accumulated in FP32 inside
+// the E kernel.  Used to generate surrogate training data and to judge
+// candidate designs; forward only.
+// ---------------------------------------------------------------------------
+
+struct Tm2Batch {
+  int ny, nx;
+  int64_t n;
+  float kh;
+  const float* __restrict__ ce;  // [B][n]
+  const float* __restrict__ by;
+  const float* __restrict__ ay;
+  const float* __restrict__ bx;
+  const float* __restrict__ ax;
+  float* __restrict__ st;  // [B][3][n]
+  int has_src;
+  Box sbox;
+  const float* __restrict__ sprof;
+  const float* __restrict__ wave;
+  Box mbox;
+  int nf;
+  const float* __restrict__ phase;  // [T][nf][2] (cos, sin)
+  float* __restrict__ acc;          // [B][nf][2][mbox cells]
+};
+
+__global__ __launch_bounds__(BX2* BY2) void k2tm_batch_h(Tm2Batch a) {
+  const int c = blockIdx.x * BX2 + threadIdx.x;
+  const int r = blockIdx.y * BY2 + threadIdx.y;
+  if (c >= a.nx || r >= a.ny) return;
+  const int64_t n = a.n, nx = a.nx;
+  float* st = a.st + int64_t(blockIdx.z) * 3 * n;
+  const int64_t i = int64_t(r) * nx + c;
+  const float ez = st[i];
+  const float dc = (c + 1 < a.nx) ? st[i + 1] - ez : 0.f;
+  const float dr = (r + 1 < a.ny) ? st[i + nx] - ez : 0.f;
+  st[n + i] = a.by[r] * st[n + i] - a.kh * (a.ay[r] * dc);
+  st[2 * n + i] = a.bx[c] * st[2 * n + i] + a.kh * (a.ax[c] * dr);
+}
+
+__global__ __launch_bounds__(BX2* BY2) void k2tm_batch_e(Tm2Batch a, int step) {
+  const int c = blockIdx.x * BX2 + threadIdx.x;
+  const int r = blockIdx.y * BY2 + threadIdx.y;
+  if (c >= a.nx || r >= a.ny) return;
+  const int64_t n = a.n, nx = a.nx;
+  const int b = blockIdx.z;
+  float* st = a.st + int64_t(b) * 3 * n;
+  const int64_t i = int64_t(r) * nx + c;
+  const float* Hx = st + n;
+  const float* Hy = st + 2 * n;
+  const float dHy = r > 0 ? Hy[i] - Hy[i - nx] : 0.f;
+  const float dHx = c > 0 ? Hx[i] - Hx[i - 1] : 0.f;
+  float ez = (a.bx[c] * a.by[r]) * st[i] +
+             a.ce[int64_t(b) * n + i] * (a.ax[c] * dHy - a.ay[r] * dHx);
+  if (a.has_src) {
+    const Box& s = a.sbox;
+    const int dr = r - s.y0, dc = c - s.x0;
+    if (dr >= 0 && dr < s.ny && dc >= 0 && dc < s.nx) ez += a.wave[step] * a.sprof[dr * s.nx + dc];
+  }
+  st[i] = ez;
+  const Box& m = a.mbox;
+  const int mr = r - m.y0, mc = c - m.x0;
+  if (a.nf > 0 && mr >= 0 && mr < m.ny && mc >= 0 && mc < m.nx) {
+    const int64_t cells = int64_t(m.ny) * m.nx;
+    const int64_t j = int64_t(mr) * m.nx + mc;
+    const float* ph = a.phase + int64_t(step) * a.nf * 2;
+    float* acc = a.acc + int64_t(b) * a.nf * 2 * cells;
+    for (int f = 0; f < a.nf; ++f) {
+      acc[(2 * f) * cells + j] += ez * ph[2 * f];
+      acc[(2 * f + 1) * cells + j] -= ez * ph[2 * f + 1];
+    }
+  }
+}
+
 }  // namespace dnx
 
 using namespace dnx;
@@ -375,4 +454,49 @@ extern "C" int32_t dnx_fdtd2d_adjoint_run(const dnx_grid2d* g, void* lam, void* 
   return adj_run2<double>(g, static_cast<double*>(lam), m_hi, m_lo,
                           static_cast<const double*>(e_lo), static_cast<const double*>(estore),
                           srcs, n_src, mons, n_mon, static_cast<double*>(S), s);
+}
+
+extern "C" int32_t dnx_fdtd2d_tm_batch(const dnx_grid2d* g, int32_t batch, void* state,
+                                       int32_t n_steps, const dnx_source* src,
+                                       const dnx_box* mbox, int32_t n_freq, const float* phase,
+                                       float* acc, void* stream) {
+  int rc = check_grid2(g);
+  if (rc) return rc;
+  DNX_REQUIRE(g->dtype == DNX_F32 && g->te == 0, "tm_batch: float32 TM only");
+  DNX_REQUIRE(batch >= 1 && batch <= 65535 && state && n_steps >= 0,
+              "tm_batch: bad batch/state/steps");
+  if (src && (rc = check_sources(src, 1, 1, g->ny, g->nx, 0))) return rc;
+  DNX_REQUIRE(n_freq == 0 || (mbox && phase && acc && box_ok(*mbox, 1, g->ny, g->nx)),
+              "tm_batch: bad monitor");
+  Tm2Batch a{};
+  a.ny = g->ny;
+  a.nx = g->nx;
+  a.n = int64_t(g->ny) * g->nx;
+  a.kh = float(g->kh);
+  a.ce = static_cast<const float*>(g->ce);
+  a.by = static_cast<const float*>(g->by);
+  a.ay = static_cast<const float*>(g->ay);
+  a.bx = static_cast<const float*>(g->bx);
+  a.ax = static_cast<const float*>(g->ax);
+  a.st = static_cast<float*>(state);
+  a.has_src = src ? 1 : 0;
+  if (src) {
+    a.sbox = make_box(src->box);
+    a.sprof = static_cast<const float*>(src->profile);
+    a.wave = static_cast<const float*>(src->wave);
+  }
+  a.nf = n_freq;
+  if (n_freq > 0) {
+    a.mbox = make_box(*mbox);
+    a.phase = phase;
+    a.acc = acc;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid(cdiv(g->nx, BX2), cdiv(g->ny, BY2), batch), block(BX2, BY2);
+  for (int k = 0; k < n_steps; ++k) {
+    hipLaunchKernelGGL(k2tm_batch_h, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k2tm_batch_e, grid, block, 0, s, a, k);
+  }
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
 }

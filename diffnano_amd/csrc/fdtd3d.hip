@@ -22,6 +22,11 @@
 
 #include "fdtd3d_fused.h"
 
+#include <type_traits>
+
+// No multiply-add contraction (same rounding in every FDTD kernel variant).
+#pragma clang fp contract(off)
+
 namespace dnx {
 
 template <typename T>
@@ -297,6 +302,68 @@ static Fused3Args<T> fused_args(const dnx_grid3d* gd, const Fused3Tiles& t) {
   return a;
 }
 
+// Two-step kernels: float32, fused variant, at most one source (fused) and
+// FUSED2_MAX_MON monitors (fused), no probe.  DNX_FDTD3D_STEPS=1 forces the
+// one-step kernels.
+template <typename T>
+static bool use_pairs(const dnx_grid3d* gd, bool fused, int n_src, int n_mon, int64_t probe) {
+  if (!std::is_same<T, float>::value || !fused || gd->dtype != DNX_F32) return false;
+  if (n_src > 1 || n_mon > FUSED2_MAX_MON || probe >= 0) return false;
+  const char* v = getenv("DNX_FDTD3D_STEPS");
+  return !(v && strcmp(v, "1") == 0);
+}
+
+template <typename T>
+static Fused2Args<T> fused2_args(const dnx_grid3d* gd, const Fused3Tiles& t,
+                                 const dnx_source* srcs, int n_src, const dnx_monitor* mons,
+                                 int n_mon, T* side, int subs) {
+  Fused2Args<T> f{};
+  f.a = fused_args<T>(gd, t);
+  if (n_src > 0) {
+    f.a.src_comp = srcs[0].comp;
+    f.a.src_box = make_box(srcs[0].box);
+    f.a.src_prof = static_cast<const T*>(srcs[0].profile);
+    f.a.src_wave = static_cast<const T*>(srcs[0].wave);
+  }
+  f.n_mon = n_mon;
+  int64_t off = 0;
+  for (int j = 0; j < n_mon; ++j) {
+    FusedMon& m = f.mon[j];
+    m.box = make_box(mons[j].box);
+    m.mask = mons[j].comp_mask;
+    m.ncomp = (m.mask & 1) + ((m.mask >> 1) & 1) + ((m.mask >> 2) & 1);
+    f.side[j] = side + off;
+    off += int64_t(subs) * m.ncomp * m.box.cells();
+  }
+  return f;
+}
+
+// Elements of the compact monitor buffers of the two-step kernels.
+static int64_t side_elems(const dnx_monitor* mons, int n_mon, int subs) {
+  int64_t n = 0;
+  for (int j = 0; j < n_mon; ++j) {
+    const int mk = mons[j].comp_mask;
+    n += int64_t(subs) * ((mk & 1) + ((mk >> 1) & 1) + ((mk >> 2) & 1)) *
+         make_box(mons[j].box).cells();
+  }
+  return n;
+}
+
+// Stream-ordered scratch (the compact monitor buffers), released in stream
+// order when the call returns.
+struct StreamScratch {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  int alloc(size_t bytes, hipStream_t st) {
+    s = st;
+    DNX_CHECK_HIP(hipMallocAsync(&p, bytes, st));
+    return DNX_OK;
+  }
+  ~StreamScratch() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+};
+
 template <typename T>
 static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
                 const dnx_source* srcs, int n_src, const dnx_monitor* mons, int n_mon, T* ckpt,
@@ -317,12 +384,46 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
   T* cur = st;
   T* nxt = fused ? work : st;
   const dim3 grid = grid3_launch(gd), block(TX, TY);
+  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, probe);
+  Fused2Args<T> f2{};
+  int ty2 = 0;
+  StreamScratch side;
+  if (pairs) {
+    const Fused3Tiles t2 = fused3_tiles(gd->dtype, 2);
+    const int64_t se = side_elems(mons, n_mon, 2);
+    if (se > 0) {
+      int rc = side.alloc(sizeof(T) * se, s);
+      if (rc) return rc;
+    }
+    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, n_mon, static_cast<T*>(side.p), 2);
+    ty2 = t2.ty;
+  }
   for (int k = 0; k < n_steps; ++k) {
     const int n = n0 + k;
     T* slot = es ? es + int64_t(k) * 3 * N : nullptr;
     if (ckpt && k % ckpt_every == 0) {
       DNX_CHECK_HIP(hipMemcpyAsync(ckpt + int64_t(k / ckpt_every) * 6 * N, cur,
                                    sizeof(T) * 6 * N, hipMemcpyDeviceToDevice, s));
+    }
+    if (pairs && k + 1 < n_steps && !(ckpt && (k + 1) % ckpt_every == 0)) {
+      if constexpr (std::is_same<T, float>::value) {
+        f2.a.in = cur;
+        f2.a.out = nxt;
+        f2.a.es = slot;
+        f2.es2 = slot ? slot + 3 * N : nullptr;
+        f2.n = n;
+        TimedLaunch tl = timed_begin(s);
+        int rc = launch_fused2_fwd<T>(f2, ty2, s);
+        timed_end(slot ? "fdtd3d_fwd2_store" : "fdtd3d_fwd2", s, tl);
+        if (rc) return rc;
+        for (int j = 0; j < n_mon; ++j)
+          if ((rc = launch_monitor_pair<T>(f2.side[j], mons[j], n, s))) return rc;
+      }
+      T* t = cur;
+      cur = nxt;
+      nxt = t;
+      ++k;  // the pair covered steps k and k+1
+      continue;
     }
     if (fused) {
       fa.in = cur;
@@ -401,8 +502,56 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   T* cur = lam;
   T* nxt = fused ? work : lam;
   const dim3 grid = grid3_launch(gd), block(TX, TY);
+  // The two-step adjoint is exact but slower than two one-step launches on
+  // gfx950 (its state does not fit the register file at two waves per
+  // SIMD), so it runs only when DNX_FDTD3D_ADJ2=1.
+  const char* adj2 = getenv("DNX_FDTD3D_ADJ2");
+  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, -1) && adj2 && strcmp(adj2, "1") == 0;
+  Fused2Args<T> f2{};
+  int ty2 = 0;
+  StreamScratch side;
+  if (pairs) {
+    const Fused3Tiles t2 = fused3_tiles(gd->dtype, 3);
+    const int64_t se = side_elems(mons, n_mon, 1);
+    if (se > 0) {
+      int rc = side.alloc(sizeof(T) * se, s);
+      if (rc) return rc;
+    }
+    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, n_mon, static_cast<T*>(side.p), 1);
+    f2.a.S = S;
+    ty2 = t2.ty;
+  }
+  auto estep = [&](int mm) -> const T* {
+    return (mm == m_lo) ? e_lo : es + int64_t(mm - m_lo - 1) * 3 * N;
+  };
   for (int m = m_hi - 1; m >= m_lo && m >= 1; --m) {
-    const T* em = (m == m_lo) ? e_lo : es + int64_t(m - m_lo - 1) * 3 * N;
+    const T* em = estep(m);
+    if (pairs && m - 1 >= m_lo && m - 1 >= 1) {
+      if constexpr (std::is_same<T, float>::value) {
+        f2.a.in = cur;
+        f2.a.out = nxt;
+        f2.a.em = em;
+        f2.em2 = estep(m - 1);
+        f2.n = m;
+        int rc;
+        for (int j = 0; j < n_mon; ++j)
+          if ((rc = launch_monitor_d<T>(f2.side[j], mons[j], m - 1, s))) return rc;
+        TimedLaunch tl = timed_begin(s);
+        rc = launch_fused2_adj<T>(f2, ty2, s);
+        timed_end("fdtd3d_adj2", s, tl);
+        if (rc) return rc;
+        T* e2 = const_cast<T*>(f2.em2);
+        Comps<T> L2{{nxt, nxt + N, nxt + 2 * N}};
+        Comps<T> E2{{e2, e2 + N, e2 + 2 * N}};
+        for (int j = 0; j < n_mon; ++j)
+          if ((rc = launch_monitor_adj<T>(L2, E2, S, g.ny, g.nx, mons[j], m - 2, s))) return rc;
+      }
+      T* t = cur;
+      cur = nxt;
+      nxt = t;
+      --m;  // the pair covered steps m and m-1
+      continue;
+    }
     for (int j = j0; j < n_src; ++j) {
       int rc = launch_source_adj<T>(S, cur + srcs[j].comp * N, g.ny, g.nx, srcs[j], m, s);
       if (rc) return rc;

@@ -9,6 +9,9 @@
 
 #include "fdtd3d_fused.h"
 
+// No multiply-add contraction (same rounding in every FDTD kernel variant).
+#pragma clang fp contract(off)
+
 struct dnx_batch3d_plan {
   int dtype = 0, n = 0, T = 0, K = 0;
   int ty_f = 0, ty_a = 0, max_w = 0, grid_f = 0, grid_a = 0;

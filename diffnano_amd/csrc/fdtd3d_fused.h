@@ -83,6 +83,42 @@ struct Fused3Tiles {
 // set by dnx_fdtd3d_set_tiles.
 Fused3Tiles fused3_tiles(int dtype, int kernel);
 
+// ---- Two-step (temporally blocked) kernels --------------------------------
+// One launch advances two time steps: state (n) -> (n+2) for the forward,
+// lam^{m+1} -> lam^{m-1} for the adjoint, reading and writing the state
+// once.  One soft source is fused.  DFT monitors (at most FUSED2_MAX_MON)
+// go through compact side buffers so that no kernel waits on them: the
+// forward stores both sub-steps' monitor-box values ([2][ncomp][cells]),
+// accumulated afterwards by launch_monitor_pair; the adjoint reads the
+// intermediate sample's adjoint source precomputed by launch_monitor_d
+// ([ncomp][cells]) and adds it in-kernel (the next sub-step needs it); the
+// last sample's term is added by launch_monitor_adj after the launch.
+
+constexpr int FUSED2_MAX_MON = 1;
+
+struct FusedMon {
+  Box box;
+  int mask, ncomp;
+};
+
+template <typename T>
+struct Fused2Args {
+  Fused3Args<T> a;  // geometry, tiles, in/out, ce, source (a.src_wave = wave base);
+                    // forward: a.es = E store of sub-step 1; adjoint: a.em = E^m, a.S
+  int n;            // forward: first step index n; adjoint: m (steps m, m-1)
+  T* es2;           // forward: E store of sub-step 2 (or null)
+  const T* em2;     // adjoint: E^{m-1}
+  int n_mon;
+  FusedMon mon[FUSED2_MAX_MON];
+  T* side[FUSED2_MAX_MON];  // compact monitor buffers (see above)
+};
+
+// Two-step tile configuration (kernel 2 forward, 3 adjoint); float32 only.
+template <typename T>
+int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s);
+template <typename T>
+int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s);
+
 template <typename T>
 int launch_fused3_fwd(const Fused3Args<T>& a, int ty, hipStream_t s);
 

@@ -28,17 +28,23 @@
 
 #include <atomic>
 
+// Every FDTD kernel rounds each operation as written (no fused
+// multiply-add contraction), so the one-step, two-step, simple and batched
+// kernels produce bitwise the same fields whatever code surrounds them.
+#pragma clang fp contract(off)
+
 namespace dnx {
 
 // [dtype][kernel: 0 forward, 1 adjoint] -> (ty, zc).  Defaults from a sweep
 // at 256^3 on MI355X (float32: forward 4 x 32 ~5.0 TB/s algorithmic,
 // adjoint 2 x 16 ~4.6 TB/s); float64 uses ty = 2 to stay within registers.
-static std::atomic<int> g_ty[2][2] = {{{4}, {2}}, {{2}, {2}}};
-static std::atomic<int> g_zc[2][2] = {{{32}, {16}}, {{32}, {16}}};
+// Kernels 2 and 3 are the two-step forward and adjoint (float32 only).
+static std::atomic<int> g_ty[2][4] = {{{4}, {2}, {2}, {2}}, {{2}, {2}, {2}, {2}}};
+static std::atomic<int> g_zc[2][4] = {{{32}, {16}, {64}, {16}}, {{32}, {16}, {64}, {16}}};
 
 Fused3Tiles fused3_tiles(int dtype, int kernel) {
   const int d = dtype == DNX_F64 ? 1 : 0;
-  const int k = kernel ? 1 : 0;
+  const int k = (kernel >= 0 && kernel < 4) ? kernel : 0;
   return {g_ty[d][k].load(), g_zc[d][k].load()};
 }
 
@@ -758,8 +764,11 @@ extern "C" int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t t
                                         int32_t zc) {
   using namespace dnx;
   DNX_REQUIRE(dtype == DNX_F32 || dtype == DNX_F64, "set_tiles: bad dtype %d", dtype);
-  DNX_REQUIRE(kernel == 0 || kernel == 1, "set_tiles: kernel must be 0 (forward) or 1 (adjoint)");
-  DNX_REQUIRE(ty == 2 || ty == 4 || ty == 8, "set_tiles: ty must be 2, 4 or 8 (got %d)", ty);
+  DNX_REQUIRE(kernel >= 0 && kernel <= 3,
+              "set_tiles: kernel must be 0 (forward), 1 (adjoint), 2 (two-step forward) or 3 "
+              "(two-step adjoint)");
+  DNX_REQUIRE(ty == 2 || ty == 4 || (ty == 8 && kernel < 2),
+              "set_tiles: ty must be 2, 4 or 8 (2 or 4 for the two-step kernels; got %d)", ty);
   DNX_REQUIRE(zc >= 1 && zc <= 4096, "set_tiles: zc must be in [1, 4096] (got %d)", zc);
   const int d = dtype == DNX_F64 ? 1 : 0;
   g_ty[d][kernel] = ty;

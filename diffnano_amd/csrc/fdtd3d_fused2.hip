@@ -1,0 +1,913 @@
+// Two-step (temporally blocked) fused 3D Yee kernels on gfx950.
+//
+// The one-step fused kernels move 52 bytes per cell-update (forward) and
+// 72 (adjoint) and already run at the device's streaming bandwidth, so the
+// remaining lever is to touch the state once per two steps.  A work-group
+// owns TY full rows of a chunk of z-planes (one thread per x column, as in
+// the one-step kernels) and marches z upward computing
+//
+//   stage A at plane p:   H1(p) from E0(p), E0(p+1), H0(p)       rows -2..TY
+//                         E1(p) from H1(p), H1(p-1), E0(p)       rows -1..TY
+//   stage B at plane p-1: H2(p-1) from E1(p-1), E1(p), H1(p-1)   rows -1..TY-1
+//                         E2(p-1) from H2(p-1), H2(p-2), E1(p-1) rows  0..TY-1
+//
+// (rows relative to the tile).  The intermediate state (E1, H1) never
+// leaves the chip: its halo rows are recomputed by both neighbouring tiles
+// (bitwise the same arithmetic), its z-neighbours stay in registers, and
+// x-neighbours go through LDS: four exchanges per plane, two barriers
+// (E0 and E1 double-buffered by plane parity).  A chunk starting at z0 > 0
+// starts its march at z0 - 2; values of the planes below z0 that depend on
+// data the march has not seen are computed but never stored.
+//
+// Per sub-step the cell arithmetic is exactly that of the one-step kernels
+// (fdtd3d_fused.hip), including the fused soft source and the DFT monitor
+// accumulation, so the two-step path reproduces the one-step path bitwise.
+#include "fdtd3d_fused.h"
+
+// No multiply-add contraction: the same rounding as the one-step kernels.
+#pragma clang fp contract(off)
+
+namespace dnx {
+
+template <typename T>
+__device__ __forceinline__ bool in_box(const Box& b, int z, int y, int x, int64_t& i) {
+  const int dz = z - b.z0, dy = y - b.y0, dx = x - b.x0;
+  if (dz < 0 || dz >= b.nz || dy < 0 || dy >= b.ny || dx < 0 || dx >= b.nx) return false;
+  i = (int64_t(dz) * b.ny + dy) * b.nx + dx;
+  return true;
+}
+
+// Stage the monitor components of E for sub-step `sub` (0, 1) into the
+// compact buffer side = [2][ncomp][cells] (plain stores; launch_monitor_pair
+// accumulates the DFT afterwards).
+template <typename T>
+__device__ __forceinline__ void mon_stage(const FusedMon& m, T* side, int sub, int z, int y,
+                                          int x, const T (&e)[3]) {
+  int64_t i;
+  if (!in_box<T>(m.box, z, y, x, i)) return;
+  const int64_t cells = m.box.cells();
+  int ci = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!((m.mask >> c) & 1)) continue;
+    side[(int64_t(sub) * m.ncomp + ci) * cells + i] = e[c];
+    ++ci;
+  }
+}
+
+// A box membership test split by axis for the tile loops: the x part once
+// per thread, the y part once per row of the tile (rows r <-> y = yrow0 + r),
+// the z part once per plane, so the per-cell cost is one AND.
+template <int R>
+struct RowBox {
+  int dx;
+  bool yin[R];
+  int dy[R];
+  __device__ void init(const Box& b, int x, int yrow0, bool enabled) {
+    dx = x - b.x0;
+    const bool xin = enabled && dx >= 0 && dx < b.nx;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      dy[r] = yrow0 + r - b.y0;
+      yin[r] = xin && dy[r] >= 0 && dy[r] < b.ny;
+    }
+  }
+  __device__ static bool zin(const Box& b, int z, int& dz) {
+    dz = z - b.z0;
+    return dz >= 0 && dz < b.nz;
+  }
+  __device__ int64_t index(const Box& b, int dz, int r) const {
+    return (int64_t(dz) * b.ny + dy[r]) * b.nx + dx;
+  }
+};
+
+// e[comp] += w * prof[i] (the fused soft source, one component).
+template <typename T>
+__device__ __forceinline__ void add_src_at(int comp, T w, const T* prof, int64_t i, T (&e)[3]) {
+  const T s = w * prof[i];
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc)
+    if (cc == comp) e[cc] += s;
+}
+
+// e[comp] += s (the fused soft source with its profile value prefetched).
+template <typename T>
+__device__ __forceinline__ void add_src_val(int comp, T s, T (&e)[3]) {
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc)
+    if (cc == comp) e[cc] += s;
+}
+
+// Monitor staging at a known in-box cell index i (see mon_stage).
+template <typename T>
+__device__ __forceinline__ void mon_stage_at(const FusedMon& m, T* side, int sub, int64_t i,
+                                             const T (&e)[3]) {
+  const int64_t cells = m.box.cells();
+  int ci = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!((m.mask >> c) & 1)) continue;
+    side[(int64_t(sub) * m.ncomp + ci) * cells + i] = e[c];
+    ++ci;
+  }
+}
+
+// Soft source of step `step`: e[comp] += wave[step] * prof(z, y, x).
+template <typename T>
+__device__ __forceinline__ void add_source(const Fused3Args<T>& a, T w, int z, int y, int x,
+                                           T (&e)[3]) {
+  int64_t i;
+  if (!in_box<T>(a.src_box, z, y, x, i)) return;
+  const T s = w * a.src_prof[i];
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc)
+    if (cc == a.src_comp) e[cc] += s;
+}
+
+template <typename T, int TY>
+__device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
+                                          T* sm) {
+  const Fused3Args<T>& a = A2.a;
+  const int W = a.nx;
+  constexpr int R = TY + 4;  // rows r = 0 .. TY+3  <->  y = y0 - 2 + r
+  T* sA = sm;                           // E0 (Ey, Ez), rows 0..TY+2, by parity
+  T* sB = sA + 4 * (TY + 3) * W;        // H1 (Hy, Hz), rows 1..TY+2
+  T* sC = sB + 2 * (TY + 2) * W;        // E1 (Ey, Ez), rows 1..TY+1, by parity
+  T* sD = sC + 4 * (TY + 1) * W;        // H2 (Hy, Hz), rows 2..TY+1
+#define LA(p, c, r) sA[(((p) * 2 + (c)) * (TY + 3) + (r)) * W]
+#define LB(c, r) sB[((c) * (TY + 2) + (r) - 1) * W]
+#define LC(p, c, r) sC[(((p) * 2 + (c)) * (TY + 1) + (r) - 1) * W]
+#define LD(c, r) sD[((c) * TY + (r) - 2) * W]
+
+  const int z1 = min(a.nz, z0 + a.zc);
+  const int x = threadIdx.x;
+  const bool xv = x < W;
+  const int64_t N = a.N, P = a.P;
+  const T* __restrict__ Ein = a.in;
+  const T* __restrict__ Hin = a.in + 3 * N;
+  T* __restrict__ Eout = a.out;
+  T* __restrict__ Hout = a.out + 3 * N;
+
+  const T kh = a.kh;
+  const T bxv = xv ? a.bx[x] : T(0);
+  const T axv = xv ? a.ax[x] : T(0);
+  const bool xf = xv && x >= 1, xl = x + 1 < W;
+
+  T byr[R], ayr[R];
+  bool rv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int y = y0 - 2 + r;
+    rv[r] = y >= 0 && y < a.ny;
+    byr[r] = rv[r] ? a.by[y] : T(0);
+    ayr[r] = rv[r] ? a.ay[y] : T(0);
+  }
+  const int64_t colbase = int64_t(y0 - 2) * W + x;  // row r at colbase + r*W
+
+  const bool has_src = a.src_comp >= 0;
+  const T w1 = has_src ? a.src_wave[A2.n] : T(0);
+  const T w2 = has_src ? a.src_wave[A2.n + 1] : T(0);
+  RowBox<R> sbx, mbx;  // source and monitor box tests
+  sbx.init(a.src_box, x, y0 - 2, has_src && xv);
+  mbx.init(A2.mon[0].box, x, y0 - 2, A2.n_mon > 0 && xv);
+
+  T e0C[3][R], e0N[3][R];
+  T h1F[3][TY + 2];  // H1 at plane p-1, rows 1..TY+2 (index r-1)
+  T e1Q[3][TY + 2];  // E1 at plane p-1, rows 1..TY+2 (index r-1)
+  T h2P[2][TY];      // H2 (x, y) at plane p-2, rows 2..TY+1 (index r-2)
+  T ceQ[TY];         // ce at plane p-1, rows 2..TY+1 (index r-2)
+  T spQ[TY];         // source profile at plane p-1, rows 2..TY+1 (0 outside the box)
+  const int zs = z0 >= 2 ? z0 - 2 : 0;
+  {
+    const int64_t pb = int64_t(zs) * P + colbase;
+#pragma unroll
+    for (int r = 0; r < TY; ++r) spQ[r] = T(0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) e0C[c][r] = (rv[r] && xv) ? Ein[c * N + pb + r * W] : T(0);
+    }
+    if (xv) {
+#pragma unroll
+      for (int r = 0; r <= TY + 2; ++r) {
+        (&LA(zs & 1, 0, r))[x] = e0C[1][r];
+        (&LA(zs & 1, 1, r))[x] = e0C[2][r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) h1F[c][r] = e1Q[c][r] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < TY; ++r) h2P[0][r] = h2P[1][r] = ceQ[r] = T(0);
+  }
+  __syncthreads();
+
+#pragma unroll 1
+  for (int p = zs; p <= z1; ++p) {
+    const bool doA = p < a.nz;       // stage A at plane p
+    const bool doB = p - 1 >= zs;    // stage B at plane q = p - 1
+    const int q = p - 1;
+    const bool zl = p + 1 < a.nz, zf = p >= 1;
+    T h1N[3][TY + 3];  // H1 at plane p, rows 0..TY+2
+    T e1P[3][TY + 2];  // E1 at plane p, rows 1..TY+2 (index r-1)
+    T ceP[TY + 2];     // ce at plane p, rows 1..TY+2 (index r-1)
+    T spP[TY + 2];     // source profile at plane p, rows 1..TY+2 (0 outside the box)
+    // ---------------- stage A, part 1: loads and H1(p)
+    if (doA) {
+      T h0[3][TY + 3];
+      const int64_t pn = int64_t(p + 1) * P + colbase;
+      const int64_t pz = int64_t(p) * P + colbase;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          e0N[c][r] = (zl && rv[r] && xv) ? Ein[c * N + pn + r * W] : T(0);
+      }
+#pragma unroll
+      for (int r = 0; r <= TY + 2; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h0[c][r] = (rv[r] && xv) ? Hin[c * N + pz + r * W] : T(0);
+      }
+#pragma unroll
+      for (int r = 1; r <= TY + 2; ++r) ceP[r - 1] = (rv[r] && xv) ? a.ce[pz + r * W] : T(0);
+      {
+        int dzs;
+        const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
+#pragma unroll
+        for (int r = 1; r <= TY + 2; ++r)
+          spP[r - 1] = (sz && sbx.yin[r]) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
+      }
+      const int pb = p & 1;
+#pragma unroll
+      for (int r = 0; r <= TY + 2; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yl = y + 1 < a.ny;
+        const T ex = e0C[0][r], ey = e0C[1][r], ez = e0C[2][r];
+        const T ey_x1 = xl ? (&LA(pb, 0, r))[x + 1] : T(0);
+        const T ez_x1 = xl ? (&LA(pb, 1, r))[x + 1] : T(0);
+        const T dEz_dy = yl ? e0C[2][r + 1] - ez : T(0);
+        const T dEx_dy = yl ? e0C[0][r + 1] - ex : T(0);
+        const T dEy_dz = zl ? e0N[1][r] - ey : T(0);
+        const T dEx_dz = zl ? e0N[0][r] - ex : T(0);
+        const T dEz_dx = xl ? ez_x1 - ez : T(0);
+        const T dEy_dx = xl ? ey_x1 - ey : T(0);
+        h1N[0][r] = byr[r] * h0[0][r] - kh * (ayr[r] * dEz_dy - axv * dEy_dz);
+        h1N[1][r] = bxv * h0[1][r] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
+        h1N[2][r] = bxv * h0[2][r] - kh * (axv * dEy_dx - ayr[r] * dEx_dy);
+      }
+      if (xv) {
+#pragma unroll
+        for (int r = 1; r <= TY + 2; ++r) {
+          (&LB(0, r))[x] = h1N[1][r];
+          (&LB(1, r))[x] = h1N[2][r];
+        }
+#pragma unroll
+        for (int r = 0; r <= TY + 2; ++r) {
+          (&LA(pb ^ 1, 0, r))[x] = e0N[1][r];
+          (&LA(pb ^ 1, 1, r))[x] = e0N[2][r];
+        }
+      }
+    }
+    __syncthreads();
+    // ---------------- stage A, part 2: E1(p) (all rows the tile needs)
+    if (doA) {
+      const bool own = p >= z0 && p < z1;
+      const int64_t pz = int64_t(p) * P + colbase;
+      int dzsA, dzmA;
+      const bool szA = RowBox<R>::zin(a.src_box, p, dzsA);
+      const bool mzA = RowBox<R>::zin(A2.mon[0].box, p, dzmA);
+      (void)dzsA;
+#pragma unroll
+      for (int r = 1; r <= TY + 2; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yf = y >= 1;
+        const T hx = h1N[0][r], hy = h1N[1][r], hz = h1N[2][r];
+        const T hy_x0 = xf ? (&LB(0, r))[x - 1] : T(0);
+        const T hz_x0 = xf ? (&LB(1, r))[x - 1] : T(0);
+        const T dHz_dy = yf ? hz - h1N[2][r - 1] : T(0);
+        const T dHx_dy = yf ? hx - h1N[0][r - 1] : T(0);
+        const T dHy_dz = zf ? hy - h1F[1][r - 1] : T(0);
+        const T dHx_dz = zf ? hx - h1F[0][r - 1] : T(0);
+        const T dHz_dx = xf ? hz - hz_x0 : T(0);
+        const T dHy_dx = xf ? hy - hy_x0 : T(0);
+        const T c = ceP[r - 1];
+        const T bxy = bxv * byr[r];
+        T e[3];
+        e[0] = bxy * e0C[0][r] + c * (ayr[r] * dHz_dy - axv * dHy_dz);
+        e[1] = bxy * e0C[1][r] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
+        e[2] = bxy * e0C[2][r] + c * (axv * dHy_dx - ayr[r] * dHx_dy);
+        if (szA && sbx.yin[r]) add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) e1P[cc][r - 1] = e[cc];
+        if (own && r >= 2 && r <= TY + 1 && rv[r] && xv) {
+          if (a.es) {
+            const int64_t i = pz + r * W;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) a.es[cc * N + i] = e[cc];
+          }
+          if (mzA && mbx.yin[r])
+            mon_stage_at<T>(A2.mon[0], A2.side[0], 0, mbx.index(A2.mon[0].box, dzmA, r), e);
+        }
+      }
+    }
+    // ---------------- stage B, part 1: H2(q)
+    T h2N[3][TY + 1];  // H2 at plane q, rows 1..TY+1 (index r-1)
+    const bool zlq = q + 1 < a.nz, zfq = q >= 1;
+    if (doB) {
+      const int qb = q & 1;
+#pragma unroll
+      for (int r = 1; r <= TY + 1; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yl = y + 1 < a.ny;
+        const T ex = e1Q[0][r - 1], ey = e1Q[1][r - 1], ez = e1Q[2][r - 1];
+        const T ey_x1 = xl ? (&LC(qb, 0, r))[x + 1] : T(0);
+        const T ez_x1 = xl ? (&LC(qb, 1, r))[x + 1] : T(0);
+        const T dEz_dy = yl ? e1Q[2][r] - ez : T(0);
+        const T dEx_dy = yl ? e1Q[0][r] - ex : T(0);
+        const T dEy_dz = zlq ? e1P[1][r - 1] - ey : T(0);
+        const T dEx_dz = zlq ? e1P[0][r - 1] - ex : T(0);
+        const T dEz_dx = xl ? ez_x1 - ez : T(0);
+        const T dEy_dx = xl ? ey_x1 - ey : T(0);
+        h2N[0][r - 1] = byr[r] * h1F[0][r - 1] - kh * (ayr[r] * dEz_dy - axv * dEy_dz);
+        h2N[1][r - 1] = bxv * h1F[1][r - 1] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
+        h2N[2][r - 1] = bxv * h1F[2][r - 1] - kh * (axv * dEy_dx - ayr[r] * dEx_dy);
+      }
+      if (xv) {
+#pragma unroll
+        for (int r = 2; r <= TY + 1; ++r) {
+          (&LD(0, r))[x] = h2N[1][r - 1];
+          (&LD(1, r))[x] = h2N[2][r - 1];
+        }
+      }
+    }
+    if (doA && xv) {
+#pragma unroll
+      for (int r = 1; r <= TY + 1; ++r) {
+        (&LC(p & 1, 0, r))[x] = e1P[1][r - 1];
+        (&LC(p & 1, 1, r))[x] = e1P[2][r - 1];
+      }
+    }
+    __syncthreads();
+    // ---------------- stage B, part 2: E2(q), store
+    if (doB) {
+      const bool own = q >= z0;
+      const int64_t pz = int64_t(q) * P + colbase;
+      int dzsB, dzmB;
+      const bool szB = RowBox<R>::zin(a.src_box, q, dzsB);
+      const bool mzB = RowBox<R>::zin(A2.mon[0].box, q, dzmB);
+      (void)dzsB;
+#pragma unroll
+      for (int r = 2; r <= TY + 1; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yf = y >= 1;
+        const T hx = h2N[0][r - 1], hy = h2N[1][r - 1], hz = h2N[2][r - 1];
+        const T hy_x0 = xf ? (&LD(0, r))[x - 1] : T(0);
+        const T hz_x0 = xf ? (&LD(1, r))[x - 1] : T(0);
+        const T dHz_dy = yf ? hz - h2N[2][r - 2] : T(0);
+        const T dHx_dy = yf ? hx - h2N[0][r - 2] : T(0);
+        const T dHy_dz = zfq ? hy - h2P[1][r - 2] : T(0);
+        const T dHx_dz = zfq ? hx - h2P[0][r - 2] : T(0);
+        const T dHz_dx = xf ? hz - hz_x0 : T(0);
+        const T dHy_dx = xf ? hy - hy_x0 : T(0);
+        const T c = ceQ[r - 2];
+        const T bxy = bxv * byr[r];
+        T e[3];
+        e[0] = bxy * e1Q[0][r - 1] + c * (ayr[r] * dHz_dy - axv * dHy_dz);
+        e[1] = bxy * e1Q[1][r - 1] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
+        e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ayr[r] * dHx_dy);
+        if (szB && sbx.yin[r]) add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
+        if (own && rv[r] && xv) {
+          const int64_t i = pz + r * W;
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            Eout[cc * N + i] = e[cc];
+            Hout[cc * N + i] = h2N[cc][r - 1];
+          }
+          if (A2.es2) {
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) A2.es2[cc * N + i] = e[cc];
+          }
+          if (mzB && mbx.yin[r])
+            mon_stage_at<T>(A2.mon[0], A2.side[0], 1, mbx.index(A2.mon[0].box, dzmB, r), e);
+        }
+      }
+#pragma unroll
+      for (int r = 2; r <= TY + 1; ++r) {
+        h2P[0][r - 2] = h2N[0][r - 1];
+        h2P[1][r - 2] = h2N[1][r - 1];
+      }
+    }
+    // ---------------- advance the march
+    if (doA) {
+#pragma unroll
+      for (int r = 1; r <= TY + 2; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          h1F[c][r - 1] = h1N[c][r];
+          e1Q[c][r - 1] = e1P[c][r - 1];
+        }
+      }
+#pragma unroll
+      for (int r = 2; r <= TY + 1; ++r) {
+        ceQ[r - 2] = ceP[r - 1];
+        spQ[r - 2] = spP[r - 1];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e0C[c][r] = e0N[c][r];
+      }
+    }
+  }
+#undef LA
+#undef LB
+#undef LC
+#undef LD
+}
+
+template <int TY>
+__device__ inline void block_tile2(int nblocks, int ty_tiles, int zc, int& y0, int& z0,
+                                   bool& live) {
+  const int nb = gridDim.x;  // multiple of 8
+  const int b = blockIdx.x;
+  const int logical = (b & 7) * (nb >> 3) + (b >> 3);
+  live = logical < nblocks;
+  y0 = (logical % ty_tiles) * TY;
+  z0 = (logical / ty_tiles) * zc;
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256, TY == 2 ? 2 : 1) void k3_fused2_fwd(Fused2Args<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  int y0, z0;
+  bool live;
+  block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
+  if (!live) return;
+  fwd2_tile<T, TY>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+}
+
+template <int TY>
+constexpr int fwd2_lds_elems() {
+  return 4 * (TY + 3) + 2 * (TY + 2) + 4 * (TY + 1) + 2 * TY;
+}
+
+template <typename T, int TY>
+static int launch_fwd2_ty(const Fused2Args<T>& a, hipStream_t s) {
+  const int threads = ((a.a.nx + 63) / 64) * 64;
+  const int nb = ((a.a.nblocks + 7) / 8) * 8;
+  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * a.a.nx;
+  static bool attr_set = false;
+  if (!attr_set) {
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k3_fused2_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, a);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
+  if (ty == 4) return launch_fwd2_ty<T, 4>(a, s);
+  if (ty == 2) return launch_fwd2_ty<T, 2>(a, s);
+  set_error("fused2: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template int launch_fused2_fwd<float>(const Fused2Args<float>&, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Adjoint, two steps: lam^{m+1} -> lam^m -> lam^{m-1}.  Same march and row
+// sets as the forward, with u = ce lamE in the role of E, muH in the role
+// of H.  Per sub-step the arithmetic is that of adj_tile followed by the
+// monitor adjoint (k_monitor_adj): lamE^m += Re(g e^{i phi}) over the
+// monitor box (on every row the tile computes, halos included, since the
+// next sub-step reads them) and S += E^m . d on owned cells.  S is read in
+// stage A and written in stage B, accumulating in the one-step order.
+// ---------------------------------------------------------------------------
+
+// Prefetch of the precomputed adjoint source d = [ncomp][cells] of monitor
+// m at cell (z, y, x): dv[c] for the components in the mask (0 elsewhere);
+// returns whether the cell is in the box.
+template <typename T>
+__device__ __forceinline__ bool mon_fetch(const FusedMon& m, const T* d, int z, int y, int x,
+                                          T (&dv)[3]) {
+  int64_t i;
+  dv[0] = dv[1] = dv[2] = T(0);
+  if (!in_box<T>(m.box, z, y, x, i)) return false;
+  const int64_t cells = m.box.cells();
+  int ci = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!((m.mask >> c) & 1)) continue;
+    dv[c] = d[int64_t(ci) * cells + i];
+    ++ci;
+  }
+  return true;
+}
+
+// k_monitor_adj with d prefetched: lam_c += d_c; returns sum_c em_c d_c in
+// the same order.
+template <typename T>
+__device__ __forceinline__ T mon_apply(int mask, const T (&dv)[3], T (&lam)[3],
+                                       const T (&em)[3]) {
+  T sacc = T(0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!((mask >> c) & 1)) continue;
+    lam[c] += dv[c];
+    sacc += em[c] * dv[c];
+  }
+  return sacc;
+}
+
+template <typename T, int TY>
+__device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
+                                          T* sm) {
+  const Fused3Args<T>& a = A2.a;
+  const int W = a.nx;
+  constexpr int R = TY + 4;  // rows r = 0 .. TY+3  <->  y = y0 - 2 + r
+  T* sA = sm;                           // u0 (uy, uz), rows 0..TY+2, by parity
+  T* sB = sA + 4 * (TY + 3) * W;        // mu1 (y, z), rows 1..TY+2
+  T* sC = sB + 2 * (TY + 2) * W;        // u1 (uy, uz), rows 1..TY+1, by parity
+  T* sD = sC + 4 * (TY + 1) * W;        // mu2 (y, z), rows 2..TY+1
+#define LA(p, c, r) sA[(((p) * 2 + (c)) * (TY + 3) + (r)) * W]
+#define LB(c, r) sB[((c) * (TY + 2) + (r) - 1) * W]
+#define LC(p, c, r) sC[(((p) * 2 + (c)) * (TY + 1) + (r) - 1) * W]
+#define LD(c, r) sD[((c) * TY + (r) - 2) * W]
+
+  const int z1 = min(a.nz, z0 + a.zc);
+  const int x = threadIdx.x;
+  const bool xv = x < W;
+  const int64_t N = a.N, P = a.P;
+  const T* __restrict__ Lin = a.in;
+  const T* __restrict__ Min = a.in + 3 * N;
+  T* __restrict__ Lout = a.out;
+  T* __restrict__ Mout = a.out + 3 * N;
+
+  const T kh = a.kh;
+  const T bxv = xv ? a.bx[x] : T(0);
+  const T axv = xv ? a.ax[x] : T(0);
+  const T axp = (x + 1 < W) ? a.ax[x + 1] : T(0);
+  const T axm = (xv && x >= 1) ? a.ax[x - 1] : T(0);
+  const bool xf = xv && x >= 1, xl = x + 1 < W, xin = x <= W - 2;
+
+  T byr[R], ayr[R];
+  bool rv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int y = y0 - 2 + r;
+    rv[r] = y >= 0 && y < a.ny;
+    byr[r] = rv[r] ? a.by[y] : T(0);
+    ayr[r] = rv[r] ? a.ay[y] : T(0);
+  }
+  const int64_t colbase = int64_t(y0 - 2) * W + x;
+
+  const int m = A2.n;  // sub-steps m (stage A) and m-1 (stage B)
+  const bool has_src = a.src_comp >= 0;
+  const T w1 = has_src ? a.src_wave[m] : T(0);
+  const T w2 = has_src ? a.src_wave[m - 1] : T(0);
+
+  T uC[3][R], uN[3][R];   // u = ce lamE^{m+1} at planes p, p+1
+  T lC[3][TY + 2], lN[3][TY + 2];  // raw lamE^{m+1}, rows 1..TY+2 (index r-1)
+  T ceC[TY + 2], ceN[TY + 2];      // ce at planes p, p+1, rows 1..TY+2
+  T m1F[3][TY + 2];       // mu^m at plane p-1, rows 1..TY+2
+  T l1Q[3][TY + 2];       // lamE^m (after the monitor term) at plane p-1, rows 1..TY+2
+  T u1Q[3][TY + 2];       // ce lamE^m at plane p-1
+  T m2P[2][TY];           // mu^{m-1} (x, y) at plane p-2, rows 2..TY+1
+  T sQ[TY];               // S of plane p-1 after stage A (owned rows 2..TY+1)
+  const int zs = z0 >= 2 ? z0 - 2 : 0;
+  {
+    const int64_t pb = int64_t(zs) * P + colbase;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool ok = rv[r] && xv;
+      const T c = ok ? a.ce[pb + r * W] : T(0);
+      if (r >= 1 && r <= TY + 2) ceC[r - 1] = c;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T l = ok ? Lin[k * N + pb + r * W] : T(0);
+        uC[k][r] = c * l;
+        if (r >= 1 && r <= TY + 2) lC[k][r - 1] = l;
+      }
+    }
+    if (xv) {
+#pragma unroll
+      for (int r = 0; r <= TY + 2; ++r) {
+        (&LA(zs & 1, 0, r))[x] = uC[1][r];
+        (&LA(zs & 1, 1, r))[x] = uC[2][r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) m1F[k][r] = l1Q[k][r] = u1Q[k][r] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < TY; ++r) m2P[0][r] = m2P[1][r] = sQ[r] = T(0);
+  }
+  __syncthreads();
+
+#pragma unroll 1
+  for (int p = zs; p <= z1; ++p) {
+    const bool doA = p < a.nz;
+    const bool doB = p - 1 >= zs;
+    const int q = p - 1;
+    const bool zl = p + 1 < a.nz, zf = p >= 1, zin = p <= a.nz - 2;
+    T m1N[3][TY + 3];  // mu^m at plane p, rows 0..TY+2
+    T l1P[3][TY + 2];  // lamE^m at plane p, rows 1..TY+2 (index r-1)
+    T u1P[3][TY + 2];
+    T sP[TY];
+    T dpre[FUSED2_MAX_MON][TY + 2][3];  // monitor adjoint source, rows 1..TY+2
+    bool dhit[FUSED2_MAX_MON][TY + 2];
+    // ---------------- stage A, part 1: loads and mu^m(p)
+    if (doA) {
+      T m0[3][TY + 3];
+      const int64_t pn = int64_t(p + 1) * P + colbase;
+      const int64_t pz = int64_t(p) * P + colbase;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const bool ok = zl && rv[r] && xv;
+        const T c = ok ? a.ce[pn + r * W] : T(0);
+        if (r >= 1 && r <= TY + 2) ceN[r - 1] = c;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const T l = ok ? Lin[k * N + pn + r * W] : T(0);
+          uN[k][r] = c * l;
+          if (r >= 1 && r <= TY + 2) lN[k][r - 1] = l;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r <= TY + 2; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) m0[k][r] = (rv[r] && xv) ? Min[k * N + pz + r * W] : T(0);
+      }
+      // adjoint source of sample m-1 for lamE^m(p), rows 1..TY+2 (issued
+      // with the field loads, consumed after the barrier)
+#pragma unroll
+      for (int j = 0; j < FUSED2_MAX_MON; ++j) {
+#pragma unroll
+        for (int r = 1; r <= TY + 2; ++r) {
+          dhit[j][r - 1] = j < A2.n_mon && rv[r] && xv &&
+                           mon_fetch<T>(A2.mon[j], A2.side[j], p, y0 - 2 + r, x, dpre[j][r - 1]);
+        }
+      }
+      const int pb = p & 1;
+#pragma unroll
+      for (int r = 0; r <= TY + 2; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yf = y >= 1, yl = y + 1 < a.ny;
+        const T ux = uC[0][r], uy = uC[1][r], uz = uC[2][r];
+        const T uy_x1 = xl ? (&LA(pb, 0, r))[x + 1] : T(0);
+        const T uz_x1 = xl ? (&LA(pb, 1, r))[x + 1] : T(0);
+        const T tz_y = (zf ? uy : T(0)) - (zl ? uN[1][r] : T(0));
+        const T tz_x = (zf ? ux : T(0)) - (zl ? uN[0][r] : T(0));
+        const T ty_z = (yf ? ayr[r] * uz : T(0)) - (yl ? ayr[r + 1] * uC[2][r + 1] : T(0));
+        const T ty_x = (yf ? ayr[r] * ux : T(0)) - (yl ? ayr[r + 1] * uC[0][r + 1] : T(0));
+        const T tx_z = (xf ? axv * uz : T(0)) - (xl ? axp * uz_x1 : T(0));
+        const T tx_y = (xf ? uy : T(0)) - (xl ? uy_x1 : T(0));
+        m1N[0][r] = byr[r] * m0[0][r] + axv * tz_y - ty_z;
+        m1N[1][r] = bxv * m0[1][r] + tx_z - axv * tz_x;
+        m1N[2][r] = bxv * m0[2][r] + ty_x - ayr[r] * tx_y;
+      }
+      if (xv) {
+#pragma unroll
+        for (int r = 1; r <= TY + 2; ++r) {
+          (&LB(0, r))[x] = m1N[1][r];
+          (&LB(1, r))[x] = m1N[2][r];
+        }
+#pragma unroll
+        for (int r = 0; r <= TY + 2; ++r) {
+          (&LA(pb ^ 1, 0, r))[x] = uN[1][r];
+          (&LA(pb ^ 1, 1, r))[x] = uN[2][r];
+        }
+      }
+    }
+    __syncthreads();
+    // ---------------- stage A, part 2: lamE^m(p), S partial, monitor term
+    if (doA) {
+      const bool own = p >= z0 && p < z1;
+      const int64_t pz = int64_t(p) * P + colbase;
+#pragma unroll
+      for (int r = 1; r <= TY + 2; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yf = y >= 1, yin = y <= a.ny - 2;
+        const bool mine = own && r >= 2 && r <= TY + 1 && rv[r] && xv;
+        const T mx = m1N[0][r], my = m1N[1][r], mz = m1N[2][r];
+        const T my_x0 = xf ? (&LB(0, r))[x - 1] : T(0);
+        const T mz_x0 = xf ? (&LB(1, r))[x - 1] : T(0);
+        const T y_mz = (yf ? ayr[r - 1] * m1N[2][r - 1] : T(0)) - (yin ? ayr[r] * mz : T(0));
+        const T y_mx = (yf ? ayr[r - 1] * m1N[0][r - 1] : T(0)) - (yin ? ayr[r] * mx : T(0));
+        const T z_my = (zf ? m1F[1][r - 1] : T(0)) - (zin ? my : T(0));
+        const T z_mx = (zf ? m1F[0][r - 1] : T(0)) - (zin ? mx : T(0));
+        const T x_mz = (xf ? axm * mz_x0 : T(0)) - (xin ? axv * mz : T(0));
+        const T x_my = (xf ? my_x0 : T(0)) - (xin ? my : T(0));
+        T t[3];
+        t[0] = kh * (y_mz - axv * z_my);
+        t[1] = kh * (axv * z_mx - x_mz);
+        t[2] = kh * (ayr[r] * x_my - y_mx);
+        const T bxy = bxv * byr[r];
+        T l[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) l[k] = bxy * lC[k][r - 1] + t[k];
+        T em[3] = {T(0), T(0), T(0)};
+        T s = T(0);
+        if (mine) {
+          const int64_t i = pz + r * W;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) em[k] = a.em[k * N + i];
+          s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
+          if (has_src) {
+            int64_t si;
+            if (in_box<T>(a.src_box, p, y, x, si)) {
+              const T w = w1 * a.src_prof[si];
+#pragma unroll
+              for (int cc = 0; cc < 3; ++cc)
+                if (cc == a.src_comp) s -= w * lC[cc][r - 1];
+            }
+          }
+          s = a.S[i] + s;
+        }
+#pragma unroll
+        for (int j = 0; j < FUSED2_MAX_MON; ++j) {
+          if (j < A2.n_mon && dhit[j][r - 1]) {
+            const T sacc = mon_apply<T>(A2.mon[j].mask, dpre[j][r - 1], l, em);
+            if (mine) s += sacc;
+          }
+        }
+        if (r >= 2 && r <= TY + 1) sP[r - 2] = s;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          l1P[k][r - 1] = l[k];
+          u1P[k][r - 1] = ceC[r - 1] * l[k];
+        }
+      }
+    }
+    // ---------------- stage B, part 1: mu^{m-1}(q)
+    T m2N[3][TY + 1];  // rows 1..TY+1 (index r-1)
+    const bool zlq = q + 1 < a.nz, zfq = q >= 1, zinq = q <= a.nz - 2;
+    if (doB) {
+      const int qb = q & 1;
+#pragma unroll
+      for (int r = 1; r <= TY + 1; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yf = y >= 1, yl = y + 1 < a.ny;
+        const T ux = u1Q[0][r - 1], uy = u1Q[1][r - 1], uz = u1Q[2][r - 1];
+        const T uy_x1 = xl ? (&LC(qb, 0, r))[x + 1] : T(0);
+        const T uz_x1 = xl ? (&LC(qb, 1, r))[x + 1] : T(0);
+        const T tz_y = (zfq ? uy : T(0)) - (zlq ? u1P[1][r - 1] : T(0));
+        const T tz_x = (zfq ? ux : T(0)) - (zlq ? u1P[0][r - 1] : T(0));
+        const T ty_z = (yf ? ayr[r] * uz : T(0)) - (yl ? ayr[r + 1] * u1Q[2][r] : T(0));
+        const T ty_x = (yf ? ayr[r] * ux : T(0)) - (yl ? ayr[r + 1] * u1Q[0][r] : T(0));
+        const T tx_z = (xf ? axv * uz : T(0)) - (xl ? axp * uz_x1 : T(0));
+        const T tx_y = (xf ? uy : T(0)) - (xl ? uy_x1 : T(0));
+        m2N[0][r - 1] = byr[r] * m1F[0][r - 1] + axv * tz_y - ty_z;
+        m2N[1][r - 1] = bxv * m1F[1][r - 1] + tx_z - axv * tz_x;
+        m2N[2][r - 1] = bxv * m1F[2][r - 1] + ty_x - ayr[r] * tx_y;
+      }
+      if (xv) {
+#pragma unroll
+        for (int r = 2; r <= TY + 1; ++r) {
+          (&LD(0, r))[x] = m2N[1][r - 1];
+          (&LD(1, r))[x] = m2N[2][r - 1];
+        }
+      }
+    }
+    if (doA && xv) {
+#pragma unroll
+      for (int r = 1; r <= TY + 1; ++r) {
+        (&LC(p & 1, 0, r))[x] = u1P[1][r - 1];
+        (&LC(p & 1, 1, r))[x] = u1P[2][r - 1];
+      }
+    }
+    __syncthreads();
+    // ---------------- stage B, part 2: lamE^{m-1}(q), S, store
+    if (doB) {
+      const bool own = q >= z0;
+      const int64_t pz = int64_t(q) * P + colbase;
+#pragma unroll
+      for (int r = 2; r <= TY + 1; ++r) {
+        const int y = y0 - 2 + r;
+        const bool yf = y >= 1, yin = y <= a.ny - 2;
+        const bool mine = own && rv[r] && xv;
+        const T mx = m2N[0][r - 1], my = m2N[1][r - 1], mz = m2N[2][r - 1];
+        const T my_x0 = xf ? (&LD(0, r))[x - 1] : T(0);
+        const T mz_x0 = xf ? (&LD(1, r))[x - 1] : T(0);
+        const T y_mz = (yf ? ayr[r - 1] * m2N[2][r - 2] : T(0)) - (yin ? ayr[r] * mz : T(0));
+        const T y_mx = (yf ? ayr[r - 1] * m2N[0][r - 2] : T(0)) - (yin ? ayr[r] * mx : T(0));
+        const T z_my = (zfq ? m2P[1][r - 2] : T(0)) - (zinq ? my : T(0));
+        const T z_mx = (zfq ? m2P[0][r - 2] : T(0)) - (zinq ? mx : T(0));
+        const T x_mz = (xf ? axm * mz_x0 : T(0)) - (xin ? axv * mz : T(0));
+        const T x_my = (xf ? my_x0 : T(0)) - (xin ? my : T(0));
+        T t[3];
+        t[0] = kh * (y_mz - axv * z_my);
+        t[1] = kh * (axv * z_mx - x_mz);
+        t[2] = kh * (ayr[r] * x_my - y_mx);
+        const T bxy = bxv * byr[r];
+        T l[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) l[k] = bxy * l1Q[k][r - 1] + t[k];
+        if (mine) {
+          const int64_t i = pz + r * W;
+          T em[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) em[k] = A2.em2[k * N + i];
+          T s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
+          if (has_src) {
+            int64_t si;
+            if (in_box<T>(a.src_box, q, y, x, si)) {
+              const T w = w2 * a.src_prof[si];
+#pragma unroll
+              for (int cc = 0; cc < 3; ++cc)
+                if (cc == a.src_comp) s -= w * l1Q[cc][r - 1];
+            }
+          }
+          s = sQ[r - 2] + s;  // the last sample's monitor term follows the launch
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            Lout[k * N + i] = l[k];
+            Mout[k * N + i] = m2N[k][r - 1];
+          }
+          a.S[i] = s;
+        }
+      }
+#pragma unroll
+      for (int r = 2; r <= TY + 1; ++r) {
+        m2P[0][r - 2] = m2N[0][r - 1];
+        m2P[1][r - 2] = m2N[1][r - 1];
+      }
+    }
+    // ---------------- advance the march
+    if (doA) {
+#pragma unroll
+      for (int r = 1; r <= TY + 2; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          m1F[k][r - 1] = m1N[k][r];
+          l1Q[k][r - 1] = l1P[k][r - 1];
+          u1Q[k][r - 1] = u1P[k][r - 1];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < TY; ++r) sQ[r] = sP[r];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) uC[k][r] = uN[k][r];
+      }
+#pragma unroll
+      for (int r = 0; r < TY + 2; ++r) {
+        ceC[r] = ceN[r];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lC[k][r] = lN[k][r];
+      }
+    }
+  }
+#undef LA
+#undef LB
+#undef LC
+#undef LD
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k3_fused2_adj(Fused2Args<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  int y0, z0;
+  bool live;
+  block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
+  if (!live) return;
+  adj2_tile<T, TY>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+}
+
+template <typename T, int TY>
+static int launch_adj2_ty(const Fused2Args<T>& a, hipStream_t s) {
+  const int threads = ((a.a.nx + 63) / 64) * 64;
+  const int nb = ((a.a.nblocks + 7) / 8) * 8;
+  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * a.a.nx;
+  static bool attr_set = false;
+  if (!attr_set) {
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_adj<T, TY>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k3_fused2_adj<T, TY>), dim3(nb), dim3(threads), smem, s, a);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
+  if (ty == 4) return launch_adj2_ty<T, 4>(a, s);
+  if (ty == 2) return launch_adj2_ty<T, 2>(a, s);
+  set_error("fused2: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template int launch_fused2_adj<float>(const Fused2Args<float>&, int, hipStream_t);
+
+}  // namespace dnx

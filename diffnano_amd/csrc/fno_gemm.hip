@@ -19,6 +19,9 @@
 // lane l holds rows (l/16)*4 .. +4 of column l%16.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <type_traits>
+
 #include "../../include/dnx_fno.h"
 #include "dnx_common.h"
 
@@ -148,6 +151,517 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   }
 }
 
+// ---- NT fast path: both operands contiguous along K ----------------------
+//
+// A[m][k] = A[m*sAm + k], B[k][n] = B[n*sBn + k] (B stored as [n][k]).  Tile
+// 128 x 128 x 64, 256 threads (4 waves, 2 x 2, 64 x 64 each).  Each K step
+// loads the next tile into registers with 16-byte global loads (FP32 inputs
+// are rounded to BF16 on the way), computes 2 x 16 MFMAs per wave from the
+// current LDS buffer, then writes the registers into the other LDS buffer:
+// one barrier per K step.  Preconditions (checked by the dispatcher): K % 64
+// == 0; sAm, sBn, sAb, sBb multiples of 8 elements; A, B 16-byte aligned.
+
+constexpr int NBK = 64;
+constexpr int NLD = NBK + 8;  // padded LDS row: 144 bytes
+
+template <bool BF>
+struct TileLoad;
+
+template <>
+struct TileLoad<false> {  // FP32 in memory: 128 rows x 64 k = 2048 float4, 8 per thread
+  float4 v[8];
+  __device__ void load(const float* base, int64_t ld, int row0, int nrows, int k0, int t) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 4, c4 = e & 15;
+      if (row0 + row < nrows)
+        v[r] = *reinterpret_cast<const float4*>(base + int64_t(row0 + row) * ld + k0 + c4 * 4);
+      else
+        v[r] = float4{0.f, 0.f, 0.f, 0.f};
+    }
+  }
+  __device__ void store(unsigned short* lds, int t) const {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 4, c4 = e & 15;
+      uint2 p;
+      p.x = uint32_t(f2bf(v[r].x)) | (uint32_t(f2bf(v[r].y)) << 16);
+      p.y = uint32_t(f2bf(v[r].z)) | (uint32_t(f2bf(v[r].w)) << 16);
+      *reinterpret_cast<uint2*>(lds + row * NLD + c4 * 4) = p;
+    }
+  }
+};
+
+template <>
+struct TileLoad<true> {  // BF16 in memory: 128 rows x 64 k = 1024 x 16 B, 4 per thread
+  uint4 v[4];
+  __device__ void load(const unsigned short* base, int64_t ld, int row0, int nrows, int k0,
+                       int t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 3, c8 = e & 7;
+      if (row0 + row < nrows)
+        v[r] = *reinterpret_cast<const uint4*>(base + int64_t(row0 + row) * ld + k0 + c8 * 8);
+      else
+        v[r] = uint4{0u, 0u, 0u, 0u};
+    }
+  }
+  __device__ void store(unsigned short* lds, int t) const {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 3, c8 = e & 7;
+      *reinterpret_cast<uint4*>(lds + row * NLD + c8 * 8) = v[r];
+    }
+  }
+};
+
+template <bool ABF, bool BBF>
+__global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
+  __shared__ unsigned short As[2][GBM * NLD];
+  __shared__ unsigned short Bs[2][GBN * NLD];
+  using TA = typename std::conditional<ABF, unsigned short, float>::type;
+  using TB = typename std::conditional<BBF, unsigned short, float>::type;
+  const int b = blockIdx.z;
+  const int m0 = blockIdx.y * GBM;
+  const int n0 = blockIdx.x * GBN;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = (wave >> 1) * 64;
+  const int wn = (wave & 1) * 64;
+  const TA* A = static_cast<const TA*>(g.A) + int64_t(b) * g.sAb;
+  const TB* B = static_cast<const TB*>(g.B) + int64_t(b) * g.sBb;
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  TileLoad<ABF> la;
+  TileLoad<BBF> lb;
+  la.load(A, g.sAm, m0, g.M, 0, t);
+  lb.load(B, g.sBn, n0, g.N, 0, t);
+  la.store(As[0], t);
+  lb.store(Bs[0], t);
+  __syncthreads();
+  const int nk = g.K / NBK;
+  const int kq = (lane >> 4) * 8;
+  const int lr = lane & 15;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(A, g.sAm, m0, g.M, (kt + 1) * NBK, t);
+      lb.load(B, g.sBn, n0, g.N, (kt + 1) * NBK, t);
+    }
+#pragma unroll
+    for (int ks = 0; ks < NBK; ks += 32) {
+      bf16x8 af[4], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        af[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wm + i * 16 + lr) * NLD + ks + kq]);
+        bfr[i] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wn + i * 16 + lr) * NLD + ks + kq]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      la.store(As[cur ^ 1], t);
+      lb.store(Bs[cur ^ 1], t);
+    }
+    __syncthreads();
+  }
+
+  const int64_t coff = int64_t(b) * g.sCb;
+  const int64_t ioff = int64_t(b) * g.sIb;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + wn + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        if (m >= g.M || n >= g.N) continue;
+        float v = g.alpha * acc[i][j][r];
+        if (g.beta != 0.f) v += g.beta * g.Cin[ioff + m * g.sIm + n * g.sIn];
+        if (g.bias_mode == 1) v += g.bias[n];
+        else if (g.bias_mode == 2) v += g.bias[m];
+        const int64_t ci = coff + m * g.sCm + n * g.sCn;
+        if (g.pre) g.pre[ci] = v;
+        if (g.act == 1) v = gelu(v);
+        g.C[ci] = v;
+      }
+    }
+  }
+}
+
+// ---- Layout-specialised tile kernel --------------------------------------
+//
+// Same 128 x 128 x 64 tile, LDS layout ([row][k], padded), register
+// prefetch and one barrier per K step as k_gemm_nt, for any operand layout:
+// an operand contiguous along K is loaded 16 bytes at a time along K; an
+// operand contiguous along its row index (M for A, N for B) is loaded as
+// 8 scalar loads along K per thread, consecutive lanes reading consecutive
+// rows (so every load instruction is one contiguous 256-byte span per wave),
+// and packed to 8 BF16 = one 16-byte LDS store.  K need not be a multiple
+// of 64 (the tail is zero-filled); a K-contiguous operand needs K, its row
+// stride and batch stride to be multiples of 8 and a 16-byte aligned base.
+
+template <bool BF, bool KFAST>
+struct TileLoadG;
+
+template <bool BF>
+struct TileLoadG<BF, true> {  // contiguous along K: 8 k per 16 bytes (BF16) or 2 x float4
+  using TE = typename std::conditional<BF, unsigned short, float>::type;
+  uint4 v[4];
+  __device__ void load(const TE* base, int64_t ld, int64_t /*sk*/, int row0, int nrows, int k0,
+                       int K, int t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 3, c8 = e & 7;
+      const int k = k0 + c8 * 8;
+      const bool ok = row0 + row < nrows && k < K;
+      const TE* p = base + int64_t(row0 + row) * ld + k;
+      if constexpr (BF) {
+        v[r] = ok ? *reinterpret_cast<const uint4*>(p) : uint4{0u, 0u, 0u, 0u};
+      } else {
+        float4 a = float4{0.f, 0.f, 0.f, 0.f}, b = a;
+        if (ok) {
+          a = reinterpret_cast<const float4*>(p)[0];
+          b = reinterpret_cast<const float4*>(p)[1];
+        }
+        v[r].x = uint32_t(f2bf(a.x)) | (uint32_t(f2bf(a.y)) << 16);
+        v[r].y = uint32_t(f2bf(a.z)) | (uint32_t(f2bf(a.w)) << 16);
+        v[r].z = uint32_t(f2bf(b.x)) | (uint32_t(f2bf(b.y)) << 16);
+        v[r].w = uint32_t(f2bf(b.z)) | (uint32_t(f2bf(b.w)) << 16);
+      }
+    }
+  }
+  __device__ void store(unsigned short* lds, int t) const {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = r * 256 + t;
+      *reinterpret_cast<uint4*>(lds + (e >> 3) * NLD + (e & 7) * 8) = v[r];
+    }
+  }
+};
+
+template <bool BF>
+struct TileLoadG<BF, false> {  // contiguous along rows: lanes walk rows, 8 loads along K
+  using TE = typename std::conditional<BF, unsigned short, float>::type;
+  uint4 v[4];
+  __device__ void load(const TE* base, int64_t /*ld*/, int64_t sk, int row0, int nrows, int k0,
+                       int K, int t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = r * 256 + t;
+      const int row = e & 127, kg = e >> 7;  // kg = 0..7
+      const int k = k0 + kg * 8;
+      const bool rok = row0 + row < nrows;
+      const TE* p = base + (row0 + row) + int64_t(k) * sk;
+      unsigned short h[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool ok = rok && k + j < K;
+        if constexpr (BF) {
+          h[j] = ok ? p[int64_t(j) * sk] : (unsigned short)0;
+        } else {
+          h[j] = ok ? f2bf(p[int64_t(j) * sk]) : (unsigned short)0;
+        }
+      }
+      v[r].x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
+      v[r].y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+      v[r].z = uint32_t(h[4]) | (uint32_t(h[5]) << 16);
+      v[r].w = uint32_t(h[6]) | (uint32_t(h[7]) << 16);
+    }
+  }
+  __device__ void store(unsigned short* lds, int t) const {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = r * 256 + t;
+      *reinterpret_cast<uint4*>(lds + (e & 127) * NLD + (e >> 7) * 8) = v[r];
+    }
+  }
+};
+
+template <bool ABF, bool BBF, bool AK, bool BK>
+__global__ __launch_bounds__(256) void k_gemm_tile(GemmArgs g) {
+  __shared__ unsigned short As[2][GBM * NLD];
+  __shared__ unsigned short Bs[2][GBN * NLD];
+  using TA = typename std::conditional<ABF, unsigned short, float>::type;
+  using TB = typename std::conditional<BBF, unsigned short, float>::type;
+  const int b = blockIdx.z;
+  const int m0 = blockIdx.y * GBM;
+  const int n0 = blockIdx.x * GBN;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = (wave >> 1) * 64;
+  const int wn = (wave & 1) * 64;
+  const TA* A = static_cast<const TA*>(g.A) + int64_t(b) * g.sAb;
+  const TB* B = static_cast<const TB*>(g.B) + int64_t(b) * g.sBb;
+  // row stride / k stride of each operand in its [row][k] view
+  const int64_t a_ld = g.sAm, a_sk = g.sAk;
+  const int64_t b_ld = g.sBn, b_sk = g.sBk;
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  TileLoadG<ABF, AK> la;
+  TileLoadG<BBF, BK> lb;
+  la.load(A, a_ld, a_sk, m0, g.M, 0, g.K, t);
+  lb.load(B, b_ld, b_sk, n0, g.N, 0, g.K, t);
+  la.store(As[0], t);
+  lb.store(Bs[0], t);
+  __syncthreads();
+  const int nk = (g.K + NBK - 1) / NBK;
+  const int kq = (lane >> 4) * 8;
+  const int lr = lane & 15;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(A, a_ld, a_sk, m0, g.M, (kt + 1) * NBK, g.K, t);
+      lb.load(B, b_ld, b_sk, n0, g.N, (kt + 1) * NBK, g.K, t);
+    }
+#pragma unroll
+    for (int ks = 0; ks < NBK; ks += 32) {
+      bf16x8 af[4], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        af[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wm + i * 16 + lr) * NLD + ks + kq]);
+        bfr[i] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wn + i * 16 + lr) * NLD + ks + kq]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      la.store(As[cur ^ 1], t);
+      lb.store(Bs[cur ^ 1], t);
+    }
+    __syncthreads();
+  }
+
+  const int64_t coff = int64_t(b) * g.sCb;
+  const int64_t ioff = int64_t(b) * g.sIb;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + wn + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        if (m >= g.M || n >= g.N) continue;
+        float v = g.alpha * acc[i][j][r];
+        if (g.beta != 0.f) v += g.beta * g.Cin[ioff + m * g.sIm + n * g.sIn];
+        if (g.bias_mode == 1) v += g.bias[n];
+        else if (g.bias_mode == 2) v += g.bias[m];
+        const int64_t ci = coff + m * g.sCm + n * g.sCn;
+        if (g.pre) g.pre[ci] = v;
+        if (g.act == 1) v = gelu(v);
+        g.C[ci] = v;
+      }
+    }
+  }
+}
+
+// Dispatch of the tile kernel; returns false when an operand's layout fits
+// neither specialisation (then the element-wise kernel runs).
+template <bool ABF, bool BBF>
+static bool launch_tile(const GemmArgs& g, dim3 grid, hipStream_t s) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  const bool ak = g.sAk == 1 && g.K % 8 == 0 && g.sAm % 8 == 0 && g.sAb % 8 == 0 && al(g.A);
+  const bool am = g.sAm == 1;
+  const bool bk = g.sBk == 1 && g.K % 8 == 0 && g.sBn % 8 == 0 && g.sBb % 8 == 0 && al(g.B);
+  const bool bn = g.sBn == 1;
+  if (!(ak || am) || !(bk || bn)) return false;
+  if (ak && bk) hipLaunchKernelGGL((k_gemm_tile<ABF, BBF, true, true>), grid, dim3(256), 0, s, g);
+  else if (ak) hipLaunchKernelGGL((k_gemm_tile<ABF, BBF, true, false>), grid, dim3(256), 0, s, g);
+  else if (bk) hipLaunchKernelGGL((k_gemm_tile<ABF, BBF, false, true>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((k_gemm_tile<ABF, BBF, false, false>), grid, dim3(256), 0, s, g);
+  return true;
+}
+
+// ---- Channel-mixing products (small M, K; huge N) ------------------------
+//
+// C[M][n] = act(alpha A[M][K] X[K][n] + beta Cin + bias) with M, K <= 128 and
+// n = batch x pixels in the millions: the FNO's lifting, pointwise and
+// projection layers and their data gradients.  They move O(n (M + K)) bytes
+// for O(n M K) flops with M K <= 4096, i.e. they are HBM-bound, so they run
+// on the vector ALU in FP32 (operands rounded to BF16 first, as on the
+// matrix cores): each thread owns 4 consecutive columns (16-byte loads and
+// stores), A sits in LDS and is read by broadcast.  With 4M <= 128 the
+// accumulators stay in registers while X streams along K; otherwise
+// (M = 128) the thread's 4K inputs stay in registers while M streams.
+
+__device__ inline float bfr(float v) { return __uint_as_float(uint32_t(f2bf(v)) << 16); }
+
+template <int M, int K, bool BBF>
+__global__ __launch_bounds__(256) void k_cmix(GemmArgs g) {
+  __shared__ float Ws[M * K];
+  {
+    const bool abf = g.a_bf16 != 0;
+    for (int e = threadIdx.x; e < M * K; e += 256) {
+      const int m = e / K, k = e % K;
+      const int64_t i = int64_t(m) * g.sAm + int64_t(k) * g.sAk;
+      Ws[e] = abf ? __uint_as_float(uint32_t(static_cast<const unsigned short*>(g.A)[i]) << 16)
+                  : bfr(static_cast<const float*>(g.A)[i]);
+    }
+  }
+  __syncthreads();
+  const int64_t n = (int64_t(blockIdx.x) * 256 + threadIdx.x) * 4;
+  if (n >= g.N) return;
+  auto loadx = [&](int k) -> float4 {
+    if constexpr (BBF) {
+      const uint2 u = *reinterpret_cast<const uint2*>(static_cast<const unsigned short*>(g.B) +
+                                                        int64_t(k) * g.sBk + n);
+      return float4{__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                    __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u)};
+    } else {
+      const float4 v =
+          *reinterpret_cast<const float4*>(static_cast<const float*>(g.B) + int64_t(k) * g.sBk + n);
+      return float4{bfr(v.x), bfr(v.y), bfr(v.z), bfr(v.w)};
+    }
+  };
+  auto emit = [&](int m, float4 a) {
+    float v[4] = {g.alpha * a.x, g.alpha * a.y, g.alpha * a.z, g.alpha * a.w};
+    const int64_t ci = int64_t(m) * g.sCm + n;
+    if (g.beta != 0.f) {
+      const float4 c = *reinterpret_cast<const float4*>(g.Cin + int64_t(m) * g.sIm + n);
+      v[0] += g.beta * c.x;
+      v[1] += g.beta * c.y;
+      v[2] += g.beta * c.z;
+      v[3] += g.beta * c.w;
+    }
+    if (g.bias_mode == 2) {
+      const float b = g.bias[m];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] += b;
+    } else if (g.bias_mode == 1) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] += g.bias[n + j];
+    }
+    if (g.pre) *reinterpret_cast<float4*>(g.pre + ci) = float4{v[0], v[1], v[2], v[3]};
+    if (g.act == 1) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = gelu(v[j]);
+    }
+    *reinterpret_cast<float4*>(g.C + ci) = float4{v[0], v[1], v[2], v[3]};
+  };
+  if constexpr (4 * M <= 128) {
+    float4 acc[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int k = 0; k < K; ++k) {
+      const float4 x = loadx(k);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const float w = Ws[m * K + k];
+        acc[m].x += w * x.x;
+        acc[m].y += w * x.y;
+        acc[m].z += w * x.z;
+        acc[m].w += w * x.w;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) emit(m, acc[m]);
+  } else {
+    float4 x[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] = loadx(k);
+#pragma unroll 2
+    for (int m = 0; m < M; ++m) {
+      float4 a = float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float w = Ws[m * K + k];
+        a.x += w * x[k].x;
+        a.y += w * x[k].y;
+        a.z += w * x[k].z;
+        a.w += w * x[k].w;
+      }
+      emit(m, a);
+    }
+  }
+}
+
+template <int M, int K>
+static bool cmix_mk(const GemmArgs& g, hipStream_t s) {
+  if (g.M != M || g.K != K) return false;
+  const dim3 grid(cdiv(g.N, 1024));
+  if (g.b_bf16) hipLaunchKernelGGL((k_cmix<M, K, true>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((k_cmix<M, K, false>), grid, dim3(256), 0, s, g);
+  return true;
+}
+
+// The channel-mixing shapes of the FNO (width 32, hidden 128, 9 inputs,
+// 6 outputs); other shapes take the tile kernel.
+static bool launch_cmix(const GemmArgs& g, hipStream_t s) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  if (g.batch != 1 || g.sBn != 1 || g.sCn != 1 || g.N % 4 != 0) return false;
+  if (g.sBk % 4 != 0 || g.sCm % 4 != 0 || !al(g.B) || !al(g.C)) return false;
+  if (g.beta != 0.f && (g.sIn != 1 || g.sIm % 4 != 0 || !al(g.Cin))) return false;
+  if (g.pre && !al(g.pre)) return false;
+  if (g.b_bf16 && g.sBk % 8 != 0) return false;
+  return cmix_mk<32, 9>(g, s) || cmix_mk<32, 32>(g, s) || cmix_mk<128, 32>(g, s) ||
+         cmix_mk<6, 128>(g, s) || cmix_mk<128, 6>(g, s) || cmix_mk<32, 128>(g, s);
+}
+
+// DNX_GEMM_GENERIC=1 routes every call through the generic kernel (tests).
+static bool getenv_flag(const char* name) {
+  const char* e = getenv(name);
+  return e && *e && *e != '0';
+}
+
+static bool nt_ok(const GemmArgs& g) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  return g.sAk == 1 && g.sBk == 1 && g.K % NBK == 0 && g.sAm % 8 == 0 && g.sBn % 8 == 0 &&
+         g.sAb % 8 == 0 && g.sBb % 8 == 0 && al(g.A) && al(g.B);
+}
+
+// out[i] = sum_j part[j*n + i] + beta * out[i]
+__global__ void k_sum_partials(const float* __restrict__ part, int s, int64_t n,
+                               float* __restrict__ out, float beta) {
+  const int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+  if (i >= n) return;
+  if (i + 4 <= n && (n & 3) == 0) {
+    float4 a = beta != 0.f ? *reinterpret_cast<const float4*>(out + i) : float4{0.f, 0.f, 0.f, 0.f};
+    if (beta != 0.f) a = float4{a.x * beta, a.y * beta, a.z * beta, a.w * beta};
+    for (int j = 0; j < s; ++j) {
+      const float4 p = *reinterpret_cast<const float4*>(part + int64_t(j) * n + i);
+      a.x += p.x;
+      a.y += p.y;
+      a.z += p.z;
+      a.w += p.w;
+    }
+    *reinterpret_cast<float4*>(out + i) = a;
+    return;
+  }
+  for (int64_t q = i; q < i + 4 && q < n; ++q) {
+    float a = beta != 0.f ? beta * out[q] : 0.f;
+    for (int j = 0; j < s; ++j) a += part[int64_t(j) * n + q];
+    out[q] = a;
+  }
+}
+
 // Elementwise helpers of the FNO (FP32).
 // GELU backward: dx = dy * gelu'(pre)
 __global__ void k_gelu_bwd(const float* __restrict__ dy, const float* __restrict__ pre,
@@ -171,7 +685,34 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
   DNX_REQUIRE(g->bias_mode == 0 || g->bias, "gemm: bias missing");
   DNX_REQUIRE(g->batch <= 65535, "gemm: batch > 65535");
   dim3 grid(cdiv(g->N, GBN), cdiv(g->M, GBM), g->batch);
-  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, static_cast<hipStream_t>(stream), *g);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool generic = getenv_flag("DNX_GEMM_GENERIC");
+  bool done = false;
+  if (!generic && launch_cmix(*g, s)) {
+    done = true;
+  } else if (!generic && nt_ok(*g)) {
+    if (g->a_bf16 && g->b_bf16) hipLaunchKernelGGL((k_gemm_nt<true, true>), grid, dim3(256), 0, s, *g);
+    else if (g->a_bf16) hipLaunchKernelGGL((k_gemm_nt<true, false>), grid, dim3(256), 0, s, *g);
+    else if (g->b_bf16) hipLaunchKernelGGL((k_gemm_nt<false, true>), grid, dim3(256), 0, s, *g);
+    else hipLaunchKernelGGL((k_gemm_nt<false, false>), grid, dim3(256), 0, s, *g);
+    done = true;
+  } else if (!generic) {
+    if (g->a_bf16 && g->b_bf16) done = launch_tile<true, true>(*g, grid, s);
+    else if (g->a_bf16) done = launch_tile<true, false>(*g, grid, s);
+    else if (g->b_bf16) done = launch_tile<false, true>(*g, grid, s);
+    else done = launch_tile<false, false>(*g, grid, s);
+  }
+  if (!done) hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, *g);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out,
+                                    float beta, void* stream) {
+  DNX_REQUIRE(part && out && s > 0 && n > 0, "sum_partials: bad arguments");
+  const int64_t threads = cdiv(n, 4);
+  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(threads, 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), part, s, n, out, beta);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }

@@ -76,7 +76,9 @@ def gelu_bwd(dy, pre, dx):
 def dft_matrices(H, W, m1, m2, device):
     """Forward F (H W x 2 M) and inverse G (2 M x H W), M = 2 m1 m2 modes,
     column/row n = ri * M + mu, mu = kyi * m2 + kx, ky = kyi (kyi < m1) or
-    kyi - 2 m1 (negative half).  Built in float64, stored BF16."""
+    kyi - 2 m1 (negative half).  Built in float64, stored BF16, each also
+    transposed (FT, GT) so that every large product reads both operands
+    contiguously along its reduction axis."""
     kyi = np.arange(2 * m1)
     ky = np.where(kyi < m1, kyi, kyi - 2 * m1).astype(np.float64)
     kx = np.arange(m2, dtype=np.float64)
@@ -89,8 +91,37 @@ def dft_matrices(H, W, m1, m2, device):
     ckx = np.where(kx == 0, 1.0, 2.0)
     cw = np.tile(ckx, 2 * m1)[None, :] / (H * W)
     G = np.concatenate([np.cos(theta) * cw, -np.sin(theta) * cw], axis=1).T
-    return (torch.tensor(F, device=device).to(torch.bfloat16).contiguous(),
-            torch.tensor(G, device=device).to(torch.bfloat16).contiguous())
+
+    def bf(a):
+        return torch.tensor(np.ascontiguousarray(a), device=device).to(torch.bfloat16).contiguous()
+
+    return bf(F), bf(G), bf(F.T), bf(G.T)
+
+
+def sum_partials(part, s, n, out, beta=0.0):
+    """out[i] = sum_j part[j n + i] + beta out[i] (split-K reduction)."""
+    L.check(L.lib().dnx_sum_partials(part.data_ptr(), int(s), int(n), out.data_ptr(),
+                                     float(beta), L.stream_handle(out.device)))
+    return out
+
+
+def gemm_nt(M, N, K, A, B, out, *, target_blocks=1024, beta=0.0, Cin=None):
+    """out[M x N] = A[M x K] B[N x K]^T (+ beta Cin), both operands row-major
+    with K contiguous, split along K into a batch of partial products (then
+    summed) when the output alone has too few 128 x 128 tiles to fill the
+    GPU."""
+    tiles = -(-M // 128) * -(-N // 128)
+    s = 1
+    while s < 8 and tiles * s * 2 <= target_blocks and K % (s * 2 * 64) == 0:
+        s *= 2
+    if s == 1:
+        return gemm(M, N, K, A, (K, 1, 0), B, (1, K, 0), out, (N, 1, 0), beta=beta, Cin=Cin)
+    kc = K // s
+    part = torch.empty(s, M * N, device=out.device)
+    gemm(M, N, kc, A, (K, 1, kc), B, (1, K, kc), part, (N, 1, M * N), batch=s)
+    if beta != 0.0 and Cin is not out:
+        out.copy_(Cin)
+    return sum_partials(part, s, M * N, out, beta=beta)
 
 
 class FNO2d:
@@ -104,7 +135,8 @@ class FNO2d:
         self.layers, self.hidden = layers, hidden
         self.dev = torch.device(device)
         self.nmodes = 2 * self.m1 * self.m2
-        self.F, self.G = dft_matrices(self.H, self.W, self.m1, self.m2, self.dev)
+        self.F, self.G, self.FT, self.GT = dft_matrices(self.H, self.W, self.m1, self.m2,
+                                                         self.dev)
         g = torch.Generator().manual_seed(seed)
 
         def u(shape, fan_in):
@@ -166,14 +198,14 @@ class FNO2d:
         saved = {"x": x2, "h": [], "Z": [], "pre": []}
         for l in range(self.layers):
             Z = torch.empty(c * B, M2, device=dev)
-            gemm(c * B, M2, P, h, (P, 1, 0), self.F, (M2, 1, 0), Z, (M2, 1, 0))
+            gemm_nt(c * B, M2, P, h, self.FT, Z)
             Y = torch.empty(c * B, M2, device=dev)
             self._mix(Z, p[f"sr{l}"], p[f"si{l}"], Y, B)
             S = torch.empty(c, n, device=dev)
-            gemm(c * B, P, M2, Y, (M2, 1, 0), self.G, (P, 1, 0), S, (P, 1, 0))
+            gemm(c * B, P, M2, Y, (M2, 1, 0), self.GT, (1, M2, 0), S, (P, 1, 0))
             last = l == self.layers - 1
             hn = torch.empty(c, n, device=dev)
-            pre = None if last else torch.empty(c, n, device=dev)
+            pre = None if (last or not save) else torch.empty(c, n, device=dev)
             self._pointwise(p[f"w{l}"], h, hn, bias=p[f"b{l}"], act=0 if last else 1, pre=pre,
                             Cin=S, beta=1.0)
             if save:
@@ -182,7 +214,7 @@ class FNO2d:
                 saved["pre"].append(pre)
             h = hn
         q = torch.empty(self.hidden, n, device=dev)
-        qpre = torch.empty(self.hidden, n, device=dev)
+        qpre = torch.empty(self.hidden, n, device=dev) if save else None
         self._pointwise(p["w_p1"], h, q, bias=p["b_p1"], act=1, pre=qpre)
         out = torch.empty(self.out_ch, n, device=dev)
         self._pointwise(p["w_p2"], q, out, bias=p["b_p2"])
@@ -206,9 +238,7 @@ class FNO2d:
         s, chunk = self._split(n)
         part = torch.empty(s, O * I, device=self.dev)
         gemm(O, I, chunk, dY, (n, 1, chunk), X, (1, n, chunk), part, (I, 1, O * I), batch=s)
-        ones = self._ones(s)
-        gemm(1, O * I, s, ones, (0, 1, 0), part, (O * I, 1, 0), out.view(1, O * I),
-             (O * I, 1, 0))
+        sum_partials(part, s, O * I, out)
 
     def _rowsum(self, dY, out):
         O, n = dY.shape
@@ -216,7 +246,7 @@ class FNO2d:
         ones = self._ones(n)
         part = torch.empty(s, O, device=self.dev)
         gemm(O, 1, chunk, dY, (n, 1, chunk), ones, (1, 0, chunk), part, (1, 1, O), batch=s)
-        gemm(1, O, s, self._ones(s), (0, 1, 0), part, (O, 1, 0), out.view(1, O), (O, 1, 0))
+        sum_partials(part, s, O, out)
 
     def _ones(self, n):
         o = getattr(self, "_ones_buf", None)
@@ -256,7 +286,7 @@ class FNO2d:
             gemm(c, n, c, p[f"w{l}"], (1, c, 0), dpre, (n, 1, 0), dhn, (n, 1, 0))
             # spectral branch: dY = dS G^T
             dY = torch.empty(c * B, M2, device=dev)
-            gemm(c * B, M2, P, dpre, (P, 1, 0), self.G, (1, P, 0), dY, (M2, 1, 0))
+            gemm_nt(c * B, M2, P, dpre, self.G, dY)
             sr, si = p[f"sr{l}"], p[f"si{l}"]
             ld = M2
             sZ = (ld, B * ld, 1)       # Z/dZ/dY as [b][i] per mode

@@ -23,13 +23,51 @@ EPS_SI = 12.1
 EPS_SIO2 = 2.085
 
 
+def blob_noise(seed, size=80):
+    """The seeded uniform noise a blob design is filtered from."""
+    g = torch.Generator().manual_seed(int(seed))
+    return torch.rand(size, size, dtype=torch.float64, generator=g)
+
+
 def blob_eps_2d(seed, H=200, W=200, lo=60, size=80, sigma=3.0):
     """200x200 field with an 80x80 blob design window at rows/cols 60..139."""
-    g = torch.Generator().manual_seed(int(seed))
-    noise = torch.rand(size, size, dtype=torch.float64, generator=g).numpy()
+    noise = blob_noise(seed, size).numpy()
     blobs = gaussian_filter(noise, sigma, mode="reflect", truncate=4.0) > 0.5
     eps = np.full((H, W), EPS_SIO2)
     eps[lo:lo + size, lo:lo + size] = np.where(blobs, EPS_SI, EPS_SIO2)
+    return eps
+
+
+def _gauss_taps(sigma, truncate=4.0):
+    r = int(truncate * sigma + 0.5)
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * (x / sigma) ** 2)
+    return w / w.sum(), r
+
+
+def blob_eps_batch(noise, H, W, lo, sigma=3.0):
+    """Batched ``blob_eps_2d`` on the noise's device: noise [B, S, S] float64
+    -> eps [B, H, W] float64.  Separable Gaussian filter with half-sample
+    symmetric padding (scipy.ndimage ``mode="reflect"``, truncate 4), then
+    threshold 0.5 to Si / SiO2 inside the window at rows/cols lo..lo+S-1."""
+    B, S, _ = noise.shape
+    w, r = _gauss_taps(sigma)
+    k = torch.as_tensor(w, dtype=torch.float64, device=noise.device)
+    # half-sample symmetric index map: -1 -> 0, -2 -> 1, S -> S-1, ...
+    idx = np.arange(-r, S + r)
+    period = 2 * S
+    idx = np.mod(idx, period)
+    idx = np.where(idx >= S, period - 1 - idx, idx)
+    it = torch.as_tensor(idx, device=noise.device)
+    x = noise[:, :, it]                                   # pad x
+    x = torch.nn.functional.conv1d(x.reshape(B * S, 1, -1), k.view(1, 1, -1)).view(B, S, S)
+    x = x[:, it, :].transpose(1, 2)                       # pad y, filter along y
+    x = torch.nn.functional.conv1d(x.reshape(B * S, 1, -1), k.view(1, 1, -1)).view(B, S, S)
+    x = x.transpose(1, 2)
+    eps = torch.full((B, H, W), EPS_SIO2, dtype=torch.float64, device=noise.device)
+    hi_ = torch.tensor(EPS_SI, dtype=torch.float64, device=noise.device)
+    lo_ = torch.tensor(EPS_SIO2, dtype=torch.float64, device=noise.device)
+    eps[:, lo:lo + S, lo:lo + S] = torch.where(x > 0.5, hi_, lo_)
     return eps
 
 
@@ -108,6 +146,38 @@ def config_c0(seed=0):
     src = {"type": "gaussian_pulse", "pos": [40, 100], "spread": spread, "t0": 3 * spread}
     mons = [{"box": (160, 161, 70, 130), "wavelengths": [lam0]}]
     return eps, solver, src, mons
+
+
+class SurrogateConfig:
+    """128x128 TM surrogate workload with the C0 physics: dl 20 nm, 10-cell
+    CPML, courant 0.5, 2000 steps, point Ez source at (16, 64) with the
+    200 nm-bandwidth pulse at 1550 nm; an 80x80 blob window at rows/cols
+    24..103; the surrogate target is the complex DFT of Ez over the whole
+    grid at 1500/1550/1600 nm; the figure of merit (maximised) is
+    sum |DFT Ez(1550 nm)|^2 over the 1x60 line at row 112, cols 34..93."""
+
+    H = W = 128
+    lo, size = 24, 80
+    wavelengths = (1500.0, 1550.0, 1600.0)
+    fom_index = 1
+    line = (112, 113, 34, 94)
+
+    def __init__(self, n_steps=2000):
+        lam0 = 1550.0
+        spread = lam0**2 / (2 * math.pi * 200.0)
+        self.solver = dict(grid_shape=(self.H, self.W), dl=20.0, wavelength_nm=lam0,
+                           pml_layers=10, n_steps=n_steps, courant=0.5)
+        self.source = {"type": "gaussian_pulse", "pos": [16, 64], "spread": spread,
+                       "t0": 3 * spread}
+
+    def eps(self, noise):
+        return blob_eps_batch(noise, self.H, self.W, self.lo)
+
+    def fom(self, fields):
+        """FoM of full-grid DFT fields [B, 3, H, W] (complex)."""
+        y0, y1, x0, x1 = self.line
+        f = fields[:, self.fom_index, y0:y1, x0:x1]
+        return (f.abs() ** 2).flatten(1).sum(1)
 
 
 def monitor_loss(monitors_out, comp_index=0):

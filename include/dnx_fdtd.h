@@ -167,9 +167,11 @@ int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work, int32
                                const dnx_monitor* mons, int32_t n_mon, void* S,
                                void* stream, int32_t* final_in_work);
 
-/* Tile shape of the fused single-pass 3D kernels for one dtype and kernel
- * (0 forward/replay, 1 adjoint): ty rows (2, 4 or 8) and zc z-planes per
- * work-group.  Process-wide; set by the autotuner.  No reference
+/* Tile shape of the fused 3D kernels for one dtype and kernel (0 forward/
+ * replay, 1 adjoint, 2 two-step forward/replay, 3 two-step adjoint; the
+ * two-step kernels run for float32 only): ty rows (2, 4 or 8; 2 or 4 for
+ * the two-step kernels) and zc z-planes per work-group.  Process-wide; set
+ * by the autotuner.  No reference
  * counterpart (the reference has no kernels); it changes performance only,
  * never results. */
 int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t ty, int32_t zc);
@@ -244,6 +246,19 @@ int32_t dnx_fdtd2d_adjoint_run(const dnx_grid2d* g, void* lam, void* work, int32
                                const dnx_source* srcs, int32_t n_src,
                                const dnx_monitor* mons, int32_t n_mon, void* S,
                                void* stream, int32_t* final_in_work);
+
+/* Batched 2D TM forward (float32) over `batch` designs of one shape: g->ce
+ * points at batch*N coefficients, state at batch*3*N ([Ez, Hx, Hy] per
+ * design, zeroed by the caller), one shared soft source (src may be NULL)
+ * and a running DFT over mbox with n_freq frequencies accumulated in
+ * float32: acc[b][f][re, im][mbox cells] += Ez (cos, -sin)(phase[n][f]),
+ * phase float32 [(n*n_freq + f)*2 + {0,1}].  Semantics per design are
+ * those of dnx_fdtd2d_run (FDTDSolver2D._time_step_tm, fdtd2d.py:233-267);
+ * no reference counterpart for the batching (surrogate data / candidate
+ * judging). */
+int32_t dnx_fdtd2d_tm_batch(const dnx_grid2d* g, int32_t batch, void* state, int32_t n_steps,
+                            const dnx_source* src, const dnx_box* mbox, int32_t n_freq,
+                            const float* phase, float* acc, void* stream);
 
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop

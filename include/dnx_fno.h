@@ -48,6 +48,11 @@ typedef struct dnx_gemm_args {
 
 int32_t dnx_gemm_bf16(const dnx_gemm_args* args, void* stream);
 
+/* out[i] = sum_{j<s} part[j*n + i] + beta * out[i], FP32: the reduction
+ * step of a split-K product (partial products written as a batch). */
+int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out, float beta,
+                         void* stream);
+
 /* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32. */
 int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, int64_t n, void* stream);
 

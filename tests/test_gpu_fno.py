@@ -56,6 +56,82 @@ def test_gemm_bf16_operands_gelu_and_pre():
     assert rel_l2(C.cpu().numpy(), gel.cpu().numpy()) < 2e-6
 
 
+@pytest.mark.parametrize("abf,bbf", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("M,N,K,batch", [(128, 128, 64, 1), (200, 72, 640, 2), (9, 300, 128, 3)])
+def test_gemm_nt_fast_path_matches_generic_and_fp32(M, N, K, batch, abf, bbf, monkeypatch):
+    """Both operands contiguous along K (the fast path), BF16 or FP32 in
+    memory, ragged M/N, several K steps and batches."""
+    from diffnano_amd.fno import gemm
+    g = torch.Generator(device="cuda").manual_seed(M + N + K)
+    A = torch.randn(batch, M, K, device="cuda", generator=g)
+    B = torch.randn(batch, N, K, device="cuda", generator=g)
+    if abf:
+        A = A.to(torch.bfloat16)
+    if bbf:
+        B = B.to(torch.bfloat16)
+    bias = torch.randn(M, device="cuda", generator=g)
+    C = torch.empty(batch, M, N, device="cuda")
+    pre = torch.empty_like(C)
+    args = (M, N, K, A, (K, 1, M * K), B, (1, K, N * K), C, (N, 1, M * N))
+    gemm(*args, batch=batch, alpha=0.75, bias=bias, bias_mode=2, act=1, pre=pre)
+    ref = 0.75 * torch.bmm(_bf(A.float()), _bf(B.float()).transpose(1, 2).contiguous()) \
+        + bias[:, None]
+    torch.cuda.synchronize()
+    assert rel_l2(pre.cpu().numpy(), ref.cpu().numpy()) < 2e-6
+    C2 = torch.empty_like(C)
+    monkeypatch.setenv("DNX_GEMM_GENERIC", "1")
+    gemm(M, N, K, A, (K, 1, M * K), B, (1, K, N * K), C2, (N, 1, M * N), batch=batch,
+         alpha=0.75, bias=bias, bias_mode=2, act=1)
+    torch.cuda.synchronize()
+    assert rel_l2(C.cpu().numpy(), C2.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("abf,bbf", [(False, False), (True, True), (False, True)])
+@pytest.mark.parametrize("M,N,K", [(32, 1000, 96), (130, 70, 100), (6, 257, 128)])
+def test_gemm_tile_layouts_match_generic(M, N, K, abf, bbf, ta, tb, monkeypatch):
+    """Layout-specialised tile kernel (K- or row-contiguous operands, K tails)
+    against the element-wise kernel, bitwise, and against FP32 products."""
+    from diffnano_amd.fno import gemm
+    g = torch.Generator(device="cuda").manual_seed(M * 3 + N + K)
+    A = torch.randn(K, M, device="cuda", generator=g) if ta else \
+        torch.randn(M, K, device="cuda", generator=g)
+    B = torch.randn(N, K, device="cuda", generator=g) if tb else \
+        torch.randn(K, N, device="cuda", generator=g)
+    if abf:
+        A = A.to(torch.bfloat16)
+    if bbf:
+        B = B.to(torch.bfloat16)
+    sA = (1, M, 0) if ta else (K, 1, 0)
+    sB = (1, K, 0) if tb else (N, 1, 0)
+    Cin = torch.randn(M, N, device="cuda", generator=g)
+    C1 = torch.empty(M, N, device="cuda")
+    C2 = torch.empty(M, N, device="cuda")
+    gemm(M, N, K, A, sA, B, sB, C1, (N, 1, 0), beta=0.5, Cin=Cin)
+    monkeypatch.setenv("DNX_GEMM_GENERIC", "1")
+    gemm(M, N, K, A, sA, B, sB, C2, (N, 1, 0), beta=0.5, Cin=Cin)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    Am = _bf(A.float()).t() if ta else _bf(A.float())
+    Bm = _bf(B.float()).t() if tb else _bf(B.float())
+    ref = Am.contiguous() @ Bm.contiguous() + 0.5 * Cin
+    assert rel_l2(C1.cpu().numpy(), ref.cpu().numpy()) < 2e-6
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 1024, 16384), (96, 64, 4096)])
+def test_gemm_nt_split_k(M, N, K):
+    from diffnano_amd.fno import gemm_nt
+    g = torch.Generator(device="cuda").manual_seed(K)
+    A = torch.randn(M, K, device="cuda", generator=g)
+    B = torch.randn(N, K, device="cuda", generator=g).to(torch.bfloat16)
+    Cin = torch.randn(M, N, device="cuda", generator=g)
+    out = torch.empty(M, N, device="cuda")
+    gemm_nt(M, N, K, A, B, out, beta=0.5, Cin=Cin)
+    ref = _bf(A) @ B.float().t() + 0.5 * Cin
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref.cpu().numpy()) < 2e-6
+
+
 def _small_model(layers=2):
     from diffnano_amd.fno import FNO2d
     return FNO2d(32, 32, in_ch=3, out_ch=6, width=8, modes=(4, 4), layers=layers, hidden=16,

@@ -353,6 +353,57 @@ def test_tile_shapes_give_identical_results():
         assert torch.equal(f, outs[0][0]) and torch.equal(g, outs[0][1])
 
 
+def _run_monitored(shape, T, ckpt, n_mon):
+    rng = np.random.default_rng(5)
+    eps = torch.tensor(1.5 + 3.0 * rng.random(shape), requires_grad=True)
+    s = _solver3d(grid_shape=shape, dl=25.0, pml_layers=3, n_steps=T, courant=0.45,
+                  checkpoint_every=ckpt, backward="time_reversal")
+    src = {"type": "gaussian_pulse", "plane": "yz", "x": 4, "component": "Ey",
+           "profile": torch.from_numpy(rng.random((shape[0], shape[1], 1)))}
+    mons = [{"box": (2, 9, 3, 9, 10, 40), "components": "xz", "wavelengths": [1600.0, 1500.0]},
+            {"box": (0, shape[0], 0, shape[1], shape[2] - 6, shape[2] - 5),
+             "components": ["Ey"], "wavelengths": [1550.0]}][:n_mon]
+    r = s.forward(eps, source=src, monitors=mons)
+    F = r.metadata.get("monitors", [])
+    w = torch.from_numpy(rng.standard_normal(r.field.shape))
+    loss = (r.field * w).sum() + sum((f.abs() ** 2).sum() for f in F)
+    loss.backward()
+    return [r.field.detach().clone()] + [f.detach().clone() for f in F] + [eps.grad.clone()]
+
+
+@pytest.mark.parametrize("ty,zc,T,ckpt,n_mon", [(4, 32, 37, 6, 1), (2, 3, 24, 5, 1),
+                                                (4, 5, 25, 64, 1), (2, 4, 20, 7, 0),
+                                                (2, 6, 21, 6, 2)])
+def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, monkeypatch):
+    """The temporally blocked (two steps per launch) kernels with fused
+    source and monitor against the one-step kernels: ragged grid, several
+    z chunks, odd step counts and checkpoint intervals (single steps at the
+    seams); two monitors take the one-step path."""
+    import diffnano_amd._lib as L
+    import ctypes as C
+    shape = (19, 13, 70)
+    saved = []
+    for k in (2, 3):
+        t, z = C.c_int32(), C.c_int32()
+        L.check(L.lib().dnx_fdtd3d_get_tiles(L.DNX_F32, k, C.byref(t), C.byref(z)))
+        saved.append((k, t.value, z.value))
+    try:
+        for k in (2, 3):
+            L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, ty, zc))
+        two = _run_monitored(shape, T, ckpt, n_mon)            # two-step forward
+        monkeypatch.setenv("DNX_FDTD3D_ADJ2", "1")
+        two_adj = _run_monitored(shape, T, ckpt, n_mon)        # and two-step adjoint
+        monkeypatch.delenv("DNX_FDTD3D_ADJ2")
+    finally:
+        for k, t, z in saved:
+            L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, t, z))
+    monkeypatch.setenv("DNX_FDTD3D_STEPS", "1")
+    one = _run_monitored(shape, T, ckpt, n_mon)
+    for a, b, c in zip(two, two_adj, one):
+        assert torch.equal(a, c), (a - c).abs().max()
+        assert torch.equal(b, c), (b - c).abs().max()
+
+
 def test_determinism_bitwise():
     g = golden("grad3d.npz")
     outs = []

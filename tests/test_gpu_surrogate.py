@@ -1,0 +1,72 @@
+"""GPU: the batched 2D TM forward against the single-design solver, and the
+surrogate pieces (dataset, training, prescreen loop bookkeeping)."""
+
+import numpy as np
+import pytest
+import torch
+
+from tests._helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(n_steps):
+    from diffnano_amd.workloads import SurrogateConfig
+    return SurrogateConfig(n_steps=n_steps)
+
+
+def test_batched_tm_matches_single_design_solver():
+    from diffnano_amd import FDTDSolver2D
+    from diffnano_amd.batched2d import BatchedTM2D
+    from diffnano_amd.workloads import blob_noise
+    cfg = _cfg(500)
+    noise = torch.stack([blob_noise(s) for s in (3, 4, 5)])
+    eps = cfg.eps(noise)
+    mon = {"box": (0, cfg.H, 0, cfg.W), "wavelengths": list(cfg.wavelengths)}
+    bt = BatchedTM2D(**cfg.solver, source=cfg.source, monitor=mon, device="cuda")
+    F, Ez = bt.forward(eps, return_field=True)
+    solver = FDTDSolver2D(**cfg.solver, device="cuda")
+    for b in range(3):
+        r = solver.forward(eps[b], source=cfg.source, monitors=[mon])
+        ref = r.metadata["monitors"][0].cpu().numpy()          # [nf, H, W]
+        got = F[b].cpu().numpy()
+        assert rel_l2(got, ref) < 1e-5, b
+        assert rel_l2(Ez[b].cpu().numpy(), r.field[0].cpu().numpy()) < 1e-6, b
+
+
+def test_batched_tm_line_monitor_and_chunking():
+    from diffnano_amd.batched2d import BatchedTM2D
+    from diffnano_amd.workloads import blob_noise
+    cfg = _cfg(300)
+    eps = cfg.eps(torch.stack([blob_noise(s) for s in range(5)]))
+    y0, y1, x0, x1 = cfg.line
+    full = BatchedTM2D(**cfg.solver, source=cfg.source, device="cuda",
+                       monitor={"box": (0, cfg.H, 0, cfg.W), "wavelengths": [1550.0]})
+    line = BatchedTM2D(**cfg.solver, source=cfg.source, device="cuda", max_batch=2,
+                       monitor={"box": (y0, y1, x0, x1), "wavelengths": [1550.0]})
+    a = full.forward(eps)[:, :, y0:y1, x0:x1]
+    b = line.forward(eps)
+    assert torch.equal(a, b)
+    with pytest.raises(RuntimeError):
+        line.forward(torch.ones(2, 64, 64))
+
+
+def test_surrogate_fit_and_prescreen_bookkeeping():
+    from diffnano_amd.surrogate import FieldSurrogate, PrescreenDesigner, generate_dataset
+    cfg = _cfg(400)
+    noise, eps, fields = generate_dataset(cfg, range(96), device="cuda", batch=64)
+    assert fields.shape == (96, 3, cfg.H, cfg.W) and fields.dtype == torch.complex64
+    sur = FieldSurrogate(cfg, device="cuda", layers=2)
+    hist = sur.fit(eps, fields, epochs=4, batch=16, lr=3e-3)
+    assert hist[-1] < hist[0]
+    err = sur.rel_error(eps[:8], fields[:8])
+    assert err.shape == (8,) and torch.isfinite(err).all()
+    d = PrescreenDesigner(cfg, sur, n_candidates=32, top_k=4, max_rel_err=0.0, seed=1)
+    f0 = d.parent_fom
+    d.run(2)
+    # max_rel_err 0: every step falls back to judging all candidates
+    assert d.stats["fallbacks"] == 2 and d.stats["fdtd_solves"] == 1 + 2 * (4 + 32)
+    assert d.parent_fom >= f0 and len(d.history) == 3
+    r = PrescreenDesigner(cfg, None, mode="random", n_candidates=32, top_k=4, seed=1)
+    r.run(1)
+    assert r.stats["fdtd_solves"] == 1 + 4
