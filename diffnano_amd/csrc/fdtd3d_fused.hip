@@ -310,19 +310,29 @@ __device__ __forceinline__ void adj_tile(const Fused3Args<T>& a, const int y0, c
     ayr[r] = rv[r] ? a.ay[y] : T(0);
   }
   const int64_t colbase = int64_t(y0 - 1) * W + x;
+  // Branch-free loads: clamped row / column (always in bounds), masked after.
+  const int xc = xv ? x : W - 1;
+  int yrow[TY + 2];
+#pragma unroll
+  for (int r = 0; r < TY + 2; ++r) yrow[r] = min(max(y0 - 1 + r, 0), a.ny - 1);
+  auto ld = [&](const T* base, int64_t plane, int r) -> T {
+    return (base + plane * P + int64_t(yrow[r]) * W)[xc];
+  };
 
   T uC[3][TY + 2], uN[3][TY + 2];
   T lC[3][TY], lN[3][TY];  // lamE own rows 1..TY at planes z, z+1
   T mp[2][TY];             // muHx', muHy' of plane z-1, rows 1..TY
   const int zs = z0 > 0 ? z0 - 1 : z0;
   {
-    const int64_t pb = int64_t(zs) * P + colbase;
 #pragma unroll
     for (int r = 0; r < TY + 2; ++r) {
-      const T c = (rv[r] && xv) ? a.ce[pb + r * W] : T(0);
+      const bool ok = rv[r] && xv;
+      const T cv = ld(a.ce, zs, r);
+      const T c = ok ? cv : T(0);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const T l = (rv[r] && xv) ? Lin[k * N + pb + r * W] : T(0);
+        const T lv = ld(Lin + k * N, zs, r);
+        const T l = ok ? lv : T(0);
         uC[k][r] = c * l;
         if (r >= 1 && r <= TY) lC[k][r - 1] = l;
       }
@@ -351,15 +361,16 @@ __device__ __forceinline__ void adj_tile(const Fused3Args<T>& a, const int y0, c
     T m[3][TY + 1];
     T em[3][TY];
     {
-      const int64_t pn = int64_t(z + 1) * P + colbase;
-      const int64_t pz = int64_t(z) * P + colbase;
+      const int zn = zl ? z + 1 : z;
 #pragma unroll
       for (int r = 0; r < TY + 2; ++r) {
         const bool ok = zl && rv[r] && xv;
-        const T c = ok ? a.ce[pn + r * W] : T(0);
+        const T cv = ld(a.ce, zn, r);
+        const T c = ok ? cv : T(0);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const T l = ok ? Lin[k * N + pn + r * W] : T(0);
+          const T lv = ld(Lin + k * N, zn, r);
+          const T l = ok ? lv : T(0);
           uN[k][r] = c * l;
           if (r >= 1 && r <= TY) lN[k][r - 1] = l;
         }
@@ -367,14 +378,23 @@ __device__ __forceinline__ void adj_tile(const Fused3Args<T>& a, const int y0, c
 #pragma unroll
       for (int r = 0; r < TY + 1; ++r) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          m[k][r] = (rv[r] && xv) ? Min[k * N + pz + r * W] : T(0);
+        for (int k = 0; k < 3; ++k) {
+          const T v = ld(Min + k * N, z, r);
+          m[k][r] = (rv[r] && xv) ? v : T(0);
+        }
       }
+      if (doB) {
 #pragma unroll
-      for (int r = 1; r <= TY; ++r) {
+        for (int r = 1; r <= TY; ++r) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          em[k][r - 1] = (doB && rv[r] && xv) ? a.em[k * N + pz + r * W] : T(0);
+          for (int k = 0; k < 3; ++k) {
+            const T v = ld(a.em + k * N, z, r);
+            em[k][r - 1] = (rv[r] && xv) ? v : T(0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < TY; ++r) em[0][r] = em[1][r] = em[2][r] = T(0);
       }
     }
     // ---- mu update, rows 0..TY
@@ -767,8 +787,9 @@ extern "C" int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t t
   DNX_REQUIRE(kernel >= 0 && kernel <= 3,
               "set_tiles: kernel must be 0 (forward), 1 (adjoint), 2 (two-step forward) or 3 "
               "(two-step adjoint)");
-  DNX_REQUIRE(ty == 2 || ty == 4 || (ty == 8 && kernel < 2),
-              "set_tiles: ty must be 2, 4 or 8 (2 or 4 for the two-step kernels; got %d)", ty);
+  DNX_REQUIRE(ty == 2 || ty == 4 || (ty == 8 && kernel < 2) || (ty == 1 && kernel == 3),
+              "set_tiles: ty must be 2, 4 or 8 (2 or 4 for the two-step forward, 1, 2 or 4 "
+              "for the two-step adjoint; got %d)", ty);
   DNX_REQUIRE(zc >= 1 && zc <= 4096, "set_tiles: zc must be in [1, 4096] (got %d)", zc);
   const int d = dtype == DNX_F64 ? 1 : 0;
   g_ty[d][kernel] = ty;

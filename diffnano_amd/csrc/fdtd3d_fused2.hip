@@ -24,6 +24,8 @@
 // accumulation, so the two-step path reproduces the one-step path bitwise.
 #include "fdtd3d_fused.h"
 
+#include <cstdlib>
+
 // No multiply-add contraction: the same rounding as the one-step kernels.
 #pragma clang fp contract(off)
 
@@ -163,6 +165,16 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     ayr[r] = rv[r] ? a.ay[y] : T(0);
   }
   const int64_t colbase = int64_t(y0 - 2) * W + x;  // row r at colbase + r*W
+  // Loads use a clamped row and column so that every one is in bounds and
+  // none needs a branch; values of rows / columns outside the grid are
+  // masked afterwards.  Row pointers are uniform, the column is the lane.
+  const int xc = xv ? x : W - 1;
+  int yrow[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) yrow[r] = min(max(y0 - 2 + r, 0), a.ny - 1);
+  auto ld = [&](const T* base, int64_t plane, int r) -> T {
+    return (base + plane * P + int64_t(yrow[r]) * W)[xc];
+  };
 
   const bool has_src = a.src_comp >= 0;
   const T w1 = has_src ? a.src_wave[A2.n] : T(0);
@@ -179,13 +191,15 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   T spQ[TY];         // source profile at plane p-1, rows 2..TY+1 (0 outside the box)
   const int zs = z0 >= 2 ? z0 - 2 : 0;
   {
-    const int64_t pb = int64_t(zs) * P + colbase;
 #pragma unroll
     for (int r = 0; r < TY; ++r) spQ[r] = T(0);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) e0C[c][r] = (rv[r] && xv) ? Ein[c * N + pb + r * W] : T(0);
+      for (int c = 0; c < 3; ++c) {
+        const T v = ld(Ein + c * N, zs, r);
+        e0C[c][r] = (rv[r] && xv) ? v : T(0);
+      }
     }
     if (xv) {
 #pragma unroll
@@ -217,21 +231,28 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     // ---------------- stage A, part 1: loads and H1(p)
     if (doA) {
       T h0[3][TY + 3];
-      const int64_t pn = int64_t(p + 1) * P + colbase;
-      const int64_t pz = int64_t(p) * P + colbase;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          e0N[c][r] = (zl && rv[r] && xv) ? Ein[c * N + pn + r * W] : T(0);
+        {
+          const T v = ld(Ein + c * N, zl ? p + 1 : p, r);
+          e0N[c][r] = (zl && rv[r] && xv) ? v : T(0);
+        }
       }
 #pragma unroll
       for (int r = 0; r <= TY + 2; ++r) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) h0[c][r] = (rv[r] && xv) ? Hin[c * N + pz + r * W] : T(0);
+        for (int c = 0; c < 3; ++c) {
+          const T v = ld(Hin + c * N, p, r);
+          h0[c][r] = (rv[r] && xv) ? v : T(0);
+        }
       }
 #pragma unroll
-      for (int r = 1; r <= TY + 2; ++r) ceP[r - 1] = (rv[r] && xv) ? a.ce[pz + r * W] : T(0);
+      for (int r = 1; r <= TY + 2; ++r) {
+        const T v = ld(a.ce, p, r);
+        ceP[r - 1] = (rv[r] && xv) ? v : T(0);
+      }
       {
         int dzs;
         const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
@@ -571,6 +592,11 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
   const bool has_src = a.src_comp >= 0;
   const T w1 = has_src ? a.src_wave[m] : T(0);
   const T w2 = has_src ? a.src_wave[m - 1] : T(0);
+  RowBox<R> sbx;
+  sbx.init(a.src_box, x, y0 - 2, has_src && xv);
+  T spQ[TY];  // source profile at plane p-1, owned rows 2..TY+1 (0 outside the box)
+#pragma unroll
+  for (int r = 0; r < TY; ++r) spQ[r] = T(0);
 
   T uC[3][R], uN[3][R];   // u = ce lamE^{m+1} at planes p, p+1
   T lC[3][TY + 2], lN[3][TY + 2];  // raw lamE^{m+1}, rows 1..TY+2 (index r-1)
@@ -622,6 +648,7 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
     T l1P[3][TY + 2];  // lamE^m at plane p, rows 1..TY+2 (index r-1)
     T u1P[3][TY + 2];
     T sP[TY];
+    T spP[TY];  // source profile at plane p, owned rows 2..TY+1
     T dpre[FUSED2_MAX_MON][TY + 2][3];  // monitor adjoint source, rows 1..TY+2
     bool dhit[FUSED2_MAX_MON][TY + 2];
     // ---------------- stage A, part 1: loads and mu^m(p)
@@ -645,6 +672,13 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
       for (int r = 0; r <= TY + 2; ++r) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) m0[k][r] = (rv[r] && xv) ? Min[k * N + pz + r * W] : T(0);
+      }
+      {
+        int dzs;
+        const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
+#pragma unroll
+        for (int r = 2; r <= TY + 1; ++r)
+          spP[r - 2] = (sz && sbx.yin[r]) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
       }
       // adjoint source of sample m-1 for lamE^m(p), rows 1..TY+2 (issued
       // with the field loads, consumed after the barrier)
@@ -721,10 +755,10 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
           for (int k = 0; k < 3; ++k) em[k] = a.em[k * N + i];
           s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
-          if (has_src) {
-            int64_t si;
-            if (in_box<T>(a.src_box, p, y, x, si)) {
-              const T w = w1 * a.src_prof[si];
+          if (has_src && sbx.yin[r] && r >= 2 && r <= TY + 1) {
+            int dzs;
+            if (RowBox<R>::zin(a.src_box, p, dzs)) {
+              const T w = w1 * spP[r - 2];
 #pragma unroll
               for (int cc = 0; cc < 3; ++cc)
                 if (cc == a.src_comp) s -= w * lC[cc][r - 1];
@@ -817,10 +851,10 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
           for (int k = 0; k < 3; ++k) em[k] = A2.em2[k * N + i];
           T s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
-          if (has_src) {
-            int64_t si;
-            if (in_box<T>(a.src_box, q, y, x, si)) {
-              const T w = w2 * a.src_prof[si];
+          if (has_src && sbx.yin[r]) {
+            int dzs;
+            if (RowBox<R>::zin(a.src_box, q, dzs)) {
+              const T w = w2 * spQ[r - 2];
 #pragma unroll
               for (int cc = 0; cc < 3; ++cc)
                 if (cc == a.src_comp) s -= w * l1Q[cc][r - 1];
@@ -853,7 +887,10 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         }
       }
 #pragma unroll
-      for (int r = 0; r < TY; ++r) sQ[r] = sP[r];
+      for (int r = 0; r < TY; ++r) {
+        sQ[r] = sP[r];
+        spQ[r] = spP[r];
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
 #pragma unroll
@@ -873,8 +910,8 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #undef LD
 }
 
-template <typename T, int TY>
-__global__ __launch_bounds__(256) void k3_fused2_adj(Fused2Args<T> a) {
+template <typename T, int TY, int MINW>
+__global__ __launch_bounds__(256, MINW) void k3_fused2_adj(Fused2Args<T> a) {
   extern __shared__ unsigned char smem_raw[];
   int y0, z0;
   bool live;
@@ -883,27 +920,30 @@ __global__ __launch_bounds__(256) void k3_fused2_adj(Fused2Args<T> a) {
   adj2_tile<T, TY>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int MINW>
 static int launch_adj2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
   const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * a.a.nx;
   static bool attr_set = false;
   if (!attr_set) {
-    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_adj<T, TY>),
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_adj<T, TY, MINW>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k3_fused2_adj<T, TY>), dim3(nb), dim3(threads), smem, s, a);
+  hipLaunchKernelGGL((k3_fused2_adj<T, TY, MINW>), dim3(nb), dim3(threads), smem, s, a);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
 
 template <typename T>
 int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
-  if (ty == 4) return launch_adj2_ty<T, 4>(a, s);
-  if (ty == 2) return launch_adj2_ty<T, 2>(a, s);
+  const char* occ = getenv("DNX_FDTD3D_ADJ2_OCC");
+  const bool two = occ && occ[0] == '2';
+  if (ty == 4) return launch_adj2_ty<T, 4, 1>(a, s);
+  if (ty == 2) return two ? launch_adj2_ty<T, 2, 2>(a, s) : launch_adj2_ty<T, 2, 1>(a, s);
+  if (ty == 1) return two ? launch_adj2_ty<T, 1, 2>(a, s) : launch_adj2_ty<T, 1, 1>(a, s);
   set_error("fused2: unsupported tile height %d", ty);
   return DNX_EINVAL;
 }

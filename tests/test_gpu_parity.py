@@ -404,6 +404,29 @@ def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, monkeyp
         assert torch.equal(b, c), (b - c).abs().max()
 
 
+@pytest.mark.parametrize("ty,zc,occ", [(1, 5, "1"), (1, 64, "2"), (2, 7, "2"), (4, 3, "1")])
+def test_two_step_adjoint_tiles_bitwise(ty, zc, occ, monkeypatch):
+    """Every tile height / occupancy variant of the two-step adjoint gives
+    the one-step gradient bitwise."""
+    import diffnano_amd._lib as L
+    import ctypes as C
+    shape = (17, 11, 66)
+    t0, z0 = C.c_int32(), C.c_int32()
+    L.check(L.lib().dnx_fdtd3d_get_tiles(L.DNX_F32, 3, C.byref(t0), C.byref(z0)))
+    monkeypatch.setenv("DNX_FDTD3D_STEPS", "1")
+    one = _run_monitored(shape, 23, 8, 1)
+    monkeypatch.delenv("DNX_FDTD3D_STEPS")
+    monkeypatch.setenv("DNX_FDTD3D_ADJ2", "1")
+    monkeypatch.setenv("DNX_FDTD3D_ADJ2_OCC", occ)
+    try:
+        L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, 3, ty, zc))
+        two = _run_monitored(shape, 23, 8, 1)
+    finally:
+        L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, 3, t0.value, z0.value))
+    for a, b in zip(two, one):
+        assert torch.equal(a, b), (a - b).abs().max()
+
+
 def test_determinism_bitwise():
     g = golden("grad3d.npz")
     outs = []

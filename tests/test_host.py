@@ -161,3 +161,15 @@ def test_autotune_cache_corrupt_file_is_discarded(tmp_path):
     autotune._save_cache(str(p), {"k": {"ty": 4, "zc": 16}})
     assert autotune._load_cache(str(p)) == {"k": {"ty": 4, "zc": 16}}
     assert autotune._valid({"ty": 4, "zc": 16}) and not autotune._valid({"ty": 3, "zc": 16})
+
+
+def test_blob_batch_generator_matches_scipy_blobs():
+    """The batched (torch) blob generator gives the thresholded maps of the
+    scipy-filtered reference generator, for the 200x200 and 128x128 layouts."""
+    from diffnano_amd import workloads as W
+    seeds = list(range(24))
+    z = torch.stack([W.blob_noise(s) for s in seeds])
+    for (H, lo) in ((200, 60), (128, 24)):
+        e = W.blob_eps_batch(z, H, H, lo).numpy()
+        for i, s in enumerate(seeds):
+            np.testing.assert_array_equal(e[i], W.blob_eps_2d(s, H=H, W=H, lo=lo))
