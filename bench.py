@@ -16,8 +16,13 @@ kernel of known size).  cpu_baseline: the FP64 NumPy oracle (a port of the
 reference's CPU path), fwd+adj over a few steps of the same grid.
 
 ``--pmc-probe``: run a short fixed sequence (a 256 MiB device copy three
-times, then 4 fused forward and 4 fused adjoint steps of the workload grid)
-for hardware-counter collection; prints what it ran and exits.
+times, then 4 forward and 3 adjoint steps of the workload grid) for
+hardware-counter collection; prints what it ran and exits.
+
+``--config c3``: batched vs one-design-at-a-time Adam loop over 48 uneven 3D
+grids.  ``--config c4``: the surrogate-prescreened 2D design loop against
+the exhaustive batched and the plain loops (``--fno-probe``: the FNO
+kernels alone, for profiling).
 """
 
 from __future__ import annotations
@@ -123,9 +128,46 @@ def _pmc_probe(torch, dev, solver_kw):
     print(json.dumps({"pmc_probe": {"copy_bytes_read": a.numel() * 4,
                                     "copy_bytes_written": a.numel() * 4, "grid": list(shape),
                                     "launches": "4 forward steps (2 two-step launches), 3 "
-                                                "adjoint steps (1 two-step, 1 one-step launch)",
-                                    "fwd2_alg_bytes": 104 * N, "adj2_alg_bytes": 144 * N,
-                                    "adj_alg_bytes": 72 * N}}))
+                                                "adjoint steps (3 one-step launches)",
+                                    "fwd2_alg_bytes": 104 * N, "adj_alg_bytes": 72 * N}}))
+
+
+def _fno_probe():
+    """Five FNO inferences on 256 samples and five training steps on 128
+    (random weights and inputs) for kernel-level profiling of the MFMA
+    kernels; prints the times and the product FLOP rate."""
+    import torch
+
+    from diffnano_amd.fno import FNO2d
+
+    m = FNO2d(128, 128, in_ch=9, out_ch=6, width=32, modes=(16, 16), layers=4, hidden=128,
+              device="cuda")
+    x = torch.randn(256, 9, 128, 128, device="cuda")
+    with torch.no_grad():
+        m(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        for _ in range(5):
+            m(x)
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - t0) / 5
+    xb = x[:128]
+    y = m(xb)
+    (y * y).mean().backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        y = m(xb)
+        (y * y).mean().backward()
+    torch.cuda.synchronize()
+    t_tr = (time.perf_counter() - t0) / 5
+    fl = fno_flops_per_sample(m)
+    print(json.dumps({"fno_probe": {"predict_256_ms": round(t_inf * 1e3, 2),
+                                    "predict_tflops": round(256 * fl / t_inf / 1e12, 1),
+                                    "train_step_128_ms": round(t_tr * 1e3, 2),
+                                    "train_tflops": round(3 * 128 * fl / t_tr / 1e12, 1),
+                                    "flops_per_sample_fwd": fl}}), flush=True)
 
 
 def _bench_batched(args):
@@ -310,6 +352,7 @@ def main():
     ap.add_argument("--samples", type=int, default=20000, help="c4: surrogate training samples")
     ap.add_argument("--epochs", type=int, default=30, help="c4: surrogate training epochs")
     ap.add_argument("--finetune", type=int, default=4, help="c4: fine-tune steps per iteration")
+    ap.add_argument("--fno-probe", action="store_true", help="c4: FNO kernels only (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the FDTD "
@@ -319,7 +362,10 @@ def main():
         _bench_batched(args)
         return
     if args.config == "c4":
-        _bench_surrogate(args)
+        if args.fno_probe:
+            _fno_probe()
+        else:
+            _bench_surrogate(args)
         return
 
     import torch
