@@ -16,8 +16,9 @@ kernel of known size).  cpu_baseline: the FP64 NumPy oracle (a port of the
 reference's CPU path), fwd+adj over a few steps of the same grid.
 
 ``--pmc-probe``: run a short fixed sequence (a 256 MiB device copy three
-times, then 4 forward and 3 adjoint steps of the workload grid) for
-hardware-counter collection; prints what it ran and exits.
+times, then 4 forward and 4 adjoint steps of the workload grid, i.e. two
+launches of each two-step kernel) for hardware-counter collection; prints
+what it ran and exits.
 
 ``--config c3``: batched vs one-design-at-a-time Adam loop over 48 uneven 3D
 grids.  ``--config c4``: the surrogate-prescreened 2D design loop against
@@ -120,16 +121,16 @@ def _pmc_probe(torch, dev, solver_kw):
     state = torch.randn(6 * N, device=dev) * 1e-3
     work = torch.empty_like(state)
     eng.run(state, 0, 4, (None, 0), (None, 0), work=work)
-    estore = torch.empty(3 * 3 * N, device=dev)
+    estore = torch.empty(4 * 3 * N, device=dev)
     lam = torch.randn(6 * N, device=dev) * 1e-3
     S = torch.zeros(N, device=dev)
-    eng.adjoint_run(lam, 4, 0, None, estore, (None, 0), (None, 0), S, torch.empty_like(lam))
+    eng.adjoint_run(lam, 5, 0, None, estore, (None, 0), (None, 0), S, torch.empty_like(lam))
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": {"copy_bytes_read": a.numel() * 4,
                                     "copy_bytes_written": a.numel() * 4, "grid": list(shape),
-                                    "launches": "4 forward steps (2 two-step launches), 3 "
-                                                "adjoint steps (3 one-step launches)",
-                                    "fwd2_alg_bytes": 104 * N, "adj_alg_bytes": 72 * N}}))
+                                    "launches": "4 forward steps (2 two-step launches), 4 "
+                                                "adjoint steps (2 two-step launches)",
+                                    "fwd2_alg_bytes": 104 * N, "adj2_alg_bytes": 144 * N}}))
 
 
 def _fno_probe():

@@ -309,6 +309,9 @@ template <typename T>
 static bool use_pairs(const dnx_grid3d* gd, bool fused, int n_src, int n_mon, int64_t probe) {
   if (!std::is_same<T, float>::value || !fused || gd->dtype != DNX_F32) return false;
   if (n_src > 1 || n_mon > FUSED2_MAX_MON || probe >= 0) return false;
+  // the two-step forward addresses the six fields through one 32-bit buffer
+  // resource: 6 * N * 4 bytes must stay below 2^31
+  if (int64_t(gd->nx) * gd->ny * gd->nz * 6 * int64_t(sizeof(T)) >= (int64_t(1) << 31)) return false;
   const char* v = getenv("DNX_FDTD3D_STEPS");
   return !(v && strcmp(v, "1") == 0);
 }
@@ -502,11 +505,10 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   T* cur = lam;
   T* nxt = fused ? work : lam;
   const dim3 grid = grid3_launch(gd), block(TX, TY);
-  // The two-step adjoint is exact but slower than two one-step launches on
-  // gfx950 (its state does not fit the register file at two waves per
-  // SIMD), so it runs only when DNX_FDTD3D_ADJ2=1.
+  // Step pairs run the two-step adjoint (DNX_FDTD3D_ADJ2=0 turns it off;
+  // DNX_FDTD3D_STEPS=1 turns off both two-step kernels).
   const char* adj2 = getenv("DNX_FDTD3D_ADJ2");
-  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, -1) && adj2 && strcmp(adj2, "1") == 0;
+  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, -1) && !(adj2 && strcmp(adj2, "0") == 0);
   Fused2Args<T> f2{};
   int ty2 = 0;
   StreamScratch side;

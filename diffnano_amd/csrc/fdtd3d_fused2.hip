@@ -83,6 +83,26 @@ struct RowBox {
   }
 };
 
+// The same test with the per-row flags packed into one uniform word and the
+// column test kept per lane (fewer scalar registers in the forward tile).
+struct RowBits {
+  int dx, dy0;    // lane column and tile row 0 relative to the box corner
+  bool xin;       // the lane's column lies inside the box
+  uint32_t yin;   // bit r: row r lies inside the box (uniform)
+  __device__ void init(const Box& b, int x, int yrow0, int rows, bool enabled) {
+    dx = x - b.x0;
+    dy0 = yrow0 - b.y0;
+    xin = enabled && dx >= 0 && dx < b.nx;
+    yin = 0;
+    for (int r = 0; r < rows; ++r)
+      if (dy0 + r >= 0 && dy0 + r < b.ny) yin |= 1u << r;
+  }
+  __device__ bool row(int r) const { return xin && ((yin >> r) & 1u); }
+  __device__ int64_t index(const Box& b, int dz, int r) const {
+    return (int64_t(dz) * b.ny + dy0 + r) * b.nx + dx;
+  }
+};
+
 // e[comp] += w * prof[i] (the fused soft source, one component).
 template <typename T>
 __device__ __forceinline__ void add_src_at(int comp, T w, const T* prof, int64_t i, T (&e)[3]) {
@@ -126,7 +146,7 @@ __device__ __forceinline__ void add_source(const Fused3Args<T>& a, T w, int z, i
     if (cc == a.src_comp) e[cc] += s;
 }
 
-template <typename T, int TY>
+template <typename T, int TY, bool PF>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
@@ -145,10 +165,6 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   const int x = threadIdx.x;
   const bool xv = x < W;
   const int64_t N = a.N, P = a.P;
-  const T* __restrict__ Ein = a.in;
-  const T* __restrict__ Hin = a.in + 3 * N;
-  T* __restrict__ Eout = a.out;
-  T* __restrict__ Hout = a.out + 3 * N;
 
   const T kh = a.kh;
   const T bxv = xv ? a.bx[x] : T(0);
@@ -156,32 +172,51 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   const bool xf = xv && x >= 1, xl = x + 1 < W;
 
   T byr[R], ayr[R];
-  bool rv[R];
+  uint32_t rvm = 0;  // bit r: row r lies inside the grid
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int y = y0 - 2 + r;
-    rv[r] = y >= 0 && y < a.ny;
-    byr[r] = rv[r] ? a.by[y] : T(0);
-    ayr[r] = rv[r] ? a.ay[y] : T(0);
+    const bool in = y >= 0 && y < a.ny;
+    if (in) rvm |= 1u << r;
+    byr[r] = in ? a.by[y] : T(0);
+    ayr[r] = in ? a.ay[y] : T(0);
   }
-  const int64_t colbase = int64_t(y0 - 2) * W + x;  // row r at colbase + r*W
+  auto rv = [&](int r) { return ((rvm >> r) & 1u) != 0; };
   // Loads use a clamped row and column so that every one is in bounds and
-  // none needs a branch; values of rows / columns outside the grid are
-  // masked afterwards.  Row pointers are uniform, the column is the lane.
+  // none needs a branch.  Values of rows or columns outside the grid are
+  // never combined into a cell of the grid (every difference across the
+  // grid edge is selected away), so they are not masked.  An access is a
+  // uniform plane base plus the lane's unsigned 32-bit byte offset within
+  // the plane; stores (rows and columns inside the grid only) use the same
+  // offsets, where clamped and true coordinates agree.
   const int xc = xv ? x : W - 1;
-  int yrow[R];
+  uint32_t voff[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) yrow[r] = min(max(y0 - 2 + r, 0), a.ny - 1);
-  auto ld = [&](const T* base, int64_t plane, int r) -> T {
-    return (base + plane * P + int64_t(yrow[r]) * W)[xc];
+  for (int r = 0; r < R; ++r)
+    voff[r] = uint32_t((min(max(y0 - 2 + r, 0), a.ny - 1) * W + xc) * int(sizeof(T)));
+  // Buffer accesses: the descriptor base is the plane of an array (uniform),
+  // the component offset is scalar, the lane's in-plane offset the vector
+  // part.  Every offset stays below 2^31 bytes (checked on the host).
+  static_assert(sizeof(T) == 4, "the two-step forward kernel is float32");
+  const int cbytes = int(N * int64_t(sizeof(T)));  // one component
+  auto rsrc = [&](const T* base, int64_t plane, int ncomp) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base + plane * P), 0,
+                                             int((ncomp * N - plane * P) * int64_t(sizeof(T))),
+                                             0x00020000);
+  };
+  auto ld = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r) -> T {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, int(voff[r]), c * cbytes, 0));
+  };
+  auto st = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r, T v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r]), c * cbytes, 0);
   };
 
   const bool has_src = a.src_comp >= 0;
   const T w1 = has_src ? a.src_wave[A2.n] : T(0);
   const T w2 = has_src ? a.src_wave[A2.n + 1] : T(0);
-  RowBox<R> sbx, mbx;  // source and monitor box tests
-  sbx.init(a.src_box, x, y0 - 2, has_src && xv);
-  mbx.init(A2.mon[0].box, x, y0 - 2, A2.n_mon > 0 && xv);
+  RowBits sbx, mbx;  // source and monitor box tests
+  sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
+  mbx.init(A2.mon[0].box, x, y0 - 2, R, A2.n_mon > 0 && xv);
 
   T e0C[3][R], e0N[3][R];
   T h1F[3][TY + 2];  // H1 at plane p-1, rows 1..TY+2 (index r-1)
@@ -196,10 +231,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const T v = ld(Ein + c * N, zs, r);
-        e0C[c][r] = (rv[r] && xv) ? v : T(0);
-      }
+      for (int c = 0; c < 3; ++c) e0C[c][r] = ld(rsrc(a.in, zs, 6), c, r);
     }
     if (xv) {
 #pragma unroll
@@ -216,6 +248,28 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
     for (int r = 0; r < TY; ++r) h2P[0][r] = h2P[1][r] = ceQ[r] = T(0);
   }
+  // PF: the loads of stage A (E at p+1, H and ce at p) are issued one plane
+  // ahead, right after stage A of the previous plane, so that they are in
+  // flight during stage B and the barrier.
+  T pfE[3][R], pfH[3][TY + 3], pfC[TY + 2];
+  auto prefetch = [&](int pn) {
+    const int pe = pn + 1 < a.nz ? pn + 1 : pn;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pfE[c][r] = ld(rsrc(a.in, pe, 6), c, r);
+    }
+#pragma unroll
+    for (int r = 0; r <= TY + 2; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pfH[c][r] = ld(rsrc(a.in, pn, 6), 3 + c, r);
+    }
+#pragma unroll
+    for (int r = 1; r <= TY + 2; ++r) pfC[r - 1] = ld(rsrc(a.ce, pn, 1), 0, r);
+  };
+  if constexpr (PF) {
+    if (zs < a.nz) prefetch(zs);
+  }
   __syncthreads();
 
 #pragma unroll 1
@@ -231,34 +285,39 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     // ---------------- stage A, part 1: loads and H1(p)
     if (doA) {
       T h0[3][TY + 3];
+      if constexpr (PF) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+        for (int r = 0; r < R; ++r) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-        {
-          const T v = ld(Ein + c * N, zl ? p + 1 : p, r);
-          e0N[c][r] = (zl && rv[r] && xv) ? v : T(0);
+          for (int c = 0; c < 3; ++c) e0N[c][r] = pfE[c][r];
         }
-      }
 #pragma unroll
-      for (int r = 0; r <= TY + 2; ++r) {
+        for (int r = 0; r <= TY + 2; ++r) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T v = ld(Hin + c * N, p, r);
-          h0[c][r] = (rv[r] && xv) ? v : T(0);
+          for (int c = 0; c < 3; ++c) h0[c][r] = pfH[c][r];
         }
-      }
 #pragma unroll
-      for (int r = 1; r <= TY + 2; ++r) {
-        const T v = ld(a.ce, p, r);
-        ceP[r - 1] = (rv[r] && xv) ? v : T(0);
+        for (int r = 1; r <= TY + 2; ++r) ceP[r - 1] = pfC[r - 1];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) e0N[c][r] = ld(rsrc(a.in, zl ? p + 1 : p, 6), c, r);
+        }
+#pragma unroll
+        for (int r = 0; r <= TY + 2; ++r) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) h0[c][r] = ld(rsrc(a.in, p, 6), 3 + c, r);
+        }
+#pragma unroll
+        for (int r = 1; r <= TY + 2; ++r) ceP[r - 1] = ld(rsrc(a.ce, p, 1), 0, r);
       }
       {
         int dzs;
         const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
 #pragma unroll
         for (int r = 1; r <= TY + 2; ++r)
-          spP[r - 1] = (sz && sbx.yin[r]) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
+          spP[r - 1] = (sz && sbx.row(r)) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
       }
       const int pb = p & 1;
 #pragma unroll
@@ -295,7 +354,6 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     // ---------------- stage A, part 2: E1(p) (all rows the tile needs)
     if (doA) {
       const bool own = p >= z0 && p < z1;
-      const int64_t pz = int64_t(p) * P + colbase;
       int dzsA, dzmA;
       const bool szA = RowBox<R>::zin(a.src_box, p, dzsA);
       const bool mzA = RowBox<R>::zin(A2.mon[0].box, p, dzmA);
@@ -319,19 +377,21 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         e[0] = bxy * e0C[0][r] + c * (ayr[r] * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e0C[1][r] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
         e[2] = bxy * e0C[2][r] + c * (axv * dHy_dx - ayr[r] * dHx_dy);
-        if (szA && sbx.yin[r]) add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);
+        if (szA && sbx.row(r)) add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) e1P[cc][r - 1] = e[cc];
-        if (own && r >= 2 && r <= TY + 1 && rv[r] && xv) {
+        if (own && r >= 2 && r <= TY + 1 && rv(r) && xv) {
           if (a.es) {
-            const int64_t i = pz + r * W;
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) a.es[cc * N + i] = e[cc];
+            for (int cc = 0; cc < 3; ++cc) st(rsrc(a.es, p, 3), cc, r, e[cc]);
           }
-          if (mzA && mbx.yin[r])
+          if (mzA && mbx.row(r))
             mon_stage_at<T>(A2.mon[0], A2.side[0], 0, mbx.index(A2.mon[0].box, dzmA, r), e);
         }
       }
+    }
+    if constexpr (PF) {
+      if (p + 1 < a.nz) prefetch(p + 1);
     }
     // ---------------- stage B, part 1: H2(q)
     T h2N[3][TY + 1];  // H2 at plane q, rows 1..TY+1 (index r-1)
@@ -374,7 +434,6 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     // ---------------- stage B, part 2: E2(q), store
     if (doB) {
       const bool own = q >= z0;
-      const int64_t pz = int64_t(q) * P + colbase;
       int dzsB, dzmB;
       const bool szB = RowBox<R>::zin(a.src_box, q, dzsB);
       const bool mzB = RowBox<R>::zin(A2.mon[0].box, q, dzmB);
@@ -398,19 +457,18 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         e[0] = bxy * e1Q[0][r - 1] + c * (ayr[r] * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e1Q[1][r - 1] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
         e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ayr[r] * dHx_dy);
-        if (szB && sbx.yin[r]) add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
-        if (own && rv[r] && xv) {
-          const int64_t i = pz + r * W;
+        if (szB && sbx.row(r)) add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
+        if (own && rv(r) && xv) {
 #pragma unroll
           for (int cc = 0; cc < 3; ++cc) {
-            Eout[cc * N + i] = e[cc];
-            Hout[cc * N + i] = h2N[cc][r - 1];
+            st(rsrc(a.out, q, 6), cc, r, e[cc]);
+            st(rsrc(a.out, q, 6), 3 + cc, r, h2N[cc][r - 1]);
           }
           if (A2.es2) {
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) A2.es2[cc * N + i] = e[cc];
+            for (int cc = 0; cc < 3; ++cc) st(rsrc(A2.es2, q, 3), cc, r, e[cc]);
           }
-          if (mzB && mbx.yin[r])
+          if (mzB && mbx.row(r))
             mon_stage_at<T>(A2.mon[0], A2.side[0], 1, mbx.index(A2.mon[0].box, dzmB, r), e);
         }
       }
@@ -459,14 +517,14 @@ __device__ inline void block_tile2(int nblocks, int ty_tiles, int zc, int& y0, i
   z0 = (logical / ty_tiles) * zc;
 }
 
-template <typename T, int TY>
-__global__ __launch_bounds__(256, TY == 2 ? 2 : 1) void k3_fused2_fwd(Fused2Args<T> a) {
+template <typename T, int TY, bool PF>
+__global__ __launch_bounds__(256, 2) void k3_fused2_fwd(Fused2Args<T> a) {
   extern __shared__ unsigned char smem_raw[];
   int y0, z0;
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  fwd2_tile<T, TY>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  fwd2_tile<T, TY, PF>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
 
 template <int TY>
@@ -474,27 +532,31 @@ constexpr int fwd2_lds_elems() {
   return 4 * (TY + 3) + 2 * (TY + 2) + 4 * (TY + 1) + 2 * TY;
 }
 
-template <typename T, int TY>
+template <typename T, int TY, bool PF>
 static int launch_fwd2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
   const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * a.a.nx;
   static bool attr_set = false;
   if (!attr_set) {
-    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY>),
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k3_fused2_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, a);
+  hipLaunchKernelGGL((k3_fused2_fwd<T, TY, PF>), dim3(nb), dim3(threads), smem, s, a);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
 
 template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
-  if (ty == 4) return launch_fwd2_ty<T, 4>(a, s);
-  if (ty == 2) return launch_fwd2_ty<T, 2>(a, s);
+  static const int pf = [] {
+    const char* v = getenv("DNX_FDTD3D_FWD2_PF");
+    return v ? atoi(v) : 0;
+  }();
+  if (ty == 4) return pf ? launch_fwd2_ty<T, 4, true>(a, s) : launch_fwd2_ty<T, 4, false>(a, s);
+  if (ty == 2) return pf ? launch_fwd2_ty<T, 2, true>(a, s) : launch_fwd2_ty<T, 2, false>(a, s);
   set_error("fused2: unsupported tile height %d", ty);
   return DNX_EINVAL;
 }
@@ -565,10 +627,6 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
   const int x = threadIdx.x;
   const bool xv = x < W;
   const int64_t N = a.N, P = a.P;
-  const T* __restrict__ Lin = a.in;
-  const T* __restrict__ Min = a.in + 3 * N;
-  T* __restrict__ Lout = a.out;
-  T* __restrict__ Mout = a.out + 3 * N;
 
   const T kh = a.kh;
   const T bxv = xv ? a.bx[x] : T(0);
@@ -578,22 +636,48 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
   const bool xf = xv && x >= 1, xl = x + 1 < W, xin = x <= W - 2;
 
   T byr[R], ayr[R];
-  bool rv[R];
+  uint32_t rvm = 0;  // bit r: row r lies inside the grid
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int y = y0 - 2 + r;
-    rv[r] = y >= 0 && y < a.ny;
-    byr[r] = rv[r] ? a.by[y] : T(0);
-    ayr[r] = rv[r] ? a.ay[y] : T(0);
+    const bool in = y >= 0 && y < a.ny;
+    if (in) rvm |= 1u << r;
+    byr[r] = in ? a.by[y] : T(0);
+    ayr[r] = in ? a.ay[y] : T(0);
   }
-  const int64_t colbase = int64_t(y0 - 2) * W + x;
+  auto rv = [&](int r) { return ((rvm >> r) & 1u) != 0; };
+  // Accesses as in fwd2_tile: clamped row and column (every load in bounds,
+  // no branch), a uniform buffer descriptor per array plane, the component
+  // as scalar offset and the lane's in-plane byte offset as vector offset.
+  // Planes are always in [0, nz): stage A touches plane p < nz, stage B
+  // plane q = p - 1 >= 0.  Values of rows or columns outside the grid are
+  // never combined into a cell of the grid (each difference across the grid
+  // edge is selected away); stores cover grid cells only.
+  const int xc = xv ? x : W - 1;
+  uint32_t voff[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    voff[r] = uint32_t((min(max(y0 - 2 + r, 0), a.ny - 1) * W + xc) * int(sizeof(T)));
+  static_assert(sizeof(T) == 4, "the two-step adjoint kernel is float32");
+  const int cbytes = int(N * int64_t(sizeof(T)));
+  auto rsrc = [&](const T* base, int64_t plane, int ncomp) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base + plane * P), 0,
+                                             int((ncomp * N - plane * P) * int64_t(sizeof(T))),
+                                             0x00020000);
+  };
+  auto ld = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r) -> T {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, int(voff[r]), c * cbytes, 0));
+  };
+  auto st = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r, T v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r]), c * cbytes, 0);
+  };
 
   const int m = A2.n;  // sub-steps m (stage A) and m-1 (stage B)
   const bool has_src = a.src_comp >= 0;
   const T w1 = has_src ? a.src_wave[m] : T(0);
   const T w2 = has_src ? a.src_wave[m - 1] : T(0);
-  RowBox<R> sbx;
-  sbx.init(a.src_box, x, y0 - 2, has_src && xv);
+  RowBits sbx;
+  sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
   T spQ[TY];  // source profile at plane p-1, owned rows 2..TY+1 (0 outside the box)
 #pragma unroll
   for (int r = 0; r < TY; ++r) spQ[r] = T(0);
@@ -608,15 +692,15 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
   T sQ[TY];               // S of plane p-1 after stage A (owned rows 2..TY+1)
   const int zs = z0 >= 2 ? z0 - 2 : 0;
   {
-    const int64_t pb = int64_t(zs) * P + colbase;
+    const auto rc = rsrc(a.ce, zs, 1);
+    const auto rl = rsrc(a.in, zs, 6);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const bool ok = rv[r] && xv;
-      const T c = ok ? a.ce[pb + r * W] : T(0);
+      const T c = ld(rc, 0, r);
       if (r >= 1 && r <= TY + 2) ceC[r - 1] = c;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const T l = ok ? Lin[k * N + pb + r * W] : T(0);
+        const T l = ld(rl, k, r);
         uC[k][r] = c * l;
         if (r >= 1 && r <= TY + 2) lC[k][r - 1] = l;
       }
@@ -651,42 +735,57 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
     T spP[TY];  // source profile at plane p, owned rows 2..TY+1
     T dpre[FUSED2_MAX_MON][TY + 2][3];  // monitor adjoint source, rows 1..TY+2
     bool dhit[FUSED2_MAX_MON][TY + 2];
+    // E^m, S at plane p and E^{m-1} at plane q (owned rows 2..TY+1) are
+    // loaded here and consumed after the barriers.
+    T emA[3][TY], sIn[TY], emB[3][TY];
     // ---------------- stage A, part 1: loads and mu^m(p)
     if (doA) {
       T m0[3][TY + 3];
-      const int64_t pn = int64_t(p + 1) * P + colbase;
-      const int64_t pz = int64_t(p) * P + colbase;
+      {
+        const int pn = zl ? p + 1 : p;  // (the values of plane p+1 are unused when p+1 = nz)
+        const auto rc = rsrc(a.ce, pn, 1);
+        const auto rl = rsrc(a.in, pn, 6);
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const bool ok = zl && rv[r] && xv;
-        const T c = ok ? a.ce[pn + r * W] : T(0);
-        if (r >= 1 && r <= TY + 2) ceN[r - 1] = c;
+        for (int r = 0; r < R; ++r) {
+          const T c = ld(rc, 0, r);
+          if (r >= 1 && r <= TY + 2) ceN[r - 1] = c;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const T l = ok ? Lin[k * N + pn + r * W] : T(0);
-          uN[k][r] = c * l;
-          if (r >= 1 && r <= TY + 2) lN[k][r - 1] = l;
+          for (int k = 0; k < 3; ++k) {
+            const T l = ld(rl, k, r);
+            uN[k][r] = c * l;
+            if (r >= 1 && r <= TY + 2) lN[k][r - 1] = l;
+          }
         }
       }
+      {
+        const auto rm = rsrc(a.in, p, 6);
 #pragma unroll
-      for (int r = 0; r <= TY + 2; ++r) {
+        for (int r = 0; r <= TY + 2; ++r) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) m0[k][r] = (rv[r] && xv) ? Min[k * N + pz + r * W] : T(0);
+          for (int k = 0; k < 3; ++k) m0[k][r] = ld(rm, 3 + k, r);
+        }
+        const auto re = rsrc(a.em, p, 3);
+        const auto rs = rsrc(a.S, p, 1);
+#pragma unroll
+        for (int r = 2; r <= TY + 1; ++r) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) emA[k][r - 2] = ld(re, k, r);
+          sIn[r - 2] = ld(rs, 0, r);
+        }
       }
       {
         int dzs;
         const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
 #pragma unroll
         for (int r = 2; r <= TY + 1; ++r)
-          spP[r - 2] = (sz && sbx.yin[r]) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
+          spP[r - 2] = (sz && sbx.row(r)) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
       }
-      // adjoint source of sample m-1 for lamE^m(p), rows 1..TY+2 (issued
-      // with the field loads, consumed after the barrier)
+      // adjoint source of sample m-1 for lamE^m(p), rows 1..TY+2
 #pragma unroll
       for (int j = 0; j < FUSED2_MAX_MON; ++j) {
 #pragma unroll
         for (int r = 1; r <= TY + 2; ++r) {
-          dhit[j][r - 1] = j < A2.n_mon && rv[r] && xv &&
+          dhit[j][r - 1] = j < A2.n_mon && rv(r) && xv &&
                            mon_fetch<T>(A2.mon[j], A2.side[j], p, y0 - 2 + r, x, dpre[j][r - 1]);
         }
       }
@@ -721,16 +820,23 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         }
       }
     }
+    if (doB) {
+      const auto re = rsrc(A2.em2, q, 3);
+#pragma unroll
+      for (int r = 2; r <= TY + 1; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) emB[k][r - 2] = ld(re, k, r);
+      }
+    }
     __syncthreads();
     // ---------------- stage A, part 2: lamE^m(p), S partial, monitor term
     if (doA) {
       const bool own = p >= z0 && p < z1;
-      const int64_t pz = int64_t(p) * P + colbase;
 #pragma unroll
       for (int r = 1; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
         const bool yf = y >= 1, yin = y <= a.ny - 2;
-        const bool mine = own && r >= 2 && r <= TY + 1 && rv[r] && xv;
+        const bool mine = own && r >= 2 && r <= TY + 1 && rv(r) && xv;
         const T mx = m1N[0][r], my = m1N[1][r], mz = m1N[2][r];
         const T my_x0 = xf ? (&LB(0, r))[x - 1] : T(0);
         const T mz_x0 = xf ? (&LB(1, r))[x - 1] : T(0);
@@ -750,21 +856,22 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         for (int k = 0; k < 3; ++k) l[k] = bxy * lC[k][r - 1] + t[k];
         T em[3] = {T(0), T(0), T(0)};
         T s = T(0);
-        if (mine) {
-          const int64_t i = pz + r * W;
+        if (r >= 2 && r <= TY + 1) {
+          if (mine) {
 #pragma unroll
-          for (int k = 0; k < 3; ++k) em[k] = a.em[k * N + i];
-          s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
-          if (has_src && sbx.yin[r] && r >= 2 && r <= TY + 1) {
-            int dzs;
-            if (RowBox<R>::zin(a.src_box, p, dzs)) {
-              const T w = w1 * spP[r - 2];
+            for (int k = 0; k < 3; ++k) em[k] = emA[k][r - 2];
+            s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
+            if (sbx.row(r)) {
+              int dzs;
+              if (RowBox<R>::zin(a.src_box, p, dzs)) {
+                const T w = w1 * spP[r - 2];
 #pragma unroll
-              for (int cc = 0; cc < 3; ++cc)
-                if (cc == a.src_comp) s -= w * lC[cc][r - 1];
+                for (int cc = 0; cc < 3; ++cc)
+                  if (cc == a.src_comp) s -= w * lC[cc][r - 1];
+              }
             }
+            s = sIn[r - 2] + s;
           }
-          s = a.S[i] + s;
         }
 #pragma unroll
         for (int j = 0; j < FUSED2_MAX_MON; ++j) {
@@ -822,12 +929,13 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
     // ---------------- stage B, part 2: lamE^{m-1}(q), S, store
     if (doB) {
       const bool own = q >= z0;
-      const int64_t pz = int64_t(q) * P + colbase;
+      const auto ro = rsrc(a.out, q, 6);
+      const auto rs = rsrc(a.S, q, 1);
 #pragma unroll
       for (int r = 2; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
         const bool yf = y >= 1, yin = y <= a.ny - 2;
-        const bool mine = own && rv[r] && xv;
+        const bool mine = own && rv(r) && xv;
         const T mx = m2N[0][r - 1], my = m2N[1][r - 1], mz = m2N[2][r - 1];
         const T my_x0 = xf ? (&LD(0, r))[x - 1] : T(0);
         const T mz_x0 = xf ? (&LD(1, r))[x - 1] : T(0);
@@ -846,12 +954,11 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
         for (int k = 0; k < 3; ++k) l[k] = bxy * l1Q[k][r - 1] + t[k];
         if (mine) {
-          const int64_t i = pz + r * W;
           T em[3];
 #pragma unroll
-          for (int k = 0; k < 3; ++k) em[k] = A2.em2[k * N + i];
+          for (int k = 0; k < 3; ++k) em[k] = emB[k][r - 2];
           T s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
-          if (has_src && sbx.yin[r]) {
+          if (sbx.row(r)) {
             int dzs;
             if (RowBox<R>::zin(a.src_box, q, dzs)) {
               const T w = w2 * spQ[r - 2];
@@ -863,10 +970,10 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
           s = sQ[r - 2] + s;  // the last sample's monitor term follows the launch
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            Lout[k * N + i] = l[k];
-            Mout[k * N + i] = m2N[k][r - 1];
+            st(ro, k, r, l[k]);
+            st(ro, 3 + k, r, m2N[k][r - 1]);
           }
-          a.S[i] = s;
+          st(rs, 0, r, s);
         }
       }
 #pragma unroll
@@ -939,8 +1046,10 @@ static int launch_adj2_ty(const Fused2Args<T>& a, hipStream_t s) {
 
 template <typename T>
 int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
+  // occupancy 2 (launch bounds for two work-groups per CU) unless
+  // DNX_FDTD3D_ADJ2_OCC=1; TY = 4 fits the register file only at occupancy 1
   const char* occ = getenv("DNX_FDTD3D_ADJ2_OCC");
-  const bool two = occ && occ[0] == '2';
+  const bool two = !(occ && occ[0] == '1');
   if (ty == 4) return launch_adj2_ty<T, 4, 1>(a, s);
   if (ty == 2) return two ? launch_adj2_ty<T, 2, 2>(a, s) : launch_adj2_ty<T, 2, 1>(a, s);
   if (ty == 1) return two ? launch_adj2_ty<T, 1, 2>(a, s) : launch_adj2_ty<T, 1, 1>(a, s);

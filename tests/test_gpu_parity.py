@@ -390,10 +390,10 @@ def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, monkeyp
     try:
         for k in (2, 3):
             L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, ty, zc))
+        monkeypatch.setenv("DNX_FDTD3D_ADJ2", "0")
         two = _run_monitored(shape, T, ckpt, n_mon)            # two-step forward
-        monkeypatch.setenv("DNX_FDTD3D_ADJ2", "1")
-        two_adj = _run_monitored(shape, T, ckpt, n_mon)        # and two-step adjoint
         monkeypatch.delenv("DNX_FDTD3D_ADJ2")
+        two_adj = _run_monitored(shape, T, ckpt, n_mon)        # and two-step adjoint
     finally:
         for k, t, z in saved:
             L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, t, z))
