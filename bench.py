@@ -182,7 +182,7 @@ def _bench_batched(args):
     probs_b = inverse_design.batched_problems(args.designs, seed=0)
     classes = sorted({p.shape for p in probs_b})
     tiles_plain = {s: autotune.tune(s, apply=False) for s in classes}
-    tiles_batch = autotune.tune_batch(classes * 2, apply=False)
+    tiles_batch = autotune.tune_batch([p.shape for p in probs_b], apply=False)
     res_b = inverse_design.adam_loop(probs_b, iterations=args.iters, n_steps=T, mode="batched",
                                      device=dev, tiles=tiles_batch)
     probs_p = inverse_design.batched_problems(args.designs, seed=0)

@@ -100,6 +100,8 @@ _SIGS = {
     "dnx_batch3d_forward": ([_P, _I, _P], _I),
     "dnx_batch3d_backward": ([_P, _P], _I),
     "dnx_batch3d_uses_graph": ([_P], _I),
+    "dnx_batch3d_final_slot": ([_P], _I),
+    "dnx_batch3d_uses_pairs": ([_P], _I),
     "dnx_batch3d_destroy": ([_P], _I),
     "dnx_fdtd2d_run": ([C.POINTER(Grid2D), _P, _P, _I, _I, C.POINTER(Source), _I,
                         C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),

@@ -147,6 +147,12 @@ class BatchedFDTD3D:
     def uses_graph(self):
         return bool(L.lib().dnx_batch3d_uses_graph(self._plan))
 
+    @property
+    def uses_pairs(self):
+        """(forward, adjoint) run the two-step kernels."""
+        v = int(L.lib().dnx_batch3d_uses_pairs(self._plan))
+        return bool(v & 1), bool(v & 2)
+
     # -- device steps ---------------------------------------------------------
     def _set_eps(self, eps_list):
         st = L.stream_handle(self.dev)
@@ -184,7 +190,7 @@ class BatchedFDTD3D:
 
     def final_fields(self):
         """(Ez, Ex, Ey) of every design after the last forward, float64."""
-        buf = self.state[self.n_steps & 1]
+        buf = self.state[int(L.lib().dnx_batch3d_final_slot(self._plan))]
         out = []
         for d, shape in enumerate(self.shapes):
             N = int(np.prod(shape))

@@ -306,11 +306,15 @@ static Fused3Args<T> fused_args(const dnx_grid3d* gd, const Fused3Tiles& t) {
 // FUSED2_MAX_MON monitors (fused), no probe.  DNX_FDTD3D_STEPS=1 forces the
 // one-step kernels.
 template <typename T>
-static bool use_pairs(const dnx_grid3d* gd, bool fused, int n_src, int n_mon, int64_t probe) {
+static bool use_pairs(const dnx_grid3d* gd, bool fused, int n_src, int n_mon, int64_t probe,
+                      int kernel) {
   if (!std::is_same<T, float>::value || !fused || gd->dtype != DNX_F32) return false;
   if (n_src > 1 || n_mon > FUSED2_MAX_MON || probe >= 0) return false;
-  // the two-step forward addresses the six fields through one 32-bit buffer
-  // resource: 6 * N * 4 bytes must stay below 2^31
+  // tile height 0 for kernel 2 / 3: two-step path off (the autotuner's
+  // choice where the one-step kernels are faster)
+  if (fused3_tiles(gd->dtype, kernel).ty <= 0) return false;
+  // the two-step kernels address the six fields through one 32-bit buffer
+  // offset: 6 * N * 4 bytes must stay below 2^31
   if (int64_t(gd->nx) * gd->ny * gd->nz * 6 * int64_t(sizeof(T)) >= (int64_t(1) << 31)) return false;
   const char* v = getenv("DNX_FDTD3D_STEPS");
   return !(v && strcmp(v, "1") == 0);
@@ -387,7 +391,7 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
   T* cur = st;
   T* nxt = fused ? work : st;
   const dim3 grid = grid3_launch(gd), block(TX, TY);
-  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, probe);
+  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, probe, 2);
   Fused2Args<T> f2{};
   int ty2 = 0;
   StreamScratch side;
@@ -508,7 +512,8 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   // Step pairs run the two-step adjoint (DNX_FDTD3D_ADJ2=0 turns it off;
   // DNX_FDTD3D_STEPS=1 turns off both two-step kernels).
   const char* adj2 = getenv("DNX_FDTD3D_ADJ2");
-  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, -1) && !(adj2 && strcmp(adj2, "0") == 0);
+  const bool pairs =
+      use_pairs<T>(gd, fused, n_src, n_mon, -1, 3) && !(adj2 && strcmp(adj2, "0") == 0);
   Fused2Args<T> f2{};
   int ty2 = 0;
   StreamScratch side;

@@ -5,6 +5,9 @@
 // replay and the adjoint.  Semantics per design are exactly those of
 // dnx_fdtd3d_run / dnx_fdtd3d_adjoint_run (same tile bodies).
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fdtd3d_fused.h"
@@ -15,25 +18,37 @@
 struct dnx_batch3d_plan {
   int dtype = 0, n = 0, T = 0, K = 0;
   int ty_f = 0, ty_a = 0, max_w = 0, grid_f = 0, grid_a = 0;
+  // two-step path (float32): forward / adjoint in step pairs, buffer layout 1
+  bool fpairs = false, apairs = false;
+  int ty_f2 = 0, ty_a2 = 0, grid_f2 = 0, grid_a2 = 0;
+  int lam1_slot = 1;  // adjoint pool holding lam^1 after the backward sweep
   int64_t max_mon_cells = 0, sum6 = 0, sum3 = 0, sumN = 0;
-  std::vector<unsigned char> host_tables;  // staging of the two descriptor tables
+  std::vector<unsigned char> host_tables;  // staging of the four descriptor tables
   void* d_fwd = nullptr;
   void* d_adj = nullptr;
+  void* d_fwd2 = nullptr;
+  void* d_adj2 = nullptr;
+  void* side_mem = nullptr;  // per-design monitor buffers of the two-step path
   int* d_ctr = nullptr;
-  dnx::Batch3Launch Lf{}, La{};
+  dnx::Batch3Launch Lf{}, La{}, Lf2{}, La2{};
   dnx_batch3d_pools pools{};
   std::vector<dnx_design3d> designs;
   std::vector<int64_t> off;  // sum_{e<d} N_e
+  std::vector<int64_t> side_off;  // element offset of design d's monitor buffer
   bool use_graph = false;
   hipGraphExec_t g_fwd = nullptr, g_rep = nullptr, g_adj = nullptr;
 };
 
 namespace dnx {
 
+// Descriptor table of one launch kind: kernel 0 / 1 (one-step forward /
+// adjoint) or 2 / 3 (two-step), whose tile shape sets the tiles per design.
 template <typename T>
-static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, bool adjoint,
+static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, int kernel,
                        Batch3Design<T>* out, int* total_tiles) {
-  const Fused3Tiles t = fused3_tiles(p->dtype, adjoint ? 1 : 0);
+  const bool adjoint = kernel == 1 || kernel == 3;
+  Fused3Tiles t = fused3_tiles(p->dtype, kernel);
+  if (t.ty <= 0) t.ty = 2;  // two-step kernel switched off: table unused
   int start = 0;
   for (int d = 0; d < p->n; ++d) {
     const dnx_design3d& de = ds[d];
@@ -77,6 +92,7 @@ static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, bool adjoint
       b.m_acc = de.mon.acc;
       b.m_grad = de.mon.grad;
     }
+    b.m_side = p->side_mem ? static_cast<T*>(p->side_mem) + p->side_off[d] : nullptr;
     out[d] = b;
   }
   *total_tiles = start;
@@ -84,19 +100,47 @@ static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, bool adjoint
 
 template <typename T>
 static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hipStream_t s) {
+  // Two-step path: float32, not switched off, every design addressable by
+  // the two-step kernels' 32-bit buffer offsets (6 N 4 bytes < 2^31), and
+  // tile heights the batched two-step kernels are built for.
+  if (std::is_same<T, float>::value) {
+    const char* st = getenv("DNX_FDTD3D_STEPS");
+    const char* a2 = getenv("DNX_FDTD3D_ADJ2");
+    bool ok = !(st && strcmp(st, "1") == 0);
+    for (int d = 0; d < p->n && ok; ++d) {
+      const dnx_grid3d& g = ds[d].grid;
+      ok = int64_t(g.nz) * g.ny * g.nx * 6 * int64_t(sizeof(T)) < (int64_t(1) << 31);
+    }
+    const int tf2 = fused3_tiles(p->dtype, 2).ty, ta2 = fused3_tiles(p->dtype, 3).ty;
+    p->fpairs = ok && (tf2 == 2 || tf2 == 4);
+    p->apairs = ok && (ta2 == 2 || ta2 == 4) && !(a2 && strcmp(a2, "0") == 0);
+  }
+  if ((p->fpairs || p->apairs) && p->max_mon_cells > 0) {
+    int64_t acc = 0;
+    for (int d = 0; d < p->n; ++d) {
+      p->side_off.push_back(acc);
+      const dnx_monitor& m = ds[d].mon;
+      if (m.n_freq > 0) {
+        const int mk = m.comp_mask;
+        acc += 2 * int64_t((mk & 1) + ((mk >> 1) & 1) + ((mk >> 2) & 1)) * box_cells(m.box);
+      }
+    }
+    DNX_CHECK_HIP(hipMalloc(&p->side_mem, sizeof(T) * size_t(std::max<int64_t>(acc, 1))));
+  }
   const size_t tb = sizeof(Batch3Design<T>) * p->n;
-  p->host_tables.resize(2 * tb);
-  auto* tf = reinterpret_cast<Batch3Design<T>*>(p->host_tables.data());
-  auto* ta = reinterpret_cast<Batch3Design<T>*>(p->host_tables.data() + tb);
-  int tot_f = 0, tot_a = 0;
-  fill_table<T>(p, ds, false, tf, &tot_f);
-  fill_table<T>(p, ds, true, ta, &tot_a);
+  const size_t tbr = ((tb + 255) / 256) * 256;
+  p->host_tables.assign(4 * tb, 0);
+  Batch3Design<T>* tab[4];
+  int tot[4] = {0, 0, 0, 0};
   auto* base = static_cast<unsigned char*>(ws);
-  p->d_fwd = base;
-  p->d_adj = base + ((tb + 255) / 256) * 256;
-  p->d_ctr = reinterpret_cast<int*>(base + 2 * ((tb + 255) / 256) * 256);
-  DNX_CHECK_HIP(hipMemcpyAsync(p->d_fwd, tf, tb, hipMemcpyHostToDevice, s));
-  DNX_CHECK_HIP(hipMemcpyAsync(p->d_adj, ta, tb, hipMemcpyHostToDevice, s));
+  void** dst[4] = {&p->d_fwd, &p->d_adj, &p->d_fwd2, &p->d_adj2};
+  for (int k = 0; k < 4; ++k) {
+    tab[k] = reinterpret_cast<Batch3Design<T>*>(p->host_tables.data() + k * tb);
+    fill_table<T>(p, ds, k, tab[k], &tot[k]);
+    *dst[k] = base + k * tbr;
+    DNX_CHECK_HIP(hipMemcpyAsync(*dst[k], tab[k], tb, hipMemcpyHostToDevice, s));
+  }
+  p->d_ctr = reinterpret_cast<int*>(base + 4 * tbr);
   int dev = 0, cus = 256;
   DNX_CHECK_HIP(hipGetDevice(&dev));
   DNX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -108,42 +152,102 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
   int rc = prepare_batch3<T>(p->ty_f, false, p->max_w, &bf);
   if (rc == DNX_OK) rc = prepare_batch3<T>(p->ty_a, true, p->max_w, &ba);
   if (rc) return rc;
-  p->grid_f = std::min(tot_f, bf * cus);
-  p->grid_a = std::min(tot_a, ba * cus);
-  p->Lf = Batch3Launch{p->d_fwd, p->n, tot_f, p->K, 0, p->d_ctr};
-  p->La = Batch3Launch{p->d_adj, p->n, tot_a, p->K, 0, p->d_ctr};
+  p->grid_f = std::min(tot[0], bf * cus);
+  p->grid_a = std::min(tot[1], ba * cus);
+  const int fl = p->fpairs ? 1 : 0, al = p->apairs ? 1 : 0;
+  p->Lf = Batch3Launch{p->d_fwd, p->n, tot[0], p->K, 0, p->d_ctr, fl};
+  p->La = Batch3Launch{p->d_adj, p->n, tot[1], p->K, 0, p->d_ctr, al};
+  if constexpr (std::is_same<T, float>::value) {
+    if (p->fpairs) {
+      p->ty_f2 = fused3_tiles(p->dtype, 2).ty;
+      int b2 = 1;
+      if ((rc = prepare_batch3_pair<T>(p->ty_f2, false, p->max_w, &b2))) return rc;
+      p->grid_f2 = std::min(tot[2], b2 * cus);
+      p->Lf2 = Batch3Launch{p->d_fwd2, p->n, tot[2], p->K, 0, p->d_ctr, 1};
+    }
+    if (p->apairs) {
+      p->ty_a2 = fused3_tiles(p->dtype, 3).ty;
+      int b2 = 1;
+      if ((rc = prepare_batch3_pair<T>(p->ty_a2, true, p->max_w, &b2))) return rc;
+      p->grid_a2 = std::min(tot[3], b2 * cus);
+      p->La2 = Batch3Launch{p->d_adj2, p->n, tot[3], p->K, 0, p->d_ctr, 1};
+    }
+  }
   return DNX_OK;
 }
 
 // ---- step sequences (each reads the device counter) ----------------------
 
+// Forward (store = replay with E storage): step pairs on the two-step
+// path, a single step when one remains (only at the end of a sequence, so
+// every pair starts at an even step).
 template <typename T>
 static int seq_forward(dnx_batch3d_plan* p, int steps, bool store, hipStream_t s,
                        bool timed = true) {
+  int k = 0;
+  int rc;
+  if constexpr (std::is_same<T, float>::value) {
+    if (p->fpairs) {
+      Batch3Launch L = p->Lf2;
+      L.store = store ? 1 : 0;
+      for (; k + 1 < steps; k += 2) {
+        TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+        rc = launch_batch3_pair<T>(L, false, p->ty_f2, p->max_w, p->grid_f2, s);
+        timed_end(store ? "batch3d_fwd2_store" : "batch3d_fwd2", s, tl);
+        if (rc) return rc;
+        if (!store && (rc = launch_batch_monitor_pair<T>(p->d_fwd2, p->n, p->max_mon_cells,
+                                                          p->d_ctr, false, s)))
+          return rc;
+        if ((rc = launch_counter_add(p->d_ctr, 2, s))) return rc;
+      }
+    }
+  }
   Batch3Launch L = p->Lf;
   L.store = store ? 1 : 0;
-  for (int k = 0; k < steps; ++k) {
+  for (; k < steps; ++k) {
     TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
-    int rc = launch_batch3<T>(L, false, p->ty_f, p->max_w, p->grid_f, s);
+    rc = launch_batch3<T>(L, false, p->ty_f, p->max_w, p->grid_f, s);
     timed_end(store ? "batch3d_fwd_store" : "batch3d_fwd", s, tl);
     if (rc) return rc;
     if (!store && (rc = launch_batch_monitor<T>(p->d_fwd, p->n, p->max_mon_cells, p->K,
-                                                 p->d_ctr, false, s)))
+                                                 p->d_ctr, false, 0, L.layout, s)))
       return rc;
     if ((rc = launch_counter_add(p->d_ctr, 1, s))) return rc;
   }
   return DNX_OK;
 }
 
+// Adjoint from the counter m downward: pairs (m, m-1) on the two-step path
+// (adjoint source of sample m-1 precomputed, that of m-2 applied after the
+// launch), a single step when one remains.
 template <typename T>
 static int seq_adjoint(dnx_batch3d_plan* p, int steps, hipStream_t s, bool timed = true) {
-  for (int k = 0; k < steps; ++k) {
+  int k = 0;
+  int rc;
+  if constexpr (std::is_same<T, float>::value) {
+    if (p->apairs) {
+      for (; k + 1 < steps; k += 2) {
+        if ((rc = launch_batch_monitor_pair<T>(p->d_adj2, p->n, p->max_mon_cells, p->d_ctr, true,
+                                               s)))
+          return rc;
+        TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+        rc = launch_batch3_pair<T>(p->La2, true, p->ty_a2, p->max_w, p->grid_a2, s);
+        timed_end("batch3d_adj2", s, tl);
+        if (rc) return rc;
+        if ((rc = launch_batch_monitor<T>(p->d_adj2, p->n, p->max_mon_cells, p->K, p->d_ctr,
+                                          true, -1, 1, s)))
+          return rc;
+        if ((rc = launch_counter_add(p->d_ctr, -2, s))) return rc;
+      }
+    }
+  }
+  for (; k < steps; ++k) {
     TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
-    int rc = launch_batch3<T>(p->La, true, p->ty_a, p->max_w, p->grid_a, s);
+    rc = launch_batch3<T>(p->La, true, p->ty_a, p->max_w, p->grid_a, s);
     timed_end("batch3d_adj", s, tl);
     if (rc) return rc;
-    if ((rc = launch_batch_monitor<T>(p->d_adj, p->n, p->max_mon_cells, p->K, p->d_ctr, true,
-                                      s)))
+    if ((rc = launch_batch_monitor<T>(p->d_adj, p->n, p->max_mon_cells, p->K, p->d_ctr, true, 0,
+                                      p->La.layout, s)))
       return rc;
     if ((rc = launch_counter_add(p->d_ctr, -1, s))) return rc;
   }
@@ -213,8 +317,8 @@ static int batch_forward(dnx_batch3d_plan* p, bool record, hipStream_t s) {
     const int len = std::min(p->K, p->T - k0);
     if (record) {
       DNX_CHECK_HIP(hipMemcpyAsync(static_cast<T*>(p->pools.ckpt) + int64_t(k0 / p->K) * p->sum6,
-                                   p->pools.state[0], sizeof(T) * p->sum6,
-                                   hipMemcpyDeviceToDevice, s));
+                                   p->pools.state[batch_in_slot(p->fpairs, k0)],
+                                   sizeof(T) * p->sum6, hipMemcpyDeviceToDevice, s));
     }
     if ((rc = run_seq<T>(p, 0, len, s))) return rc;
   }
@@ -231,8 +335,9 @@ static int batch_backward(dnx_batch3d_plan* p, hipStream_t s) {
   if ((rc = launch_zero<T>(static_cast<T*>(p->pools.lam[1]), p->sum6, s))) return rc;
   for (int d = 0; d < p->n; ++d) {
     const dnx_design3d& de = p->designs[d];
-    T* lam = static_cast<T*>(p->pools.lam[T_ & 1]) + 6 * p->off[d];
-    const T* eT = static_cast<const T*>(p->pools.state[T_ & 1]) + 6 * p->off[d];
+    T* lam = static_cast<T*>(p->pools.lam[batch_in_slot(p->apairs, T_)]) + 6 * p->off[d];
+    const T* eT =
+        static_cast<const T*>(p->pools.state[batch_in_slot(p->fpairs, T_)]) + 6 * p->off[d];
     T* S = static_cast<T*>(p->pools.S) + p->off[d];
     rc = dnx_fdtd3d_adjoint_init(&de.grid, lam, eT, de.mon.n_freq > 0 ? &de.mon : nullptr,
                                  de.mon.n_freq > 0 ? 1 : 0, T_, S, s);
@@ -243,7 +348,7 @@ static int batch_backward(dnx_batch3d_plan* p, hipStream_t s) {
     const int a = k * p->K;
     const int len = std::min(p->K, T_ - a);
     if (len > 1) {
-      DNX_CHECK_HIP(hipMemcpyAsync(p->pools.state[0],
+      DNX_CHECK_HIP(hipMemcpyAsync(p->pools.state[batch_in_slot(p->fpairs, a)],
                                    static_cast<const T*>(p->pools.ckpt) + int64_t(k) * p->sum6,
                                    sizeof(T) * p->sum6, hipMemcpyDeviceToDevice, s));
       if ((rc = launch_counter_set(p->d_ctr, a, s))) return rc;
@@ -252,12 +357,17 @@ static int batch_backward(dnx_batch3d_plan* p, hipStream_t s) {
     if ((rc = launch_counter_set(p->d_ctr, a + len - 1, s))) return rc;
     const int steps = k == 0 ? len - 1 : len;  // m runs down to max(a, 1)
     if ((rc = run_seq<T>(p, 2, steps, s))) return rc;
+    if (k == 0) {
+      // lam^1 sits where the last launch wrote it: layout 1 puts a final
+      // single step (m = 1, odd) in the other pool than a final pair
+      p->lam1_slot = (p->apairs && steps > 0 && (steps & 1)) ? 0 : 1;
+    }
   }
   // step-0 source term with lam^1 (in adjoint buffer 1)
   for (int d = 0; d < p->n; ++d) {
     const dnx_design3d& de = p->designs[d];
     if (de.src.comp < 0) continue;
-    T* lam1 = static_cast<T*>(p->pools.lam[1]) + 6 * p->off[d];
+    T* lam1 = static_cast<T*>(p->pools.lam[p->lam1_slot]) + 6 * p->off[d];
     rc = launch_source_adj<T>(static_cast<T*>(p->pools.S) + p->off[d],
                               lam1 + int64_t(de.src.comp) * (int64_t(de.grid.nz) * de.grid.ny *
                                                              de.grid.nx),
@@ -273,7 +383,7 @@ using namespace dnx;
 
 extern "C" int64_t dnx_batch3d_workspace_bytes(int32_t n_designs) {
   const size_t tb = sizeof(Batch3Design<double>) * size_t(std::max(n_designs, 1));
-  return int64_t(2 * ((tb + 255) / 256) * 256 + 256);
+  return int64_t(4 * ((tb + 255) / 256) * 256 + 256);  // four tables + counter
 }
 
 extern "C" int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, int32_t n_steps,
@@ -353,11 +463,20 @@ extern "C" int32_t dnx_batch3d_uses_graph(const dnx_batch3d_plan* p) {
   return (p && p->use_graph) ? 1 : 0;
 }
 
+extern "C" int32_t dnx_batch3d_final_slot(const dnx_batch3d_plan* p) {
+  return p ? batch_in_slot(p->fpairs ? 1 : 0, p->T) : 0;
+}
+
+extern "C" int32_t dnx_batch3d_uses_pairs(const dnx_batch3d_plan* p) {
+  return p ? (p->fpairs ? 1 : 0) | (p->apairs ? 2 : 0) : 0;
+}
+
 extern "C" int32_t dnx_batch3d_destroy(dnx_batch3d_plan* p) {
   if (!p) return DNX_OK;
   if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
   if (p->g_rep) (void)hipGraphExecDestroy(p->g_rep);
   if (p->g_adj) (void)hipGraphExecDestroy(p->g_adj);
+  if (p->side_mem) (void)hipFree(p->side_mem);
   delete p;
   return DNX_OK;
 }

@@ -52,7 +52,18 @@ struct Batch3Design {
   const double* m_phase;
   double* m_acc;
   const double* m_grad;
+  T* m_side;              // two-step path: staged monitor values / adjoint source
 };
+
+// Buffer layouts of the batched state and adjoint pools.  Layout 0 (one-step
+// path): state n in buf[n & 1], lam^m in buf[m & 1].  Layout 1 (two-step
+// path, launches of two steps starting at even n): state n in
+// buf[((n + 1) >> 1) & 1] and lam^m in buf[((m + 1) >> 1) & 1]; a launch at
+// counter c reads buf[in] and writes buf[in ^ 1] with in = batch_in_slot(c)
+// (forward) or batch_in_slot(c + 1) (adjoint, which reads lam^{c+1}).
+__host__ __device__ inline int batch_in_slot(int layout, int j) {
+  return layout ? ((j + 1) >> 1) & 1 : j & 1;
+}
 
 struct Batch3Launch {
   const void* designs;  // device array of Batch3Design<T>
@@ -61,6 +72,7 @@ struct Batch3Launch {
   int K;                // checkpoint interval
   int store;            // forward: also store E (replay)
   const int* step;      // device step counter
+  int layout;           // buffer layout (batch_in_slot)
 };
 
 template <typename T>
@@ -70,7 +82,7 @@ int launch_batch3(const Batch3Launch& L, bool adjoint, int ty, int max_w, int gr
                   hipStream_t s);
 template <typename T>
 int launch_batch_monitor(const void* designs, int n_designs, int64_t max_cells, int K,
-                         const int* step, bool adjoint, hipStream_t s);
+                         const int* step, bool adjoint, int koff, int layout, hipStream_t s);
 int launch_counter_add(int* c, int delta, hipStream_t s);
 int launch_counter_set(int* c, int v, hipStream_t s);
 
@@ -118,6 +130,22 @@ template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s);
 template <typename T>
 int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s);
+
+// Batched two-step launches over a design table whose tiles follow kernel 2
+// (forward) or 3 (adjoint); float32 only.  prepare_batch3_pair sets the
+// kernel attributes (never during capture) and reports work-groups per CU.
+template <typename T>
+int prepare_batch3_pair(int ty, bool adjoint, int max_w, int* blocks_per_cu);
+template <typename T>
+int launch_batch3_pair(const Batch3Launch& L, bool adjoint, int ty, int max_w, int grid,
+                       hipStream_t s);
+// Monitors of the batched two-step path: after a forward pair at counter n,
+// accumulate the two staged samples (phase rows n, n+1); before an adjoint
+// pair at counter m, precompute the adjoint source of sample m-1 (phase row
+// m-1) into each design's m_side.
+template <typename T>
+int launch_batch_monitor_pair(const void* designs, int n_designs, int64_t max_cells,
+                              const int* step, bool adjoint, hipStream_t s);
 
 template <typename T>
 int launch_fused3_fwd(const Fused3Args<T>& a, int ty, hipStream_t s);

@@ -510,8 +510,9 @@ __global__ __launch_bounds__(256) void k3_fused_adj(Fused3Args<T> a) {
 // design d are [tile_start[d], tile_start[d+1])), XCD-remapped like the
 // single-grid kernels.  The step index comes from a device counter, so the
 // same launch can be replayed from a captured graph for every segment.
-// Buffer of the state at step n: buf[n & 1] (forward); adjoint input
-// lam^{m+1} in buf[(m+1) & 1], output lam^m in buf[m & 1].
+// A launch at counter n (forward) reads the state from
+// buf[batch_in_slot(layout, n)] and writes the other buffer; an adjoint
+// launch at m reads lam^{m+1} from buf[batch_in_slot(layout, m + 1)].
 // ---------------------------------------------------------------------------
 
 template <typename T>
@@ -527,14 +528,15 @@ __global__ __launch_bounds__(256) void k3_batch_fwd(Batch3Launch L) {
   T* sm = reinterpret_cast<T*>(smem_raw);
   const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
   const int n = *L.step;
+  const int ib = batch_in_slot(L.layout, n);
   const int G = gridDim.x;
   const int b = blockIdx.x;
   const int lw = (b & 7) * (G >> 3) + (b >> 3);
   for (int t = lw; t < L.total_tiles; t += G) {
     const int d = batch_design(D, L.n_designs, t);
     Fused3Args<T> a = D[d].a;
-    a.in = D[d].buf[n & 1];
-    a.out = D[d].buf[(n & 1) ^ 1];
+    a.in = D[d].buf[ib];
+    a.out = D[d].buf[ib ^ 1];
     a.es = L.store ? D[d].estore + int64_t(n % L.K) * D[d].estore_stride : nullptr;
     if (a.src_comp >= 0) a.src_wave = D[d].wave + n;
     const int lt = t - D[d].tile_start;
@@ -549,14 +551,15 @@ __global__ __launch_bounds__(256) void k3_batch_adj(Batch3Launch L) {
   T* sm = reinterpret_cast<T*>(smem_raw);
   const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
   const int m = *L.step;
+  const int ib = batch_in_slot(L.layout, m + 1);
   const int G = gridDim.x;
   const int b = blockIdx.x;
   const int lw = (b & 7) * (G >> 3) + (b >> 3);
   for (int t = lw; t < L.total_tiles; t += G) {
     const int d = batch_design(D, L.n_designs, t);
     Fused3Args<T> a = D[d].a;
-    a.in = D[d].buf[(m + 1) & 1];
-    a.out = D[d].buf[m & 1];
+    a.in = D[d].buf[ib];
+    a.out = D[d].buf[ib ^ 1];
     const int r = m % L.K;
     a.em = r == 0 ? D[d].ckpt + int64_t(m / L.K) * D[d].ckpt_stride
                   : D[d].estore + int64_t(r - 1) * D[d].estore_stride;
@@ -579,14 +582,19 @@ __global__ void k_counter_set(int* c, int v) {
 // n, acc += E^{n+1} (cos, -sin)(phase[n]).  adjoint = 1: after adjoint step
 // m, d = Re(g e^{i phase[m-1]}); lam^m_c += d; S += E^m_c d.
 template <typename T>
-__global__ void k_batch_monitor(const Batch3Design<T>* D, int K, const int* step, int adjoint) {
+__global__ void k_batch_monitor(const Batch3Design<T>* D, int K, const int* step, int adjoint,
+                                int koff, int layout) {
   const Batch3Design<T>& dz = D[blockIdx.y];
   const Box b = dz.mbox;
   if (dz.m_nf <= 0) return;
   const int64_t cells = b.cells();
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= cells) return;
-  const int k = *step;
+  // sample k of the launch at counter c (k = c + koff; the adjoint pair's
+  // second sample has koff = -1), read from that launch's output buffer
+  const int c = *step;
+  const int k = c + koff;
+  const int obuf = batch_in_slot(layout, adjoint ? c + 1 : c) ^ 1;
   const int bx = int(i % b.nx);
   const int64_t rr = i / b.nx;
   const int by = int(rr % b.ny);
@@ -598,7 +606,7 @@ __global__ void k_batch_monitor(const Batch3Design<T>* D, int K, const int* step
   const int ncomp = (mask & 1) + ((mask >> 1) & 1) + ((mask >> 2) & 1);
   const int nf = dz.m_nf;
   if (!adjoint) {
-    const T* E = dz.buf[(k + 1) & 1];
+    const T* E = dz.buf[obuf];
     const double* ph = dz.m_phase + int64_t(k) * nf * 2;
     int ci = 0;
     for (int c = 0; c < 3; ++c) {
@@ -612,7 +620,7 @@ __global__ void k_batch_monitor(const Batch3Design<T>* D, int K, const int* step
       ++ci;
     }
   } else {
-    T* Lm = dz.buf[k & 1];
+    T* Lm = dz.buf[obuf];
     const int r = k % K;
     const T* Em = r == 0 ? dz.ckpt + int64_t(k / K) * dz.ckpt_stride
                          : dz.estore + int64_t(r - 1) * dz.estore_stride;
@@ -747,10 +755,11 @@ int launch_batch3(const Batch3Launch& L, bool adjoint, int ty, int max_w, int gr
 
 template <typename T>
 int launch_batch_monitor(const void* designs, int n_designs, int64_t max_cells, int K,
-                         const int* step, bool adjoint, hipStream_t s) {
+                         const int* step, bool adjoint, int koff, int layout, hipStream_t s) {
   if (max_cells <= 0) return DNX_OK;
   hipLaunchKernelGGL(k_batch_monitor<T>, dim3(cdiv(max_cells, 256), n_designs), dim3(256), 0, s,
-                     static_cast<const Batch3Design<T>*>(designs), K, step, adjoint ? 1 : 0);
+                     static_cast<const Batch3Design<T>*>(designs), K, step, adjoint ? 1 : 0,
+                     koff, layout);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
@@ -775,10 +784,10 @@ template int prepare_batch3<float>(int, bool, int, int*);
 template int prepare_batch3<double>(int, bool, int, int*);
 template int launch_batch3<float>(const Batch3Launch&, bool, int, int, int, hipStream_t);
 template int launch_batch3<double>(const Batch3Launch&, bool, int, int, int, hipStream_t);
-template int launch_batch_monitor<float>(const void*, int, int64_t, int, const int*, bool,
-                                         hipStream_t);
-template int launch_batch_monitor<double>(const void*, int, int64_t, int, const int*, bool,
-                                          hipStream_t);
+template int launch_batch_monitor<float>(const void*, int, int64_t, int, const int*, bool, int,
+                                         int, hipStream_t);
+template int launch_batch_monitor<double>(const void*, int, int64_t, int, const int*, bool, int,
+                                          int, hipStream_t);
 
 }  // namespace dnx
 
@@ -789,9 +798,10 @@ extern "C" int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t t
   DNX_REQUIRE(kernel >= 0 && kernel <= 3,
               "set_tiles: kernel must be 0 (forward), 1 (adjoint), 2 (two-step forward) or 3 "
               "(two-step adjoint)");
-  DNX_REQUIRE(ty == 2 || ty == 4 || (ty == 8 && kernel < 2) || (ty == 1 && kernel == 3),
+  DNX_REQUIRE(ty == 2 || ty == 4 || (ty == 8 && kernel < 2) || (ty == 1 && kernel == 3) ||
+                  (ty == 0 && kernel >= 2),
               "set_tiles: ty must be 2, 4 or 8 (2 or 4 for the two-step forward, 1, 2 or 4 "
-              "for the two-step adjoint; got %d)", ty);
+              "for the two-step adjoint, 0 to switch a two-step kernel off; got %d)", ty);
   DNX_REQUIRE(zc >= 1 && zc <= 4096, "set_tiles: zc must be in [1, 4096] (got %d)", zc);
   const int d = dtype == DNX_F64 ? 1 : 0;
   g_ty[d][kernel] = ty;

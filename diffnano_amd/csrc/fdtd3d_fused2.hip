@@ -1059,4 +1059,197 @@ int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
 
 template int launch_fused2_adj<float>(const Fused2Args<float>&, int, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Batched two-step launches: one persistent launch advances every design of
+// a batch by two steps, walking the global tile list (tiles of design d are
+// [tile_start[d], tile_start[d+1]), tile shapes of kernel 2 / 3) with
+// stride gridDim.x, XCD-remapped like the single-grid kernels.  Buffers
+// follow layout 1 of batch_in_slot; E^m of the adjoint comes from the
+// checkpoint (m % K == 0) or the replay store, as in k3_batch_adj.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ inline int batch2_design(const Batch3Design<T>* D, int n_designs, int t) {
+  int d = 0;
+  while (d + 1 < n_designs && D[d + 1].tile_start <= t) ++d;
+  return d;
+}
+
+template <typename T>
+__device__ inline const T* batch2_em(const Batch3Design<T>& Dd, int m, int K) {
+  const int r = m % K;
+  return r == 0 ? Dd.ckpt + int64_t(m / K) * Dd.ckpt_stride
+                : Dd.estore + int64_t(r - 1) * Dd.estore_stride;
+}
+
+template <typename T>
+__device__ inline void batch2_mon(const Batch3Design<T>& Dd, bool on, Fused2Args<T>& f) {
+  f.n_mon = (on && Dd.m_nf > 0) ? 1 : 0;
+  f.mon[0].box = Dd.mbox;
+  f.mon[0].mask = Dd.m_mask;
+  f.mon[0].ncomp = (Dd.m_mask & 1) + ((Dd.m_mask >> 1) & 1) + ((Dd.m_mask >> 2) & 1);
+  f.side[0] = Dd.m_side;
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
+  const int n = *L.step;
+  const int ib = batch_in_slot(1, n);
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int lw = (b & 7) * (G >> 3) + (b >> 3);
+  for (int t = lw; t < L.total_tiles; t += G) {
+    const int d = batch2_design(D, L.n_designs, t);
+    const Batch3Design<T>& Dd = D[d];
+    Fused2Args<T> f{};
+    f.a = Dd.a;
+    f.a.in = Dd.buf[ib];
+    f.a.out = Dd.buf[ib ^ 1];
+    f.a.es = L.store ? Dd.estore + int64_t(n % L.K) * Dd.estore_stride : nullptr;
+    f.es2 = L.store ? Dd.estore + int64_t(n % L.K + 1) * Dd.estore_stride : nullptr;
+    f.a.src_wave = Dd.wave;
+    f.n = n;
+    batch2_mon(Dd, !L.store, f);
+    const int lt = t - Dd.tile_start;
+    fwd2_tile<T, TY, false>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc, sm);
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
+  const int m = *L.step;  // steps m and m-1
+  const int ib = batch_in_slot(1, m + 1);
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int lw = (b & 7) * (G >> 3) + (b >> 3);
+  for (int t = lw; t < L.total_tiles; t += G) {
+    const int d = batch2_design(D, L.n_designs, t);
+    const Batch3Design<T>& Dd = D[d];
+    Fused2Args<T> f{};
+    f.a = Dd.a;
+    f.a.in = Dd.buf[ib];
+    f.a.out = Dd.buf[ib ^ 1];
+    f.a.em = batch2_em(Dd, m, L.K);
+    f.em2 = batch2_em(Dd, m - 1, L.K);
+    f.a.src_wave = Dd.wave;
+    f.n = m;
+    batch2_mon(Dd, true, f);
+    const int lt = t - Dd.tile_start;
+    adj2_tile<T, TY>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc, sm);
+    __syncthreads();
+  }
+}
+
+// After a forward pair at counter n: acc += (v1 at phase row n) then (v2 at
+// row n+1), the arithmetic of k_monitor_pair.  Before an adjoint pair at
+// counter m: m_side = T(sum_f Re(g e^{i phase[m-1]})), that of k_monitor_d.
+template <typename T>
+__global__ void k_batch_monitor2(const Batch3Design<T>* D, const int* step, int adjoint) {
+  const Batch3Design<T>& dz = D[blockIdx.y];
+  if (dz.m_nf <= 0) return;
+  const int64_t cells = dz.mbox.cells();
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  const int c = *step;
+  const int mask = dz.m_mask;
+  const int ncomp = (mask & 1) + ((mask >> 1) & 1) + ((mask >> 2) & 1);
+  const int nf = dz.m_nf;
+  if (!adjoint) {
+    const double* ph = dz.m_phase + int64_t(c) * nf * 2;
+    for (int ci = 0; ci < ncomp; ++ci) {
+      const double v1 = double(dz.m_side[int64_t(ci) * cells + i]);
+      const double v2 = double(dz.m_side[(int64_t(ncomp) + ci) * cells + i]);
+      for (int f = 0; f < nf; ++f) {
+        double* a = dz.m_acc + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+        a[0] += v1 * ph[2 * f];
+        a[1] -= v1 * ph[2 * f + 1];
+        a[0] += v2 * ph[2 * nf + 2 * f];
+        a[1] -= v2 * ph[2 * nf + 2 * f + 1];
+      }
+    }
+  } else {
+    const double* ph = dz.m_phase + int64_t(c - 1) * nf * 2;
+    for (int ci = 0; ci < ncomp; ++ci) {
+      double sv = 0.0;
+      for (int f = 0; f < nf; ++f) {
+        const double* gg = dz.m_grad + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+        sv += gg[0] * ph[2 * f] - gg[1] * ph[2 * f + 1];
+      }
+      dz.m_side[int64_t(ci) * cells + i] = T(sv);
+    }
+  }
+}
+
+template <typename T, int TY>
+static size_t batch2_smem(int max_w) {
+  return sizeof(T) * size_t(fwd2_lds_elems<TY>()) * size_t(max_w);
+}
+
+template <typename T, int TY>
+static int prepare_batch2_ty(bool adjoint, int max_w, int* blocks_per_cu) {
+  const void* f = adjoint ? reinterpret_cast<const void*>(&k3_batch2_adj<T, TY>)
+                          : reinterpret_cast<const void*>(&k3_batch2_fwd<T, TY>);
+  DNX_CHECK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(batch2_smem<T, TY>(FUSED_MAX_W))));
+  const int threads = ((max_w + 63) / 64) * 64;
+  int nb = 0;
+  DNX_CHECK_HIP(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, batch2_smem<T, TY>(max_w)));
+  *blocks_per_cu = nb > 0 ? nb : 1;
+  return DNX_OK;
+}
+
+template <typename T, int TY>
+static int launch_batch2_ty(const Batch3Launch& L, bool adjoint, int max_w, int grid,
+                            hipStream_t s) {
+  const int threads = ((max_w + 63) / 64) * 64;
+  const int nb = ((grid + 7) / 8) * 8;
+  const size_t smem = batch2_smem<T, TY>(max_w);
+  if (adjoint)
+    hipLaunchKernelGGL((k3_batch2_adj<T, TY>), dim3(nb), dim3(threads), smem, s, L);
+  else
+    hipLaunchKernelGGL((k3_batch2_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, L);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template <typename T>
+int prepare_batch3_pair(int ty, bool adjoint, int max_w, int* blocks_per_cu) {
+  if (ty == 4) return prepare_batch2_ty<T, 4>(adjoint, max_w, blocks_per_cu);
+  if (ty == 2) return prepare_batch2_ty<T, 2>(adjoint, max_w, blocks_per_cu);
+  set_error("batch3 pair: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template <typename T>
+int launch_batch3_pair(const Batch3Launch& L, bool adjoint, int ty, int max_w, int grid,
+                       hipStream_t s) {
+  if (ty == 4) return launch_batch2_ty<T, 4>(L, adjoint, max_w, grid, s);
+  if (ty == 2) return launch_batch2_ty<T, 2>(L, adjoint, max_w, grid, s);
+  set_error("batch3 pair: unsupported tile height %d", ty);
+  return DNX_EINVAL;
+}
+
+template <typename T>
+int launch_batch_monitor_pair(const void* designs, int n_designs, int64_t max_cells,
+                              const int* step, bool adjoint, hipStream_t s) {
+  if (max_cells <= 0) return DNX_OK;
+  hipLaunchKernelGGL(k_batch_monitor2<T>, dim3(cdiv(max_cells, 256), n_designs), dim3(256), 0,
+                     s, static_cast<const Batch3Design<T>*>(designs), step, adjoint ? 1 : 0);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+template int prepare_batch3_pair<float>(int, bool, int, int*);
+template int launch_batch3_pair<float>(const Batch3Launch&, bool, int, int, int, hipStream_t);
+template int launch_batch_monitor_pair<float>(const void*, int, int64_t, const int*, bool,
+                                              hipStream_t);
+
 }  // namespace dnx

@@ -63,16 +63,15 @@ def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layer
               tiles=None):
     """Run `iterations` Adam steps on every design.  Returns dict with the
     FoM history (iterations x designs), final rho, wall time (s) and
-    design-iterations per second.  ``tiles``: batched mode, one
-    {"forward": (ty, zc), "adjoint": (ty, zc)} applied before the plan is
-    built; plain mode, a dict shape -> such a config applied before each
-    design's solve."""
+    design-iterations per second.  ``tiles``: batched mode, one autotuner
+    result ({"forward": (ty, zc), "adjoint": ..., "forward2": ...,
+    "adjoint2": ...}) applied before the plan is built; plain mode, a dict
+    shape -> such a result applied before each design's solve."""
     from . import autotune
 
     dev = torch.device(device)
     if mode == "batched" and tiles:
-        autotune.set_tiles(torch.float32, 0, *tiles["forward"])
-        autotune.set_tiles(torch.float32, 1, *tiles["adjoint"])
+        autotune.apply_tiles(torch.float32, tiles)
     rhos = [torch.tensor(p.rho0, dtype=torch.float64, device=dev, requires_grad=True)
             for p in problems]
     bgs = [torch.tensor(p.eps_bg, dtype=torch.float64, device=dev) for p in problems]
@@ -103,8 +102,7 @@ def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layer
             foms = []
             for p, r, bg in zip(problems, rhos, bgs):
                 if tiles and p.shape in tiles:
-                    autotune.set_tiles(torch.float32, 0, *tiles[p.shape]["forward"])
-                    autotune.set_tiles(torch.float32, 1, *tiles[p.shape]["adjoint"])
+                    autotune.apply_tiles(torch.float32, tiles[p.shape])
                 F = solvers[p.shape].forward(p.eps(r, bg), source=p.source,
                                              monitors=[p.monitor]).metadata["monitors"][0]
                 fom = (F.abs() ** 2).sum()

@@ -170,10 +170,11 @@ int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work, int32
 /* Tile shape of the fused 3D kernels for one dtype and kernel (0 forward/
  * replay, 1 adjoint, 2 two-step forward/replay, 3 two-step adjoint; the
  * two-step kernels run for float32 only): ty rows (2, 4 or 8; 2 or 4 for
- * the two-step kernels) and zc z-planes per work-group.  Process-wide; set
- * by the autotuner.  No reference
- * counterpart (the reference has no kernels); it changes performance only,
- * never results. */
+ * the two-step forward; 1, 2 or 4 for the two-step adjoint, 2 or 4 in
+ * batched plans; 0 switches a two-step kernel off, so the one-step kernel
+ * runs every step) and zc z-planes per work-group.  Process-wide; set by
+ * the autotuner.  No reference counterpart (the reference has no kernels);
+ * it changes performance only, never results. */
 int32_t dnx_fdtd3d_set_tiles(int32_t dtype, int32_t kernel, int32_t ty, int32_t zc);
 int32_t dnx_fdtd3d_get_tiles(int32_t dtype, int32_t kernel, int32_t* ty, int32_t* zc);
 
@@ -215,12 +216,19 @@ int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, int32_t n_ste
                            void* stream, dnx_batch3d_plan** out);
 /* Forward T steps from the state in pools.state[0] (caller zeroes it and
  * the monitor accumulators); record_ckpt: store checkpoints.  Final state
- * of every design in pools.state[T & 1]. */
+ * of every design in pools.state[dnx_batch3d_final_slot(plan)]. */
 int32_t dnx_batch3d_forward(dnx_batch3d_plan* plan, int32_t record_ckpt, void* stream);
 /* Gradient of a loss on the monitors (grads in each design's mon.grad):
  * replay + adjoint for every design; S in pools.S (dL/deps = -S/eps). */
 int32_t dnx_batch3d_backward(dnx_batch3d_plan* plan, void* stream);
 int32_t dnx_batch3d_uses_graph(const dnx_batch3d_plan* plan);
+/* Which state pool (0 or 1) holds the state after the forward's last step:
+ * T & 1 on the one-step path; the float32 two-step path (launches of two
+ * steps) uses its own ping-pong order.  (Batched form of the reference's
+ * per-design state, fdtd3d.py:497-505; no direct counterpart.) */
+int32_t dnx_batch3d_final_slot(const dnx_batch3d_plan* plan);
+/* Whether the plan runs the two-step kernels (bit 0 forward, bit 1 adjoint). */
+int32_t dnx_batch3d_uses_pairs(const dnx_batch3d_plan* plan);
 int32_t dnx_batch3d_destroy(dnx_batch3d_plan* plan);
 
 /* ---- 2D -------------------------------------------------------------- */

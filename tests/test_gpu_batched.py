@@ -63,15 +63,80 @@ def test_batched_equals_single_design(use_graph):
 def test_autotuner_measures_caches_and_applies(tmp_path, monkeypatch):
     from diffnano_amd import autotune
     monkeypatch.setenv("DNX_AUTOTUNE_CACHE", str(tmp_path / "tiles.json"))
-    saved = [autotune.get_tiles(torch.float32, k) for k in (0, 1)]
+    saved = [autotune.get_tiles(torch.float32, k) for k in (0, 1, 2, 3)]
     try:
         out = autotune.tune((24, 40, 64), steps=4)
         assert out["forward"][0] in (2, 4, 8) and out["adjoint"][0] in (2, 4, 8)
-        assert autotune.get_tiles(torch.float32, 0) == out["forward"]
+        assert out["forward2"][0] in (0, 2, 4) and out["adjoint2"][0] in (0, 1, 2, 4)
+        for k, name in enumerate(("forward", "adjoint", "forward2", "adjoint2")):
+            assert autotune.get_tiles(torch.float32, k) == out[name]
         (tmp_path / "tiles.json").write_text("garbage")  # corrupt cache -> re-tune
         out2 = autotune.tune((24, 40, 64), steps=4)
-        assert set(out2) == {"forward", "adjoint"}
+        assert set(out2) == {"forward", "adjoint", "forward2", "adjoint2"}
         assert autotune._load_cache(str(tmp_path / "tiles.json"))
+        # a switched-off two-step kernel: the solver runs one-step kernels only
+        autotune.set_tiles(torch.float32, 2, 0, 1)
+        autotune.set_tiles(torch.float32, 3, 0, 1)
+        F0, g0 = _plain((12, 14, 40), 2.0 + np.zeros((12, 14, 40)), {"type": "gaussian_pulse",
+                        "pos": [6, 7, 20]}, {"box": (0, 12, 0, 14, 30, 31)}, 10, 4)
+        for k, (ty, zc) in enumerate(saved):
+            autotune.set_tiles(torch.float32, k, ty, zc)
+        F1, g1 = _plain((12, 14, 40), 2.0 + np.zeros((12, 14, 40)), {"type": "gaussian_pulse",
+                        "pos": [6, 7, 20]}, {"box": (0, 12, 0, 14, 30, 31)}, 10, 4)
+        assert torch.equal(F0, F1) and torch.equal(g0, g1)
     finally:
         for k, (ty, zc) in enumerate(saved):
             autotune.set_tiles(torch.float32, k, ty, zc)
+
+
+def _batched_run(T, K, monkeypatch, env):
+    from diffnano_amd.batched import BatchedFDTD3D
+    shapes, eps, srcs, mons = _cases()
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    b = BatchedFDTD3D(shapes, [dict(s) for s in srcs], mons, dl=25.0, pml_layers=3, n_steps=T,
+                      courant=0.4, checkpoint_every=K, device="cuda", use_graph=True)
+    for k in env:
+        monkeypatch.delenv(k)
+    es = [torch.tensor(e, device="cuda", requires_grad=True) for e in eps]
+    outs = b.forward(es)
+    fields = [f.clone() for f in b.final_fields()]
+    sum((F.abs() ** 2).sum() for F in outs).backward()
+    return b.uses_pairs, [o.detach().clone() for o in outs], fields, [e.grad.clone() for e in es]
+
+
+@pytest.mark.parametrize("T,K", [(50, 8), (37, 6), (21, 4), (9, 2)])
+def test_batched_two_step_matches_one_step_bitwise(T, K, monkeypatch):
+    """The batched two-step kernels (pairs of steps per persistent launch,
+    their own buffer order, single steps at odd tails) reproduce the
+    one-step batched path bitwise: monitors, final fields and gradients, for
+    step counts and checkpoint intervals that leave odd segments."""
+    p2, F2, E2, G2 = _batched_run(T, K, monkeypatch, {})
+    pm, Fm, Em, Gm = _batched_run(T, K, monkeypatch, {"DNX_FDTD3D_ADJ2": "0"})
+    p1, F1, E1, G1 = _batched_run(T, K, monkeypatch, {"DNX_FDTD3D_STEPS": "1"})
+    assert p2 == (True, True) and pm == (True, False) and p1 == (False, False)
+    for run in ((F2, E2, G2), (Fm, Em, Gm)):
+        for a, b in zip(run[0], F1):
+            assert torch.equal(a, b)
+        for a, b in zip(run[1], E1):
+            assert torch.equal(a, b)
+        for a, b in zip(run[2], G1):
+            assert torch.equal(a, b), (a - b).abs().max()
+
+
+def test_batched_two_step_switched_off_by_tiles(monkeypatch):
+    """Tile height 0 for the two-step kernels (the autotuner's choice when
+    the one-step kernels are faster) gives a plan on the one-step path."""
+    from diffnano_amd import autotune
+    saved = [autotune.get_tiles(torch.float32, k) for k in (2, 3)]
+    try:
+        autotune.set_tiles(torch.float32, 2, 0, 1)
+        autotune.set_tiles(torch.float32, 3, 0, 1)
+        p0, F0, E0, G0 = _batched_run(21, 4, monkeypatch, {})
+    finally:
+        for k, (ty, zc) in zip((2, 3), saved):
+            autotune.set_tiles(torch.float32, k, ty, zc)
+    p1, F1, E1, G1 = _batched_run(21, 4, monkeypatch, {"DNX_FDTD3D_STEPS": "1"})
+    assert p0 == (False, False)
+    for a, b in zip(F0 + E0 + G0, F1 + E1 + G1):
+        assert torch.equal(a, b)
