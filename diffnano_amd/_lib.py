@@ -14,7 +14,7 @@ import threading
 import torch
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdnx_fdtd.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 DNX_F32 = 0
 DNX_F64 = 1
@@ -55,7 +55,7 @@ class Design3D(C.Structure):
 
 class Pools3D(C.Structure):
     _fields_ = [("state", C.c_void_p * 2), ("lam", C.c_void_p * 2), ("ckpt", C.c_void_p),
-                ("estore", C.c_void_p), ("S", C.c_void_p)]
+                ("estore", C.c_void_p), ("S", C.c_void_p), ("side", C.c_void_p)]
 
 
 class GemmArgs(C.Structure):
@@ -95,6 +95,7 @@ _SIGS = {
     "dnx_gelu_bwd": ([_P, _P, _P, _L, _P], _I),
     "dnx_sum_partials": ([_P, _I, _L, _P, C.c_float, _P], _I),
     "dnx_batch3d_workspace_bytes": ([_I], _L),
+    "dnx_batch3d_side_elems": ([C.POINTER(Design3D), _I], _L),
     "dnx_batch3d_create": ([C.POINTER(Design3D), _I, _I, _I, C.POINTER(Pools3D), _P, _L, _I, _P,
                             C.POINTER(_P)], _I),
     "dnx_batch3d_forward": ([_P, _I, _P], _I),

@@ -124,6 +124,9 @@ class BatchedFDTD3D:
         pools.ckpt = self.ckpt.data_ptr()
         pools.estore = self.estore.data_ptr()
         pools.S = self.S.data_ptr()
+        side_n = int(lib.dnx_batch3d_side_elems(arr, self.n))
+        self.side = torch.empty(max(side_n, 1), dtype=dt, device=dev)
+        pools.side = self.side.data_ptr()
         self._pools = pools
         nbytes = lib.dnx_batch3d_workspace_bytes(self.n)
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)

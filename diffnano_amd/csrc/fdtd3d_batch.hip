@@ -28,7 +28,7 @@ struct dnx_batch3d_plan {
   void* d_adj = nullptr;
   void* d_fwd2 = nullptr;
   void* d_adj2 = nullptr;
-  void* side_mem = nullptr;  // per-design monitor buffers of the two-step path
+  void* side_mem = nullptr;  // pools.side when the two-step path stages monitors
   int* d_ctr = nullptr;
   dnx::Batch3Launch Lf{}, La{}, Lf2{}, La2{};
   dnx_batch3d_pools pools{};
@@ -98,6 +98,20 @@ static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, int kernel,
   *total_tiles = start;
 }
 
+// Element offset of design d's monitor staging in pools.side (two samples
+// of its monitor-box components); d = n gives the total.
+static int64_t side_offset(const dnx_design3d* ds, int d) {
+  int64_t acc = 0;
+  for (int e = 0; e < d; ++e) {
+    const dnx_monitor& m = ds[e].mon;
+    if (m.n_freq > 0) {
+      const int mk = m.comp_mask;
+      acc += 2 * int64_t((mk & 1) + ((mk >> 1) & 1) + ((mk >> 2) & 1)) * box_cells(m.box);
+    }
+  }
+  return acc;
+}
+
 template <typename T>
 static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hipStream_t s) {
   // Two-step path: float32, not switched off, every design addressable by
@@ -116,16 +130,12 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
     p->apairs = ok && (ta2 == 2 || ta2 == 4) && !(a2 && strcmp(a2, "0") == 0);
   }
   if ((p->fpairs || p->apairs) && p->max_mon_cells > 0) {
-    int64_t acc = 0;
-    for (int d = 0; d < p->n; ++d) {
-      p->side_off.push_back(acc);
-      const dnx_monitor& m = ds[d].mon;
-      if (m.n_freq > 0) {
-        const int mk = m.comp_mask;
-        acc += 2 * int64_t((mk & 1) + ((mk >> 1) & 1) + ((mk >> 2) & 1)) * box_cells(m.box);
-      }
+    if (!p->pools.side) {
+      p->fpairs = p->apairs = false;  // no staging buffer: one-step kernels
+    } else {
+      for (int d = 0; d < p->n; ++d) p->side_off.push_back(side_offset(ds, d));
+      p->side_mem = p->pools.side;
     }
-    DNX_CHECK_HIP(hipMalloc(&p->side_mem, sizeof(T) * size_t(std::max<int64_t>(acc, 1))));
   }
   const size_t tb = sizeof(Batch3Design<T>) * p->n;
   const size_t tbr = ((tb + 255) / 256) * 256;
@@ -381,6 +391,11 @@ static int batch_backward(dnx_batch3d_plan* p, hipStream_t s) {
 
 using namespace dnx;
 
+extern "C" int64_t dnx_batch3d_side_elems(const dnx_design3d* designs, int32_t n) {
+  if (!designs || n <= 0) return 0;
+  return dnx::side_offset(designs, n);
+}
+
 extern "C" int64_t dnx_batch3d_workspace_bytes(int32_t n_designs) {
   const size_t tb = sizeof(Batch3Design<double>) * size_t(std::max(n_designs, 1));
   return int64_t(4 * ((tb + 255) / 256) * 256 + 256);  // four tables + counter
@@ -476,7 +491,6 @@ extern "C" int32_t dnx_batch3d_destroy(dnx_batch3d_plan* p) {
   if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
   if (p->g_rep) (void)hipGraphExecDestroy(p->g_rep);
   if (p->g_adj) (void)hipGraphExecDestroy(p->g_adj);
-  if (p->side_mem) (void)hipFree(p->side_mem);
   delete p;
   return DNX_OK;
 }

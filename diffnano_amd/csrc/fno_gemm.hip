@@ -637,6 +637,230 @@ static bool nt_ok(const GemmArgs& g) {
          g.sAb % 8 == 0 && g.sBb % 8 == 0 && al(g.A) && al(g.B);
 }
 
+// ---- Large NT kernel: 256 x 256 x 64 tiles on 32x32x16 MFMA -------------
+//
+// For the big spectral products with long reductions (M, N >= 256,
+// K >= 2048, both operands contiguous along K, K % 64 == 0).  256 threads = 4 waves in 2 x 2, each owning a
+// 128 x 128 block of C as 4 x 4 tiles of v_mfma_f32_32x32x16_bf16 (256 FP32
+// accumulators per lane), so each 16-byte LDS read feeds four MFMAs of
+// 32 x 32 x 16: half the LDS bytes per flop of the 64 x 64 x 16x16 layout.
+// LDS: A and B tiles [row][k] (row padded to 72 elements, so 8 lanes
+// reading 8 consecutive rows hit 8 disjoint bank groups), double-buffered
+// (144 KiB, one work-group per CU); the next K step is prefetched into
+// registers (FP32 inputs converted to BF16 with v_cvt_pk_bf16_f32, round
+// to nearest even, on the LDS store) with one barrier per K step.  Blocks
+// are remapped so that consecutive logical tiles, which share the A panel
+// (all N tiles of one M panel), run on one XCD and share its L2.  MFMA
+// 32x32x16 lane layout: A lane l holds row l%32, k = (l/32)*8 .. +8; B lane
+// l holds column l%32, the same k; accumulator element v of lane l is row
+// 8*(v/4) + 4*(l/32) + v%4, column l%32.
+
+constexpr int XBM = 256, XBN = 256, XBK = 64, XLD = XBK + 8;
+// Used for long reductions only: with one work-group per CU and a one-step
+// register prefetch, short K (few steps to hide the load latency and a
+// large epilogue share) runs faster on the 128 x 128 kernel (measured at
+// 8192 x 16384 x 1024: 704 vs 485 us; at 8192 x 1024 x 16384: 466 vs 604).
+constexpr int XBIG_K = 2048;
+constexpr size_t XSMEM = size_t(2) * (XBM + XBN) * XLD * sizeof(unsigned short);
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <bool BF>
+struct BigLoad;
+
+template <>
+struct BigLoad<true> {  // 256 rows x 64 k BF16 = 2048 x 16 B, 8 per thread
+  u32x4 v[8];
+  __device__ __forceinline__ void load(const unsigned short* base, int64_t ld, int row0,
+                                       int nrows, int k0, int t) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 3, c8 = e & 7;
+      const int gr = min(row0 + row, nrows - 1);  // clamped: rows past the edge are unused
+      v[r] = *reinterpret_cast<const u32x4*>(base + int64_t(gr) * ld + k0 + c8 * 8);
+    }
+  }
+  __device__ __forceinline__ void store(unsigned short* lds, int t) const {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 3, c8 = e & 7;
+      *reinterpret_cast<u32x4*>(lds + row * XLD + c8 * 8) = v[r];
+    }
+  }
+};
+
+template <>
+struct BigLoad<false> {  // 256 rows x 64 k FP32 = 4096 x 16 B, 16 per thread
+  f32x4 v[16];
+  __device__ __forceinline__ void load(const float* base, int64_t ld, int row0, int nrows,
+                                       int k0, int t) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 4, c4 = e & 15;
+      const int gr = min(row0 + row, nrows - 1);
+      v[r] = *reinterpret_cast<const f32x4*>(base + int64_t(gr) * ld + k0 + c4 * 4);
+    }
+  }
+  __device__ __forceinline__ void store(unsigned short* lds, int t) const {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = r * 256 + t;
+      const int row = e >> 4, c4 = e & 15;
+      bf16x2 lo, hi;
+      lo[0] = (__bf16)v[r][0];
+      lo[1] = (__bf16)v[r][1];
+      hi[0] = (__bf16)v[r][2];
+      hi[1] = (__bf16)v[r][3];
+      u32x2 p;
+      p[0] = __builtin_bit_cast(uint32_t, lo);
+      p[1] = __builtin_bit_cast(uint32_t, hi);
+      *reinterpret_cast<u32x2*>(lds + row * XLD + c4 * 4) = p;
+    }
+  }
+};
+
+// Epilogue of four consecutive columns n .. n+3 of row m: vector loads and
+// stores when C (and Cin) are contiguous along n and the run is inside the
+// matrix, element by element otherwise.
+__device__ __forceinline__ void big_epilogue(const GemmArgs& g, int m, int n, int64_t coff,
+                                          int64_t ioff, f32x4 a) {
+  if (m >= g.M || n >= g.N) return;
+  const int64_t ci = coff + m * g.sCm + n * g.sCn;
+  const bool vec = n + 3 < g.N && g.sCn == 1 && (ci & 3) == 0 &&
+                   (g.beta == 0.f || (g.sIn == 1 && ((ioff + m * g.sIm + n) & 3) == 0));
+  if (vec) {
+    f32x4 x = a * g.alpha;
+    if (g.beta != 0.f) x += g.beta * *reinterpret_cast<const f32x4*>(g.Cin + ioff + m * g.sIm + n);
+    if (g.bias_mode == 1) x += f32x4{g.bias[n], g.bias[n + 1], g.bias[n + 2], g.bias[n + 3]};
+    else if (g.bias_mode == 2) x += g.bias[m];
+    if (g.pre) *reinterpret_cast<f32x4*>(g.pre + ci) = x;
+    if (g.act == 1) x = f32x4{gelu(x[0]), gelu(x[1]), gelu(x[2]), gelu(x[3])};
+    *reinterpret_cast<f32x4*>(g.C + ci) = x;
+    return;
+  }
+  for (int e = 0; e < 4 && n + e < g.N; ++e) {
+    float x = g.alpha * a[e];
+    if (g.beta != 0.f) x += g.beta * g.Cin[ioff + m * g.sIm + (n + e) * g.sIn];
+    if (g.bias_mode == 1) x += g.bias[n + e];
+    else if (g.bias_mode == 2) x += g.bias[m];
+    const int64_t c = coff + m * g.sCm + (n + e) * g.sCn;
+    if (g.pre) g.pre[c] = x;
+    if (g.act == 1) x = gelu(x);
+    g.C[c] = x;
+  }
+}
+
+template <bool ABF, bool BBF>
+__global__ __launch_bounds__(256, 1) void k_gemm_big(GemmArgs g, int tiles_n, int tiles) {
+  extern __shared__ unsigned short xsm[];
+  // XCD-aware remap (the grid is a multiple of 8; block b runs on XCD b % 8)
+  const int nb = gridDim.x, bx = blockIdx.x;
+  const int logical = (bx & 7) * (nb >> 3) + (bx >> 3);
+  if (logical >= tiles) return;
+  const int m0 = (logical / tiles_n) * XBM;
+  const int n0 = (logical % tiles_n) * XBN;
+  const int b = blockIdx.z;
+  unsigned short* As = xsm;
+  unsigned short* Bs = xsm + 2 * XBM * XLD;
+  using TA = typename std::conditional<ABF, unsigned short, float>::type;
+  using TB = typename std::conditional<BBF, unsigned short, float>::type;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = (wave >> 1) * 128;
+  const int wn = (wave & 1) * 128;
+  const TA* A = static_cast<const TA*>(g.A) + int64_t(b) * g.sAb;
+  const TB* B = static_cast<const TB*>(g.B) + int64_t(b) * g.sBb;
+
+  f32x16 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+  BigLoad<ABF> la;
+  BigLoad<BBF> lb;
+  la.load(A, g.sAm, m0, g.M, 0, t);
+  lb.load(B, g.sBn, n0, g.N, 0, t);
+  la.store(As, t);
+  lb.store(Bs, t);
+  __syncthreads();
+  const int nk = g.K / XBK;
+  const int kq = (lane >> 5) * 8;
+  const int lr = lane & 31;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(A, g.sAm, m0, g.M, (kt + 1) * XBK, t);
+      lb.load(B, g.sBn, n0, g.N, (kt + 1) * XBK, t);
+    }
+    const unsigned short* Ac = As + cur * XBM * XLD;
+    const unsigned short* Bc = Bs + cur * XBN * XLD;
+#pragma unroll
+    for (int ks = 0; ks < XBK; ks += 16) {
+      bf16x8 af[4], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        af[i] = *reinterpret_cast<const bf16x8*>(&Ac[(wm + i * 32 + lr) * XLD + ks + kq]);
+        bfr[i] = *reinterpret_cast<const bf16x8*>(&Bc[(wn + i * 32 + lr) * XLD + ks + kq]);
+      }
+      // operands swapped: each accumulator holds a 32 x 32 tile of C^T, so a
+      // lane's values are 4-column runs of one row of C (vector stores)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[j], af[i], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      la.store(As + (cur ^ 1) * XBM * XLD, t);
+      lb.store(Bs + (cur ^ 1) * XBN * XLD, t);
+    }
+    __syncthreads();
+  }
+
+  const int64_t coff = int64_t(b) * g.sCb;
+  const int64_t ioff = int64_t(b) * g.sIb;
+  // (written out per accumulator so that every index is a literal and the
+  // accumulators stay in registers)
+  auto tile = [&](const f32x16& a, int i, int j) {
+    const int m = m0 + wm + i * 32 + lr;
+    const int n = n0 + wn + j * 32 + 4 * (lane >> 5);
+    big_epilogue(g, m, n, coff, ioff, f32x4{a[0], a[1], a[2], a[3]});
+    big_epilogue(g, m, n + 8, coff, ioff, f32x4{a[4], a[5], a[6], a[7]});
+    big_epilogue(g, m, n + 16, coff, ioff, f32x4{a[8], a[9], a[10], a[11]});
+    big_epilogue(g, m, n + 24, coff, ioff, f32x4{a[12], a[13], a[14], a[15]});
+  };
+  tile(acc[0][0], 0, 0); tile(acc[0][1], 0, 1); tile(acc[0][2], 0, 2); tile(acc[0][3], 0, 3);
+  tile(acc[1][0], 1, 0); tile(acc[1][1], 1, 1); tile(acc[1][2], 1, 2); tile(acc[1][3], 1, 3);
+  tile(acc[2][0], 2, 0); tile(acc[2][1], 2, 1); tile(acc[2][2], 2, 2); tile(acc[2][3], 2, 3);
+  tile(acc[3][0], 3, 0); tile(acc[3][1], 3, 1); tile(acc[3][2], 3, 2); tile(acc[3][3], 3, 3);
+}
+
+template <bool ABF, bool BBF>
+static void launch_big(const GemmArgs& g, hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gemm_big<ABF, BBF>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(XSMEM));
+    attr = true;
+  }
+  const int tn = int(cdiv(g.N, XBN)), tiles = int(cdiv(g.M, XBM)) * tn;
+  const int nb = ((tiles + 7) / 8) * 8;
+  hipLaunchKernelGGL((k_gemm_big<ABF, BBF>), dim3(nb, 1, g.batch), dim3(256), XSMEM, s, g, tn,
+                     tiles);
+}
+
 // out[i] = sum_j part[j*n + i] + beta * out[i]
 __global__ void k_sum_partials(const float* __restrict__ part, int s, int64_t n,
                                float* __restrict__ out, float beta) {
@@ -689,6 +913,13 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
   const bool generic = getenv_flag("DNX_GEMM_GENERIC");
   bool done = false;
   if (!generic && launch_cmix(*g, s)) {
+    done = true;
+  } else if (!generic && nt_ok(*g) && g->M >= XBM && g->N >= XBN && g->K >= XBIG_K &&
+             !getenv_flag("DNX_GEMM_SMALL")) {
+    if (g->a_bf16 && g->b_bf16) launch_big<true, true>(*g, s);
+    else if (g->a_bf16) launch_big<true, false>(*g, s);
+    else if (g->b_bf16) launch_big<false, true>(*g, s);
+    else launch_big<false, false>(*g, s);
     done = true;
   } else if (!generic && nt_ok(*g)) {
     if (g->a_bf16 && g->b_bf16) hipLaunchKernelGGL((k_gemm_nt<true, true>), grid, dim3(256), 0, s, *g);

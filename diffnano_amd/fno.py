@@ -105,14 +105,20 @@ def sum_partials(part, s, n, out, beta=0.0):
     return out
 
 
-def gemm_nt(M, N, K, A, B, out, *, target_blocks=1024, beta=0.0, Cin=None):
+def gemm_nt(M, N, K, A, B, out, *, target_blocks=None, beta=0.0, Cin=None):
     """out[M x N] = A[M x K] B[N x K]^T (+ beta Cin), both operands row-major
     with K contiguous, split along K into a batch of partial products (then
-    summed) when the output alone has too few 128 x 128 tiles to fill the
-    GPU."""
-    tiles = -(-M // 128) * -(-N // 128)
+    summed) when the output alone has too few tiles to fill the GPU (256 x
+    256 tiles, one work-group per CU, for M, N >= 256 and long K; 128 x 128
+    tiles otherwise)."""
+    big = M >= 256 and N >= 256 and K >= 4096  # split chunks keep K >= 2048
+    tb = 256 if big else 128
+    if target_blocks is None:
+        target_blocks = 512 if big else 1024
+    tiles = -(-M // tb) * -(-N // tb)
     s = 1
-    while s < 8 and tiles * s * 2 <= target_blocks and K % (s * 2 * 64) == 0:
+    while (s < 8 and tiles * s * 2 <= target_blocks and K % (s * 2 * 64) == 0
+           and (not big or K // (s * 2) >= 2048)):
         s *= 2
     if s == 1:
         return gemm(M, N, K, A, (K, 1, 0), B, (1, K, 0), out, (N, 1, 0), beta=beta, Cin=Cin)

@@ -167,3 +167,42 @@ def test_fno_backward_matches_fft_oracle():
         print(f"grad {k}: {e:.3e}")
         assert e < 5e-2, k
     assert worst < 5e-2
+
+
+@pytest.mark.parametrize("abf,bbf", [(False, True), (True, True), (False, False)])
+@pytest.mark.parametrize("M,N,K,batch", [(256, 256, 2048, 1), (300, 522, 2112, 2),
+                                         (512, 1024, 4096, 1)])
+def test_gemm_large_tile_kernel_matches_small_and_fp32(M, N, K, batch, abf, bbf, monkeypatch):
+    """The 256 x 256 MFMA 32x32x16 kernel (M, N >= 256, K >= 2048, K-contiguous operands):
+    ragged M and N (vector and element epilogue paths), both bias modes,
+    GELU with the pre-activation, beta * Cin, against FP32 products of the
+    BF16-rounded operands and against the 128 x 128 kernel."""
+    from diffnano_amd.fno import gemm
+    g = torch.Generator(device="cuda").manual_seed(M + 3 * N + K)
+    A = torch.randn(batch, M, K, device="cuda", generator=g)
+    B = torch.randn(batch, N, K, device="cuda", generator=g)
+    if abf:
+        A = A.to(torch.bfloat16)
+    if bbf:
+        B = B.to(torch.bfloat16)
+    Cin = torch.randn(batch, M, N, device="cuda", generator=g)
+    ref0 = torch.bmm(_bf(A.float()), _bf(B.float()).transpose(1, 2).contiguous())
+    for mode, bias in ((1, torch.randn(N, device="cuda", generator=g)),
+                       (2, torch.randn(M, device="cuda", generator=g))):
+        C = torch.empty(batch, M, N, device="cuda")
+        pre = torch.empty_like(C)
+        args = (M, N, K, A, (K, 1, M * K), B, (1, K, N * K), C, (N, 1, M * N))
+        gemm(*args, batch=batch, alpha=0.75, beta=0.5, Cin=Cin, bias=bias, bias_mode=mode,
+             act=1, pre=pre)
+        bb = bias[None, None, :] if mode == 1 else bias[None, :, None]
+        ref = 0.75 * ref0 + 0.5 * Cin + bb
+        torch.cuda.synchronize()
+        assert rel_l2(pre.cpu().numpy(), ref.cpu().numpy()) < 2e-6, mode
+        assert rel_l2(C.cpu().numpy(), torch.nn.functional.gelu(ref).cpu().numpy()) < 2e-6
+        C2 = torch.empty_like(C)
+        monkeypatch.setenv("DNX_GEMM_SMALL", "1")
+        gemm(M, N, K, A, (K, 1, M * K), B, (1, K, N * K), C2, (N, 1, M * N), batch=batch,
+             alpha=0.75, beta=0.5, Cin=Cin, bias=bias, bias_mode=mode, act=1)
+        monkeypatch.delenv("DNX_GEMM_SMALL")
+        torch.cuda.synchronize()
+        assert rel_l2(C.cpu().numpy(), C2.cpu().numpy()) < 1e-6
