@@ -2,8 +2,9 @@
 
 Surrogate for the 2D TM FDTD: maps a permittivity map to complex DFT
 fields.  Every dense operation runs through ``dnx_gemm_bf16`` (BF16
-operands, FP32 accumulation, fused epilogues) and the GELU backward through
-``dnx_gelu_bwd``; forward and backward are written out by hand (no torch
+operands, FP32 accumulation, fused epilogues), the per-mode complex channel
+mixing through ``dnx_fno_mode_mix`` / ``dnx_fno_mode_wgrad`` (FP32) and the
+GELU backward through ``dnx_gelu_bwd``; forward and backward are written out by hand (no torch
 autograd inside), wrapped in one ``torch.autograd.Function`` so a torch
 optimizer can train the FP32 master weights.
 
@@ -15,7 +16,7 @@ mode with complex weights, and returns to space (real part, Hermitian
 weights over kx, 1/(H W) normalisation, as irfft2 of the truncated
 spectrum).  Here the truncated 2D DFT and its inverse are dense matrices
 (H W x 2 m1 m2 complex, stored BF16), so each spectral convolution is two
-large-K GEMMs on MFMA plus 4 small per-mode GEMMs.  The activation after
+large-K GEMMs on MFMA plus the per-mode complex channel mixing.  The activation after
 the last Fourier layer is omitted, as in the original FNO.
 
 Layout: activations are [C][B][H*W] FP32 (channel-major over the batch), so
@@ -65,6 +66,23 @@ def gemm(M, N, K, A, sA, B, sB, Cout, sC, *, batch=1, a_off=0, b_off=0, c_off=0,
             raise TypeError("gemm operands must be float32 or bfloat16")
     L.check(L.lib().dnx_gemm_bf16(C.byref(g), L.stream_handle(Cout.device)))
     return Cout
+
+
+def mode_mix(X, Wr, Wi, Y, B, M, ld, c, conj=False):
+    """Y[q][b][r*M+mu] = sum_p X[p][b][.] (Wr + i s Wi)[p][q][mu] (FP32,
+    s = -1 if conj); Wr, Wi are [p][q][M] contiguous."""
+    L.check(L.lib().dnx_fno_mode_mix(X.data_ptr(), Wr.data_ptr(), Wi.data_ptr(), Y.data_ptr(),
+                                     int(B), int(M), int(ld), int(c), 1 if conj else 0,
+                                     L.stream_handle(Y.device)))
+    return Y
+
+
+def mode_wgrad(X, dY, dWr, dWi, B, M, ld, c, nsplit):
+    """dWr[s] + i dWi[s] = sum over sample split s of conj(X[p][b]) dY[q][b], per
+    mode, as [nsplit][p][q][M]."""
+    L.check(L.lib().dnx_fno_mode_wgrad(X.data_ptr(), dY.data_ptr(), dWr.data_ptr(),
+                                       dWi.data_ptr(), int(B), int(M), int(ld), int(c),
+                                       int(nsplit), L.stream_handle(dY.device)))
 
 
 def gelu_bwd(dy, pre, dx):
@@ -175,10 +193,19 @@ class FNO2d:
         return gemm(O, n, I, Wt, (I, 1, 0), x, (n, 1, 0), out, (n, 1, 0), bias=bias,
                     bias_mode=2 if bias is not None else 0, act=act, pre=pre, Cin=Cin, beta=beta)
 
+    fast_mix = True  # spectral mixing on the dedicated kernels where shapes allow
+
+    def _fast_mix_ok(self, B):
+        return self.fast_mix and self.width == 32 and B % 8 == 0 and self.nmodes % 16 == 0
+
     def _mix(self, Z, sr, si, Y, B):
         """Y[(o,b)][ri*M+mu] = sum_i Z[(i,b)][.] (Wr + i Wi)[mu][i][o] per mode."""
         c, M = self.width, self.nmodes
         ld = 2 * M
+        if self._fast_mix_ok(B):
+            mode_mix(Z, sr.permute(1, 2, 0).contiguous(), si.permute(1, 2, 0).contiguous(), Y,
+                     B, M, ld, c, conj=False)
+            return
         sA = (ld, B * ld, 1)
         sB = (c, 1, c * c)
         sC = (ld, B * ld, 1)
@@ -299,6 +326,21 @@ class FNO2d:
             sWt = (1, c, c * c)        # W^T: B[k=o][n=i] = W[mu][i][o]
             sWc = (c, 1, c * c)        # dW[mu][i][o]
             dZ = torch.empty(c * B, M2, device=dev)
+            if self._fast_mix_ok(B):
+                # dZ = dY W^H per mode; dW = sum_b conj(Z) dY
+                mode_mix(dY, sr.permute(2, 1, 0).contiguous(), si.permute(2, 1, 0).contiguous(),
+                         dZ, B, M, ld, c, conj=True)
+                chunks = B // 8
+                ns = max(d for d in range(1, min(chunks, 16) + 1) if chunks % d == 0)
+                pr = torch.empty(ns, c, c, M, device=dev)
+                pi = torch.empty(ns, c, c, M, device=dev)
+                mode_wgrad(Z, dY, pr, pi, B, M, ld, c, ns)
+                g[f"sr{l}"] = pr.sum(0).permute(2, 0, 1).contiguous()
+                g[f"si{l}"] = pi.sum(0).permute(2, 0, 1).contiguous()
+                gemm(c * B, P, M2, dZ, (M2, 1, 0), self.F, (1, M2, 0), dhn, (P, 1, 0), beta=1.0,
+                     Cin=dhn)
+                dh = dhn
+                continue
             kw = dict(batch=M)
             # dZr = dYr Wr^T + dYi Wi^T ; dZi = -dYr Wi^T + dYi Wr^T
             gemm(B, c, c, dY, sZ, sr, sWt, dZ, sZ, a_off=0, c_off=0, **kw)

@@ -35,7 +35,7 @@ extern "C" {
 #pragma GCC visibility push(default)
 #endif
 
-#define DNX_ABI_VERSION 1
+#define DNX_ABI_VERSION 2
 
 enum { DNX_F32 = 0, DNX_F64 = 1 };
 enum {
@@ -193,19 +193,26 @@ typedef struct dnx_design3d {
 
 /* Device pools shared by all designs; design d starts at element offset
  * 6*Nd (state, lam), 3*Nd (estore slot), Nd (S), Nd = sum_{e<d} N_e.
- * ckpt: ceil(T/K) slots of sum 6N; estore: K-1 slots of sum 3N. */
+ * ckpt: ceil(T/K) slots of sum 6N; estore: K-1 slots of sum 3N.  side:
+ * dnx_batch3d_side_elems() elements (monitor staging of the float32
+ * two-step kernels); null runs the one-step kernels when a design has a
+ * monitor. */
 typedef struct dnx_batch3d_pools {
   void* state[2];
   void* lam[2];
   void* ckpt;
   void* estore;
   void* S;
+  void* side;
 } dnx_batch3d_pools;
 
 typedef struct dnx_batch3d_plan dnx_batch3d_plan;  /* opaque host object */
 
 /* Device workspace the plan needs (descriptor tables + step counter). */
 int64_t dnx_batch3d_workspace_bytes(int32_t n_designs);
+/* Elements (of the designs' dtype) of the pools.side buffer: two samples of
+ * every design's monitor-box components. */
+int64_t dnx_batch3d_side_elems(const dnx_design3d* designs, int32_t n_designs);
 /* Build a plan (uploads descriptor tables into `workspace` on `stream`).
  * ckpt_every must be even.  use_graph: capture one K-step segment of each
  * sequence (forward, replay, adjoint) into a HIP graph on first use and

@@ -56,6 +56,21 @@ int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out, fl
 /* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32. */
 int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, int64_t n, void* stream);
 
+/* Spectral channel mixing of an FNO layer, per Fourier mode, complex, FP32:
+ *   Y[q][b][r*M + mu] (r = 0 real, 1 imaginary part)
+ *     = sum_p X[p][b][.] (Wr + i s Wi)[p][q][mu],   s = -1 if conj else +1
+ * X rows (p, b) and Y rows (q, b) have length ld (>= 2M, multiple of 4; X
+ * 16-byte aligned); Wr, Wi are [p][q][mu] (mode fastest).  32 channels,
+ * B % 8 == 0, M % 16 == 0.  (The spectral convolution of an FNO layer,
+ * no reference counterpart.) */
+int32_t dnx_fno_mode_mix(const float* X, const float* Wr, const float* Wi, float* Y, int32_t B,
+                         int32_t M, int64_t ld, int32_t c, int32_t conj, void* stream);
+/* Its weight gradient in nsplit partial sums over sample chunks:
+ *   dWr[s][p][q][mu] + i dWi[s][p][q][mu] = sum_{b in split s} conj(X[p][b][mu]) dY[q][b][mu]
+ * (nsplit divides B / 8). */
+int32_t dnx_fno_mode_wgrad(const float* X, const float* dY, float* dWr, float* dWi, int32_t B,
+                           int32_t M, int64_t ld, int32_t c, int32_t nsplit, void* stream);
+
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop
 #endif
