@@ -938,9 +938,27 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
   return DNX_OK;
 }
 
+// Few outputs, many partials (weight-gradient reductions): one output per
+// thread, the partials of consecutive outputs read by consecutive lanes.
+__global__ void k_sum_partials1(const float* __restrict__ part, int s, int64_t n,
+                                float* __restrict__ out, float beta) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float a = beta != 0.f ? beta * out[i] : 0.f;
+#pragma unroll 8
+  for (int j = 0; j < s; ++j) a += part[int64_t(j) * n + i];
+  out[i] = a;
+}
+
 extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out,
                                     float beta, void* stream) {
   DNX_REQUIRE(part && out && s > 0 && n > 0, "sum_partials: bad arguments");
+  if (n < (int64_t(1) << 16)) {
+    hipLaunchKernelGGL(k_sum_partials1, dim3(cdiv(n, 64)), dim3(64), 0,
+                       static_cast<hipStream_t>(stream), part, s, n, out, beta);
+    DNX_CHECK_HIP(hipGetLastError());
+    return DNX_OK;
+  }
   const int64_t threads = cdiv(n, 4);
   hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(threads, 256)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), part, s, n, out, beta);

@@ -138,6 +138,76 @@ __global__ __launch_bounds__(512) void k_mode_wgrad(const float* __restrict__ X,
   }
 }
 
+// Weight and bias gradients of a pointwise (1x1) layer over n = B*H*W
+// columns, in partial sums over column splits:
+//   part[s][o][i] = sum_{n in split s} dY[o][n] X[i][n],  part_b[s][o] = sum dY[o][n]
+// A work-group owns one split and a 32 x 32 (o, i) tile; per 64-column step
+// it stages dY[o][k] and X[i][k] (transposed to [k][i]) in LDS, and each of
+// 256 threads accumulates 4 outputs (one o, four consecutive i).  FP32.
+__global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY,
+                                               const float* __restrict__ X,
+                                               float* __restrict__ part,
+                                               float* __restrict__ part_b, int O, int I,
+                                               int64_t n, int64_t chunk) {
+  __shared__ float Ds[32][68];
+  __shared__ float Xs[64][36];
+  const int t = threadIdx.x;
+  const int o = t >> 3, i4 = (t & 7) * 4;
+  const int o0 = blockIdx.y * 32, i0 = blockIdx.z * 32;
+  const int64_t k_begin = int64_t(blockIdx.x) * chunk;
+  const int64_t k_end = min(n, k_begin + chunk);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f}, accb = 0.f;
+  // register prefetch of the next 64-column step (loads in flight during
+  // the current step's arithmetic); n % 64 == 0 and 64-aligned splits, so a
+  // step is either entirely inside the split or past its end
+  float4 dv[2], xv[2];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int idx = j * 256 + t;
+      const int r = idx >> 4, c4 = (idx & 15) * 4;
+      dv[j] = xv[j] = float4{0.f, 0.f, 0.f, 0.f};
+      if (k0 < k_end) {
+        if (o0 + r < O) dv[j] = *reinterpret_cast<const float4*>(dY + int64_t(o0 + r) * n + k0 + c4);
+        if (i0 + r < I) xv[j] = *reinterpret_cast<const float4*>(X + int64_t(i0 + r) * n + k0 + c4);
+      }
+    }
+  };
+  fetch(k_begin);
+  for (int64_t k0 = k_begin; k0 < k_end; k0 += 64) {
+    __syncthreads();  // the previous step's LDS reads are done
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int idx = j * 256 + t;
+      const int r = idx >> 4, c4 = (idx & 15) * 4;
+      *reinterpret_cast<float4*>(&Ds[r][c4]) = dv[j];
+      Xs[c4][r] = xv[j].x;
+      Xs[c4 + 1][r] = xv[j].y;
+      Xs[c4 + 2][r] = xv[j].z;
+      Xs[c4 + 3][r] = xv[j].w;
+    }
+    __syncthreads();
+    fetch(k0 + 64);
+#pragma unroll 8
+    for (int k = 0; k < 64; ++k) {
+      const float d = Ds[o][k];
+      const float4 x = *reinterpret_cast<const float4*>(&Xs[k][i4]);
+      acc[0] += d * x.x;
+      acc[1] += d * x.y;
+      acc[2] += d * x.z;
+      acc[3] += d * x.w;
+      accb += d;
+    }
+  }
+  const int64_t base = int64_t(blockIdx.x) * O * I;
+  if (o0 + o < O) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (i0 + i4 + e < I) part[base + int64_t(o0 + o) * I + i0 + i4 + e] = acc[e];
+    if (part_b && blockIdx.z == 0 && i4 == 0) part_b[int64_t(blockIdx.x) * O + o0 + o] = accb;
+  }
+}
+
 }  // namespace dnx
 
 using namespace dnx;
@@ -174,6 +244,24 @@ extern "C" int32_t dnx_fno_mode_wgrad(const float* X, const float* dY, float* dW
   hipLaunchKernelGGL(k_mode_wgrad, dim3(M / MIX_MB, nsplit), dim3(512), 0,
                      static_cast<hipStream_t>(stream), X, dY, dWr, dWi, B, M, ld,
                      chunks / nsplit);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_fno_wgrad(const float* dY, const float* X, float* part, float* part_b,
+                                 int32_t O, int32_t I, int64_t n, int32_t nsplit, void* stream) {
+  DNX_REQUIRE(dY && X && part, "fno_wgrad: null operand");
+  DNX_REQUIRE(O > 0 && I > 0 && n > 0 && n % 64 == 0 && nsplit > 0 && nsplit <= 65535 &&
+                  (reinterpret_cast<uintptr_t>(dY) & 15u) == 0 &&
+                  (reinterpret_cast<uintptr_t>(X) & 15u) == 0,
+              "fno_wgrad: needs n %% 64 == 0 and 16-byte aligned operands (O=%d I=%d n=%lld)", O,
+              I, (long long)n);
+  const int64_t chunk = ((n / nsplit + 63) / 64) * 64;
+  const int splits = int((n + chunk - 1) / chunk);
+  DNX_REQUIRE(splits == nsplit, "fno_wgrad: nsplit %d does not split n = %lld into 64-column "
+              "multiples", nsplit, (long long)n);
+  hipLaunchKernelGGL(k_wgrad, dim3(nsplit, (O + 31) / 32, (I + 31) / 32), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), dY, X, part, part_b, O, I, n, chunk);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }

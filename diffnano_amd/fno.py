@@ -273,6 +273,27 @@ class FNO2d:
         gemm(O, I, chunk, dY, (n, 1, chunk), X, (1, n, chunk), part, (I, 1, O * I), batch=s)
         sum_partials(part, s, O * I, out)
 
+    def _wgrad_bias(self, dY, X, gw, gb):
+        """gw[o][i] = sum_n dY[o][n] X[i][n], gb[o] = sum_n dY[o][n] (FP32
+        partial sums over column splits, then summed)."""
+        O, n = dY.shape
+        I = X.shape[0]
+        if n % 64 or dY.data_ptr() % 16 or X.data_ptr() % 16:
+            self._wgrad(dY, X, gw)
+            self._rowsum(dY, gb)
+            return
+        tiles = -(-O // 32) * -(-I // 32)
+        q = n // 64
+        target = max(1, min(q, 512 // tiles))  # ~2 work-groups per CU
+        ns = max(d for d in range(1, target + 1) if q % d == 0)
+        part = torch.empty(ns, O * I, device=self.dev)
+        pb = torch.empty(ns, O, device=self.dev)
+        L.check(L.lib().dnx_fno_wgrad(dY.data_ptr(), X.data_ptr(), part.data_ptr(),
+                                      pb.data_ptr(), int(O), int(I), int(n), int(ns),
+                                      L.stream_handle(self.dev)))
+        sum_partials(part, ns, O * I, gw)
+        sum_partials(pb, ns, O, gb)
+
     def _rowsum(self, dY, out):
         O, n = dY.shape
         s, chunk = self._split(n)
@@ -299,22 +320,19 @@ class FNO2d:
         g = {k: torch.zeros_like(v) for k, v in p.items()}
         d2 = dout.reshape(self.out_ch, n).contiguous()
         # projection
-        self._wgrad(d2, saved["q"], g["w_p2"])
-        self._rowsum(d2, g["b_p2"])
+        self._wgrad_bias(d2, saved["q"], g["w_p2"], g["b_p2"])
         dq = torch.empty(self.hidden, n, device=dev)
         gemm(self.hidden, n, self.out_ch, p["w_p2"], (1, self.hidden, 0), d2, (n, 1, 0), dq,
              (n, 1, 0))
         gelu_bwd(dq, saved["qpre"], dq)
-        self._wgrad(dq, saved["hL"], g["w_p1"])
-        self._rowsum(dq, g["b_p1"])
+        self._wgrad_bias(dq, saved["hL"], g["w_p1"], g["b_p1"])
         dh = torch.empty(c, n, device=dev)
         gemm(c, n, self.hidden, p["w_p1"], (1, c, 0), dq, (n, 1, 0), dh, (n, 1, 0))
         del dq
         for l in range(self.layers - 1, -1, -1):
             h, Z, pre = saved["h"][l], saved["Z"][l], saved["pre"][l]
             dpre = dh if pre is None else gelu_bwd(dh, pre, torch.empty_like(dh))
-            self._wgrad(dpre, h, g[f"w{l}"])
-            self._rowsum(dpre, g[f"b{l}"])
+            self._wgrad_bias(dpre, h, g[f"w{l}"], g[f"b{l}"])
             dhn = torch.empty(c, n, device=dev)
             gemm(c, n, c, p[f"w{l}"], (1, c, 0), dpre, (n, 1, 0), dhn, (n, 1, 0))
             # spectral branch: dY = dS G^T
@@ -360,8 +378,7 @@ class FNO2d:
             gemm(c * B, P, M2, dZ, (M2, 1, 0), self.F, (1, M2, 0), dhn, (P, 1, 0), beta=1.0,
                  Cin=dhn)
             dh = dhn
-        self._wgrad(dh, saved["x"], g["w_lift"])
-        self._rowsum(dh, g["b_lift"])
+        self._wgrad_bias(dh, saved["x"], g["w_lift"], g["b_lift"])
         return g
 
     def __call__(self, x):

@@ -70,6 +70,12 @@ int32_t dnx_fno_mode_mix(const float* X, const float* Wr, const float* Wi, float
  * (nsplit divides B / 8). */
 int32_t dnx_fno_mode_wgrad(const float* X, const float* dY, float* dWr, float* dWi, int32_t B,
                            int32_t M, int64_t ld, int32_t c, int32_t nsplit, void* stream);
+/* Weight and bias gradients of a pointwise layer in nsplit partial sums over
+ * the n columns (n % 64 == 0, split into equal 64-column multiples):
+ *   part[s][o][i] = sum_{n in split s} dY[o][n] X[i][n],  part_b[s][o] = sum dY[o][n]
+ * (part_b may be NULL).  dY [O][n], X [I][n], FP32. */
+int32_t dnx_fno_wgrad(const float* dY, const float* X, float* part, float* part_b, int32_t O,
+                      int32_t I, int64_t n, int32_t nsplit, void* stream);
 
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop

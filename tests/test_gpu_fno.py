@@ -265,3 +265,21 @@ def test_fno_fast_mix_matches_generic_and_oracle():
     for k in m1.names:
         assert rel_l2(g1[k].numpy(), p64[k].grad.numpy()) < 5e-2, k
         assert rel_l2(g1[k].numpy(), g2[k].numpy()) < 5e-2, k
+
+
+@pytest.mark.parametrize("O,I,n,ns", [(40, 9, 64 * 37, 37), (32, 32, 64 * 64, 8), (6, 128, 4096, 1)])
+def test_pointwise_weight_and_bias_gradient_kernel(O, I, n, ns):
+    """dnx_fno_wgrad: ragged 32 x 32 tiles, several column splits, against
+    FP64 sums."""
+    import diffnano_amd._lib as L
+    g = torch.Generator(device="cuda").manual_seed(O + I + n)
+    dY = torch.randn(O, n, device="cuda", generator=g)
+    X = torch.randn(I, n, device="cuda", generator=g)
+    part = torch.empty(ns, O * I, device="cuda")
+    pb = torch.empty(ns, O, device="cuda")
+    L.check(L.lib().dnx_fno_wgrad(dY.data_ptr(), X.data_ptr(), part.data_ptr(), pb.data_ptr(),
+                                  O, I, n, ns, L.stream_handle(dY.device)))
+    ref = dY.double().cpu() @ X.double().cpu().t()
+    # FP32 running sums over up to 4096 columns per thread: ~sqrt(n) eps
+    assert rel_l2(part.sum(0).view(O, I).double().cpu().numpy(), ref.numpy()) < 1e-5
+    assert rel_l2(pb.sum(0).double().cpu().numpy(), dY.double().cpu().sum(1).numpy()) < 1e-5
