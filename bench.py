@@ -348,8 +348,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--designs", type=int, default=48, help="c3: number of designs")
-    ap.add_argument("--iters", type=int, default=2,
-                    help="c3: Adam iterations per mode; c4: design iterations of the reference loop")
+    ap.add_argument("--iters", type=int, default=None,
+                    help="c3: Adam iterations per mode (default 2); c4: design iterations of the "
+                         "reference loop (default 10)")
     ap.add_argument("--samples", type=int, default=20000, help="c4: surrogate training samples")
     ap.add_argument("--epochs", type=int, default=30, help="c4: surrogate training epochs")
     ap.add_argument("--finetune", type=int, default=4, help="c4: fine-tune steps per iteration")
@@ -359,6 +360,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the FDTD "
                                    "domain is designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.iters is None:
+        args.iters = 10 if args.config == "c4" else 2
     if args.config == "c3":
         _bench_batched(args)
         return
