@@ -236,7 +236,7 @@ def _bench_surrogate(args):
            "dtype": "f32 FDTD, bf16 MFMA (fp32 accumulate) FNO", "data": "synthetic (seeded blobs)",
            "config": {"workload": "2D TM 128x128, dl 20 nm, 10-cell CPML, 2000 steps; FNO 4 "
                                   "layers, width 32, 16 modes; 256 candidates, top 8",
-                      "samples": args.samples, "epochs": args.epochs}}
+                      "samples": args.samples, "epochs": args.epochs, "weight_decay": args.wd}}
     sync()
     t0 = time.perf_counter()
     noise, eps, fields = SG.generate_dataset(cfg, range(args.samples), device=dev)
@@ -248,7 +248,8 @@ def _bench_surrogate(args):
     ntr = args.samples - nval
     sur = SG.FieldSurrogate(cfg, device=dev)
     t0 = time.perf_counter()
-    hist = sur.fit(eps[:ntr], fields[:ntr], epochs=args.epochs, batch=128, lr=3e-3)
+    hist = sur.fit(eps[:ntr], fields[:ntr], epochs=args.epochs, batch=128, lr=3e-3,
+                   weight_decay=args.wd)
     sync()
     out["train_s"] = round(time.perf_counter() - t0, 1)
     out["train_loss"] = [round(h, 5) for h in hist[-3:]]
@@ -351,8 +352,9 @@ def main():
     ap.add_argument("--iters", type=int, default=None,
                     help="c3: Adam iterations per mode (default 2); c4: design iterations of the "
                          "reference loop (default 10)")
-    ap.add_argument("--samples", type=int, default=20000, help="c4: surrogate training samples")
+    ap.add_argument("--samples", type=int, default=40000, help="c4: surrogate training samples")
     ap.add_argument("--epochs", type=int, default=30, help="c4: surrogate training epochs")
+    ap.add_argument("--wd", type=float, default=0.0, help="c4: AdamW weight decay")
     ap.add_argument("--finetune", type=int, default=4, help="c4: fine-tune steps per iteration")
     ap.add_argument("--fno-probe", action="store_true", help="c4: FNO kernels only (profiling)")
     args = ap.parse_args()

@@ -284,7 +284,7 @@ class FNO2d:
             return
         tiles = -(-O // 32) * -(-I // 32)
         q = n // 64
-        target = max(1, min(q, 512 // tiles))  # ~2 work-groups per CU
+        target = max(1, min(q, 1024 // tiles))  # ~4 work-groups per CU
         ns = max(d for d in range(1, target + 1) if q % d == 0)
         part = torch.empty(ns, O * I, device=self.dev)
         pb = torch.empty(ns, O, device=self.dev)
