@@ -186,16 +186,25 @@ class Engine:
 
     # -- kernels ------------------------------------------------------------
     def run(self, state, n0, n_steps, srcs, mons, ckpt=None, ckpt_every=0, estore=None,
-            probe=-1, probe_out=None, work=None):
-        """Advance ``state``; returns the buffer (``state`` or ``work``) that
-        holds the final state (the fused kernels ping-pong between them)."""
+            probe=-1, probe_out=None, work=None, init=None):
+        """Advance ``state`` (or, 3D, the state in ``init``, left unchanged);
+        returns the buffer (``state`` or ``work``) that holds the final state
+        (the fused kernels ping-pong between them)."""
         if work is None:
             work = torch.empty_like(state)
         fin = C.c_int32(0)
-        fn = L.lib().dnx_fdtd3d_run if self.dim == 3 else L.lib().dnx_fdtd2d_run
-        L.check(fn(self.grid(), L.ptr(state), L.ptr(work), int(n0), int(n_steps), srcs[0],
-                   srcs[1], mons[0], mons[1], L.ptr(ckpt), int(ckpt_every), L.ptr(estore),
-                   int(probe), L.ptr(probe_out), self.stream(), C.byref(fin)))
+        if self.dim == 3:
+            L.check(L.lib().dnx_fdtd3d_run_from(
+                self.grid(), L.ptr(init), L.ptr(state), L.ptr(work), int(n0), int(n_steps),
+                srcs[0], srcs[1], mons[0], mons[1], L.ptr(ckpt), int(ckpt_every), L.ptr(estore),
+                int(probe), L.ptr(probe_out), self.stream(), C.byref(fin)))
+            return work if fin.value else state
+        if init is not None:
+            state.copy_(init)
+        L.check(L.lib().dnx_fdtd2d_run(self.grid(), L.ptr(state), L.ptr(work), int(n0),
+                                       int(n_steps), srcs[0], srcs[1], mons[0], mons[1],
+                                       L.ptr(ckpt), int(ckpt_every), L.ptr(estore), int(probe),
+                                       L.ptr(probe_out), self.stream(), C.byref(fin)))
         return work if fin.value else state
 
     def adjoint_init(self, lam, e_T, mons, n_total, S):
@@ -335,8 +344,8 @@ class _TimeReversalFDTD(torch.autograd.Function):
             slot = ctx.ckpt[k * ns * N:(k + 1) * ns * N]
             if b - a > 1:
                 _begin("replay")
-                work.copy_(slot)
-                eng.run(work, a, b - a - 1, srcs[:2], (None, 0), estore=estore, work=work2)
+                eng.run(work, a, b - a - 1, srcs[:2], (None, 0), estore=estore, work=work2,
+                        init=slot)
                 _end("replay")
             _begin("adjoint")
             lam, lam_spare = eng.adjoint_run(lam, b, a, slot[:ne * N], estore, srcs[:2], mons[:2],

@@ -83,6 +83,8 @@ _SIGS = {
     "dnx_finalize_grad": ([_P, _P, _P, _L, _I, _P], _I),
     "dnx_fdtd3d_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, C.POINTER(Source), _I,
                         C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),
+    "dnx_fdtd3d_run_from": ([C.POINTER(Grid3D), _P, _P, _P, _I, _I, C.POINTER(Source), _I,
+                             C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),
     "dnx_fdtd3d_adjoint_init": ([C.POINTER(Grid3D), _P, _P, C.POINTER(Monitor), _I, _I, _P, _P],
                                 _I),
     "dnx_fdtd3d_adjoint_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, _P, _P, C.POINTER(Source), _I,

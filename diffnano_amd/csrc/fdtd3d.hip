@@ -374,7 +374,8 @@ struct StreamScratch {
 template <typename T>
 static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
                 const dnx_source* srcs, int n_src, const dnx_monitor* mons, int n_mon, T* ckpt,
-                int ckpt_every, T* es, int64_t probe, T* probe_out, hipStream_t s, int* fin) {
+                int ckpt_every, T* es, int64_t probe, T* probe_out, hipStream_t s, int* fin,
+                const T* init = nullptr) {
   G3<T> g = make_g3<T>(gd);
   const int64_t N = g.n;
   const bool fused = use_fused(gd, work);
@@ -388,8 +389,26 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
     fa.src_prof = static_cast<const T*>(srcs[0].profile);
     j0 = 1;
   }
-  T* cur = st;
+  // The fused kernels ping-pong between st and work ("pp" buffers); a launch
+  // whose output is a checkpointed step writes straight into its
+  // checkpoint slot (no copy), and the next launch reads it from there.
+  // init (if given) is read by the first launch in place of st.
+  if (init && (!fused || n_steps == 0)) {
+    DNX_CHECK_HIP(hipMemcpyAsync(st, init, sizeof(T) * 6 * N, hipMemcpyDeviceToDevice, s));
+    init = nullptr;
+  }
+  T* cur = init ? const_cast<T*>(init) : st;
   T* nxt = fused ? work : st;
+  auto ckpt_slot = [&](int k) { return ckpt + int64_t(k / ckpt_every) * 6 * N; };
+  // output of a fused launch that produces step k_out (relative)
+  auto out_for = [&](int k_out) -> T* {
+    if (fused && ckpt && k_out < n_steps && k_out % ckpt_every == 0) return ckpt_slot(k_out);
+    return nxt;
+  };
+  auto advance = [&](T* out) {
+    cur = out;
+    if (out == st || out == work) nxt = (out == st) ? work : st;
+  };
   const dim3 grid = grid3_launch(gd), block(TX, TY);
   const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, probe, 2);
   Fused2Args<T> f2{};
@@ -408,14 +427,15 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
   for (int k = 0; k < n_steps; ++k) {
     const int n = n0 + k;
     T* slot = es ? es + int64_t(k) * 3 * N : nullptr;
-    if (ckpt && k % ckpt_every == 0) {
-      DNX_CHECK_HIP(hipMemcpyAsync(ckpt + int64_t(k / ckpt_every) * 6 * N, cur,
-                                   sizeof(T) * 6 * N, hipMemcpyDeviceToDevice, s));
+    if (ckpt && k % ckpt_every == 0 && cur != ckpt_slot(k)) {
+      DNX_CHECK_HIP(hipMemcpyAsync(ckpt_slot(k), cur, sizeof(T) * 6 * N,
+                                   hipMemcpyDeviceToDevice, s));
     }
     if (pairs && k + 1 < n_steps && !(ckpt && (k + 1) % ckpt_every == 0)) {
+      T* out = out_for(k + 2);
       if constexpr (std::is_same<T, float>::value) {
         f2.a.in = cur;
-        f2.a.out = nxt;
+        f2.a.out = out;
         f2.a.es = slot;
         f2.es2 = slot ? slot + 3 * N : nullptr;
         f2.n = n;
@@ -426,15 +446,14 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
         for (int j = 0; j < n_mon; ++j)
           if ((rc = launch_monitor_pair<T>(f2.side[j], mons[j], n, s))) return rc;
       }
-      T* t = cur;
-      cur = nxt;
-      nxt = t;
+      advance(out);
       ++k;  // the pair covered steps k and k+1
       continue;
     }
+    T* out = out_for(k + 1);  // (the in-place simple kernels write cur == nxt)
     if (fused) {
       fa.in = cur;
-      fa.out = nxt;
+      fa.out = out;
       fa.es = slot;
       if (fa.src_comp >= 0) fa.src_wave = static_cast<const T*>(srcs[0].wave) + n;
       TimedLaunch tl = timed_begin(s);
@@ -447,27 +466,23 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
       DNX_CHECK_HIP(hipGetLastError());
     }
     for (int j = j0; j < n_src; ++j) {
-      int rc = launch_source<T>(nxt + srcs[j].comp * N, g.ny, g.nx, srcs[j], n, s);
+      int rc = launch_source<T>(out + srcs[j].comp * N, g.ny, g.nx, srcs[j], n, s);
       if (rc) return rc;
       if (slot && (rc = launch_source<T>(slot + srcs[j].comp * N, g.ny, g.nx, srcs[j], n, s)))
         return rc;
     }
     if (probe >= 0) {
-      DNX_CHECK_HIP(hipMemcpyAsync(probe_out + k, nxt + probe, sizeof(T),
+      DNX_CHECK_HIP(hipMemcpyAsync(probe_out + k, out + probe, sizeof(T),
                                    hipMemcpyDeviceToDevice, s));
     }
-    Comps<T> E{{nxt, nxt + N, nxt + 2 * N}};
+    Comps<T> E{{out, out + N, out + 2 * N}};
     for (int j = 0; j < n_mon; ++j) {
       int rc = launch_monitor<T>(E, g.ny, g.nx, mons[j], n, s);
       if (rc) return rc;
     }
-    if (fused) {
-      T* t = cur;
-      cur = nxt;
-      nxt = t;
-    }
+    if (fused) advance(out);
   }
-  *fin = (cur != st) ? 1 : 0;
+  *fin = (cur != st) ? 1 : 0;  // the last output is never a checkpoint slot
   return DNX_OK;
 }
 
@@ -607,11 +622,12 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
 
 using namespace dnx;
 
-extern "C" int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, int32_t n0,
-                                  int32_t n_steps, const dnx_source* srcs, int32_t n_src,
-                                  const dnx_monitor* mons, int32_t n_mon, void* ckpt,
-                                  int32_t ckpt_every, void* estore, int64_t probe_index,
-                                  void* probe_out, void* stream, int32_t* final_in_work) {
+extern "C" int32_t dnx_fdtd3d_run_from(const dnx_grid3d* g, const void* init, void* state,
+                                       void* work, int32_t n0, int32_t n_steps,
+                                       const dnx_source* srcs, int32_t n_src,
+                                       const dnx_monitor* mons, int32_t n_mon, void* ckpt,
+                                       int32_t ckpt_every, void* estore, int64_t probe_index,
+                                       void* probe_out, void* stream, int32_t* final_in_work) {
   int rc = check_grid3(g);
   if (rc) return rc;
   DNX_REQUIRE(state != nullptr, "fdtd3d_run: null state");
@@ -624,20 +640,32 @@ extern "C" int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, 
                                    probe_index < int64_t(6) * g->nz * g->ny * g->nx),
               "run: bad probe index %lld", (long long)probe_index);
   DNX_REQUIRE(work != state, "fdtd3d_run: work must differ from state");
+  DNX_REQUIRE(init != state && init != work, "fdtd3d_run_from: init must differ from state and "
+              "work");
   int fin = 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (g->dtype == DNX_F32)
     rc = run3<float>(g, static_cast<float*>(state), static_cast<float*>(work), n0, n_steps, srcs,
                      n_src, mons, n_mon, static_cast<float*>(ckpt), ckpt_every,
                      static_cast<float*>(estore), probe_index, static_cast<float*>(probe_out), s,
-                     &fin);
+                     &fin, static_cast<const float*>(init));
   else
     rc = run3<double>(g, static_cast<double*>(state), static_cast<double*>(work), n0, n_steps,
                       srcs, n_src, mons, n_mon, static_cast<double*>(ckpt), ckpt_every,
                       static_cast<double*>(estore), probe_index, static_cast<double*>(probe_out),
-                      s, &fin);
+                      s, &fin, static_cast<const double*>(init));
   if (final_in_work) *final_in_work = fin;
   return rc;
+}
+
+extern "C" int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, int32_t n0,
+                                  int32_t n_steps, const dnx_source* srcs, int32_t n_src,
+                                  const dnx_monitor* mons, int32_t n_mon, void* ckpt,
+                                  int32_t ckpt_every, void* estore, int64_t probe_index,
+                                  void* probe_out, void* stream, int32_t* final_in_work) {
+  const void* init = nullptr;
+  return dnx_fdtd3d_run_from(g, init, state, work, n0, n_steps, srcs, n_src, mons, n_mon, ckpt,
+                             ckpt_every, estore, probe_index, probe_out, stream, final_in_work);
 }
 
 extern "C" int32_t dnx_fdtd3d_adjoint_init(const dnx_grid3d* g, void* lam, const void* e_T,

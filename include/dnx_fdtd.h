@@ -146,6 +146,15 @@ int32_t dnx_fdtd3d_run(const dnx_grid3d* g, void* state, void* work, int32_t n0,
                        const dnx_monitor* mons, int32_t n_mon, void* ckpt,
                        int32_t ckpt_every, void* estore, int64_t probe_index,
                        void* probe_out, void* stream, int32_t* final_in_work);
+/* As dnx_fdtd3d_run, with the initial state read from init (6*N, left
+ * unchanged) instead of state: the replay of a checkpoint segment starts
+ * from the checkpoint slot without copying it (init NULL: as
+ * dnx_fdtd3d_run).  Same reference functions as dnx_fdtd3d_run. */
+int32_t dnx_fdtd3d_run_from(const dnx_grid3d* g, const void* init, void* state, void* work,
+                            int32_t n0, int32_t n_steps, const dnx_source* srcs, int32_t n_src,
+                            const dnx_monitor* mons, int32_t n_mon, void* ckpt,
+                            int32_t ckpt_every, void* estore, int64_t probe_index,
+                            void* probe_out, void* stream, int32_t* final_in_work);
 
 /* Adjoint initialisation at m = T: lam[0:3N] holds dL/dE^T on entry;
  * adds the monitor terms of sample T (n_total = T), clears lam[3N:6N]
