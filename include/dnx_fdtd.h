@@ -5,7 +5,7 @@
  * (diffnano/solvers/fdtd3d.py, fdtd2d.py).  Those are pure PyTorch, so the
  * reference has no FFI of its own; each entry point below names the
  * reference function whose semantics it implements.  The Python host
- * (diffnano_amd) binds them with ctypes; INTEGRATION.md shows the binding.
+ * (diffnano_amd/_lib.py) binds them with ctypes.
  *
  * Conventions
  *  - Arrays are row-major [axis0][axis1][axis2] = [z][y][x] (3D) or [y][x]
