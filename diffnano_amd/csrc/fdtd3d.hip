@@ -320,6 +320,30 @@ static bool use_pairs(const dnx_grid3d* gd, bool fused, int n_src, int n_mon, in
   return !(v && strcmp(v, "1") == 0);
 }
 
+static int device_cus() {
+  static const int cus = [] {
+    int dev = 0, n = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      n = 256;
+    return n;
+  }();
+  return cus;
+}
+
+// The tile shape of `kernel` for this grid: the configured one, with the
+// z-chunk halved (down to 8 planes) while the grid would have fewer than
+// two work-groups per CU, so that a default shape tuned at 256^3 still
+// fills the device on smaller grids.  Results do not depend on tiles.
+static Fused3Tiles grid_tiles(const dnx_grid3d* gd, int kernel) {
+  Fused3Tiles t = fused3_tiles(gd->dtype, kernel);
+  if (t.ty <= 0) return t;
+  const int64_t target = 2 * int64_t(device_cus());
+  const int64_t ty_tiles = cdiv(gd->ny, t.ty);
+  while (t.zc > 8 && ty_tiles * cdiv(gd->nz, t.zc) < target) t.zc = (t.zc + 1) / 2;
+  return t;
+}
+
 template <typename T>
 static Fused2Args<T> fused2_args(const dnx_grid3d* gd, const Fused3Tiles& t,
                                  const dnx_source* srcs, int n_src, const dnx_monitor* mons,
@@ -379,7 +403,7 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
   G3<T> g = make_g3<T>(gd);
   const int64_t N = g.n;
   const bool fused = use_fused(gd, work);
-  const Fused3Tiles tiles = fused3_tiles(gd->dtype, 0);
+  const Fused3Tiles tiles = grid_tiles(gd, 0);
   Fused3Args<T> fa = fused ? fused_args<T>(gd, tiles) : Fused3Args<T>{};
   const int ty = tiles.ty;
   int j0 = 0;  // sources handled by the side kernel start at j0
@@ -415,7 +439,7 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
   int ty2 = 0;
   StreamScratch side;
   if (pairs) {
-    const Fused3Tiles t2 = fused3_tiles(gd->dtype, 2);
+    const Fused3Tiles t2 = grid_tiles(gd, 2);
     const int64_t se = side_elems(mons, n_mon, 2);
     if (se > 0) {
       int rc = side.alloc(sizeof(T) * se, s);
@@ -510,7 +534,7 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   G3<T> g = make_g3<T>(gd);
   const int64_t N = g.n;
   const bool fused = use_fused(gd, work);
-  const Fused3Tiles tiles = fused3_tiles(gd->dtype, 1);
+  const Fused3Tiles tiles = grid_tiles(gd, 1);
   Fused3Args<T> fa = fused ? fused_args<T>(gd, tiles) : Fused3Args<T>{};
   const int ty = tiles.ty;
   int j0 = 0;
@@ -533,7 +557,7 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   int ty2 = 0;
   StreamScratch side;
   if (pairs) {
-    const Fused3Tiles t2 = fused3_tiles(gd->dtype, 3);
+    const Fused3Tiles t2 = grid_tiles(gd, 3);
     const int64_t se = side_elems(mons, n_mon, 1);
     if (se > 0) {
       int rc = side.alloc(sizeof(T) * se, s);
