@@ -446,7 +446,8 @@ def main():
     # Dominant kernel: the timed launch class with the largest total time.
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     dom_k = kern[dom]
-    traffic = _read_traffic().get(dom_k["kernel"])
+    # measured bytes per launch exist for the 256^3 workload (the profiled one)
+    traffic = _read_traffic().get(dom_k["kernel"]) if args.config == "c2" else None
 
     cpu = _cpu_baseline(eps_np, solver_kw, args.cpu_steps)
 
