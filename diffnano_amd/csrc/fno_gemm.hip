@@ -19,6 +19,7 @@
 // lane l holds rows (l/16)*4 .. +4 of column l%16.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdlib>
 #include <type_traits>
 
@@ -848,17 +849,18 @@ __global__ __launch_bounds__(256, 1) void k_gemm_big(GemmArgs g, int tiles_n, in
 }
 
 template <bool ABF, bool BBF>
-static void launch_big(const GemmArgs& g, hipStream_t s) {
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gemm_big<ABF, BBF>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(XSMEM));
+static int launch_big(const GemmArgs& g, hipStream_t s) {
+  static std::atomic<bool> attr{false};
+  if (!attr.load()) {
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gemm_big<ABF, BBF>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(XSMEM)));
     attr = true;
   }
   const int tn = int(cdiv(g.N, XBN)), tiles = int(cdiv(g.M, XBM)) * tn;
   const int nb = ((tiles + 7) / 8) * 8;
   hipLaunchKernelGGL((k_gemm_big<ABF, BBF>), dim3(nb, 1, g.batch), dim3(256), XSMEM, s, g, tn,
                      tiles);
+  return DNX_OK;
 }
 
 // out[i] = sum_j part[j*n + i] + beta * out[i]
@@ -929,10 +931,12 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
     done = true;
   } else if (!generic && nt_ok(*g) && g->M >= XBM && g->N >= XBN && g->K >= XBIG_K &&
              !getenv_flag("DNX_GEMM_SMALL")) {
-    if (g->a_bf16 && g->b_bf16) launch_big<true, true>(*g, s);
-    else if (g->a_bf16) launch_big<true, false>(*g, s);
-    else if (g->b_bf16) launch_big<false, true>(*g, s);
-    else launch_big<false, false>(*g, s);
+    int rc;
+    if (g->a_bf16 && g->b_bf16) rc = launch_big<true, true>(*g, s);
+    else if (g->a_bf16) rc = launch_big<true, false>(*g, s);
+    else if (g->b_bf16) rc = launch_big<false, true>(*g, s);
+    else rc = launch_big<false, false>(*g, s);
+    if (rc) return rc;
     done = true;
   } else if (!generic && nt_ok(*g)) {
     if (g->a_bf16 && g->b_bf16) hipLaunchKernelGGL((k_gemm_nt<true, true>), grid, dim3(256), 0, s, *g);
