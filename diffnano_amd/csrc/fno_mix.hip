@@ -18,6 +18,8 @@
 // so that the 16 lanes of a mode run read 64 contiguous bytes.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/dnx_fno.h"
 #include "dnx_common.h"
 
@@ -50,10 +52,13 @@ __device__ __forceinline__ void mix_stage(const float* __restrict__ X, int B, in
   }
 }
 
+// TY = float, or unsigned short for BF16 output (round to nearest even: the
+// rounding the GEMM that consumes Y would apply to an FP32 operand).
+template <typename TY>
 __global__ __launch_bounds__(512) void k_mode_mix(const float* __restrict__ X,
                                                   const float* __restrict__ Wr,
                                                   const float* __restrict__ Wi,
-                                                  float* __restrict__ Y, int B, int M, int64_t ld,
+                                                  TY* __restrict__ Y, int B, int M, int64_t ld,
                                                   float s) {
   __shared__ float Xs[2 * MIX_BB * MIX_MB * MIX_LD];
   const int t = threadIdx.x;
@@ -86,9 +91,14 @@ __global__ __launch_bounds__(512) void k_mode_mix(const float* __restrict__ X,
           yi += ar[e] * wi[p4 + e] + ai[e] * wr[p4 + e];
         }
       }
-      float* yrow = Y + (int64_t(q) * B + b0 + b) * ld;
-      yrow[mu] = yr;
-      yrow[M + mu] = yi;
+      TY* yrow = Y + (int64_t(q) * B + b0 + b) * ld;
+      if constexpr (std::is_same<TY, float>::value) {
+        yrow[mu] = yr;
+        yrow[M + mu] = yi;
+      } else {
+        yrow[mu] = __builtin_bit_cast(unsigned short, (__bf16)yr);
+        yrow[M + mu] = __builtin_bit_cast(unsigned short, (__bf16)yi);
+      }
     }
   }
 }
@@ -217,18 +227,23 @@ static bool mix_ok(const void* X, int32_t B, int32_t M, int64_t ld, int32_t c) {
          ld % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15u) == 0;
 }
 
-extern "C" int32_t dnx_fno_mode_mix(const float* X, const float* Wr, const float* Wi, float* Y,
-                                    int32_t B, int32_t M, int64_t ld, int32_t c, int32_t conj,
-                                    void* stream) {
+extern "C" int32_t dnx_fno_mode_mix(const float* X, const float* Wr, const float* Wi, void* Y,
+                                    int32_t y_bf16, int32_t B, int32_t M, int64_t ld, int32_t c,
+                                    int32_t conj, void* stream) {
   DNX_REQUIRE(X && Wr && Wi && Y, "fno_mode_mix: null operand");
   DNX_REQUIRE(mix_ok(X, B, M, ld, c),
               "fno_mode_mix: needs 32 channels, B %% 8 == 0, M %% 16 == 0, ld >= 2M, ld %% 4 "
               "== 0, 16-byte aligned X (B=%d M=%d ld=%lld c=%d)", B, M, (long long)ld, c);
   const int chunks = B / MIX_BB;
   const int gy = chunks < 32 ? chunks : 32;
-  hipLaunchKernelGGL(k_mode_mix, dim3(M / MIX_MB, gy), dim3(512), 0,
-                     static_cast<hipStream_t>(stream), X, Wr, Wi, Y, B, M, ld,
-                     conj ? -1.f : 1.f);
+  if (y_bf16)
+    hipLaunchKernelGGL(k_mode_mix<unsigned short>, dim3(M / MIX_MB, gy), dim3(512), 0,
+                       static_cast<hipStream_t>(stream), X, Wr, Wi,
+                       static_cast<unsigned short*>(Y), B, M, ld, conj ? -1.f : 1.f);
+  else
+    hipLaunchKernelGGL(k_mode_mix<float>, dim3(M / MIX_MB, gy), dim3(512), 0,
+                       static_cast<hipStream_t>(stream), X, Wr, Wi, static_cast<float*>(Y), B, M,
+                       ld, conj ? -1.f : 1.f);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }

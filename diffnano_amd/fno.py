@@ -69,11 +69,13 @@ def gemm(M, N, K, A, sA, B, sB, Cout, sC, *, batch=1, a_off=0, b_off=0, c_off=0,
 
 
 def mode_mix(X, Wr, Wi, Y, B, M, ld, c, conj=False):
-    """Y[q][b][r*M+mu] = sum_p X[p][b][.] (Wr + i s Wi)[p][q][mu] (FP32,
-    s = -1 if conj); Wr, Wi are [p][q][M] contiguous."""
+    """Y[q][b][r*M+mu] = sum_p X[p][b][.] (Wr + i s Wi)[p][q][mu] (FP32
+    arithmetic, s = -1 if conj); Wr, Wi are [p][q][M] contiguous; Y float32
+    or bfloat16 (rounded to nearest even)."""
+    ybf = Y.dtype == torch.bfloat16
     L.check(L.lib().dnx_fno_mode_mix(X.data_ptr(), Wr.data_ptr(), Wi.data_ptr(), Y.data_ptr(),
-                                     int(B), int(M), int(ld), int(c), 1 if conj else 0,
-                                     L.stream_handle(Y.device)))
+                                     1 if ybf else 0, int(B), int(M), int(ld), int(c),
+                                     1 if conj else 0, L.stream_handle(Y.device)))
     return Y
 
 
@@ -232,7 +234,10 @@ class FNO2d:
         for l in range(self.layers):
             Z = torch.empty(c * B, M2, device=dev)
             gemm_nt(c * B, M2, P, h, self.FT, Z)
-            Y = torch.empty(c * B, M2, device=dev)
+            # the inverse-DFT product rounds Y to BF16 on load: the mixing
+            # kernel writes it rounded (same values, half the bytes)
+            Y = torch.empty(c * B, M2, device=dev,
+                            dtype=torch.bfloat16 if self._fast_mix_ok(B) else torch.float32)
             self._mix(Z, p[f"sr{l}"], p[f"si{l}"], Y, B)
             S = torch.empty(c, n, device=dev)
             gemm(c * B, P, M2, Y, (M2, 1, 0), self.GT, (1, M2, 0), S, (P, 1, 0))
@@ -343,8 +348,8 @@ class FNO2d:
             sZ = (ld, B * ld, 1)       # Z/dZ/dY as [b][i] per mode
             sWt = (1, c, c * c)        # W^T: B[k=o][n=i] = W[mu][i][o]
             sWc = (c, 1, c * c)        # dW[mu][i][o]
-            dZ = torch.empty(c * B, M2, device=dev)
             if self._fast_mix_ok(B):
+                dZ = torch.empty(c * B, M2, device=dev, dtype=torch.bfloat16)
                 # dZ = dY W^H per mode; dW = sum_b conj(Z) dY
                 mode_mix(dY, sr.permute(2, 1, 0).contiguous(), si.permute(2, 1, 0).contiguous(),
                          dZ, B, M, ld, c, conj=True)
@@ -359,6 +364,7 @@ class FNO2d:
                      Cin=dhn)
                 dh = dhn
                 continue
+            dZ = torch.empty(c * B, M2, device=dev)
             kw = dict(batch=M)
             # dZr = dYr Wr^T + dYi Wi^T ; dZi = -dYr Wi^T + dYi Wr^T
             gemm(B, c, c, dY, sZ, sr, sWt, dZ, sZ, a_off=0, c_off=0, **kw)

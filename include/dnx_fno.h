@@ -60,11 +60,14 @@ int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, int64_t n, vo
  *   Y[q][b][r*M + mu] (r = 0 real, 1 imaginary part)
  *     = sum_p X[p][b][.] (Wr + i s Wi)[p][q][mu],   s = -1 if conj else +1
  * X rows (p, b) and Y rows (q, b) have length ld (>= 2M, multiple of 4; X
- * 16-byte aligned); Wr, Wi are [p][q][mu] (mode fastest).  32 channels,
+ * 16-byte aligned); Wr, Wi are [p][q][mu] (mode fastest).  Y is FP32, or
+ * BF16 (round to nearest even) if y_bf16 -- the rounding a BF16 MFMA
+ * product applies to an FP32 operand anyway.  32 channels,
  * B % 8 == 0, M % 16 == 0.  (The spectral convolution of an FNO layer,
  * no reference counterpart.) */
-int32_t dnx_fno_mode_mix(const float* X, const float* Wr, const float* Wi, float* Y, int32_t B,
-                         int32_t M, int64_t ld, int32_t c, int32_t conj, void* stream);
+int32_t dnx_fno_mode_mix(const float* X, const float* Wr, const float* Wi, void* Y,
+                         int32_t y_bf16, int32_t B, int32_t M, int64_t ld, int32_t c,
+                         int32_t conj, void* stream);
 /* Its weight gradient in nsplit partial sums over sample chunks:
  *   dWr[s][p][q][mu] + i dWi[s][p][q][mu] = sum_{b in split s} conj(X[p][b][mu]) dY[q][b][mu]
  * (nsplit divides B / 8). */

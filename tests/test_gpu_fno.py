@@ -229,6 +229,9 @@ def test_mode_mix_kernels_match_fp64(conj):
     assert rel_l2(y[..., :M].numpy(), ref.real.numpy()) < 1e-5
     assert rel_l2(y[..., M:2 * M].numpy(), ref.imag.numpy()) < 1e-5
     assert torch.all(y[..., 2 * M:] == 7.0)  # padding untouched
+    Yb = torch.zeros((c * B, ld), device="cuda", dtype=torch.bfloat16)
+    mode_mix(X, Wr, Wi, Yb, B, M, ld, c, conj=conj)  # BF16 output: Y rounded to nearest even
+    assert torch.equal(Yb[:, :2 * M], Y[:, :2 * M].to(torch.bfloat16))
     ns = 3
     pr = torch.empty(ns, c, c, M, device="cuda")
     pi = torch.empty_like(pr)
