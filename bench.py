@@ -284,6 +284,9 @@ def _bench_surrogate(args):
     sync()
     t_ex = time.perf_counter() - t0
     target = ex.parent_fom
+    # every surrogate loop starts from the trained weights (fine-tuning
+    # changes them in place)
+    trained = sur.state_dict()
     sl = SG.PrescreenDesigner(cfg, sur, mode="surrogate", seed=7, device=dev,
                               finetune_steps=args.finetune)
     sync()
@@ -296,6 +299,7 @@ def _bench_surrogate(args):
     rnd.run(iters)
     # ablation: the same 8 FDTD solves per step, ranked by the network (no
     # fallback) instead of picked at random
+    sur.load_state_dict(trained)
     nf = SG.PrescreenDesigner(cfg, sur, mode="surrogate", seed=7, device=dev,
                               max_rel_err=float("inf"), finetune_steps=args.finetune)
     sync()
@@ -303,6 +307,11 @@ def _bench_surrogate(args):
     nf.run(iters)
     sync()
     t_nf = time.perf_counter() - t0
+    # the same without anchoring the prediction on the parent's FDTD field
+    sur.load_state_dict(trained)
+    nr = SG.PrescreenDesigner(cfg, sur, mode="surrogate", seed=7, device=dev, anchor=False,
+                              max_rel_err=float("inf"), finetune_steps=args.finetune)
+    nr.run(iters)
     pl = SG.PrescreenDesigner(cfg, mode="plain", seed=7, device=dev)
     sync()
     t0 = time.perf_counter()
@@ -331,6 +340,9 @@ def _bench_surrogate(args):
                                   "fom_gain": nf.parent_fom / nf.history[0] - 1,
                                   "wall_s": round(t_nf, 2),
                                   "mean_rel_err": round(float(np.mean(nf.stats["rel_err"])), 4)},
+        "surrogate_same_budget_unanchored": {
+            "iterations": iters, "fom": nr.parent_fom, "fom_gain": nr.parent_fom / nr.history[0] - 1,
+            "mean_rel_err": round(float(np.mean(nr.stats["rel_err"])), 4)},
         "exhaustive_fom_gain": target / ex.history[0] - 1,
         "speedup_vs_plain_at_equal_fom": round(t_pl / t_sl, 1) if reached else None,
         "speedup_vs_exhaustive_batched_at_equal_fom": round(t_ex / t_sl, 2) if reached else None,

@@ -207,10 +207,14 @@ class PrescreenDesigner:
     * ``"surrogate"``: the FNO ranks all candidates and the batched FDTD
       judges the top ``top_k`` (full-grid fields).  If the surrogate's FoM
       relative error over those exceeds ``max_rel_err``, every candidate of
-      the step is judged by the FDTD (fallback).  With ``finetune_steps`` >
-      0, every FDTD solution of the loop also goes into a replay buffer the
-      surrogate is fine-tuned on after each step (it learns the
-      neighbourhood the search is in);
+      the step is judged by the FDTD (fallback).  With ``anchor`` (default)
+      the predicted FoM-line field of a candidate is the parent's FDTD
+      field plus the network's predicted change, FNO(candidate) -
+      FNO(parent): an additive low/high-fidelity correction in which the
+      network's error common to the parent and its mutations cancels.
+      With ``finetune_steps`` > 0, every FDTD solution of the loop also goes
+      into a replay buffer the surrogate is fine-tuned on after each step
+      (it learns the neighbourhood the search is in);
     * ``"random"``: ``top_k`` candidates picked at random are judged (the
       same FDTD budget without the network);
     * ``"exhaustive"``: every candidate is judged by the batched FDTD;
@@ -223,8 +227,9 @@ class PrescreenDesigner:
 
     def __init__(self, cfg, surrogate=None, mode="surrogate", n_candidates=256, top_k=8,
                  max_rel_err=0.05, patch=24, strength=0.5, seed=0, device=None,
-                 finetune_steps=0, finetune_batch=32, finetune_lr=2e-4, buffer=1024):
-        self.cfg, self.sur, self.mode = cfg, surrogate, mode
+                 finetune_steps=0, finetune_batch=32, finetune_lr=2e-4, buffer=1024,
+                 anchor=True):
+        self.cfg, self.sur, self.mode, self.anchor = cfg, surrogate, mode, bool(anchor)
         if mode == "surrogate" and surrogate is None:
             raise ValueError("mode 'surrogate' needs a trained FieldSurrogate")
         self.nc, self.top_k, self.max_rel_err = int(n_candidates), int(top_k), float(max_rel_err)
@@ -251,15 +256,30 @@ class PrescreenDesigner:
         self.rng = torch.Generator(device="cpu").manual_seed(seed)       # candidates
         self.aux_rng = torch.Generator(device="cpu").manual_seed(seed + 1)  # picks, batches
         self.parent = workloads.blob_noise(seed, cfg.size).to(self.dev).unsqueeze(0)
-        self.parent_fom = float(self._judge(cfg.eps(self.parent))[0][0])
+        fom0, line0 = self._judge(cfg.eps(self.parent))
+        self.parent_fom, self.parent_line = float(fom0[0]), line0[:1]
         self.history = [self.parent_fom]
         self.stats = {"steps": 0, "fallbacks": 0, "fdtd_solves": 1, "rel_err": [],
                       "time_s": {"candidates": 0.0, "surrogate": 0.0, "fdtd": 0.0,
                                  "finetune": 0.0}}
 
     # -- pieces -------------------------------------------------------------
+    def _line(self, fields):
+        """FoM-line DFT field [B, 1, n] of full-grid fields [B, nf, H, W]."""
+        y0, y1, x0, x1 = self.cfg.line
+        return fields[:, self.cfg.fom_index, y0:y1, x0:x1]
+
+    def _predict_fom(self, eps):
+        """Surrogate FoM of each candidate (anchored on the parent's FDTD
+        field when ``anchor`` is set)."""
+        line = self._line(self.sur.predict(eps))
+        if self.anchor:
+            base = self._line(self.sur.predict(self.cfg.eps(self.parent)))
+            line = self.parent_line + (line - base)
+        return (line.abs() ** 2).flatten(1).sum(1).double()
+
     def _judge(self, eps):
-        """(FoM, line field) of each design from the batched FDTD."""
+        """(FoM, line field [B, 1, n]) of each design from the batched FDTD."""
         f = self.judge.forward(eps)[:, 0]
         return (f.abs() ** 2).flatten(1).sum(1).double(), f
 
@@ -318,32 +338,34 @@ class PrescreenDesigner:
         cand = self._candidates()
         eps = self.cfg.eps(cand)
         t0 = self._tick("candidates", t0)
+        line = None
         if self.mode == "plain":
             fom = plain_fom(self.cfg, eps, self.dev).to(self.dev)
             idx = torch.arange(self.nc, device=self.dev)
             st["fdtd_solves"] += self.nc
             self._tick("fdtd", t0)
         elif self.mode == "exhaustive":
-            fom, _ = self._judge(eps)
+            fom, line = self._judge(eps)
             idx = torch.arange(self.nc, device=self.dev)
             st["fdtd_solves"] += self.nc
             self._tick("fdtd", t0)
         elif self.mode == "random":
             idx = torch.randperm(self.nc, generator=self.aux_rng)[:self.top_k].to(self.dev)
-            fom, _ = self._judge(eps[idx])
+            fom, line = self._judge(eps[idx])
             st["fdtd_solves"] += self.top_k
             self._tick("fdtd", t0)
         else:
-            pfom = self.cfg.fom(self.sur.predict(eps)).double()
+            pfom = self._predict_fom(eps)
             idx = torch.argsort(pfom, descending=True)[:self.top_k]
             t0 = self._tick("surrogate", t0)
-            fom, _ = self._judge_full(eps[idx])
+            fom, full = self._judge_full(eps[idx])
+            line = self._line(full)
             st["fdtd_solves"] += self.top_k
             err = ((pfom[idx] - fom).abs() / fom.clamp_min(1e-300)).mean().item()
             st["rel_err"].append(err)
             if err > self.max_rel_err:
                 st["fallbacks"] += 1
-                fom, _ = self._judge(eps)
+                fom, line = self._judge(eps)
                 idx = torch.arange(self.nc, device=self.dev)
                 st["fdtd_solves"] += self.nc
             t0 = self._tick("fdtd", t0)
@@ -353,6 +375,8 @@ class PrescreenDesigner:
         if float(fom[best]) > self.parent_fom:
             self.parent = cand[int(idx[best])].unsqueeze(0)
             self.parent_fom = float(fom[best])
+            if line is not None:
+                self.parent_line = line[best:best + 1].clone()
         st["steps"] += 1
         self.history.append(self.parent_fom)
         return self.parent_fom

@@ -63,10 +63,14 @@ def test_surrogate_fit_and_prescreen_bookkeeping():
     assert err.shape == (8,) and torch.isfinite(err).all()
     d = PrescreenDesigner(cfg, sur, n_candidates=32, top_k=4, max_rel_err=0.0, seed=1)
     f0 = d.parent_fom
+    # anchored estimate of the parent itself: its own FDTD figure of merit
+    assert float(d._predict_fom(cfg.eps(d.parent))[0]) == f0
     d.run(2)
     # max_rel_err 0: every step falls back to judging all candidates
     assert d.stats["fallbacks"] == 2 and d.stats["fdtd_solves"] == 1 + 2 * (4 + 32)
     assert d.parent_fom >= f0 and len(d.history) == 3
+    # the parent's line field follows the accepted design
+    assert float((d.parent_line.abs() ** 2).sum()) == pytest.approx(d.parent_fom, rel=1e-6)
     r = PrescreenDesigner(cfg, None, mode="random", n_candidates=32, top_k=4, seed=1)
     r.run(1)
     assert r.stats["fdtd_solves"] == 1 + 4
@@ -83,15 +87,23 @@ def test_ablation_network_ranking_beats_random_selection():
     sur = FieldSurrogate(cfg, device="cuda")
     sur.fit(eps, fields, epochs=8, batch=64, lr=3e-3)
     del eps, fields
-    lifts, best_pct = [], []
+    lifts, best_pct, lifts_a, err, err_a = [], [], [], [], []
     for seed in (101, 202, 303):
-        d = PrescreenDesigner(cfg, mode="exhaustive", seed=seed, device="cuda")
+        d = PrescreenDesigner(cfg, sur, mode="surrogate", seed=seed, device="cuda")
         cand = cfg.eps(d._candidates())
         true_fom, _ = d._judge(cand)
         pred_fom = cfg.fom(sur.predict(cand)).double()
         top = torch.argsort(pred_fom, descending=True)[:8]
         lifts.append(float(true_fom[top].mean() / true_fom.mean()))
         best_pct.append(float((true_fom < true_fom[top].max()).double().mean()))
+        err.append(float(((pred_fom - true_fom).abs() / true_fom).mean()))
+        # anchored on the parent's FDTD field (the prescreen loop's default)
+        pa = d._predict_fom(cand)
+        ta = torch.argsort(pa, descending=True)[:8]
+        lifts_a.append(float(true_fom[ta].mean() / true_fom.mean()))
+        err_a.append(float(((pa - true_fom).abs() / true_fom).mean()))
     print("top-8 mean FoM / all-candidate mean:", lifts, "best-of-8 percentile:", best_pct)
+    print("anchored: top-8 lift", lifts_a, "FoM rel. error", err_a, "unanchored", err)
     assert np.mean(lifts) > 1.0
     assert np.mean(best_pct) > 0.5
+    assert np.mean(lifts_a) > 1.0
