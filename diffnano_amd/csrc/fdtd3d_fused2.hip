@@ -146,6 +146,16 @@ __device__ __forceinline__ void add_source(const Fused3Args<T>& a, T w, int z, i
     if (cc == a.src_comp) e[cc] += s;
 }
 
+// A uniform value kept in a vector register.  The per-row profile factors
+// are VALU operands on every plane; held in scalar registers they push the
+// tile kernels past the scalar register file, and the compiler then parks
+// scalars in vector lanes and reads them back with v_readlane every plane.
+__device__ __forceinline__ float vreg(float v) {
+  float r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
+  return r;
+}
+
 template <typename T, int TY, bool PF>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
@@ -178,8 +188,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     const int y = y0 - 2 + r;
     const bool in = y >= 0 && y < a.ny;
     if (in) rvm |= 1u << r;
-    byr[r] = in ? a.by[y] : T(0);
-    ayr[r] = in ? a.ay[y] : T(0);
+    byr[r] = vreg(in ? a.by[y] : T(0));
+    ayr[r] = vreg(in ? a.ay[y] : T(0));
   }
   auto rv = [&](int r) { return ((rvm >> r) & 1u) != 0; };
   // Loads use a clamped row and column so that every one is in bounds and
@@ -199,10 +209,14 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   // part.  Every offset stays below 2^31 bytes (checked on the host).
   static_assert(sizeof(T) == 4, "the two-step forward kernel is float32");
   const int cbytes = int(N * int64_t(sizeof(T)));  // one component
-  auto rsrc = [&](const T* base, int64_t plane, int ncomp) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base + plane * P), 0,
-                                             int((ncomp * N - plane * P) * int64_t(sizeof(T))),
-                                             0x00020000);
+  const int pbytes = int(P * int64_t(sizeof(T)));   // one plane
+  // 32-bit byte arithmetic (every offset is below 2^31): fewer scalar
+  // registers and no 64-bit multiplies per plane
+  auto rsrc = [&](const T* base, int plane, int ncomp) {
+    const int off = plane * pbytes;
+    return __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<char*>(reinterpret_cast<const char*>(base)) + off, 0, ncomp * cbytes - off,
+        0x00020000);
   };
   auto ld = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r) -> T {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, int(voff[r]), c * cbytes, 0));
@@ -642,8 +656,8 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
     const int y = y0 - 2 + r;
     const bool in = y >= 0 && y < a.ny;
     if (in) rvm |= 1u << r;
-    byr[r] = in ? a.by[y] : T(0);
-    ayr[r] = in ? a.ay[y] : T(0);
+    byr[r] = vreg(in ? a.by[y] : T(0));
+    ayr[r] = vreg(in ? a.ay[y] : T(0));
   }
   auto rv = [&](int r) { return ((rvm >> r) & 1u) != 0; };
   // Accesses as in fwd2_tile: clamped row and column (every load in bounds,
@@ -660,10 +674,12 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
     voff[r] = uint32_t((min(max(y0 - 2 + r, 0), a.ny - 1) * W + xc) * int(sizeof(T)));
   static_assert(sizeof(T) == 4, "the two-step adjoint kernel is float32");
   const int cbytes = int(N * int64_t(sizeof(T)));
-  auto rsrc = [&](const T* base, int64_t plane, int ncomp) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base + plane * P), 0,
-                                             int((ncomp * N - plane * P) * int64_t(sizeof(T))),
-                                             0x00020000);
+  const int pbytes = int(P * int64_t(sizeof(T)));
+  auto rsrc = [&](const T* base, int plane, int ncomp) {
+    const int off = plane * pbytes;
+    return __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<char*>(reinterpret_cast<const char*>(base)) + off, 0, ncomp * cbytes - off,
+        0x00020000);
   };
   auto ld = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r) -> T {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, int(voff[r]), c * cbytes, 0));

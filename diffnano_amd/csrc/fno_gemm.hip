@@ -505,12 +505,12 @@ static bool launch_tile(const GemmArgs& g, dim3 grid, hipStream_t s) {
 // C[M][n] = act(alpha A[M][K] X[K][n] + beta Cin + bias) with M, K <= 128 and
 // n = batch x pixels in the millions: the FNO's lifting, pointwise and
 // projection layers and their data gradients.  They move O(n (M + K)) bytes
-// for O(n M K) flops with M K <= 4096, i.e. they are HBM-bound, so they run
-// on the vector ALU in FP32 (operands rounded to BF16 first, as on the
-// matrix cores): each thread owns 4 consecutive columns (16-byte loads and
-// stores), A sits in LDS and is read by broadcast.  With 4M <= 128 the
-// accumulators stay in registers while X streams along K; otherwise
-// (M = 128) the thread's 4K inputs stay in registers while M streams.
+// for O(n M K) flops with M K <= 4096.  This kernel runs them on the vector
+// ALU in FP32 (operands rounded to BF16 first, as on the matrix cores): each
+// thread owns 4 consecutive columns (16-byte loads and stores), A sits in
+// LDS and is read by broadcast.  The shapes with K <= 32 run on the matrix
+// cores instead (k_pw, below); this kernel takes the K = 128 ones and the
+// cross-check.
 
 __device__ inline float bfr(float v) { return __uint_as_float(uint32_t(f2bf(v)) << 16); }
 
@@ -566,7 +566,11 @@ __global__ __launch_bounds__(256) void k_cmix(GemmArgs g) {
     }
     *reinterpret_cast<float4*>(g.C + ci) = float4{v[0], v[1], v[2], v[3]};
   };
-  if constexpr (4 * M <= 128) {
+  // Small K: all K input rows are loaded first (K loads in flight per
+  // thread), then each output row is summed and stored.  Large K with few
+  // outputs: the outputs stay in registers while K streams.  The sum over k
+  // runs in the same order either way.
+  if constexpr (K > 32 && 4 * M <= 128) {
     float4 acc[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) acc[m] = float4{0.f, 0.f, 0.f, 0.f};
@@ -913,6 +917,162 @@ __global__ void k_gelu_bwd(const float* dy, const float* __restrict__ pre, float
   for (int64_t q = i; q < i + 4 && q < n; ++q) dx[q] = dy[q] * gelu_grad(pre[q]);
 }
 
+// ---- Channel-mixing products on the matrix cores ------------------------
+//
+// The same products as k_cmix (C[m][n] = act(alpha sum_k W[m][k] X[k][n] +
+// beta Cin[m][n] + bias), n = batch x pixels), with W and X rounded to BF16
+// as k_cmix does and multiplied on v_mfma_f32_32x32x16_bf16 (FP32
+// accumulation; the order of the k sum differs from k_cmix's, so results
+// agree to FP32 rounding, not bit for bit).  On the vector ALU the products
+// cost 2 M K flops per cell (M K = 4096: 0.44 ms of FMA per 2^22 cells);
+// here the matrix cores take them and the kernel streams X, Cin and C
+// (measured per call at 2^22 / 2^21 cells, against k_cmix: 32 x 32 261 vs
+// 292 us average, 128 x 32 770 vs 834, 32 x 9 167 vs 209).
+//
+// MFMA operands: A = W, 32 rows per tile (MT tiles, rows >= M zero), kept in
+// registers for the whole launch; B = X, 32 cells, K padded to 16 KS with
+// zeros.  Lane l holds W row l%32 and cell l%32 with k = (l/32)*8 .. +8; the
+// accumulator element v of lane l is output row 8(v/4) + 4(l/32) + v%4 at
+// cell l%32, so every load and store of X, Cin, pre and C covers 32
+// consecutive cells of a row (128-byte segments of FP32).  Each wave walks
+// groups of 32 cells.
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <int M, int K, bool BBF>
+__global__ __launch_bounds__(256) void k_pw(GemmArgs g) {
+  constexpr int MT = (M + 31) / 32, KS = (K + 15) / 16;
+  __shared__ float bs[128];
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 31, half = lane >> 5, kq = half * 8;
+  const int N = g.N;
+  if (threadIdx.x < 128)
+    bs[threadIdx.x] = (g.bias_mode == 2 && int(threadIdx.x) < M) ? g.bias[threadIdx.x] : 0.f;
+  bf16x8 wa[MT][KS];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int m = mt * 32 + lr;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      u16x8 u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = ks * 16 + kq + j;
+        u[j] = (m < M && k < K) ? load_bf(g.A, int64_t(m) * g.sAm + int64_t(k) * g.sAk, g.a_bf16)
+                                : (unsigned short)0;
+      }
+      wa[mt][ks] = __builtin_bit_cast(bf16x8, u);
+    }
+  }
+  __syncthreads();
+  // Buffer accesses: a descriptor per block of rows (16 of X, 32 of C, pre
+  // and Cin) with its base in scalar registers, the row within the block as
+  // scalar offset and the cell as the lane's vector offset; byte offsets
+  // within a block stay below 2^31 (checked by the launcher).
+  constexpr int xb_bytes = BBF ? 2 : 4;
+  const int sBk = int(g.sBk) * xb_bytes, sCm = int(g.sCm) * 4, sIm = int(g.sIm) * 4;
+  auto rsrc = [](const void* base, int64_t off) {
+    return __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<char*>(static_cast<const char*>(base)) + off, 0, 0x7fffffff, 0x00020000);
+  };
+  const int groups = (N + 31) / 32;
+  const int stride = int(gridDim.x) * 4;
+  for (int gi = int(blockIdx.x) * 4 + int(threadIdx.x >> 6); gi < groups; gi += stride) {
+    const int n = gi * 32 + lr;
+    const bool nv = n < N;
+    const int nc = nv ? n : N - 1;
+    bf16x8 xb[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      u16x8 u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = ks * 16 + kq + j;
+        const auto rx = rsrc(g.B, int64_t(ks * 16) * sBk);
+        unsigned short v = 0;
+        if (k < K) {
+          // lane-dependent part (cell, k block of the lane half) in the
+          // vector offset, the row within it in the scalar offset
+          if constexpr (BBF)
+            v = __builtin_amdgcn_raw_buffer_load_b16(rx, nc * 2 + kq * sBk, j * sBk, 0);
+          else
+            v = f2bf(__uint_as_float(
+                __builtin_amdgcn_raw_buffer_load_b32(rx, nc * 4 + kq * sBk, j * sBk, 0)));
+        }
+        u[j] = v;
+      }
+      xb[ks] = __builtin_bit_cast(bf16x8, u);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int mb = mt * 32 + 4 * half;  // row of element v: mb + 8(v/4) + v%4
+      float cin[16];
+      if (g.beta != 0.f) {
+        const auto ri = rsrc(g.Cin, int64_t(mt * 32) * sIm);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int m = mb + 8 * (v >> 2) + (v & 3);
+          cin[v] = m < M ? __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                               ri, nc * 4 + 4 * half * sIm, (8 * (v >> 2) + (v & 3)) * sIm, 0))
+                         : 0.f;
+        }
+      }
+      f32x16 acc;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[mt][ks], xb[ks], acc, 0, 0, 0);
+      if (!nv) continue;
+      const int vo = n * 4 + 4 * half * sCm;
+      const auto rc = rsrc(g.C, int64_t(mt * 32) * sCm);
+      const auto rp = rsrc(g.pre ? g.pre : g.C, int64_t(mt * 32) * sCm);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int m = mb + 8 * (v >> 2) + (v & 3);
+        if (m >= M) continue;
+        const int so = (8 * (v >> 2) + (v & 3)) * sCm;
+        float r = g.alpha * acc[v];
+        if (g.beta != 0.f) r += g.beta * cin[v];
+        if (g.bias_mode == 2) r += bs[m];
+        else if (g.bias_mode == 1) r += g.bias[n];
+        if (g.pre) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rp, vo, so, 0);
+        if (g.act == 1) r = gelu(r);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rc, vo, so, 0);
+      }
+    }
+  }
+}
+
+template <int M, int K>
+static bool pw_mk(const GemmArgs& g, hipStream_t s) {
+  if (g.M != M || g.K != K) return false;
+  const int64_t groups = (int64_t(g.N) + 31) / 32;
+  const int blocks = int(std::min<int64_t>((groups + 3) / 4, 4096));
+  if (g.b_bf16) hipLaunchKernelGGL((k_pw<M, K, true>), dim3(blocks), dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((k_pw<M, K, false>), dim3(blocks), dim3(256), 0, s, g);
+  return true;
+}
+
+// The FNO's channel-mixing shapes with K <= 32 inputs (M outputs x K inputs,
+// compile-time so that the row and k bounds fold away), cells contiguous.
+// The K = 128 shapes (the projection's data gradient and output layer)
+// stay on k_cmix, which measured faster for them (420 vs 462 us, 352 vs
+// 370 us at 2^22 and 2^21 cells).  DNX_GEMM_PW_VALU=1 keeps every shape on
+// k_cmix (cross-check).
+static bool launch_pw(const GemmArgs& g, hipStream_t s) {
+  if (g.batch != 1 || g.sBn != 1 || g.sCn != 1) return false;
+  if (g.beta != 0.f && g.sIn != 1) return false;
+  if (g.M > 128 || g.K > 128 || g.N < 32) return false;
+  // byte offsets within a block of 16 rows of X or 32 rows of C, pre, Cin
+  // fit in 31 bits
+  const int64_t lim = int64_t(1) << 31;
+  if (g.sBk < 0 || g.sCm < 0 || g.sIm < 0) return false;
+  if ((16 * g.sBk + g.N) * 4 >= lim || (32 * g.sCm + g.N) * 4 >= lim) return false;
+  if (g.beta != 0.f && (32 * g.sIm + g.N) * 4 >= lim) return false;
+  return pw_mk<32, 9>(g, s) || pw_mk<32, 32>(g, s) || pw_mk<128, 32>(g, s) ||
+         pw_mk<128, 6>(g, s);
+}
+
 }  // namespace dnx
 
 using namespace dnx;
@@ -927,7 +1087,9 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool generic = getenv_flag("DNX_GEMM_GENERIC");
   bool done = false;
-  if (!generic && launch_cmix(*g, s)) {
+  if (!generic && !getenv_flag("DNX_GEMM_PW_VALU") && launch_pw(*g, s)) {
+    done = true;
+  } else if (!generic && launch_cmix(*g, s)) {
     done = true;
   } else if (!generic && nt_ok(*g) && g->M >= XBM && g->N >= XBN && g->K >= XBIG_K &&
              !getenv_flag("DNX_GEMM_SMALL")) {

@@ -56,6 +56,41 @@ def test_gemm_bf16_operands_gelu_and_pre():
     assert rel_l2(C.cpu().numpy(), gel.cpu().numpy()) < 2e-6
 
 
+@pytest.mark.parametrize("M,K", [(32, 9), (32, 32), (128, 32), (6, 128), (128, 6), (32, 128)])
+@pytest.mark.parametrize("N", [4096, 1000])
+@pytest.mark.parametrize("xbf", [False, True])
+def test_pointwise_kernels(M, K, N, xbf, monkeypatch):
+    """The FNO's channel-mixing shapes (K <= 32 on the matrix-core kernel,
+    K = 128 on the vector-ALU one): every epilogue (alpha, beta Cin, row
+    bias, pre, GELU), a ragged cell count, a weight read through a
+    transposed view, against FP32 products of the same BF16-rounded inputs
+    and against the vector-ALU kernel."""
+    from diffnano_amd.fno import gemm
+    g = torch.Generator(device="cuda").manual_seed(M * 131 + K * 7 + N + int(xbf))
+    Wt = torch.randn(K, M, device="cuda", generator=g)          # used as W = Wt^T
+    X = torch.randn(K, N, device="cuda", generator=g)
+    if xbf:
+        X = X.to(torch.bfloat16)
+    Cin = torch.randn(M, N, device="cuda", generator=g)
+    bias = torch.randn(M, device="cuda", generator=g)
+    outs = []
+    for valu in (False, True):
+        if valu:
+            monkeypatch.setenv("DNX_GEMM_PW_VALU", "1")
+        C = torch.full((M, N), float("nan"), device="cuda")
+        pre = torch.full_like(C, float("nan"))
+        gemm(M, N, K, Wt, (1, M, 0), X, (N, 1, 0), C, (N, 1, 0), alpha=0.75, beta=-0.5, Cin=Cin,
+             bias=bias, bias_mode=2, act=1, pre=pre)
+        torch.cuda.synchronize()
+        outs.append((C, pre))
+    z = 0.75 * (_bf(Wt).t().contiguous() @ _bf(X.float())) - 0.5 * Cin + bias[:, None]
+    (C, pre), (C2, pre2) = outs
+    assert rel_l2(pre.cpu().numpy(), z.cpu().numpy()) < 2e-6
+    assert rel_l2(C.cpu().numpy(), torch.nn.functional.gelu(z).cpu().numpy()) < 2e-6
+    assert rel_l2(C.cpu().numpy(), C2.cpu().numpy()) < 1e-6
+    assert rel_l2(pre.cpu().numpy(), pre2.cpu().numpy()) < 1e-6
+
+
 @pytest.mark.parametrize("abf,bbf", [(False, False), (True, False), (False, True), (True, True)])
 @pytest.mark.parametrize("M,N,K,batch", [(128, 128, 64, 1), (200, 72, 640, 2), (9, 300, 128, 3)])
 def test_gemm_nt_fast_path_matches_generic_and_fp32(M, N, K, batch, abf, bbf, monkeypatch):
