@@ -357,6 +357,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--tune", type=int, default=None,
+                    help="c1/c2: autotune the tile shapes first (default: on for c1)")
     ap.add_argument("--n-steps", type=int, default=None, help="time steps (default: config's)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--pmc-probe", action="store_true")
@@ -376,6 +378,8 @@ def main():
         sys.exit(2)
     if args.iters is None:
         args.iters = 10 if args.config == "c4" else 2
+    if args.tune is None:
+        args.tune = 1 if args.config == "c1" else 0
     if args.config == "c3":
         _bench_batched(args)
         return
@@ -402,6 +406,10 @@ def main():
     solver = FDTDSolver3D(**solver_kw, device=dev)
     eps0 = torch.tensor(eps_np, dtype=torch.float64, device=dev)
     D, H, W = solver_kw["grid_shape"]
+    tiles = None
+    if args.tune:  # per-grid tile shapes from the autotuner (default shapes are 256^3 sweeps)
+        from diffnano_amd import autotune
+        tiles = {k: list(v) for k, v in autotune.tune((D, H, W), device=dev).items()}
     N = D * H * W
     T = solver_kw["n_steps"]
     K = solver_kw["checkpoint_every"]
@@ -479,7 +487,8 @@ def main():
         "config": {"workload": f"3D Yee FDTD {D}x{H}x{W}, {T} steps, "
                                f"{solver_kw['pml_layers']}-cell CPML, fwd+time-reversal adjoint, "
                                f"checkpoint every {K}",
-                   "grid": [D, H, W], "n_steps": T, "parallelism": "single GPU"},
+                   "grid": [D, H, W], "n_steps": T, "parallelism": "single GPU",
+                   "tiles": tiles or "default"},
         "roofline": {"bound": "hbm", "kernel": f"{dom_k['kernel']} ({TIMED[dom][0]} launches)",
                      "achieved": dom_k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_k["gbs"] / HBM_PEAK_GBS, 4),
