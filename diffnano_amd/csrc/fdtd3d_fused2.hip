@@ -182,14 +182,21 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   const bool xf = xv && x >= 1, xl = x + 1 < W;
 
   T byr[R], ayr[R];
+  // ay of row r for its y differences: 0 where the difference crosses the
+  // grid edge (y + 1 = ny for ayl, y = 0 for ayf), so no per-row edge test
+  // (a lane mask per row, kept live across the march) is needed
+  T ayl[R], ayf[R];
   uint32_t rvm = 0;  // bit r: row r lies inside the grid
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int y = y0 - 2 + r;
     const bool in = y >= 0 && y < a.ny;
     if (in) rvm |= 1u << r;
+    const T ay = in ? a.ay[y] : T(0);
     byr[r] = vreg(in ? a.by[y] : T(0));
-    ayr[r] = vreg(in ? a.ay[y] : T(0));
+    ayr[r] = vreg(ay);
+    ayl[r] = y + 1 < a.ny ? ay : T(0);
+    ayf[r] = y >= 1 ? ay : T(0);
   }
   auto rv = [&](int r) { return ((rvm >> r) & 1u) != 0; };
   // Loads use a clamped row and column so that every one is in bounds and
@@ -337,19 +344,18 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 0; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
-        const bool yl = y + 1 < a.ny;
         const T ex = e0C[0][r], ey = e0C[1][r], ez = e0C[2][r];
         const T ey_x1 = xl ? (&LA(pb, 0, r))[x + 1] : T(0);
         const T ez_x1 = xl ? (&LA(pb, 1, r))[x + 1] : T(0);
-        const T dEz_dy = yl ? e0C[2][r + 1] - ez : T(0);
-        const T dEx_dy = yl ? e0C[0][r + 1] - ex : T(0);
+        const T dEz_dy = e0C[2][r + 1] - ez;
+        const T dEx_dy = e0C[0][r + 1] - ex;
         const T dEy_dz = zl ? e0N[1][r] - ey : T(0);
         const T dEx_dz = zl ? e0N[0][r] - ex : T(0);
         const T dEz_dx = xl ? ez_x1 - ez : T(0);
         const T dEy_dx = xl ? ey_x1 - ey : T(0);
-        h1N[0][r] = byr[r] * h0[0][r] - kh * (ayr[r] * dEz_dy - axv * dEy_dz);
+        h1N[0][r] = byr[r] * h0[0][r] - kh * (ayl[r] * dEz_dy - axv * dEy_dz);
         h1N[1][r] = bxv * h0[1][r] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
-        h1N[2][r] = bxv * h0[2][r] - kh * (axv * dEy_dx - ayr[r] * dEx_dy);
+        h1N[2][r] = bxv * h0[2][r] - kh * (axv * dEy_dx - ayl[r] * dEx_dy);
       }
       if (xv) {
 #pragma unroll
@@ -375,12 +381,11 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 1; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
-        const bool yf = y >= 1;
         const T hx = h1N[0][r], hy = h1N[1][r], hz = h1N[2][r];
         const T hy_x0 = xf ? (&LB(0, r))[x - 1] : T(0);
         const T hz_x0 = xf ? (&LB(1, r))[x - 1] : T(0);
-        const T dHz_dy = yf ? hz - h1N[2][r - 1] : T(0);
-        const T dHx_dy = yf ? hx - h1N[0][r - 1] : T(0);
+        const T dHz_dy = hz - h1N[2][r - 1];
+        const T dHx_dy = hx - h1N[0][r - 1];
         const T dHy_dz = zf ? hy - h1F[1][r - 1] : T(0);
         const T dHx_dz = zf ? hx - h1F[0][r - 1] : T(0);
         const T dHz_dx = xf ? hz - hz_x0 : T(0);
@@ -388,9 +393,9 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T c = ceP[r - 1];
         const T bxy = bxv * byr[r];
         T e[3];
-        e[0] = bxy * e0C[0][r] + c * (ayr[r] * dHz_dy - axv * dHy_dz);
+        e[0] = bxy * e0C[0][r] + c * (ayf[r] * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e0C[1][r] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
-        e[2] = bxy * e0C[2][r] + c * (axv * dHy_dx - ayr[r] * dHx_dy);
+        e[2] = bxy * e0C[2][r] + c * (axv * dHy_dx - ayf[r] * dHx_dy);
         if (szA && sbx.row(r)) add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) e1P[cc][r - 1] = e[cc];
@@ -415,19 +420,18 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 1; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
-        const bool yl = y + 1 < a.ny;
         const T ex = e1Q[0][r - 1], ey = e1Q[1][r - 1], ez = e1Q[2][r - 1];
         const T ey_x1 = xl ? (&LC(qb, 0, r))[x + 1] : T(0);
         const T ez_x1 = xl ? (&LC(qb, 1, r))[x + 1] : T(0);
-        const T dEz_dy = yl ? e1Q[2][r] - ez : T(0);
-        const T dEx_dy = yl ? e1Q[0][r] - ex : T(0);
+        const T dEz_dy = e1Q[2][r] - ez;
+        const T dEx_dy = e1Q[0][r] - ex;
         const T dEy_dz = zlq ? e1P[1][r - 1] - ey : T(0);
         const T dEx_dz = zlq ? e1P[0][r - 1] - ex : T(0);
         const T dEz_dx = xl ? ez_x1 - ez : T(0);
         const T dEy_dx = xl ? ey_x1 - ey : T(0);
-        h2N[0][r - 1] = byr[r] * h1F[0][r - 1] - kh * (ayr[r] * dEz_dy - axv * dEy_dz);
+        h2N[0][r - 1] = byr[r] * h1F[0][r - 1] - kh * (ayl[r] * dEz_dy - axv * dEy_dz);
         h2N[1][r - 1] = bxv * h1F[1][r - 1] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
-        h2N[2][r - 1] = bxv * h1F[2][r - 1] - kh * (axv * dEy_dx - ayr[r] * dEx_dy);
+        h2N[2][r - 1] = bxv * h1F[2][r - 1] - kh * (axv * dEy_dx - ayl[r] * dEx_dy);
       }
       if (xv) {
 #pragma unroll
@@ -455,12 +459,11 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 2; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
-        const bool yf = y >= 1;
         const T hx = h2N[0][r - 1], hy = h2N[1][r - 1], hz = h2N[2][r - 1];
         const T hy_x0 = xf ? (&LD(0, r))[x - 1] : T(0);
         const T hz_x0 = xf ? (&LD(1, r))[x - 1] : T(0);
-        const T dHz_dy = yf ? hz - h2N[2][r - 2] : T(0);
-        const T dHx_dy = yf ? hx - h2N[0][r - 2] : T(0);
+        const T dHz_dy = hz - h2N[2][r - 2];
+        const T dHx_dy = hx - h2N[0][r - 2];
         const T dHy_dz = zfq ? hy - h2P[1][r - 2] : T(0);
         const T dHx_dz = zfq ? hx - h2P[0][r - 2] : T(0);
         const T dHz_dx = xf ? hz - hz_x0 : T(0);
@@ -468,9 +471,9 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T c = ceQ[r - 2];
         const T bxy = bxv * byr[r];
         T e[3];
-        e[0] = bxy * e1Q[0][r - 1] + c * (ayr[r] * dHz_dy - axv * dHy_dz);
+        e[0] = bxy * e1Q[0][r - 1] + c * (ayf[r] * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e1Q[1][r - 1] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
-        e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ayr[r] * dHx_dy);
+        e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ayf[r] * dHx_dy);
         if (szB && sbx.row(r)) add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
         if (own && rv(r) && xv) {
 #pragma unroll
@@ -622,7 +625,11 @@ __device__ __forceinline__ T mon_apply(int mask, const T (&dv)[3], T (&lam)[3],
   return sacc;
 }
 
-template <typename T, int TY>
+// FOLD: the y-edge tests folded into per-row ay factors (fewer scalar
+// registers; the single-grid kernel), or as selects on each difference (the
+// batched kernel, whose per-design arguments leave no room for the factors).
+// The two forms differ at most in the sign of a zero.
+template <typename T, int TY, bool FOLD>
 __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
@@ -658,6 +665,21 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
     if (in) rvm |= 1u << r;
     byr[r] = vreg(in ? a.by[y] : T(0));
     ayr[r] = vreg(in ? a.ay[y] : T(0));
+  }
+  // ay factors of the y differences with the grid-edge tests folded in (0
+  // where the difference crosses the edge), so no per-row edge test (a lane
+  // mask per row, kept live across the march) is needed: af = [y >= 1]
+  // ay(y), al1 = [y + 1 < ny] ay(y + 1), afm1 = [y >= 1] ay(y - 1),
+  // ain = [y <= ny - 2] ay(y)
+  T af[R], al1[R], afm1[R], ain[R];
+#pragma unroll
+  for (int r = 0; r < R && FOLD; ++r) {
+    const int y = y0 - 2 + r;
+    auto ayv = [&](int yy) { return (yy >= 0 && yy < a.ny) ? a.ay[yy] : T(0); };
+    af[r] = vreg(y >= 1 ? ayv(y) : T(0));
+    al1[r] = y + 1 < a.ny ? ayv(y + 1) : T(0);
+    afm1[r] = y >= 1 ? ayv(y - 1) : T(0);
+    ain[r] = vreg(y <= a.ny - 2 ? ayv(y) : T(0));
   }
   auto rv = [&](int r) { return ((rvm >> r) & 1u) != 0; };
   // Accesses as in fwd2_tile: clamped row and column (every load in bounds,
@@ -809,14 +831,20 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 0; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
-        const bool yf = y >= 1, yl = y + 1 < a.ny;
         const T ux = uC[0][r], uy = uC[1][r], uz = uC[2][r];
         const T uy_x1 = xl ? (&LA(pb, 0, r))[x + 1] : T(0);
         const T uz_x1 = xl ? (&LA(pb, 1, r))[x + 1] : T(0);
         const T tz_y = (zf ? uy : T(0)) - (zl ? uN[1][r] : T(0));
         const T tz_x = (zf ? ux : T(0)) - (zl ? uN[0][r] : T(0));
-        const T ty_z = (yf ? ayr[r] * uz : T(0)) - (yl ? ayr[r + 1] * uC[2][r + 1] : T(0));
-        const T ty_x = (yf ? ayr[r] * ux : T(0)) - (yl ? ayr[r + 1] * uC[0][r + 1] : T(0));
+        T ty_z, ty_x;
+        if constexpr (FOLD) {
+          ty_z = af[r] * uz - al1[r] * uC[2][r + 1];
+          ty_x = af[r] * ux - al1[r] * uC[0][r + 1];
+        } else {
+          const bool yf = y >= 1, yl = y + 1 < a.ny;
+          ty_z = (yf ? ayr[r] * uz : T(0)) - (yl ? ayr[r + 1] * uC[2][r + 1] : T(0));
+          ty_x = (yf ? ayr[r] * ux : T(0)) - (yl ? ayr[r + 1] * uC[0][r + 1] : T(0));
+        }
         const T tx_z = (xf ? axv * uz : T(0)) - (xl ? axp * uz_x1 : T(0));
         const T tx_y = (xf ? uy : T(0)) - (xl ? uy_x1 : T(0));
         m1N[0][r] = byr[r] * m0[0][r] + axv * tz_y - ty_z;
@@ -851,13 +879,19 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 1; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
-        const bool yf = y >= 1, yin = y <= a.ny - 2;
         const bool mine = own && r >= 2 && r <= TY + 1 && rv(r) && xv;
         const T mx = m1N[0][r], my = m1N[1][r], mz = m1N[2][r];
         const T my_x0 = xf ? (&LB(0, r))[x - 1] : T(0);
         const T mz_x0 = xf ? (&LB(1, r))[x - 1] : T(0);
-        const T y_mz = (yf ? ayr[r - 1] * m1N[2][r - 1] : T(0)) - (yin ? ayr[r] * mz : T(0));
-        const T y_mx = (yf ? ayr[r - 1] * m1N[0][r - 1] : T(0)) - (yin ? ayr[r] * mx : T(0));
+        T y_mz, y_mx;
+        if constexpr (FOLD) {
+          y_mz = afm1[r] * m1N[2][r - 1] - ain[r] * mz;
+          y_mx = afm1[r] * m1N[0][r - 1] - ain[r] * mx;
+        } else {
+          const bool yf = y >= 1, yin = y <= a.ny - 2;
+          y_mz = (yf ? ayr[r - 1] * m1N[2][r - 1] : T(0)) - (yin ? ayr[r] * mz : T(0));
+          y_mx = (yf ? ayr[r - 1] * m1N[0][r - 1] : T(0)) - (yin ? ayr[r] * mx : T(0));
+        }
         const T z_my = (zf ? m1F[1][r - 1] : T(0)) - (zin ? my : T(0));
         const T z_mx = (zf ? m1F[0][r - 1] : T(0)) - (zin ? mx : T(0));
         const T x_mz = (xf ? axm * mz_x0 : T(0)) - (xin ? axv * mz : T(0));
@@ -912,14 +946,20 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 1; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
-        const bool yf = y >= 1, yl = y + 1 < a.ny;
         const T ux = u1Q[0][r - 1], uy = u1Q[1][r - 1], uz = u1Q[2][r - 1];
         const T uy_x1 = xl ? (&LC(qb, 0, r))[x + 1] : T(0);
         const T uz_x1 = xl ? (&LC(qb, 1, r))[x + 1] : T(0);
         const T tz_y = (zfq ? uy : T(0)) - (zlq ? u1P[1][r - 1] : T(0));
         const T tz_x = (zfq ? ux : T(0)) - (zlq ? u1P[0][r - 1] : T(0));
-        const T ty_z = (yf ? ayr[r] * uz : T(0)) - (yl ? ayr[r + 1] * u1Q[2][r] : T(0));
-        const T ty_x = (yf ? ayr[r] * ux : T(0)) - (yl ? ayr[r + 1] * u1Q[0][r] : T(0));
+        T ty_z, ty_x;
+        if constexpr (FOLD) {
+          ty_z = af[r] * uz - al1[r] * u1Q[2][r];
+          ty_x = af[r] * ux - al1[r] * u1Q[0][r];
+        } else {
+          const bool yf = y >= 1, yl = y + 1 < a.ny;
+          ty_z = (yf ? ayr[r] * uz : T(0)) - (yl ? ayr[r + 1] * u1Q[2][r] : T(0));
+          ty_x = (yf ? ayr[r] * ux : T(0)) - (yl ? ayr[r + 1] * u1Q[0][r] : T(0));
+        }
         const T tx_z = (xf ? axv * uz : T(0)) - (xl ? axp * uz_x1 : T(0));
         const T tx_y = (xf ? uy : T(0)) - (xl ? uy_x1 : T(0));
         m2N[0][r - 1] = byr[r] * m1F[0][r - 1] + axv * tz_y - ty_z;
@@ -950,13 +990,19 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 2; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
-        const bool yf = y >= 1, yin = y <= a.ny - 2;
         const bool mine = own && rv(r) && xv;
         const T mx = m2N[0][r - 1], my = m2N[1][r - 1], mz = m2N[2][r - 1];
         const T my_x0 = xf ? (&LD(0, r))[x - 1] : T(0);
         const T mz_x0 = xf ? (&LD(1, r))[x - 1] : T(0);
-        const T y_mz = (yf ? ayr[r - 1] * m2N[2][r - 2] : T(0)) - (yin ? ayr[r] * mz : T(0));
-        const T y_mx = (yf ? ayr[r - 1] * m2N[0][r - 2] : T(0)) - (yin ? ayr[r] * mx : T(0));
+        T y_mz, y_mx;
+        if constexpr (FOLD) {
+          y_mz = afm1[r] * m2N[2][r - 2] - ain[r] * mz;
+          y_mx = afm1[r] * m2N[0][r - 2] - ain[r] * mx;
+        } else {
+          const bool yf = y >= 1, yin = y <= a.ny - 2;
+          y_mz = (yf ? ayr[r - 1] * m2N[2][r - 2] : T(0)) - (yin ? ayr[r] * mz : T(0));
+          y_mx = (yf ? ayr[r - 1] * m2N[0][r - 2] : T(0)) - (yin ? ayr[r] * mx : T(0));
+        }
         const T z_my = (zfq ? m2P[1][r - 2] : T(0)) - (zinq ? my : T(0));
         const T z_mx = (zfq ? m2P[0][r - 2] : T(0)) - (zinq ? mx : T(0));
         const T x_mz = (xf ? axm * mz_x0 : T(0)) - (xin ? axv * mz : T(0));
@@ -1040,7 +1086,7 @@ __global__ __launch_bounds__(256, MINW) void k3_fused2_adj(Fused2Args<T> a) {
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  adj2_tile<T, TY>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  adj2_tile<T, TY, true>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
 
 template <typename T, int TY, int MINW>
@@ -1158,7 +1204,7 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
     f.n = m;
     batch2_mon(Dd, true, f);
     const int lt = t - Dd.tile_start;
-    adj2_tile<T, TY>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc, sm);
+    adj2_tile<T, TY, false>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc, sm);
     __syncthreads();
   }
 }
