@@ -437,15 +437,11 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
   const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, probe, 2);
   Fused2Args<T> f2{};
   int ty2 = 0;
-  StreamScratch side;
   if (pairs) {
+    // the forward kernel stages no monitor samples: k_monitor_pair_fwd
+    // recomputes the intermediate one after each launch
     const Fused3Tiles t2 = grid_tiles(gd, 2);
-    const int64_t se = side_elems(mons, n_mon, 2);
-    if (se > 0) {
-      int rc = side.alloc(sizeof(T) * se, s);
-      if (rc) return rc;
-    }
-    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, n_mon, static_cast<T*>(side.p), 2);
+    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, 0, nullptr, 2);
     ty2 = t2.ty;
   }
   for (int k = 0; k < n_steps; ++k) {
@@ -468,7 +464,7 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
         timed_end(slot ? "fdtd3d_fwd2_store" : "fdtd3d_fwd2", s, tl);
         if (rc) return rc;
         for (int j = 0; j < n_mon; ++j)
-          if ((rc = launch_monitor_pair<T>(f2.side[j], mons[j], n, s))) return rc;
+          if ((rc = launch_monitor_pair_fwd<T>(f2, mons[j], n, s))) return rc;
       }
       advance(out);
       ++k;  // the pair covered steps k and k+1

@@ -130,6 +130,11 @@ template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s);
 template <typename T>
 int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s);
+// DFT monitor m of the forward pair just launched with f (steps n, n+1):
+// the intermediate sample recomputed from f.a.in, the second read from
+// f.a.out (the single-grid forward kernel stages no monitor samples).
+template <typename T>
+int launch_monitor_pair_fwd(const Fused2Args<T>& f, const dnx_monitor& m, int n, hipStream_t s);
 
 // Batched two-step launches over a design table whose tiles follow kernel 2
 // (forward) or 3 (adjoint); float32 only.  prepare_batch3_pair sets the

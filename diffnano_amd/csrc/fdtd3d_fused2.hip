@@ -156,7 +156,12 @@ __device__ __forceinline__ float vreg(float v) {
   return r;
 }
 
-template <typename T, int TY, bool PF>
+// MON: stage the monitor samples of both sub-steps (the batched kernel).
+// The single-grid kernel leaves them to k_monitor_pair_fwd, which
+// recomputes the intermediate sample after the launch: staging from inside
+// the tile cost 41 us of a 242 us launch at 256^3 (one store per monitor
+// cell and sub-step, and the registers of the box tests).
+template <typename T, int TY, bool PF, bool MON>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
@@ -237,7 +242,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   const T w2 = has_src ? a.src_wave[A2.n + 1] : T(0);
   RowBits sbx, mbx;  // source and monitor box tests
   sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
-  mbx.init(A2.mon[0].box, x, y0 - 2, R, A2.n_mon > 0 && xv);
+  mbx.init(A2.mon[0].box, x, y0 - 2, R, MON && A2.n_mon > 0 && xv);
 
   T e0C[3][R], e0N[3][R];
   T h1F[3][TY + 2];  // H1 at plane p-1, rows 1..TY+2 (index r-1)
@@ -376,7 +381,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       const bool own = p >= z0 && p < z1;
       int dzsA, dzmA;
       const bool szA = RowBox<R>::zin(a.src_box, p, dzsA);
-      const bool mzA = RowBox<R>::zin(A2.mon[0].box, p, dzmA);
+      const bool mzA = MON && RowBox<R>::zin(A2.mon[0].box, p, dzmA);
       (void)dzsA;
 #pragma unroll
       for (int r = 1; r <= TY + 2; ++r) {
@@ -454,7 +459,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       const bool own = q >= z0;
       int dzsB, dzmB;
       const bool szB = RowBox<R>::zin(a.src_box, q, dzsB);
-      const bool mzB = RowBox<R>::zin(A2.mon[0].box, q, dzmB);
+      const bool mzB = MON && RowBox<R>::zin(A2.mon[0].box, q, dzmB);
       (void)dzsB;
 #pragma unroll
       for (int r = 2; r <= TY + 1; ++r) {
@@ -541,7 +546,7 @@ __global__ __launch_bounds__(256, 2) void k3_fused2_fwd(Fused2Args<T> a) {
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  fwd2_tile<T, TY, PF>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  fwd2_tile<T, TY, PF, false>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
 
 template <int TY>
@@ -566,6 +571,123 @@ static int launch_fwd2_ty(const Fused2Args<T>& a, hipStream_t s) {
   return DNX_OK;
 }
 
+// ---- DFT monitor of a forward pair without staging --------------------
+//
+// After a two-step forward launch (state n in a.in, n + 2 in a.out), the
+// sample after step n (E^{n+1}) of each monitor cell is recomputed from
+// a.in with the arithmetic of fwd2_tile's stage A (H1 at the cell and at
+// its three lower neighbours, then E1 with the fused source of step n), and
+// the sample after step n + 1 is read from a.out; both are accumulated in
+// the order of k_monitor_pair.  The recomputed values are the tile's, up to
+// the sign of a zero where a difference is cut at the lower y edge.
+template <typename T>
+struct HVec {
+  T x, y, z;
+};
+
+template <typename T>
+__device__ HVec<T> h1_cell(const Fused3Args<T>& a, int z, int y, int x) {
+  const T* in = a.in;
+  const int64_t N = a.N, P = a.P;
+  const int nx = a.nx;
+  auto at = [&](int c, int zz, int yy, int xx) {
+    return in[int64_t(c) * N + int64_t(zz) * P + int64_t(yy) * nx + xx];
+  };
+  const bool zl = z + 1 < a.nz, xl = x + 1 < nx;
+  const int yc = min(y + 1, a.ny - 1);  // the tile's clamped row
+  const T ex = at(0, z, y, x), ey = at(1, z, y, x), ez = at(2, z, y, x);
+  const T ey_x1 = xl ? at(1, z, y, x + 1) : T(0);
+  const T ez_x1 = xl ? at(2, z, y, x + 1) : T(0);
+  const T dEz_dy = at(2, z, yc, x) - ez;
+  const T dEx_dy = at(0, z, yc, x) - ex;
+  const T dEy_dz = zl ? at(1, z + 1, y, x) - ey : T(0);
+  const T dEx_dz = zl ? at(0, z + 1, y, x) - ex : T(0);
+  const T dEz_dx = xl ? ez_x1 - ez : T(0);
+  const T dEy_dx = xl ? ey_x1 - ey : T(0);
+  const T ay = a.ay[y], ayl = y + 1 < a.ny ? ay : T(0);
+  const T bx = a.bx[x], ax = a.ax[x], kh = a.kh;
+  HVec<T> h;
+  h.x = a.by[y] * at(3, z, y, x) - kh * (ayl * dEz_dy - ax * dEy_dz);
+  h.y = bx * at(4, z, y, x) - kh * (ax * dEx_dz - ay * dEz_dx);
+  h.z = bx * at(5, z, y, x) - kh * (ax * dEy_dx - ayl * dEx_dy);
+  return h;
+}
+
+// MASK: the monitored components (compile-time, so that the loads and
+// arithmetic of the components not monitored are dropped).
+template <typename T, int MASK>
+__global__ void k_monitor_pair_fwd(Fused3Args<T> a, int n, Box mb, int nf,
+                                   const double* __restrict__ ph, double* __restrict__ acc) {
+  constexpr int comp_mask = MASK;
+  const int64_t cells = mb.cells();
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  const int dx = int(i % mb.nx), dy = int((i / mb.nx) % mb.ny);
+  const int dz = int(i / (int64_t(mb.nx) * mb.ny));
+  const int x = mb.x0 + dx, y = mb.y0 + dy, z = mb.z0 + dz;
+  const int64_t c0 = int64_t(z) * a.P + int64_t(y) * a.nx + x;
+  const HVec<T> h = h1_cell(a, z, y, x);
+  const bool zf = z >= 1, xf = x >= 1;
+  const HVec<T> hym = y >= 1 ? h1_cell(a, z, y - 1, x) : h;
+  const HVec<T> hzm = zf ? h1_cell(a, z - 1, y, x) : h;
+  const HVec<T> hxm = xf ? h1_cell(a, z, y, x - 1) : h;
+  const T hy_x0 = xf ? hxm.y : T(0), hz_x0 = xf ? hxm.z : T(0);
+  const T dHz_dy = h.z - hym.z;
+  const T dHx_dy = h.x - hym.x;
+  const T dHy_dz = zf ? h.y - hzm.y : T(0);
+  const T dHx_dz = zf ? h.x - hzm.x : T(0);
+  const T dHz_dx = xf ? h.z - hz_x0 : T(0);
+  const T dHy_dx = xf ? h.y - hy_x0 : T(0);
+  const T c = a.ce[c0];
+  const T ay = a.ay[y], ayf = y >= 1 ? ay : T(0);
+  const T ax = a.ax[x];
+  const T bxy = a.bx[x] * a.by[y];
+  const T* in = a.in;
+  T e[3];
+  e[0] = bxy * in[c0] + c * (ayf * dHz_dy - ax * dHy_dz);
+  e[1] = bxy * in[a.N + c0] + c * (ax * dHx_dz - ay * dHz_dx);
+  e[2] = bxy * in[2 * a.N + c0] + c * (ax * dHy_dx - ayf * dHx_dy);
+  int64_t si;
+  if (a.src_comp >= 0 && in_box<T>(a.src_box, z, y, x, si))
+    add_src_at<T>(a.src_comp, a.src_wave[n], a.src_prof, si, e);
+  int ci = 0;
+  for (int comp = 0; comp < 3; ++comp) {
+    if (!((comp_mask >> comp) & 1)) continue;
+    const double v1 = double(e[comp]);
+    const double v2 = double(a.out[int64_t(comp) * a.N + c0]);
+    constexpr int ncomp = ((MASK >> 0) & 1) + ((MASK >> 1) & 1) + ((MASK >> 2) & 1);
+    for (int f = 0; f < nf; ++f) {
+      double* q = acc + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+      q[0] += v1 * ph[2 * f];
+      q[1] -= v1 * ph[2 * f + 1];
+      q[0] += v2 * ph[2 * nf + 2 * f];
+      q[1] -= v2 * ph[2 * nf + 2 * f + 1];
+    }
+    ++ci;
+  }
+}
+
+template <typename T>
+int launch_monitor_pair_fwd(const Fused2Args<T>& f, const dnx_monitor& m, int n, hipStream_t s) {
+  const Box b = make_box(m.box);
+  const dim3 grid(unsigned((b.cells() + 255) / 256)), block(256);
+  const double* ph = m.phase + int64_t(n) * m.n_freq * 2;
+  switch (m.comp_mask) {
+#define DNX_MPF(K)                                                                           \
+  case K:                                                                                  \
+    hipLaunchKernelGGL((k_monitor_pair_fwd<T, K>), grid, block, 0, s, f.a, n, b, m.n_freq, ph, \
+                       m.acc);                                                             \
+    break;
+    DNX_MPF(1) DNX_MPF(2) DNX_MPF(3) DNX_MPF(4) DNX_MPF(5) DNX_MPF(6) DNX_MPF(7)
+#undef DNX_MPF
+    default:
+      set_error("monitor: component mask %d", m.comp_mask);
+      return DNX_EINVAL;
+  }
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
 template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
   static const int pf = [] {
@@ -579,6 +701,8 @@ int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
 }
 
 template int launch_fused2_fwd<float>(const Fused2Args<float>&, int, hipStream_t);
+template int launch_monitor_pair_fwd<float>(const Fused2Args<float>&, const dnx_monitor&, int,
+                                            hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Adjoint, two steps: lam^{m+1} -> lam^m -> lam^{m-1}.  Same march and row
@@ -1176,7 +1300,8 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
     f.n = n;
     batch2_mon(Dd, !L.store, f);
     const int lt = t - Dd.tile_start;
-    fwd2_tile<T, TY, false>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc, sm);
+    fwd2_tile<T, TY, false, true>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc,
+                                  sm);
     __syncthreads();  // LDS is reused by the next tile
   }
 }
