@@ -547,8 +547,11 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   // Step pairs run the two-step adjoint (DNX_FDTD3D_ADJ2=0 turns it off;
   // DNX_FDTD3D_STEPS=1 turns off both two-step kernels).
   const char* adj2 = getenv("DNX_FDTD3D_ADJ2");
-  const bool pairs =
-      use_pairs<T>(gd, fused, n_src, n_mon, -1, 3) && !(adj2 && strcmp(adj2, "0") == 0);
+  // (the tiles read the monitor adjoint terms through one descriptor whose
+  // out-of-box offsets start at 2^29 bytes)
+  const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, -1, 3) &&
+                     !(adj2 && strcmp(adj2, "0") == 0) &&
+                     side_elems(mons, n_mon, 1) * int64_t(sizeof(T)) < (int64_t(1) << 29);
   Fused2Args<T> f2{};
   int ty2 = 0;
   StreamScratch side;

@@ -124,6 +124,13 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
     for (int d = 0; d < p->n && ok; ++d) {
       const dnx_grid3d& g = ds[d].grid;
       ok = int64_t(g.nz) * g.ny * g.nx * 6 * int64_t(sizeof(T)) < (int64_t(1) << 31);
+      // the tiles read the monitor adjoint terms through one descriptor
+      // whose out-of-box offsets start at 2^29 bytes
+      if (ok && ds[d].mon.n_freq > 0) {
+        const int mk = ds[d].mon.comp_mask;
+        const int nc = (mk & 1) + ((mk >> 1) & 1) + ((mk >> 2) & 1);
+        ok = make_box(ds[d].mon.box).cells() * nc * int64_t(sizeof(T)) < (int64_t(1) << 29);
+      }
     }
     const int tf2 = fused3_tiles(p->dtype, 2).ty, ta2 = fused3_tiles(p->dtype, 3).ty;
     p->fpairs = ok && (tf2 == 2 || tf2 == 4);

@@ -103,6 +103,31 @@ struct RowBits {
   }
 };
 
+// Branch-free access to the box-compact arrays (source profile, monitor
+// adjoint terms) from the tiles: the byte offset of cell (z, y, x) of box b
+// is lane + row + plane, each part replaced by kBoxOut when that coordinate
+// lies outside the box.  kBoxOut is past the end of every such array (the
+// host keeps them below 2^29 bytes) and three of them stay below 2^31, so a
+// load for a cell outside the box reads 0 through the descriptor's range
+// check, with no branch and no per-row lane mask.
+constexpr int kBoxOut = 1 << 29;
+__device__ __forceinline__ int box_lane(const Box& b, int x, bool on) {
+  return (on && x >= b.x0 && x < b.x0 + b.nx) ? (x - b.x0) * 4 : kBoxOut;
+}
+__device__ __forceinline__ int box_row(const Box& b, int y) {
+  return (y >= b.y0 && y < b.y0 + b.ny) ? (y - b.y0) * b.nx * 4 : kBoxOut;
+}
+__device__ __forceinline__ int box_plane(const Box& b, int z) {
+  return (z >= b.z0 && z < b.z0 + b.nz) ? (z - b.z0) * b.ny * b.nx * 4 : kBoxOut;
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(const void* p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, p ? int(bytes) : 0,
+                                           0x00020000);
+}
+__device__ __forceinline__ float box_ld(const __amdgpu_buffer_rsrc_t& rs, int off, int soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, soff, 0));
+}
+
 // e[comp] += w * prof[i] (the fused soft source, one component).
 template <typename T>
 __device__ __forceinline__ void add_src_at(int comp, T w, const T* prof, int64_t i, T (&e)[3]) {
@@ -240,8 +265,10 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   const bool has_src = a.src_comp >= 0;
   const T w1 = has_src ? a.src_wave[A2.n] : T(0);
   const T w2 = has_src ? a.src_wave[A2.n + 1] : T(0);
-  RowBits sbx, mbx;  // source and monitor box tests
-  sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
+  RowBits mbx;  // monitor box test (staging, MON only)
+  // source profile: branch-free loads (0 outside the box; see box_lane)
+  const int s_lane = box_lane(a.src_box, x, has_src && xv);
+  const auto rsp = box_rsrc(has_src ? a.src_prof : nullptr, a.src_box.cells() * 4);
   mbx.init(A2.mon[0].box, x, y0 - 2, R, MON && A2.n_mon > 0 && xv);
 
   T e0C[3][R], e0N[3][R];
@@ -339,11 +366,10 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         for (int r = 1; r <= TY + 2; ++r) ceP[r - 1] = ld(rsrc(a.ce, p, 1), 0, r);
       }
       {
-        int dzs;
-        const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
+        const int sp = s_lane + box_plane(a.src_box, p);
 #pragma unroll
         for (int r = 1; r <= TY + 2; ++r)
-          spP[r - 1] = (sz && sbx.row(r)) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
+          spP[r - 1] = box_ld(rsp, sp + box_row(a.src_box, y0 - 2 + r), 0);
       }
       const int pb = p & 1;
 #pragma unroll
@@ -379,10 +405,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     // ---------------- stage A, part 2: E1(p) (all rows the tile needs)
     if (doA) {
       const bool own = p >= z0 && p < z1;
-      int dzsA, dzmA;
-      const bool szA = RowBox<R>::zin(a.src_box, p, dzsA);
+      int dzmA;
       const bool mzA = MON && RowBox<R>::zin(A2.mon[0].box, p, dzmA);
-      (void)dzsA;
 #pragma unroll
       for (int r = 1; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
@@ -401,7 +425,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         e[0] = bxy * e0C[0][r] + c * (ayf[r] * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e0C[1][r] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
         e[2] = bxy * e0C[2][r] + c * (axv * dHy_dx - ayf[r] * dHx_dy);
-        if (szA && sbx.row(r)) add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);
+        add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);  // spP is 0 outside the box
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) e1P[cc][r - 1] = e[cc];
         if (own && r >= 2 && r <= TY + 1 && rv(r) && xv) {
@@ -457,10 +481,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     // ---------------- stage B, part 2: E2(q), store
     if (doB) {
       const bool own = q >= z0;
-      int dzsB, dzmB;
-      const bool szB = RowBox<R>::zin(a.src_box, q, dzsB);
+      int dzmB;
       const bool mzB = MON && RowBox<R>::zin(A2.mon[0].box, q, dzmB);
-      (void)dzsB;
 #pragma unroll
       for (int r = 2; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
@@ -479,7 +501,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         e[0] = bxy * e1Q[0][r - 1] + c * (ayf[r] * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e1Q[1][r - 1] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
         e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ayf[r] * dHx_dy);
-        if (szB && sbx.row(r)) add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
+        add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
         if (own && rv(r) && xv) {
 #pragma unroll
           for (int cc = 0; cc < 3; ++cc) {
@@ -838,8 +860,19 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
   const bool has_src = a.src_comp >= 0;
   const T w1 = has_src ? a.src_wave[m] : T(0);
   const T w2 = has_src ? a.src_wave[m - 1] : T(0);
-  RowBits sbx;
-  sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
+  // source profile and monitor adjoint terms: branch-free loads (0 outside
+  // their boxes; see box_lane)
+  const int s_lane = box_lane(a.src_box, x, has_src && xv);
+  const auto rsp = box_rsrc(has_src ? a.src_prof : nullptr, a.src_box.cells() * 4);
+  int m_lane[FUSED2_MAX_MON];
+  __amdgpu_buffer_rsrc_t rmd[FUSED2_MAX_MON];
+#pragma unroll
+  for (int j = 0; j < FUSED2_MAX_MON; ++j) {
+    const bool on = j < A2.n_mon;
+    m_lane[j] = box_lane(A2.mon[j].box, x, on && xv);
+    rmd[j] = box_rsrc(on ? A2.side[j] : nullptr, on ? A2.mon[j].box.cells() * A2.mon[j].ncomp * 4
+                                                    : 0);
+  }
   T spQ[TY];  // source profile at plane p-1, owned rows 2..TY+1 (0 outside the box)
 #pragma unroll
   for (int r = 0; r < TY; ++r) spQ[r] = T(0);
@@ -896,7 +929,6 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
     T sP[TY];
     T spP[TY];  // source profile at plane p, owned rows 2..TY+1
     T dpre[FUSED2_MAX_MON][TY + 2][3];  // monitor adjoint source, rows 1..TY+2
-    bool dhit[FUSED2_MAX_MON][TY + 2];
     // E^m, S at plane p and E^{m-1} at plane q (owned rows 2..TY+1) are
     // loaded here and consumed after the barriers.
     T emA[3][TY], sIn[TY], emB[3][TY];
@@ -936,19 +968,28 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         }
       }
       {
-        int dzs;
-        const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
+        const int sp = s_lane + box_plane(a.src_box, p);
 #pragma unroll
         for (int r = 2; r <= TY + 1; ++r)
-          spP[r - 2] = (sz && sbx.row(r)) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
+          spP[r - 2] = box_ld(rsp, sp + box_row(a.src_box, y0 - 2 + r), 0);
       }
-      // adjoint source of sample m-1 for lamE^m(p), rows 1..TY+2
+      // adjoint source of sample m-1 for lamE^m(p), rows 1..TY+2 (0 outside
+      // the monitor box)
 #pragma unroll
       for (int j = 0; j < FUSED2_MAX_MON; ++j) {
+        const FusedMon& mj = A2.mon[j];
+        const int mp = m_lane[j] + box_plane(mj.box, p);
+        const int cb = int(mj.box.cells()) * 4;
 #pragma unroll
         for (int r = 1; r <= TY + 2; ++r) {
-          dhit[j][r - 1] = j < A2.n_mon && rv(r) && xv &&
-                           mon_fetch<T>(A2.mon[j], A2.side[j], p, y0 - 2 + r, x, dpre[j][r - 1]);
+          const int off = mp + box_row(mj.box, y0 - 2 + r);
+          int ci = 0;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const bool mc = j < A2.n_mon && ((mj.mask >> c) & 1);
+            dpre[j][r - 1][c] = mc ? box_ld(rmd[j], off, ci * cb) : T(0);
+            ci += mc ? 1 : 0;
+          }
         }
       }
       const int pb = p & 1;
@@ -1035,21 +1076,18 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
             for (int k = 0; k < 3; ++k) em[k] = emA[k][r - 2];
             s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
-            if (sbx.row(r)) {
-              int dzs;
-              if (RowBox<R>::zin(a.src_box, p, dzs)) {
-                const T w = w1 * spP[r - 2];
+            {
+              const T w = w1 * spP[r - 2];  // 0 outside the source box
 #pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                  if (cc == a.src_comp) s -= w * lC[cc][r - 1];
-              }
+              for (int cc = 0; cc < 3; ++cc)
+                if (cc == a.src_comp) s -= w * lC[cc][r - 1];
             }
             s = sIn[r - 2] + s;
           }
         }
 #pragma unroll
         for (int j = 0; j < FUSED2_MAX_MON; ++j) {
-          if (j < A2.n_mon && dhit[j][r - 1]) {
+          if (j < A2.n_mon) {  // dpre is 0 outside the monitor box
             const T sacc = mon_apply<T>(A2.mon[j].mask, dpre[j][r - 1], l, em);
             if (mine) s += sacc;
           }
@@ -1144,14 +1182,11 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
           for (int k = 0; k < 3; ++k) em[k] = emB[k][r - 2];
           T s = em[0] * t[0] + em[1] * t[1] + em[2] * t[2];
-          if (sbx.row(r)) {
-            int dzs;
-            if (RowBox<R>::zin(a.src_box, q, dzs)) {
-              const T w = w2 * spQ[r - 2];
+          {
+            const T w = w2 * spQ[r - 2];  // 0 outside the source box
 #pragma unroll
-              for (int cc = 0; cc < 3; ++cc)
-                if (cc == a.src_comp) s -= w * l1Q[cc][r - 1];
-            }
+            for (int cc = 0; cc < 3; ++cc)
+              if (cc == a.src_comp) s -= w * l1Q[cc][r - 1];
           }
           s = sQ[r - 2] + s;  // the last sample's monitor term follows the launch
 #pragma unroll
