@@ -404,6 +404,38 @@ def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, monkeyp
         assert torch.equal(b, c), (b - c).abs().max()
 
 
+@pytest.mark.parametrize("box,comps", [((0, 14, 0, 11, 0, 1), "xyz"),      # x = 0 plane
+                                       ((0, 1, 0, 11, 0, 40), "yz"),       # z = 0 plane
+                                       ((5, 9, 0, 1, 3, 37), "z"),         # y = 0 rows
+                                       ((13, 14, 10, 11, 39, 40), "xz"),   # far corner
+                                       ((2, 12, 2, 9, 19, 22), "y")])      # holds the source
+def test_two_step_monitor_at_edges_and_source_bitwise(box, comps, monkeypatch):
+    """The forward pair's monitor sample recomputed after the launch (cells
+    on every grid edge, every component mask, a box holding the source)
+    against the one-step kernels, with the gradient through the monitor."""
+    shape = (14, 11, 40)
+
+    def run():
+        rng = np.random.default_rng(9)
+        eps = torch.tensor(1.5 + 3.0 * rng.random(shape), requires_grad=True)
+        s = _solver3d(grid_shape=shape, dl=25.0, pml_layers=3, n_steps=90, courant=0.45,
+                      checkpoint_every=10, backward="time_reversal")
+        src = {"type": "gaussian_pulse", "plane": "yz", "x": 20, "component": "Ez",
+               "profile": torch.from_numpy(rng.random((shape[0], shape[1], 1)))}
+        r = s.forward(eps, source=src, monitors=[{"box": box, "components": comps,
+                                                   "wavelengths": [1550.0, 1400.0]}])
+        F = r.metadata["monitors"][0]
+        ((F.abs() ** 2).sum() + r.field.sum()).backward()
+        return r.field.detach().clone(), F.detach().clone(), eps.grad.clone()
+
+    two = run()
+    monkeypatch.setenv("DNX_FDTD3D_STEPS", "1")
+    one = run()
+    for a, b in zip(two, one):
+        assert torch.equal(a, b), (a - b).abs().max()
+    assert float(two[1].abs().max()) > 0
+
+
 @pytest.mark.parametrize("ty,zc,occ", [(1, 5, "1"), (1, 64, "2"), (2, 7, "2"), (4, 3, "1")])
 def test_two_step_adjoint_tiles_bitwise(ty, zc, occ, monkeypatch):
     """Every tile height / occupancy variant of the two-step adjoint gives
