@@ -181,11 +181,14 @@ __device__ __forceinline__ float vreg(float v) {
   return r;
 }
 
-// MON: stage the monitor samples of both sub-steps (the batched kernel).
-// The single-grid kernel leaves them to k_monitor_pair_fwd, which
-// recomputes the intermediate sample after the launch: staging from inside
-// the tile cost 41 us of a 242 us launch at 256^3 (one store per monitor
-// cell and sub-step, and the registers of the box tests).
+// MON: stage the monitor samples of both sub-steps from inside the tile
+// (the batched kernel).  The single-grid kernel leaves them to
+// k_monitor_pair_fwd, which recomputes the intermediate sample after the
+// launch: staging from inside the tile cost 41 us of a 242 us launch at
+// 256^3 (one partial-line store per monitor cell and sub-step, whose
+// completion the next loads' waits include), the recompute 24 us.  With
+// the batch's many monitor planes the recompute costs more than the
+// staging, so the batched kernel stages.
 template <typename T, int TY, bool PF, bool MON>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
@@ -638,12 +641,11 @@ __device__ HVec<T> h1_cell(const Fused3Args<T>& a, int z, int y, int x) {
 // MASK: the monitored components (compile-time, so that the loads and
 // arithmetic of the components not monitored are dropped).
 template <typename T, int MASK>
-__global__ void k_monitor_pair_fwd(Fused3Args<T> a, int n, Box mb, int nf,
-                                   const double* __restrict__ ph, double* __restrict__ acc) {
+__device__ void monitor_pair_cell(const Fused3Args<T>& a, int n, const Box& mb, int nf,
+                                  const double* __restrict__ ph, double* __restrict__ acc,
+                                  int64_t i) {
   constexpr int comp_mask = MASK;
   const int64_t cells = mb.cells();
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= cells) return;
   const int dx = int(i % mb.nx), dy = int((i / mb.nx) % mb.ny);
   const int dz = int(i / (int64_t(mb.nx) * mb.ny));
   const int x = mb.x0 + dx, y = mb.y0 + dy, z = mb.z0 + dz;
@@ -687,6 +689,13 @@ __global__ void k_monitor_pair_fwd(Fused3Args<T> a, int n, Box mb, int nf,
     }
     ++ci;
   }
+}
+
+template <typename T, int MASK>
+__global__ void k_monitor_pair_fwd(Fused3Args<T> a, int n, Box mb, int nf,
+                                   const double* __restrict__ ph, double* __restrict__ acc) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < mb.cells()) monitor_pair_cell<T, MASK>(a, n, mb, nf, ph, acc, i);
 }
 
 template <typename T>
