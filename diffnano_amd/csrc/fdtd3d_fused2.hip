@@ -189,7 +189,11 @@ __device__ __forceinline__ float vreg(float v) {
 // completion the next loads' waits include), the recompute 24 us.  With
 // the batch's many monitor planes the recompute costs more than the
 // staging, so the batched kernel stages.
-template <typename T, int TY, bool PF, bool MON>
+// FOLD: the y-edge tests folded into per-row ay factors (the single-grid
+// kernel) or applied as selects where the factors are used (the batched
+// kernel, whose per-design arguments leave no scalar registers for them:
+// folded, it spills).  The forms differ at most in the sign of a zero.
+template <typename T, int TY, bool PF, bool MON, bool FOLD = true>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
@@ -226,12 +230,25 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     const bool in = y >= 0 && y < a.ny;
     if (in) rvm |= 1u << r;
     const T ay = in ? a.ay[y] : T(0);
-    byr[r] = vreg(in ? a.by[y] : T(0));
-    ayr[r] = vreg(ay);
-    ayl[r] = y + 1 < a.ny ? ay : T(0);
-    ayf[r] = y >= 1 ? ay : T(0);
+    // (vector registers for the single-grid kernel; the batched kernel
+    // keeps them scalar, its vector registers are full)
+    byr[r] = FOLD ? vreg(in ? a.by[y] : T(0)) : (in ? a.by[y] : T(0));
+    ayr[r] = FOLD ? vreg(ay) : ay;
+    if constexpr (FOLD) {
+      ayl[r] = y + 1 < a.ny ? ay : T(0);
+      ayf[r] = y >= 1 ? ay : T(0);
+    }
   }
   auto rv = [&](int r) { return ((rvm >> r) & 1u) != 0; };
+  // the y-difference factors of row r (see FOLD)
+  auto fl = [&](int r) -> T {
+    if constexpr (FOLD) return ayl[r];
+    else return (y0 - 2 + r + 1 < a.ny) ? ayr[r] : T(0);
+  };
+  auto ff = [&](int r) -> T {
+    if constexpr (FOLD) return ayf[r];
+    else return (y0 - 2 + r >= 1) ? ayr[r] : T(0);
+  };
   // Loads use a clamped row and column so that every one is in bounds and
   // none needs a branch.  Values of rows or columns outside the grid are
   // never combined into a cell of the grid (every difference across the
@@ -272,6 +289,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   // source profile: branch-free loads (0 outside the box; see box_lane)
   const int s_lane = box_lane(a.src_box, x, has_src && xv);
   const auto rsp = box_rsrc(has_src ? a.src_prof : nullptr, a.src_box.cells() * 4);
+  RowBits sbx;  // the batched kernel's source test (FOLD = false)
+  if constexpr (!FOLD) sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
   mbx.init(A2.mon[0].box, x, y0 - 2, R, MON && A2.n_mon > 0 && xv);
 
   T e0C[3][R], e0N[3][R];
@@ -369,10 +388,18 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         for (int r = 1; r <= TY + 2; ++r) ceP[r - 1] = ld(rsrc(a.ce, p, 1), 0, r);
       }
       {
-        const int sp = s_lane + box_plane(a.src_box, p);
+        if constexpr (FOLD) {
+          const int sp = s_lane + box_plane(a.src_box, p);
 #pragma unroll
-        for (int r = 1; r <= TY + 2; ++r)
-          spP[r - 1] = box_ld(rsp, sp + box_row(a.src_box, y0 - 2 + r), 0);
+          for (int r = 1; r <= TY + 2; ++r)
+            spP[r - 1] = box_ld(rsp, sp + box_row(a.src_box, y0 - 2 + r), 0);
+        } else {
+          int dzs;
+          const bool sz = RowBox<R>::zin(a.src_box, p, dzs);
+#pragma unroll
+          for (int r = 1; r <= TY + 2; ++r)
+            spP[r - 1] = (sz && sbx.row(r)) ? a.src_prof[sbx.index(a.src_box, dzs, r)] : T(0);
+        }
       }
       const int pb = p & 1;
 #pragma unroll
@@ -387,9 +414,9 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T dEx_dz = zl ? e0N[0][r] - ex : T(0);
         const T dEz_dx = xl ? ez_x1 - ez : T(0);
         const T dEy_dx = xl ? ey_x1 - ey : T(0);
-        h1N[0][r] = byr[r] * h0[0][r] - kh * (ayl[r] * dEz_dy - axv * dEy_dz);
+        h1N[0][r] = byr[r] * h0[0][r] - kh * (fl(r) * dEz_dy - axv * dEy_dz);
         h1N[1][r] = bxv * h0[1][r] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
-        h1N[2][r] = bxv * h0[2][r] - kh * (axv * dEy_dx - ayl[r] * dEx_dy);
+        h1N[2][r] = bxv * h0[2][r] - kh * (axv * dEy_dx - fl(r) * dEx_dy);
       }
       if (xv) {
 #pragma unroll
@@ -425,9 +452,9 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T c = ceP[r - 1];
         const T bxy = bxv * byr[r];
         T e[3];
-        e[0] = bxy * e0C[0][r] + c * (ayf[r] * dHz_dy - axv * dHy_dz);
+        e[0] = bxy * e0C[0][r] + c * (ff(r) * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e0C[1][r] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
-        e[2] = bxy * e0C[2][r] + c * (axv * dHy_dx - ayf[r] * dHx_dy);
+        e[2] = bxy * e0C[2][r] + c * (axv * dHy_dx - ff(r) * dHx_dy);
         add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);  // spP is 0 outside the box
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) e1P[cc][r - 1] = e[cc];
@@ -461,9 +488,9 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T dEx_dz = zlq ? e1P[0][r - 1] - ex : T(0);
         const T dEz_dx = xl ? ez_x1 - ez : T(0);
         const T dEy_dx = xl ? ey_x1 - ey : T(0);
-        h2N[0][r - 1] = byr[r] * h1F[0][r - 1] - kh * (ayl[r] * dEz_dy - axv * dEy_dz);
+        h2N[0][r - 1] = byr[r] * h1F[0][r - 1] - kh * (fl(r) * dEz_dy - axv * dEy_dz);
         h2N[1][r - 1] = bxv * h1F[1][r - 1] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
-        h2N[2][r - 1] = bxv * h1F[2][r - 1] - kh * (axv * dEy_dx - ayl[r] * dEx_dy);
+        h2N[2][r - 1] = bxv * h1F[2][r - 1] - kh * (axv * dEy_dx - fl(r) * dEx_dy);
       }
       if (xv) {
 #pragma unroll
@@ -501,9 +528,9 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T c = ceQ[r - 2];
         const T bxy = bxv * byr[r];
         T e[3];
-        e[0] = bxy * e1Q[0][r - 1] + c * (ayf[r] * dHz_dy - axv * dHy_dz);
+        e[0] = bxy * e1Q[0][r - 1] + c * (ff(r) * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e1Q[1][r - 1] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
-        e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ayf[r] * dHx_dy);
+        e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ff(r) * dHx_dy);
         add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
         if (own && rv(r) && xv) {
 #pragma unroll
@@ -1344,8 +1371,8 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
     f.n = n;
     batch2_mon(Dd, !L.store, f);
     const int lt = t - Dd.tile_start;
-    fwd2_tile<T, TY, false, true>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc,
-                                  sm);
+    fwd2_tile<T, TY, false, true, false>(f, (lt % f.a.ty_tiles) * TY,
+                                         (lt / f.a.ty_tiles) * f.a.zc, sm);
     __syncthreads();  // LDS is reused by the next tile
   }
 }
