@@ -220,7 +220,7 @@ def fno_flops_per_sample(m):
 
 def _bench_surrogate(args):
     """Surrogate-prescreened design loop (128x128 TM, 2000 steps) against the
-    same loop judged exhaustively by the batched FDTD and by the plain
+    same loop evaluated exhaustively by the batched FDTD and by the plain
     one-design-at-a-time solver; plus the surrogate's data generation,
     training and MFMA throughput."""
     import torch

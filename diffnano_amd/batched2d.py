@@ -5,7 +5,7 @@ Each design advances exactly as ``FDTDSolver2D`` in TM polarisation
 :309-337 of the reference), with one shared soft Ez source and a running
 DFT of Ez over a monitor box accumulated in float32 inside the E-update
 kernel (``dnx_fdtd2d_tm_batch``).  No gradient: this is the forward used to
-generate surrogate training fields and to judge many candidate designs at
+generate surrogate training fields and to evaluate many candidate designs at
 once; the differentiable single-design path stays ``FDTDSolver2D``.
 """
 

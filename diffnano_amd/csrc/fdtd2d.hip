@@ -310,7 +310,7 @@ static int adj_run2(const dnx_grid2d* gd, T* lam, int m_hi, int m_lo, const T* e
 // ---------------------------------------------------------------------------
 // Batched TM forward over many designs of one shape (blockIdx.z = design),
 // source shared, with a running DFT over a box accumulated in FP32 inside
-// the E kernel.  Used to generate surrogate training data and to judge
+// the E kernel.  Used to generate surrogate training data and to evaluate
 // candidate designs; forward only.
 // ---------------------------------------------------------------------------
 

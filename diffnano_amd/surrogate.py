@@ -11,11 +11,11 @@ trained with Adam on fields generated by the batched FDTD
 ``PrescreenDesigner`` is a (1 + lambda) evolution loop over blob designs
 that follows the reference's two-stage prescreen (``TwoStageOptimizer``,
 proxy_prescreen.py:147-274: the proxy ranks many candidates, the exact
-solver re-judges the top k and the best judged candidate is kept) with the
-FNO as the proxy and the FDTD as the judge, plus the correction rule of
+solver re-evaluates the top k and the best evaluated candidate is kept) with the
+FNO as the proxy and the FDTD as the evaluator, plus the correction rule of
 ``NeuralSurrogate.forward`` (surrogate.py:237-239: fall back to the full
-solver): when the surrogate's relative error on the judged candidates
-exceeds ``max_rel_err``, that step judges every candidate with the FDTD.
+solver): when the surrogate's relative error on the evaluated candidates
+exceeds ``max_rel_err``, that step evaluates every candidate with the FDTD.
 Every accepted design has been solved by the FDTD, so the figure of merit
 the loop reports is always an FDTD value.
 """
@@ -65,14 +65,14 @@ class FieldSurrogate:
         self.cfg = cfg
         H, W = cfg.H, cfg.W
         self.nf = len(cfg.wavelengths)
-        self.judge = BatchedTM2D(**cfg.solver, source=cfg.source,
+        self.fdtd = BatchedTM2D(**cfg.solver, source=cfg.source,
                                  monitor={"box": (0, H, 0, W), "wavelengths": cfg.wavelengths},
                                  device=device)
-        self.dev = self.judge.dev
+        self.dev = self.fdtd.dev
         self.fno = FNO2d(H, W, in_ch=3 + 2 * self.nf, out_ch=2 * self.nf, width=width,
                          modes=modes, layers=layers, hidden=hidden, device=self.dev, seed=seed)
         bg = torch.full((1, H, W), workloads.EPS_SIO2, dtype=torch.float64, device=self.dev)
-        self.incident = self.judge.forward(bg)[0]                  # [nf, H, W] complex64
+        self.incident = self.fdtd.forward(bg)[0]                  # [nf, H, W] complex64
         self.scale = torch.ones(self.nf, device=self.dev)
         yy, xx = torch.meshgrid(torch.arange(H, device=self.dev) / (H - 1),
                                 torch.arange(W, device=self.dev) / (W - 1), indexing="ij")
@@ -201,13 +201,13 @@ class PrescreenDesigner:
     """(1 + n_candidates) evolution of a blob design.
 
     Each step mutates the parent's noise in ``n_candidates`` random patches,
-    maps them to permittivity, ranks them and keeps the best FDTD-judged
+    maps them to permittivity, ranks them and keeps the best FDTD-evaluated
     candidate if it beats the parent.  ``mode``:
 
     * ``"surrogate"``: the FNO ranks all candidates and the batched FDTD
-      judges the top ``top_k`` (full-grid fields).  If the surrogate's FoM
+      evaluates the top ``top_k`` (full-grid fields).  If the surrogate's FoM
       relative error over those exceeds ``max_rel_err``, every candidate of
-      the step is judged by the FDTD (fallback).  With ``anchor`` (default)
+      the step is evaluated by the FDTD (fallback).  With ``anchor`` (default)
       the predicted FoM-line field of a candidate is the parent's FDTD
       field plus the network's predicted change, FNO(candidate) -
       FNO(parent): an additive low/high-fidelity correction in which the
@@ -215,10 +215,10 @@ class PrescreenDesigner:
       With ``finetune_steps`` > 0, every FDTD solution of the loop also goes
       into a replay buffer the surrogate is fine-tuned on after each step
       (it learns the neighbourhood the search is in);
-    * ``"random"``: ``top_k`` candidates picked at random are judged (the
+    * ``"random"``: ``top_k`` candidates picked at random are evaluated (the
       same FDTD budget without the network);
-    * ``"exhaustive"``: every candidate is judged by the batched FDTD;
-    * ``"plain"``: every candidate is judged by the single-design solver, one
+    * ``"exhaustive"``: every candidate is evaluated by the batched FDTD;
+    * ``"plain"``: every candidate is evaluated by the single-design solver, one
       at a time (the plain loop).
 
     All modes draw the same candidate stream for the same parent and seed;
@@ -237,11 +237,11 @@ class PrescreenDesigner:
         self.ft_steps, self.ft_batch, self.ft_lr = int(finetune_steps), int(finetune_batch), \
             float(finetune_lr)
         y0, y1, x0, x1 = cfg.line
-        self.judge = BatchedTM2D(**cfg.solver, source=cfg.source,
+        self.fdtd = BatchedTM2D(**cfg.solver, source=cfg.source,
                                  monitor={"box": (y0, y1, x0, x1),
                                           "wavelengths": [cfg.wavelengths[cfg.fom_index]]},
                                  device=device)
-        self.dev = self.judge.dev
+        self.dev = self.fdtd.dev
         self.full = None
         if mode == "surrogate":
             self.full = BatchedTM2D(**cfg.solver, source=cfg.source,
@@ -256,7 +256,7 @@ class PrescreenDesigner:
         self.rng = torch.Generator(device="cpu").manual_seed(seed)       # candidates
         self.aux_rng = torch.Generator(device="cpu").manual_seed(seed + 1)  # picks, batches
         self.parent = workloads.blob_noise(seed, cfg.size).to(self.dev).unsqueeze(0)
-        fom0, line0 = self._judge(cfg.eps(self.parent))
+        fom0, line0 = self._evaluate(cfg.eps(self.parent))
         self.parent_fom, self.parent_line = float(fom0[0]), line0[:1]
         self.history = [self.parent_fom]
         self.stats = {"steps": 0, "fallbacks": 0, "fdtd_solves": 1, "rel_err": [],
@@ -278,12 +278,12 @@ class PrescreenDesigner:
             line = self.parent_line + (line - base)
         return (line.abs() ** 2).flatten(1).sum(1).double()
 
-    def _judge(self, eps):
+    def _evaluate(self, eps):
         """(FoM, line field [B, 1, n]) of each design from the batched FDTD."""
-        f = self.judge.forward(eps)[:, 0]
+        f = self.fdtd.forward(eps)[:, 0]
         return (f.abs() ** 2).flatten(1).sum(1).double(), f
 
-    def _judge_full(self, eps):
+    def _evaluate_full(self, eps):
         """(FoM, full-grid fields) of each design; the fields feed the replay
         buffer."""
         f = self.full.forward(eps)
@@ -345,27 +345,27 @@ class PrescreenDesigner:
             st["fdtd_solves"] += self.nc
             self._tick("fdtd", t0)
         elif self.mode == "exhaustive":
-            fom, line = self._judge(eps)
+            fom, line = self._evaluate(eps)
             idx = torch.arange(self.nc, device=self.dev)
             st["fdtd_solves"] += self.nc
             self._tick("fdtd", t0)
         elif self.mode == "random":
             idx = torch.randperm(self.nc, generator=self.aux_rng)[:self.top_k].to(self.dev)
-            fom, line = self._judge(eps[idx])
+            fom, line = self._evaluate(eps[idx])
             st["fdtd_solves"] += self.top_k
             self._tick("fdtd", t0)
         else:
             pfom = self._predict_fom(eps)
             idx = torch.argsort(pfom, descending=True)[:self.top_k]
             t0 = self._tick("surrogate", t0)
-            fom, full = self._judge_full(eps[idx])
+            fom, full = self._evaluate_full(eps[idx])
             line = self._line(full)
             st["fdtd_solves"] += self.top_k
             err = ((pfom[idx] - fom).abs() / fom.clamp_min(1e-300)).mean().item()
             st["rel_err"].append(err)
             if err > self.max_rel_err:
                 st["fallbacks"] += 1
-                fom, line = self._judge(eps)
+                fom, line = self._evaluate(eps)
                 idx = torch.arange(self.nc, device=self.dev)
                 st["fdtd_solves"] += self.nc
             t0 = self._tick("fdtd", t0)

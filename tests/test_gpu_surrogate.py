@@ -66,7 +66,7 @@ def test_surrogate_fit_and_prescreen_bookkeeping():
     # anchored estimate of the parent itself: its own FDTD figure of merit
     assert float(d._predict_fom(cfg.eps(d.parent))[0]) == f0
     d.run(2)
-    # max_rel_err 0: every step falls back to judging all candidates
+    # max_rel_err 0: every step falls back to evaluating all candidates
     assert d.stats["fallbacks"] == 2 and d.stats["fdtd_solves"] == 1 + 2 * (4 + 32)
     assert d.parent_fom >= f0 and len(d.history) == 3
     # the parent's line field follows the accepted design
@@ -91,7 +91,7 @@ def test_ablation_network_ranking_beats_random_selection():
     for seed in (101, 202, 303):
         d = PrescreenDesigner(cfg, sur, mode="surrogate", seed=seed, device="cuda")
         cand = cfg.eps(d._candidates())
-        true_fom, _ = d._judge(cand)
+        true_fom, _ = d._evaluate(cand)
         pred_fom = cfg.fom(sur.predict(cand)).double()
         top = torch.argsort(pred_fom, descending=True)[:8]
         lifts.append(float(true_fom[top].mean() / true_fom.mean()))
