@@ -646,14 +646,16 @@ __device__ HVec<T> h1_cell(const Fused3Args<T>& a, int z, int y, int x) {
     return in[int64_t(c) * N + int64_t(zz) * P + int64_t(yy) * nx + xx];
   };
   const bool zl = z + 1 < a.nz, xl = x + 1 < nx;
-  const int yc = min(y + 1, a.ny - 1);  // the tile's clamped row
+  // clamped neighbours (every load in bounds, no branch); a difference
+  // across the grid edge is selected away afterwards
+  const int yc = min(y + 1, a.ny - 1), zc = min(z + 1, a.nz - 1), xc = min(x + 1, nx - 1);
   const T ex = at(0, z, y, x), ey = at(1, z, y, x), ez = at(2, z, y, x);
-  const T ey_x1 = xl ? at(1, z, y, x + 1) : T(0);
-  const T ez_x1 = xl ? at(2, z, y, x + 1) : T(0);
+  const T ey_x1 = at(1, z, y, xc), ez_x1 = at(2, z, y, xc);
+  const T ey_z1 = at(1, zc, y, x), ex_z1 = at(0, zc, y, x);
   const T dEz_dy = at(2, z, yc, x) - ez;
   const T dEx_dy = at(0, z, yc, x) - ex;
-  const T dEy_dz = zl ? at(1, z + 1, y, x) - ey : T(0);
-  const T dEx_dz = zl ? at(0, z + 1, y, x) - ex : T(0);
+  const T dEy_dz = zl ? ey_z1 - ey : T(0);
+  const T dEx_dz = zl ? ex_z1 - ex : T(0);
   const T dEz_dx = xl ? ez_x1 - ez : T(0);
   const T dEy_dx = xl ? ey_x1 - ey : T(0);
   const T ay = a.ay[y], ayl = y + 1 < a.ny ? ay : T(0);
@@ -673,15 +675,19 @@ __device__ void monitor_pair_cell(const Fused3Args<T>& a, int n, const Box& mb, 
                                   int64_t i) {
   constexpr int comp_mask = MASK;
   const int64_t cells = mb.cells();
-  const int dx = int(i % mb.nx), dy = int((i / mb.nx) % mb.ny);
-  const int dz = int(i / (int64_t(mb.nx) * mb.ny));
+  // 32-bit decode: the launcher admits boxes of fewer than 2^31 cells
+  const uint32_t ui = uint32_t(i), bnx = uint32_t(mb.nx), bny = uint32_t(mb.ny);
+  const uint32_t iy = ui / bnx;
+  const int dx = int(ui - iy * bnx), dy = int(iy % bny), dz = int(iy / bny);
   const int x = mb.x0 + dx, y = mb.y0 + dy, z = mb.z0 + dz;
   const int64_t c0 = int64_t(z) * a.P + int64_t(y) * a.nx + x;
   const HVec<T> h = h1_cell(a, z, y, x);
   const bool zf = z >= 1, xf = x >= 1;
-  const HVec<T> hym = y >= 1 ? h1_cell(a, z, y - 1, x) : h;
-  const HVec<T> hzm = zf ? h1_cell(a, z - 1, y, x) : h;
-  const HVec<T> hxm = xf ? h1_cell(a, z, y, x - 1) : h;
+  // lower neighbours at clamped indices: at the grid edge the clamp lands
+  // on the cell itself (= h), with no branch around the loads
+  const HVec<T> hym = h1_cell(a, z, max(y - 1, 0), x);
+  const HVec<T> hzm = h1_cell(a, max(z - 1, 0), y, x);
+  const HVec<T> hxm = h1_cell(a, z, y, max(x - 1, 0));
   const T hy_x0 = xf ? hxm.y : T(0), hz_x0 = xf ? hxm.z : T(0);
   const T dHz_dy = h.z - hym.z;
   const T dHx_dy = h.x - hym.x;
@@ -728,6 +734,10 @@ __global__ void k_monitor_pair_fwd(Fused3Args<T> a, int n, Box mb, int nf,
 template <typename T>
 int launch_monitor_pair_fwd(const Fused2Args<T>& f, const dnx_monitor& m, int n, hipStream_t s) {
   const Box b = make_box(m.box);
+  if (b.cells() >= (int64_t(1) << 31)) {
+    set_error("monitor: %lld cells in one box (at most 2^31 - 1)", (long long)b.cells());
+    return DNX_EINVAL;
+  }
   const dim3 grid(unsigned((b.cells() + 255) / 256)), block(256);
   const double* ph = m.phase + int64_t(n) * m.n_freq * 2;
   switch (m.comp_mask) {
