@@ -1,0 +1,109 @@
+"""CPU: benchmark result schemas, suite loop and cross-validation helpers
+(fdtd_benchmark.py:36-310) against hand values and a toy solver class."""
+
+import json
+
+import pytest
+import torch
+
+from diffnano_amd._result import SimResult
+from diffnano_amd.fdtd_benchmark import (BenchmarkConfig, BenchmarkResult, ExternalCrossValidator,
+                                         FDTDBenchmarkSuite, _estimate_cpu_peak_mb,
+                                         _make_eps_grid)
+
+
+def test_config_defaults_and_eps_grid():
+    c = BenchmarkConfig()
+    assert c.grid_sizes == [(16, 16, 16), (32, 32, 32)] and c.n_time_steps == 100
+    assert c.backward_modes == ["time_reversal", "checkpoint", "autograd"] and c.device == "cpu"
+    e = _make_eps_grid((8, 12, 16), "cpu")
+    assert e.dtype == torch.float64 and e.shape == (8, 12, 16)
+    block = torch.zeros_like(e, dtype=torch.bool)
+    block[2:6, 3:9, 4:12] = True
+    assert (e[block] == 4.0).all()
+    assert e[~block].min() >= 1.5 and e[~block].max() < 2.5
+    assert torch.equal(e, _make_eps_grid((8, 12, 16), "cpu"))     # seeded
+
+
+def test_cpu_peak_estimate_hand_value():
+    eps = torch.zeros(16, 16, 16, requires_grad=True)
+    assert _estimate_cpu_peak_mb(eps, 100, (16, 16, 16)) == pytest.approx(0.786432)
+    assert _estimate_cpu_peak_mb(eps.detach(), 100, (16, 16, 16)) == pytest.approx(0.196608)
+
+
+class ToySolver:
+    """Differentiable stand-in: field = eps^2 * n_steps; records its kwargs."""
+    seen = []
+
+    def __init__(self, **kw):
+        ToySolver.seen.append(kw)
+        self.n = kw["n_steps"]
+
+    def forward(self, eps):
+        return SimResult((eps * eps * self.n)[None], torch.tensor([1550.0]), {})
+
+
+def test_suite_loop_modes_and_cosines():
+    ToySolver.seen = []
+    cfg = BenchmarkConfig(grid_sizes=[(4, 4, 4), (6, 6, 6)], n_time_steps=3,
+                          backward_modes=["time_reversal", "autograd", "checkpoint"])
+    suite = FDTDBenchmarkSuite()
+    res = suite.run(ToySolver, cfg)
+    assert [(r.grid_size, r.backward_mode) for r in res] == [
+        (g, m) for g in cfg.grid_sizes for m in cfg.backward_modes]
+    assert ToySolver.seen[0]["backward"] == "time_reversal"
+    assert ToySolver.seen[1].get("backward") is None and "use_checkpoint" not in ToySolver.seen[1]
+    assert ToySolver.seen[2]["use_checkpoint"] and ToySolver.seen[2]["checkpoint_segments"] == 2
+    for kw in ToySolver.seen:
+        assert kw["pml_layers"] == 0 and kw["courant"] == 0.35 and kw["dl"] == 20.0
+    # modes before "autograd" have no reference yet; later ones compare with it
+    assert res[0].gradient_cosine_vs_autograd is None
+    assert res[1].gradient_cosine_vs_autograd is None
+    assert res[2].gradient_cosine_vs_autograd == 1.0
+    assert all(r.device == "cpu" and r.forward_time_ms >= 0 for r in res)
+    assert res[0].peak_memory_mb == pytest.approx(6 * 64 * 8 * 4 / 1e6, abs=1e-3)
+    lines = suite.summary_table().splitlines()
+    assert len(lines) == 2 + len(res)
+    assert "4x4x4" in lines[2] and lines[2].rstrip().endswith("cpu") and "ref" in lines[2]
+    assert "1.0000" in lines[4]
+
+
+def test_summary_table_empty_and_format():
+    s = FDTDBenchmarkSuite()
+    assert s.summary_table() == "No benchmark results."
+    s._results = [BenchmarkResult((16, 16, 16), "checkpoint", 12.345, 67.891, 1.5, 0.999876,
+                                  "cuda")]
+    row = s.summary_table().splitlines()[2]
+    assert row == ("    16x16x16 |     checkpoint |     12.3 |     67.9 |     1.50 |"
+                   "     0.9999 |   cuda")
+
+
+def test_validate_forward_and_gradient_hand_values():
+    v = ExternalCrossValidator()
+    ext = torch.tensor([3.0, 4.0])                                  # norm 5
+    r = v.validate_forward(torch.tensor([3.0, 4.5]), ext)
+    assert r["relative_error"] == pytest.approx(0.1) and r["max_absolute_error"] == 0.5
+    assert not r["passed"]
+    assert v.validate_forward(torch.tensor([3.0, 4.5]), ext, rtol=0.2)["passed"]
+    g = v.validate_gradient(torch.tensor([0.0, 1.0]), torch.tensor([1.0, 0.0]))
+    assert g["cosine_similarity"] == 0.0 and not g["passed"]
+    g = v.validate_gradient(2 * ext, ext)
+    assert g["cosine_similarity"] == pytest.approx(1.0) and g["relative_error"] == pytest.approx(1.0)
+    assert g["passed"]
+
+
+def test_test_case_and_json_round_trip(tmp_path):
+    v = ExternalCrossValidator()
+    c = v.generate_test_case((6, 8, 10))
+    assert c["source_pos"] == (3, 4, 5) and c["source_config"]["pos"] == [3, 4, 5]
+    assert c["n_steps"] == 50 and c["courant"] == 0.35 and c["seed"] == 42
+    assert c["eps"].shape == (6, 8, 10) and c["eps"].min() >= 1.5 and c["eps"].max() < 2.5
+    assert torch.equal(c["eps"], v.generate_test_case((6, 8, 10))["eps"])
+    p = tmp_path / "ext.json"
+    p.write_text(json.dumps({"field": {"shape": [2, 3], "data": [1, 2, 3, 4, 5, 6]},
+                             "gradient": {"shape": [3], "data": [0.5, 0.25, 0.125]},
+                             "solver": "external", "n_steps": 50}))
+    out = ExternalCrossValidator.load_external_results(str(p))
+    assert out["field"].dtype == torch.float64 and out["field"].shape == (2, 3)
+    assert out["field"][1, 2] == 6.0 and out["gradient"][2] == 0.125
+    assert out["solver"] == "external" and out["n_steps"] == 50
