@@ -107,3 +107,76 @@ def test_test_case_and_json_round_trip(tmp_path):
     assert out["field"].dtype == torch.float64 and out["field"].shape == (2, 3)
     assert out["field"][1, 2] == 6.0 and out["gradient"][2] == 0.125
     assert out["solver"] == "external" and out["n_steps"] == 50
+
+
+# ---------------------------------------------------------------------------
+# FDTDGPUBenchmark (fdtd_benchmark_gpu.py:59-398), with run_single replaced by
+# a stub so that the matrix logic runs without a GPU
+from diffnano_amd import fdtd_benchmark_gpu as G  # noqa: E402
+
+
+def test_gpu_schemas_and_memory_model():
+    assert [s.value for s in G.GPUMemoryStrategy] == ["time_reversal", "checkpoint",
+                                                      "full_autodiff"]
+    c = G.FDTDBenchmarkConfig()
+    assert c.grid_sizes == [(32, 32, 32), (64, 64, 64)] and c.n_time_steps == 100
+    assert c.device == "cpu" and len(c.memory_strategies) == 3
+    r = G.StabilityReport()
+    assert r.forward_errors == [] and r.memory_used_mb == {} and r.is_valid is False
+    S = G.GPUMemoryStrategy
+    cells = 8 * 8 * 8
+    assert G._estimate_peak_mb((8, 8, 8), 10, S.FULL_AUTODIFF, "cpu") == 10 * 6 * 10 * cells * 8 / 1e6
+    assert G._estimate_peak_mb((8, 8, 8), 10, S.TIME_REVERSAL, "cpu") == 3 * 10 * cells * 8 / 1e6
+    assert G._estimate_peak_mb((8, 8, 8), 10, S.CHECKPOINT, "cpu") == 4 * 6 * cells * 8 / 1e6
+    assert G._estimate_peak_mb((8, 8, 8), 10, S.CHECKPOINT, "cuda", 2.5e6) == 2.5
+
+
+def _stub(grads):
+    calls = []
+
+    def run_single(self, grid_size, strategy, device):
+        calls.append((grid_size, strategy, device))
+        rec = {"grid_size": grid_size, "strategy": strategy.value, "device": device,
+               "forward_time_ms": 1.0, "backward_time_ms": 2.0,
+               "peak_memory_mb": float(len(calls)), "gradient": grads[strategy]}
+        self._results.append(rec)
+        return rec
+    return run_single, calls
+
+
+def test_run_all_orders_full_autodiff_first_and_sets_cosines(monkeypatch):
+    S = G.GPUMemoryStrategy
+    grads = {S.FULL_AUTODIFF: torch.tensor([1.0, 0.0]), S.TIME_REVERSAL: torch.tensor([1.0, 1.0]),
+             S.CHECKPOINT: torch.tensor([2.0, 0.0])}
+    run_single, calls = _stub(grads)
+    monkeypatch.setattr(G.FDTDGPUBenchmark, "run_single", run_single)
+    cfg = G.FDTDBenchmarkConfig(grid_sizes=[(4, 4, 4), (6, 6, 6)], n_time_steps=5)
+    res = G.FDTDGPUBenchmark(cfg).run_all()
+    assert [c[1] for c in calls] == [S.FULL_AUTODIFF, S.TIME_REVERSAL, S.CHECKPOINT] * 2
+    assert all("gradient" not in r for r in res)
+    assert res[0]["gradient_cosine"] is None
+    assert res[1]["gradient_cosine"] == round(2 ** -0.5, 6) and res[2]["gradient_cosine"] == 1.0
+    # without a full-autodiff run there is no reference
+    cfg2 = G.FDTDBenchmarkConfig(grid_sizes=[(4, 4, 4)],
+                                 memory_strategies=[S.CHECKPOINT, S.TIME_REVERSAL])
+    assert all(r["gradient_cosine"] is None for r in G.FDTDGPUBenchmark(cfg2).run_all())
+
+
+def test_memory_scaling_and_stability_report(monkeypatch):
+    S = G.GPUMemoryStrategy
+    grads = {S.FULL_AUTODIFF: torch.tensor([1.0, 0.0]), S.TIME_REVERSAL: torch.tensor([3.0, 0.0]),
+             S.CHECKPOINT: torch.tensor([1.0, 0.1])}
+    run_single, calls = _stub(grads)
+    monkeypatch.setattr(G.FDTDGPUBenchmark, "run_single", run_single)
+    cfg = G.FDTDBenchmarkConfig(grid_sizes=[(4, 4, 4), (8, 8, 8)], n_time_steps=50)
+    b = G.FDTDGPUBenchmark(cfg)
+    sc = b.benchmark_memory_scaling()
+    assert [(s["grid_size"], s["strategy"]) for s in sc] == [
+        (g, s.value) for g in cfg.grid_sizes for s in cfg.memory_strategies]
+    calls.clear()
+    rep = b.cpu_fallback_benchmark()
+    assert {c[0] for c in calls} == {(4, 4, 4)} and {c[2] for c in calls} == {"cpu"}
+    assert rep.is_valid and rep.forward_errors == [0.0] and len(rep.gradient_cosines) == 2
+    assert set(rep.memory_used_mb) == {"full_autodiff", "time_reversal", "checkpoint"}
+    grads[S.CHECKPOINT] = torch.tensor([0.0, 1.0])                 # orthogonal: invalid
+    assert not b.cpu_fallback_benchmark().is_valid

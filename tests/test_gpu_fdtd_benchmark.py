@@ -50,3 +50,17 @@ def test_cross_validation_against_oracle(dtype, rtol):
     r = v.validate_forward(field, ext, rtol=rtol)
     assert r["passed"], r
     assert ext.abs().max() > 0
+
+
+def test_gpu_benchmark_matrix_and_stability_report():
+    from diffnano_amd.fdtd_benchmark_gpu import FDTDBenchmarkConfig, FDTDGPUBenchmark
+    b = FDTDGPUBenchmark(FDTDBenchmarkConfig(grid_sizes=[(12, 12, 12)], n_time_steps=30,
+                                             device="cuda"))
+    assert b.detect_gpu()
+    res = b.run_all()
+    assert [r["strategy"] for r in res] == ["full_autodiff", "time_reversal", "checkpoint"]
+    assert res[0]["gradient_cosine"] is None
+    assert all(r["gradient_cosine"] == 1.0 for r in res[1:])
+    assert all(r["device"] == "cuda" and r["peak_memory_mb"] > 0 for r in res)
+    rep = b.cpu_fallback_benchmark()                  # results on the host, kernels on the GPU
+    assert rep.is_valid and rep.gradient_cosines == [1.0, 1.0]
