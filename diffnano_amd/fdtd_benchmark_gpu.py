@@ -1,6 +1,6 @@
 """GPU benchmark matrix of ``FDTDSolver3D`` over gradient memory strategies,
 drop-in for ``FDTDGPUBenchmark`` and its schemas in DiffNano's
-``diffnano/solvers/fdtd_benchmark_gpu.py`` (lines 59-398).
+``diffnano/solvers/fdtd_benchmark_gpu.py`` (lines 59-398 and 644-870).
 
 * ``GPUMemoryStrategy`` (:59-78) and the keywords each one sets: time
   reversal -> ``backward="time_reversal"``, checkpoint ->
@@ -17,6 +17,8 @@ drop-in for ``FDTDGPUBenchmark`` and its schemas in DiffNano's
 * ``benchmark_memory_scaling`` (:340-356), ``cpu_fallback_benchmark``
   (:358-398): all strategies on the first grid with at most 20 steps and
   results on the host; valid when every cosine is at least 0.99.
+* ``FDTDXCrossValidator`` (:644-870): an injected or self-generated
+  reference field and gradient, compared per seed into a ``StabilityReport``.
 
 Here every strategy runs the same hand-written adjoint (the strategies only
 choose its checkpoint interval), so the cosines are 1, and "cpu" places
@@ -36,7 +38,8 @@ import torch
 
 from .fdtd_benchmark import _cosine_sim, _make_eps_grid
 
-__all__ = ["GPUMemoryStrategy", "FDTDBenchmarkConfig", "StabilityReport", "FDTDGPUBenchmark"]
+__all__ = ["GPUMemoryStrategy", "FDTDBenchmarkConfig", "StabilityReport", "FDTDGPUBenchmark",
+           "FDTDXCrossValidator"]
 
 
 class GPUMemoryStrategy(enum.Enum):
@@ -185,3 +188,67 @@ class FDTDGPUBenchmark:
         return StabilityReport(forward_errors=[0.0], gradient_cosines=cosines,
                                memory_used_mb=memory,
                                is_valid=bool(cosines) and all(c >= 0.99 for c in cosines))
+
+
+class FDTDXCrossValidator:
+    """Cross-validation against an external solver's field and gradient
+    (fdtd_benchmark_gpu.py:644-870).  ``reference_fn(grid_size, n_steps)``
+    returns ``{"field": (3, D, H, W), "gradient": (D, H, W)}``; without it
+    the reference is this solver itself (a self-consistency run of the
+    pipeline).  The ε grid is the seed-42 one for every seed, as in the
+    reference."""
+
+    def __init__(self, reference_fn=None) -> None:
+        self.reference_fn = reference_fn
+
+    @staticmethod
+    def _solve(grid_size, n_steps):
+        from .fdtd3d import FDTDSolver3D
+        solver = FDTDSolver3D(grid_shape=grid_size, dl=20.0, wavelength_nm=1550.0, pml_layers=0,
+                              n_steps=n_steps, device="cpu", courant=0.35)
+        eps = _make_eps_grid(grid_size, "cpu").detach().requires_grad_(True)
+        result = solver.forward(eps)
+        result.field.sum().backward()
+        grad = eps.grad.detach().clone() if eps.grad is not None else torch.zeros_like(eps)
+        return result.field.detach().clone(), grad
+
+    def generate_synthetic_reference(self, grid_size: tuple[int, int, int],
+                                     n_steps: int = 50) -> dict[str, torch.Tensor]:
+        field, grad = self._solve(grid_size, n_steps)
+        return {"field": field, "gradient": grad}
+
+    def validate_forward(self, result: torch.Tensor, reference: torch.Tensor,
+                         rtol: float = 1e-3) -> dict[str, Any]:
+        diff = result.detach().to(torch.float64) - reference.detach().to(torch.float64)
+        rel = diff.norm().item() / max(reference.detach().to(torch.float64).norm().item(), 1e-30)
+        return {"relative_error": rel, "max_absolute_error": diff.abs().max().item(),
+                "passed": rel < rtol}
+
+    def validate_gradient(self, grad: torch.Tensor, reference_grad: torch.Tensor,
+                          cosine_threshold: float = 0.99) -> dict[str, Any]:
+        g = grad.detach().to(torch.float64)
+        r = reference_grad.detach().to(torch.float64)
+        cos = _cosine_sim(g, r)
+        rel = (g - r).norm().item() / max(r.norm().item(), 1e-30)
+        return {"cosine_similarity": cos, "relative_error": rel,
+                "passed": cos >= cosine_threshold}
+
+    def run_cross_validation(self, n_seeds: int = 3, grid_size: tuple[int, int, int] = (16, 16, 16),
+                             n_steps: int = 30) -> StabilityReport:
+        """Per seed: reference, this solver's field and gradient of sum(field),
+        both compared; valid when every forward error is below 1e-3 and
+        every cosine at least 0.99."""
+        fwd, cos = [], []
+        for seed in range(n_seeds):
+            torch.manual_seed(seed * 100 + 42)
+            ref = (self.reference_fn(grid_size, n_steps) if self.reference_fn is not None
+                   else self.generate_synthetic_reference(grid_size, n_steps))
+            torch.manual_seed(seed * 100 + 42)
+            field, grad = self._solve(grid_size, n_steps)
+            fwd.append(self.validate_forward(field, ref["field"])["relative_error"])
+            cos.append(self.validate_gradient(grad, ref["gradient"])["cosine_similarity"])
+        memory = {"full_autodiff": _estimate_peak_mb(grid_size, n_steps,
+                                                     GPUMemoryStrategy.FULL_AUTODIFF, "cpu")}
+        return StabilityReport(forward_errors=fwd, gradient_cosines=cos, memory_used_mb=memory,
+                               is_valid=bool(fwd) and all(e < 1e-3 for e in fwd)
+                               and all(c >= 0.99 for c in cos))

@@ -180,3 +180,31 @@ def test_memory_scaling_and_stability_report(monkeypatch):
     assert set(rep.memory_used_mb) == {"full_autodiff", "time_reversal", "checkpoint"}
     grads[S.CHECKPOINT] = torch.tensor([0.0, 1.0])                 # orthogonal: invalid
     assert not b.cpu_fallback_benchmark().is_valid
+
+
+class _ToySolver3D:
+    """Stand-in for FDTDSolver3D: field = [eps, 2 eps, eps^2]."""
+
+    def __init__(self, **kw):
+        assert kw["pml_layers"] == 0 and kw["device"] == "cpu"
+
+    def forward(self, eps):
+        return SimResult(torch.stack([eps, 2 * eps, eps * eps]), torch.tensor([1550.0]), {})
+
+
+def test_fdtdx_cross_validation_logic(monkeypatch):
+    import diffnano_amd.fdtd3d as F3
+    monkeypatch.setattr(F3, "FDTDSolver3D", _ToySolver3D)
+    v = G.FDTDXCrossValidator()
+    ref = v.generate_synthetic_reference((4, 4, 4), 10)
+    assert ref["field"].shape == (3, 4, 4, 4)
+    assert torch.allclose(ref["gradient"], 3 + 2 * _make_eps_grid((4, 4, 4), "cpu"))
+    rep = v.run_cross_validation(n_seeds=2, grid_size=(4, 4, 4), n_steps=10)
+    assert rep.forward_errors == [0.0, 0.0] and rep.gradient_cosines == [pytest.approx(1.0)] * 2
+    assert rep.is_valid
+    assert rep.memory_used_mb == {"full_autodiff": 10 * 6 * 10 * 64 * 8 / 1e6}
+    # an external reference 1 % off in the field fails the 1e-3 forward bar
+    off = G.FDTDXCrossValidator(lambda g, n: {"field": 1.01 * ref["field"],
+                                              "gradient": ref["gradient"]})
+    rep = off.run_cross_validation(n_seeds=1, grid_size=(4, 4, 4), n_steps=10)
+    assert rep.forward_errors[0] == pytest.approx(1 / 101) and not rep.is_valid

@@ -64,3 +64,11 @@ def test_gpu_benchmark_matrix_and_stability_report():
     assert all(r["device"] == "cuda" and r["peak_memory_mb"] > 0 for r in res)
     rep = b.cpu_fallback_benchmark()                  # results on the host, kernels on the GPU
     assert rep.is_valid and rep.gradient_cosines == [1.0, 1.0]
+
+
+def test_fdtdx_self_consistency_run():
+    from diffnano_amd.fdtd_benchmark_gpu import FDTDXCrossValidator
+    rep = FDTDXCrossValidator().run_cross_validation(n_seeds=2, grid_size=(12, 12, 12),
+                                                     n_steps=20)
+    assert rep.forward_errors == [0.0, 0.0]            # deterministic kernels
+    assert all(abs(c - 1.0) < 1e-12 for c in rep.gradient_cosines) and rep.is_valid
