@@ -9,6 +9,8 @@ from __future__ import annotations
 
 import math
 
+import warnings
+
 import numpy as np
 import torch
 
@@ -175,3 +177,31 @@ def phase_table(n_steps: int, dt: float, omegas) -> np.ndarray:
     n = np.arange(1, n_steps + 1, dtype=np.float64)[:, None]
     ph = n * dt * np.asarray(omegas, dtype=np.float64)[None, :]
     return np.stack([np.cos(ph), np.sin(ph)], axis=-1)
+
+
+EPS_CHECKS = ("warn", "raise", "off")
+
+
+def check_eps_finite(eps: torch.Tensor, mode: str) -> bool:
+    """Host-side guard on the permittivity before any kernel runs.
+
+    In the reference a NaN or Inf in eps flows through the step and the
+    ``eps.sum() * 0`` term into every output (fdtd3d.py:162-163), and the
+    solve runs to the end.  ``mode``: "warn" keeps that result and warns,
+    "raise" raises ValueError before launching anything, "off" skips the
+    check (it costs one device-to-host read when eps is on the GPU).
+    Returns whether eps is finite (True when unchecked).
+    """
+    if mode not in EPS_CHECKS:
+        raise ValueError(f"check_eps must be one of {EPS_CHECKS}, got {mode!r}")
+    if mode == "off":
+        return True
+    finite = bool(torch.isfinite(eps.detach()).all())
+    if not finite:
+        bad = int((~torch.isfinite(eps.detach())).sum())
+        msg = (f"permittivity has {bad} non-finite value(s) (NaN/Inf); every field and "
+               "gradient of this solve will be non-finite")
+        if mode == "raise":
+            raise ValueError(msg)
+        warnings.warn(msg, RuntimeWarning, stacklevel=3)
+    return finite

@@ -212,13 +212,34 @@ class Engine:
         L.check(fn(self.grid(), L.ptr(lam), L.ptr(e_T), mons[0], mons[1], int(n_total), L.ptr(S),
                    self.stream()))
 
-    def adjoint_run(self, lam, m_hi, m_lo, e_lo, estore, srcs, mons, S, work):
-        """Reverse sweep; returns (final, spare) adjoint buffers."""
+    def adjoint_workspace(self, mons):
+        """Device scratch the 3D adjoint sweep needs for these monitors (a
+        torch allocation, reused by every sweep of one backward), or None."""
+        if self.dim != 3:
+            return None
+        n = int(L.lib().dnx_fdtd3d_adjoint_workspace_bytes(self.grid(), mons[0], mons[1]))
+        if n < 0:
+            raise RuntimeError("libdnx_fdtd: adjoint workspace query failed")
+        return torch.empty(n, dtype=torch.uint8, device=self.dev) if n else None
+
+    def adjoint_run(self, lam, m_hi, m_lo, e_lo, estore, srcs, mons, S, work, ws=None):
+        """Reverse sweep; returns (final, spare) adjoint buffers.  ``ws``:
+        the 3D workspace from ``adjoint_workspace`` (allocated here when
+        not given)."""
         fin = C.c_int32(0)
-        fn = L.lib().dnx_fdtd3d_adjoint_run if self.dim == 3 else L.lib().dnx_fdtd2d_adjoint_run
-        L.check(fn(self.grid(), L.ptr(lam), L.ptr(work), int(m_hi), int(m_lo), L.ptr(e_lo),
-                   L.ptr(estore), srcs[0], srcs[1], mons[0], mons[1], L.ptr(S), self.stream(),
-                   C.byref(fin)))
+        if self.dim == 3:
+            if ws is None:
+                ws = self.adjoint_workspace(mons)
+            nbytes = 0 if ws is None else ws.numel()
+            L.check(L.lib().dnx_fdtd3d_adjoint_run(
+                self.grid(), L.ptr(lam), L.ptr(work), int(m_hi), int(m_lo), L.ptr(e_lo),
+                L.ptr(estore), srcs[0], srcs[1], mons[0], mons[1], L.ptr(S), L.ptr(ws), nbytes,
+                self.stream(), C.byref(fin)))
+        else:
+            L.check(L.lib().dnx_fdtd2d_adjoint_run(
+                self.grid(), L.ptr(lam), L.ptr(work), int(m_hi), int(m_lo), L.ptr(e_lo),
+                L.ptr(estore), srcs[0], srcs[1], mons[0], mons[1], L.ptr(S), self.stream(),
+                C.byref(fin)))
         return (work, lam) if fin.value else (lam, work)
 
     def finalize_grad(self, S):
@@ -289,6 +310,9 @@ class _TimeReversalFDTD(torch.autograd.Function):
     def forward(ctx, eps, plan):
         eng = plan.engine
         ctx.eps_device = eps.device
+        # saved so that an in-place change of eps before backward raises (the
+        # engine's coefficients and -S/eps are taken from this eps)
+        ctx.save_for_backward(eps)
         eng.set_eps(eps)
         T = plan.n_steps
         K = plan.ckpt_every
@@ -318,6 +342,7 @@ class _TimeReversalFDTD(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_field, *g_mons):
+        ctx.saved_tensors  # noqa: B018 -- raises if eps was modified in place
         plan = ctx.plan
         eng = plan.engine
         T, K = plan.n_steps, plan.ckpt_every
@@ -339,6 +364,7 @@ class _TimeReversalFDTD(torch.autograd.Function):
         work = torch.empty(ns * N, dtype=dt, device=dev)
         work2 = torch.empty(ns * N, dtype=dt, device=dev)
         lam_spare = torch.empty(ns * N, dtype=dt, device=dev)
+        ws = eng.adjoint_workspace(mons[:2])
         for k in range(nseg - 1, -1, -1):
             a, b = k * K, min(T, (k + 1) * K)
             slot = ctx.ckpt[k * ns * N:(k + 1) * ns * N]
@@ -349,7 +375,7 @@ class _TimeReversalFDTD(torch.autograd.Function):
                 _end("replay")
             _begin("adjoint")
             lam, lam_spare = eng.adjoint_run(lam, b, a, slot[:ne * N], estore, srcs[:2], mons[:2],
-                                             S, lam_spare)
+                                             S, lam_spare, ws)
             _end("adjoint")
         grad = eng.finalize_grad(S)
         return grad.to(ctx.eps_device), None

@@ -7,6 +7,7 @@ RuntimeError naming the missing file when it has not been built
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import threading
@@ -14,7 +15,7 @@ import threading
 import torch
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdnx_fdtd.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 DNX_F32 = 0
 DNX_F64 = 1
@@ -77,6 +78,8 @@ _L = C.c_int64
 _SIGS = {
     "dnx_abi_version": ([], _I),
     "dnx_last_error": ([], C.c_char_p),
+    "dnx_set_option": ([_I, _I], _I),
+    "dnx_get_option": ([_I, C.POINTER(_I)], _I),
     "dnx_timing_enable": ([_I], _I),
     "dnx_timing_read": ([C.c_char_p, C.POINTER(C.c_double), C.POINTER(_L)], _I),
     "dnx_prepare_coef": ([_P, _P, _L, C.c_double, _I, _P], _I),
@@ -87,8 +90,9 @@ _SIGS = {
                              C.POINTER(Monitor), _I, _P, _I, _P, _L, _P, _P, C.POINTER(_I)], _I),
     "dnx_fdtd3d_adjoint_init": ([C.POINTER(Grid3D), _P, _P, C.POINTER(Monitor), _I, _I, _P, _P],
                                 _I),
+    "dnx_fdtd3d_adjoint_workspace_bytes": ([C.POINTER(Grid3D), C.POINTER(Monitor), _I], _L),
     "dnx_fdtd3d_adjoint_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, _P, _P, C.POINTER(Source), _I,
-                                C.POINTER(Monitor), _I, _P, _P, C.POINTER(_I)], _I),
+                                C.POINTER(Monitor), _I, _P, _P, _L, _P, C.POINTER(_I)], _I),
     "dnx_fdtd3d_set_tiles": ([_I, _I, _I, _I], _I),
     "dnx_fdtd3d_get_tiles": ([_I, _I, C.POINTER(_I), C.POINTER(_I)], _I),
     "dnx_fdtd2d_tm_batch": ([C.POINTER(Grid2D), _I, _P, _I, C.POINTER(Source), C.POINTER(Box),
@@ -144,6 +148,36 @@ def lib():
             if _lib is None:
                 _lib = load()
     return _lib
+
+
+# Kernel-selection options (DNX_OPT_* in include/dnx_fdtd.h).  They pick
+# kernel variants, never results.
+OPTIONS = {"simple": 0, "one_step": 1, "adj2": 2, "fwd2_pf": 3, "adj2_occ": 4,
+           "gemm_generic": 5, "gemm_pw_valu": 6, "gemm_small": 7}
+
+
+def get_option(name):
+    v = C.c_int32()
+    check(lib().dnx_get_option(OPTIONS[name], C.byref(v)))
+    return v.value
+
+
+def set_option(name, value):
+    check(lib().dnx_set_option(OPTIONS[name], int(value)))
+
+
+@contextlib.contextmanager
+def options(**values):
+    """Set kernel-selection options for the duration of a block, then
+    restore the previous values."""
+    saved = {k: get_option(k) for k in values}
+    try:
+        for k, v in values.items():
+            set_option(k, v)
+        yield
+    finally:
+        for k, v in saved.items():
+            set_option(k, v)
 
 
 def check(rc):

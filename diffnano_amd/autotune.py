@@ -16,7 +16,6 @@ gives bitwise-identical results.
 
 from __future__ import annotations
 
-import contextlib
 import ctypes as C
 import json
 import os
@@ -34,20 +33,6 @@ TY2_BATCH = (2, 4)                       # batched two-step kernels
 NAMES = {0: "forward", 1: "adjoint", 2: "forward2", 3: "adjoint2"}
 OFF = (0, 1)  # tile of a two-step kernel that is switched off
 _lock = threading.Lock()
-
-
-@contextlib.contextmanager
-def _env(**kv):
-    old = {k: os.environ.get(k) for k in kv}
-    os.environ.update({k: str(v) for k, v in kv.items()})
-    try:
-        yield
-    finally:
-        for k, v in old.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
 
 
 def _kernels(dtype):
@@ -143,8 +128,7 @@ def _time_configs(shape, dtype, dev, steps):
 
     try:
         for kernel in kernels:
-            env = {"DNX_FDTD3D_STEPS": "1"} if kernel < 2 else {"DNX_FDTD3D_STEPS": "2"}
-            with _env(**env):
+            with L.options(one_step=1 if kernel < 2 else 0):
                 for ty, zc in cands(kernel):
                     set_tiles(dtype, kernel, ty, zc)
                     best = float("inf")
@@ -232,8 +216,8 @@ def tune_batch(shapes, dtype=torch.float32, device=None, steps=8, force=False, a
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             times = {k: {} for k in kernels}
 
-            def measure(env):
-                with _env(**env):
+            def measure(one_step):
+                with L.options(one_step=one_step):
                     b = BatchedFDTD3D(shapes, srcs, mons, n_steps=steps, checkpoint_every=steps,
                                       pml_layers=2, device=dev, dtype=dtype)
                 b._set_eps(eps)
@@ -260,7 +244,7 @@ def tune_batch(shapes, dtype=torch.float32, device=None, steps=8, force=False, a
                     for zc in ZC_CHOICES:
                         for k in (0, 1):
                             set_tiles(dtype, k, ty, zc)
-                        tf, tb = measure({"DNX_FDTD3D_STEPS": "1"})
+                        tf, tb = measure(1)
                         times[0][(ty, zc)], times[1][(ty, zc)] = tf, tb
                 if dtype == torch.float32:
                     picked = _pick({0: times[0], 1: times[1]}, torch.float64)
@@ -270,7 +254,7 @@ def tune_batch(shapes, dtype=torch.float32, device=None, steps=8, force=False, a
                         for ty in TY2_BATCH:
                             for zc in ZC_CHOICES:
                                 set_tiles(dtype, k, ty, zc)
-                                tf, tb = measure({"DNX_FDTD3D_STEPS": "2"})
+                                tf, tb = measure(0)
                                 times[k][(ty, zc)] = tf if k == 2 else tb
                         best = min(times[k].items(), key=lambda kv: kv[1])[0]
                         set_tiles(dtype, k, *best)

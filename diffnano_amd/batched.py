@@ -73,6 +73,10 @@ class BatchedFDTD3D:
         self.ce = torch.ones(tot, dtype=dt, device=dev)
         self.eps = torch.ones(tot, dtype=torch.float64, device=dev)
         self._keep = []
+        # forward generation: the backward of a forward is valid only while no
+        # other forward has overwritten the plan's checkpoints, coefficients
+        # and eps (they live in plan-wide buffers)
+        self._gen = 0
         arr = (Design3D * self.n)()
         self.accs, self.grads, self.mon_shapes = [], [], []
         for d, (shape, src, mon) in enumerate(zip(self.shapes, sources, monitors)):
@@ -204,9 +208,19 @@ class BatchedFDTD3D:
 
     def forward(self, eps_list):
         """Monitor outputs per design: complex128 (n_freq, n_comp, bz, by, bx),
-        differentiable w.r.t. every eps."""
-        outs = _BatchedFn.apply(self, *eps_list)
+        differentiable w.r.t. every eps (checkpoints are recorded only when
+        grad mode is on and some eps requires grad)."""
+        if torch.is_grad_enabled() and any(e.requires_grad for e in eps_list):
+            outs = _BatchedFn.apply(self, *eps_list)
+        else:
+            outs = self._forward_detached(eps_list)
         return [torch.view_as_complex(o) for o in outs]
+
+    def _forward_detached(self, eps_list):
+        self._gen += 1
+        self._set_eps(eps_list)
+        self._run_forward(record=False)
+        return tuple(a.clone().to(self.out_device) for a in self.accs)
 
 
 class _BatchedFn(torch.autograd.Function):
@@ -214,12 +228,22 @@ class _BatchedFn(torch.autograd.Function):
     def forward(ctx, batch, *eps_list):
         ctx.batch = batch
         ctx.devices = [e.device for e in eps_list]
+        ctx.recorded = True  # BatchedFDTD3D.forward applies this only to record
+        batch._gen += 1
+        ctx.gen = batch._gen
         batch._set_eps(eps_list)
-        batch._run_forward(record=any(ctx.needs_input_grad[1:]))
+        batch._run_forward(record=ctx.recorded)
         return tuple(a.clone().to(batch.out_device) for a in batch.accs)
 
     @staticmethod
     def backward(ctx, *grads):
         b = ctx.batch
+        if ctx.gen != b._gen:
+            raise RuntimeError(
+                "BatchedFDTD3D: another forward ran on this batch after the one being "
+                "differentiated and overwrote its checkpoints and coefficients; call backward "
+                "before the next forward (or use a separate BatchedFDTD3D)")
+        if not ctx.recorded:
+            raise RuntimeError("BatchedFDTD3D: this forward stored no checkpoints")
         gl = b._run_backward(list(grads))
         return (None,) + tuple(g.to(dv) for g, dv in zip(gl, ctx.devices))

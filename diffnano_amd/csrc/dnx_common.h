@@ -16,6 +16,10 @@ namespace dnx {
 void set_error(const char* fmt, ...);
 const char* last_error();
 
+// Process-wide kernel-selection option (DNX_OPT_*): the environment is read
+// once, at the first query; dnx_set_option changes it afterwards.
+int option(int id);
+
 #define DNX_CHECK_HIP(expr)                                                      \
   do {                                                                           \
     hipError_t _e = (expr);                                                      \

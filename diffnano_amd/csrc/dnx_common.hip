@@ -1,6 +1,8 @@
 // libdnx_fdtd: error state, launch timing, coefficient preparation, and the
 // per-step side kernels (soft sources, running DFT monitors and adjoints).
 #include <atomic>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -24,6 +26,35 @@ void set_error(const char* fmt, ...) {
 }
 
 const char* last_error() { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Kernel-selection options: initial values from the environment, read once
+// ---------------------------------------------------------------------------
+
+static std::atomic<int> g_opt[DNX_OPT_COUNT];
+static std::once_flag g_opt_once;
+
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return (v && *v) ? atoi(v) : dflt;
+}
+
+static void init_options() {
+  const char* var = getenv("DNX_FDTD3D_VARIANT");
+  g_opt[DNX_OPT_FDTD3D_SIMPLE] = (var && strcmp(var, "simple") == 0) ? 1 : 0;
+  g_opt[DNX_OPT_FDTD3D_ONE_STEP] = env_int("DNX_FDTD3D_STEPS", 2) == 1 ? 1 : 0;
+  g_opt[DNX_OPT_FDTD3D_ADJ2] = env_int("DNX_FDTD3D_ADJ2", 1) != 0 ? 1 : 0;
+  g_opt[DNX_OPT_FDTD3D_FWD2_PF] = env_int("DNX_FDTD3D_FWD2_PF", 0) != 0 ? 1 : 0;
+  g_opt[DNX_OPT_FDTD3D_ADJ2_OCC] = env_int("DNX_FDTD3D_ADJ2_OCC", 2) == 1 ? 1 : 2;
+  g_opt[DNX_OPT_GEMM_GENERIC] = env_int("DNX_GEMM_GENERIC", 0) != 0 ? 1 : 0;
+  g_opt[DNX_OPT_GEMM_PW_VALU] = env_int("DNX_GEMM_PW_VALU", 0) != 0 ? 1 : 0;
+  g_opt[DNX_OPT_GEMM_SMALL] = env_int("DNX_GEMM_SMALL", 0) != 0 ? 1 : 0;
+}
+
+int option(int id) {
+  std::call_once(g_opt_once, init_options);
+  return (id >= 0 && id < DNX_OPT_COUNT) ? g_opt[id].load(std::memory_order_relaxed) : 0;
+}
 
 // ---------------------------------------------------------------------------
 // Per-launch timing registry
@@ -339,6 +370,19 @@ DNX_INST(double)
 extern "C" int32_t dnx_abi_version(void) { return DNX_ABI_VERSION; }
 
 extern "C" const char* dnx_last_error(void) { return dnx::last_error(); }
+
+extern "C" int32_t dnx_set_option(int32_t id, int32_t value) {
+  DNX_REQUIRE(id >= 0 && id < DNX_OPT_COUNT, "set_option: unknown option %d", id);
+  (void)dnx::option(id);  // the environment defaults are read before any change
+  dnx::g_opt[id].store(value, std::memory_order_relaxed);
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_get_option(int32_t id, int32_t* value) {
+  DNX_REQUIRE(id >= 0 && id < DNX_OPT_COUNT && value, "get_option: bad option %d", id);
+  *value = dnx::option(id);
+  return DNX_OK;
+}
 
 extern "C" int32_t dnx_timing_enable(int32_t on) {
   dnx::g_timing_on = on != 0;

@@ -273,12 +273,11 @@ static dim3 grid3_launch(const dnx_grid3d* g) {
 }
 
 // Kernel variant: the fused single-pass kernels for nx <= 256 (default), or
-// the two-kernel in-place variant ("simple") when DNX_FDTD3D_VARIANT=simple,
-// no ping-pong buffer is given, or nx > 256.
+// the two-kernel in-place variant ("simple") when DNX_OPT_FDTD3D_SIMPLE is
+// set, no ping-pong buffer is given, or nx > 256.
 static bool use_fused(const dnx_grid3d* g, const void* work) {
   if (work == nullptr || g->nx > FUSED_MAX_W) return false;
-  const char* v = getenv("DNX_FDTD3D_VARIANT");
-  return !(v && strcmp(v, "simple") == 0);
+  return option(DNX_OPT_FDTD3D_SIMPLE) == 0;
 }
 
 template <typename T>
@@ -303,8 +302,8 @@ static Fused3Args<T> fused_args(const dnx_grid3d* gd, const Fused3Tiles& t) {
 }
 
 // Two-step kernels: float32, fused variant, at most one source (fused) and
-// FUSED2_MAX_MON monitors (fused), no probe.  DNX_FDTD3D_STEPS=1 forces the
-// one-step kernels.
+// FUSED2_MAX_MON monitors (fused), no probe.  DNX_OPT_FDTD3D_ONE_STEP forces
+// the one-step kernels.
 template <typename T>
 static bool use_pairs(const dnx_grid3d* gd, bool fused, int n_src, int n_mon, int64_t probe,
                       int kernel) {
@@ -316,8 +315,7 @@ static bool use_pairs(const dnx_grid3d* gd, bool fused, int n_src, int n_mon, in
   // the two-step kernels address the six fields through one 32-bit buffer
   // offset: 6 * N * 4 bytes must stay below 2^31
   if (int64_t(gd->nx) * gd->ny * gd->nz * 6 * int64_t(sizeof(T)) >= (int64_t(1) << 31)) return false;
-  const char* v = getenv("DNX_FDTD3D_STEPS");
-  return !(v && strcmp(v, "1") == 0);
+  return option(DNX_OPT_FDTD3D_ONE_STEP) == 0;
 }
 
 static int device_cus() {
@@ -380,20 +378,15 @@ static int64_t side_elems(const dnx_monitor* mons, int n_mon, int subs) {
   return n;
 }
 
-// Stream-ordered scratch (the compact monitor buffers), released in stream
-// order when the call returns.
-struct StreamScratch {
-  void* p = nullptr;
-  hipStream_t s = nullptr;
-  int alloc(size_t bytes, hipStream_t st) {
-    s = st;
-    DNX_CHECK_HIP(hipMallocAsync(&p, bytes, st));
-    return DNX_OK;
-  }
-  ~StreamScratch() {
-    if (p) (void)hipFreeAsync(p, s);
-  }
-};
+// Bytes of caller-provided workspace the adjoint sweep needs: the compact
+// monitor adjoint buffers of the float32 two-step kernels ([ncomp][cells]
+// per monitor).  An upper bound that does not depend on the kernel-selection
+// options, so that callers can size it once per plan.
+static int64_t adj3_workspace_bytes(const dnx_grid3d* gd, const dnx_monitor* mons, int n_mon) {
+  if (gd->dtype != DNX_F32 || n_mon > FUSED2_MAX_MON) return 0;
+  const int64_t n = side_elems(mons, n_mon, 1) * int64_t(sizeof(float));
+  return n < (int64_t(1) << 29) ? n : 0;
+}
 
 template <typename T>
 static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
@@ -526,7 +519,7 @@ static int adj_init3(const dnx_grid3d* gd, T* lam, const T* eT, const dnx_monito
 template <typename T>
 static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, const T* e_lo,
                     const T* es, const dnx_source* srcs, int n_src, const dnx_monitor* mons,
-                    int n_mon, T* S, hipStream_t s, int* fin) {
+                    int n_mon, T* S, void* ws, int64_t ws_bytes, hipStream_t s, int* fin) {
   G3<T> g = make_g3<T>(gd);
   const int64_t N = g.n;
   const bool fused = use_fused(gd, work);
@@ -544,25 +537,22 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   T* cur = lam;
   T* nxt = fused ? work : lam;
   const dim3 grid = grid3_launch(gd), block(TX, TY);
-  // Step pairs run the two-step adjoint (DNX_FDTD3D_ADJ2=0 turns it off;
-  // DNX_FDTD3D_STEPS=1 turns off both two-step kernels).
-  const char* adj2 = getenv("DNX_FDTD3D_ADJ2");
+  // Step pairs run the two-step adjoint (DNX_OPT_FDTD3D_ADJ2 = 0 turns it
+  // off; DNX_OPT_FDTD3D_ONE_STEP turns off both two-step kernels).
   // (the tiles read the monitor adjoint terms through one descriptor whose
   // out-of-box offsets start at 2^29 bytes)
   const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, -1, 3) &&
-                     !(adj2 && strcmp(adj2, "0") == 0) &&
+                     option(DNX_OPT_FDTD3D_ADJ2) != 0 &&
                      side_elems(mons, n_mon, 1) * int64_t(sizeof(T)) < (int64_t(1) << 29);
   Fused2Args<T> f2{};
   int ty2 = 0;
-  StreamScratch side;
   if (pairs) {
     const Fused3Tiles t2 = grid_tiles(gd, 3);
-    const int64_t se = side_elems(mons, n_mon, 1);
-    if (se > 0) {
-      int rc = side.alloc(sizeof(T) * se, s);
-      if (rc) return rc;
-    }
-    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, n_mon, static_cast<T*>(side.p), 1);
+    const int64_t need = side_elems(mons, n_mon, 1) * int64_t(sizeof(T));
+    DNX_REQUIRE(need == 0 || (ws != nullptr && ws_bytes >= need),
+                "fdtd3d_adjoint_run: workspace of %lld bytes required, %lld given (size it with "
+                "dnx_fdtd3d_adjoint_workspace_bytes)", (long long)need, (long long)ws_bytes);
+    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, n_mon, static_cast<T*>(ws), 1);
     f2.a.S = S;
     ty2 = t2.ty;
   }
@@ -707,11 +697,18 @@ extern "C" int32_t dnx_fdtd3d_adjoint_init(const dnx_grid3d* g, void* lam, const
                            n_mon, n_total, static_cast<double*>(S), s);
 }
 
+extern "C" int64_t dnx_fdtd3d_adjoint_workspace_bytes(const dnx_grid3d* g,
+                                                     const dnx_monitor* mons, int32_t n_mon) {
+  if (check_grid3(g) || n_mon < 0 || (n_mon > 0 && mons == nullptr)) return -1;
+  return adj3_workspace_bytes(g, mons, n_mon);
+}
+
 extern "C" int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work,
                                           int32_t m_hi, int32_t m_lo, const void* e_lo,
                                           const void* estore, const dnx_source* srcs,
                                           int32_t n_src, const dnx_monitor* mons, int32_t n_mon,
-                                          void* S, void* stream, int32_t* final_in_work) {
+                                          void* S, void* workspace, int64_t ws_bytes,
+                                          void* stream, int32_t* final_in_work) {
   int rc = check_grid3(g);
   if (rc) return rc;
   DNX_REQUIRE(lam && S, "fdtd3d_adjoint_run: null pointer");
@@ -728,11 +725,13 @@ extern "C" int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* 
   if (g->dtype == DNX_F32)
     rc = adj_run3<float>(g, static_cast<float*>(lam), static_cast<float*>(work), m_hi, m_lo,
                          static_cast<const float*>(e_lo), static_cast<const float*>(estore), srcs,
-                         n_src, mons, n_mon, static_cast<float*>(S), s, &fin);
+                         n_src, mons, n_mon, static_cast<float*>(S), workspace, ws_bytes,
+                         s, &fin);
   else
     rc = adj_run3<double>(g, static_cast<double*>(lam), static_cast<double*>(work), m_hi, m_lo,
                           static_cast<const double*>(e_lo), static_cast<const double*>(estore),
-                          srcs, n_src, mons, n_mon, static_cast<double*>(S), s, &fin);
+                          srcs, n_src, mons, n_mon, static_cast<double*>(S), workspace,
+                          ws_bytes, s, &fin);
   if (final_in_work) *final_in_work = fin;
   return rc;
 }

@@ -118,9 +118,7 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
   // the two-step kernels' 32-bit buffer offsets (6 N 4 bytes < 2^31), and
   // tile heights the batched two-step kernels are built for.
   if (std::is_same<T, float>::value) {
-    const char* st = getenv("DNX_FDTD3D_STEPS");
-    const char* a2 = getenv("DNX_FDTD3D_ADJ2");
-    bool ok = !(st && strcmp(st, "1") == 0);
+    bool ok = option(DNX_OPT_FDTD3D_ONE_STEP) == 0;
     for (int d = 0; d < p->n && ok; ++d) {
       const dnx_grid3d& g = ds[d].grid;
       ok = int64_t(g.nz) * g.ny * g.nx * 6 * int64_t(sizeof(T)) < (int64_t(1) << 31);
@@ -134,7 +132,7 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
     }
     const int tf2 = fused3_tiles(p->dtype, 2).ty, ta2 = fused3_tiles(p->dtype, 3).ty;
     p->fpairs = ok && (tf2 == 2 || tf2 == 4);
-    p->apairs = ok && (ta2 == 2 || ta2 == 4) && !(a2 && strcmp(a2, "0") == 0);
+    p->apairs = ok && (ta2 == 2 || ta2 == 4) && option(DNX_OPT_FDTD3D_ADJ2) != 0;
   }
   if ((p->fpairs || p->apairs) && p->max_mon_cells > 0) {
     if (!p->pools.side) {

@@ -758,10 +758,7 @@ int launch_monitor_pair_fwd(const Fused2Args<T>& f, const dnx_monitor& m, int n,
 
 template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
-  static const int pf = [] {
-    const char* v = getenv("DNX_FDTD3D_FWD2_PF");
-    return v ? atoi(v) : 0;
-  }();
+  const bool pf = option(DNX_OPT_FDTD3D_FWD2_PF) != 0;
   if (ty == 4) return pf ? launch_fwd2_ty<T, 4, true>(a, s) : launch_fwd2_ty<T, 4, false>(a, s);
   if (ty == 2) return pf ? launch_fwd2_ty<T, 2, true>(a, s) : launch_fwd2_ty<T, 2, false>(a, s);
   set_error("fused2: unsupported tile height %d", ty);
@@ -1315,8 +1312,7 @@ template <typename T>
 int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
   // occupancy 2 (launch bounds for two work-groups per CU) unless
   // DNX_FDTD3D_ADJ2_OCC=1; TY = 4 fits the register file only at occupancy 1
-  const char* occ = getenv("DNX_FDTD3D_ADJ2_OCC");
-  const bool two = !(occ && occ[0] == '1');
+  const bool two = option(DNX_OPT_FDTD3D_ADJ2_OCC) != 1;
   if (ty == 4) return launch_adj2_ty<T, 4, 1>(a, s);
   if (ty == 2) return two ? launch_adj2_ty<T, 2, 2>(a, s) : launch_adj2_ty<T, 2, 1>(a, s);
   if (ty == 1) return two ? launch_adj2_ty<T, 1, 2>(a, s) : launch_adj2_ty<T, 1, 1>(a, s);

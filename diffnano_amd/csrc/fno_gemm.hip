@@ -630,11 +630,6 @@ static bool launch_cmix(const GemmArgs& g, hipStream_t s) {
          cmix_mk<6, 128>(g, s) || cmix_mk<128, 6>(g, s) || cmix_mk<32, 128>(g, s);
 }
 
-// DNX_GEMM_GENERIC=1 routes every call through the generic kernel (tests).
-static bool getenv_flag(const char* name) {
-  const char* e = getenv(name);
-  return e && *e && *e != '0';
-}
 
 static bool nt_ok(const GemmArgs& g) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
@@ -1085,14 +1080,14 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
   DNX_REQUIRE(g->batch <= 65535, "gemm: batch > 65535");
   dim3 grid(cdiv(g->N, GBN), cdiv(g->M, GBM), g->batch);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const bool generic = getenv_flag("DNX_GEMM_GENERIC");
+  const bool generic = option(DNX_OPT_GEMM_GENERIC) != 0;
   bool done = false;
-  if (!generic && !getenv_flag("DNX_GEMM_PW_VALU") && launch_pw(*g, s)) {
+  if (!generic && option(DNX_OPT_GEMM_PW_VALU) == 0 && launch_pw(*g, s)) {
     done = true;
   } else if (!generic && launch_cmix(*g, s)) {
     done = true;
   } else if (!generic && nt_ok(*g) && g->M >= XBM && g->N >= XBN && g->K >= XBIG_K &&
-             !getenv_flag("DNX_GEMM_SMALL")) {
+             option(DNX_OPT_GEMM_SMALL) == 0) {
     int rc;
     if (g->a_bf16 && g->b_bf16) rc = launch_big<true, true>(*g, s);
     else if (g->a_bf16) rc = launch_big<true, false>(*g, s);

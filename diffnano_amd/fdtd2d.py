@@ -21,8 +21,8 @@ from collections.abc import Sequence
 import torch
 
 from . import _lib as L
-from ._common import (cpml_profile, dense_mask, monitor_box, source_box_2d, source_profile,
-                      waveform_table)
+from ._common import (EPS_CHECKS, check_eps_finite, cpml_profile, dense_mask, monitor_box,
+                      source_box_2d, source_profile, waveform_table)
 from ._engine import Engine, MonitorSpec, Plan, SourceSpec, _TimeReversalFDTD, forward_only
 from ._result import SimResult
 
@@ -48,6 +48,7 @@ class FDTDSolver2D:
         backward: str = "auto",
         dtype: torch.dtype = torch.float32,
         checkpoint_every: int = 64,
+        check_eps: str = "warn",
     ):
         self.grid_shape = grid_shape
         self.dl = dl
@@ -62,6 +63,9 @@ class FDTDSolver2D:
         self.backward = backward
         self.dtype = dtype
         self.checkpoint_every = checkpoint_every
+        if check_eps not in EPS_CHECKS:
+            raise ValueError(f"check_eps must be one of {EPS_CHECKS}, got {check_eps!r}")
+        self.check_eps = check_eps
 
         self.dt = courant * dl / math.sqrt(2.0)
         self.omega = 2 * math.pi / wavelength_nm
@@ -152,6 +156,7 @@ class FDTDSolver2D:
         if eps_r.dim() == 3:
             eps_r = eps_r.squeeze(0)
 
+        finite = check_eps_finite(eps_r, self.check_eps)
         dev = L.compute_device(self._device)
         eng = self._engine(dev)
         probe = -1
@@ -181,6 +186,8 @@ class FDTDSolver2D:
             "courant": self.courant,
             "snapshots": snapshots,
         }
+        if not finite:
+            meta["eps_finite"] = False
         if mon_specs:
             meta["monitors"] = [torch.view_as_complex(m) for m in mon_raw]
         return SimResult(field=field, wavelengths=wavelengths, metadata=meta)

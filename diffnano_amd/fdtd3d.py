@@ -25,8 +25,8 @@ from collections.abc import Sequence
 import torch
 
 from . import _lib as L
-from ._common import (COMPONENTS_3D, comp_mask_3d, cpml_profile, dense_mask, monitor_box,
-                      source_box_3d, source_profile, waveform_table)
+from ._common import (COMPONENTS_3D, EPS_CHECKS, check_eps_finite, comp_mask_3d, cpml_profile,
+                      dense_mask, monitor_box, source_box_3d, source_profile, waveform_table)
 from ._engine import Engine, MonitorSpec, Plan, SourceSpec, _TimeReversalFDTD, forward_only
 from ._result import SimResult
 
@@ -59,6 +59,7 @@ class FDTDSolver3D:
         *,
         dtype: torch.dtype = torch.float32,
         checkpoint_every: int = 64,
+        check_eps: str = "warn",
     ):
         self.grid_shape = grid_shape
         self.dl = dl
@@ -72,6 +73,9 @@ class FDTDSolver3D:
         self.backward = backward
         self.dtype = dtype
         self.checkpoint_every = checkpoint_every
+        if check_eps not in EPS_CHECKS:
+            raise ValueError(f"check_eps must be one of {EPS_CHECKS}, got {check_eps!r}")
+        self.check_eps = check_eps
 
         self.dt = courant * dl / math.sqrt(3.0)
         self.omega = 2 * math.pi / wavelength_nm
@@ -181,6 +185,7 @@ class FDTDSolver3D:
         if eps_r.dim() == 2:
             eps_r = eps_r.unsqueeze(0)
 
+        finite = check_eps_finite(eps_r, self.check_eps)
         dev = L.compute_device(self._device)
         eng = self._engine(dev)
         mon_specs = self._monitor_specs(monitors)
@@ -199,6 +204,8 @@ class FDTDSolver3D:
         }
         if self.backward == "time_reversal":
             meta["backward"] = "time_reversal"
+        if not finite:
+            meta["eps_finite"] = False
         if mon_specs:
             meta["monitors"] = [torch.view_as_complex(m) for m in mon_raw]
         return SimResult(field=field, wavelengths=wavelengths, metadata=meta)

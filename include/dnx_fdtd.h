@@ -16,7 +16,11 @@
  *    coefficient, profile, waveform and gradient array of the call.
  *    DFT accumulators are always float64.
  *  - Calls are asynchronous on the HIP stream passed in (NULL = the null
- *    stream); the library never synchronises and allocates no device memory.
+ *    stream); the library never synchronises.
+ *  - The library allocates no device memory: every buffer, including the
+ *    scratch ("workspace") a call needs, is the caller's, sized by the
+ *    *_workspace_bytes queries.  (The HIP runtime itself creates the event
+ *    and graph objects of timed launches and of batched plans.)
  *  - Return 0 on success, a negative DNX_E* code on error; the message is in
  *    dnx_last_error() (thread-local).  No C++ exception crosses the ABI.
  */
@@ -35,7 +39,7 @@ extern "C" {
 #pragma GCC visibility push(default)
 #endif
 
-#define DNX_ABI_VERSION 2
+#define DNX_ABI_VERSION 3
 
 enum { DNX_F32 = 0, DNX_F64 = 1 };
 enum {
@@ -109,6 +113,26 @@ typedef struct dnx_monitor {
 int32_t dnx_abi_version(void);
 const char* dnx_last_error(void);
 
+/* Process-wide kernel-selection options.  They change which kernels run,
+ * never results (every variant is bitwise identical).  Initial values come
+ * from the environment variable in brackets, read once at the first query;
+ * dnx_set_option overrides them for later calls.  No reference counterpart
+ * (the reference has no kernels). */
+enum {
+  DNX_OPT_FDTD3D_SIMPLE = 0,   /* 1: two-kernel in-place 3D step [DNX_FDTD3D_VARIANT=simple] */
+  DNX_OPT_FDTD3D_ONE_STEP = 1, /* 1: one-step 3D kernels only [DNX_FDTD3D_STEPS=1] */
+  DNX_OPT_FDTD3D_ADJ2 = 2,     /* 0: two-step 3D adjoint off (default 1) [DNX_FDTD3D_ADJ2] */
+  DNX_OPT_FDTD3D_FWD2_PF = 3,  /* 1: two-step forward loads one plane ahead [DNX_FDTD3D_FWD2_PF] */
+  DNX_OPT_FDTD3D_ADJ2_OCC = 4, /* waves per SIMD of the two-step adjoint, 1 or 2 (default 2)
+                                  [DNX_FDTD3D_ADJ2_OCC] */
+  DNX_OPT_GEMM_GENERIC = 5,    /* 1: FNO products on the element-wise kernel [DNX_GEMM_GENERIC] */
+  DNX_OPT_GEMM_PW_VALU = 6,    /* 1: FNO 1x1 layers on the vector ALU [DNX_GEMM_PW_VALU] */
+  DNX_OPT_GEMM_SMALL = 7,      /* 1: FNO products on the 128x128 tile kernel [DNX_GEMM_SMALL] */
+  DNX_OPT_COUNT = 8
+};
+int32_t dnx_set_option(int32_t id, int32_t value);
+int32_t dnx_get_option(int32_t id, int32_t* value);
+
 /* Per-launch timing of the hot kernels with HIP events on the launch
  * stream (no reference counterpart; measurement only).  Names:
  * "fdtd3d_fwd", "fdtd3d_fwd_store" (replay, also stores E), "fdtd3d_adj".
@@ -169,12 +193,19 @@ int32_t dnx_fdtd3d_adjoint_init(const dnx_grid3d* g, void* lam, const void* e_T,
  * if m_lo == 0 it finally adds the step-0 source term.  This is the exact
  * discrete transpose of dnx_fdtd3d_run, i.e. the gradient the reference
  * obtains by autograd (fdtd3d.py:166-219).  dL/deps = -S / eps.
- * e_lo: E^{m_lo} (3*N); estore: E^{m_lo+1..m_hi-1} (slot j = E^{m_lo+1+j}). */
+ * e_lo: E^{m_lo} (3*N); estore: E^{m_lo+1..m_hi-1} (slot j = E^{m_lo+1+j}).
+ * workspace: device scratch of at least
+ *   dnx_fdtd3d_adjoint_workspace_bytes(g, mons, n_mon) bytes (the two-step
+ *   kernels' monitor adjoint terms); may be NULL when that is 0.  Contents
+ *   on entry do not matter; calls sharing one workspace must not overlap. */
+int64_t dnx_fdtd3d_adjoint_workspace_bytes(const dnx_grid3d* g, const dnx_monitor* mons,
+                                           int32_t n_mon);
 int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work, int32_t m_hi,
                                int32_t m_lo, const void* e_lo, const void* estore,
                                const dnx_source* srcs, int32_t n_src,
                                const dnx_monitor* mons, int32_t n_mon, void* S,
-                               void* stream, int32_t* final_in_work);
+                               void* workspace, int64_t ws_bytes, void* stream,
+                               int32_t* final_in_work);
 
 /* Tile shape of the fused 3D kernels for one dtype and kernel (0 forward/
  * replay, 1 adjoint, 2 two-step forward/replay, 3 two-step adjoint; the

@@ -21,21 +21,28 @@ def _lin(w, b, x):
     return torch.einsum("oi,bihw->bohw", w, x) + b[None, :, None, None]
 
 
+def spectral_conv(h, sr, si, m1, m2):
+    """Spectral convolution of one Fourier layer: h [B, I, H, W] -> [B, O, H, W].
+    sr, si: [modes][I][O] real and imaginary weights (mode order above)."""
+    B, _, H, W = h.shape
+    O = sr.shape[2]
+    X = torch.fft.rfft2(h)
+    out = torch.zeros(B, O, H, W // 2 + 1, dtype=torch.complex128)
+    Wc = torch.complex(sr, si)
+    for kyi in range(2 * m1):
+        ky = kyi if kyi < m1 else H + kyi - 2 * m1
+        for kx in range(m2):
+            mu = kyi * m2 + kx
+            out[:, :, ky, kx] = X[:, :, ky, kx] @ Wc[mu]
+    return torch.fft.irfft2(out, s=(H, W))
+
+
 def fno_forward(p, x, layers, m1, m2):
     """p: dict of float64 CPU tensors (names as FNO2d.params); x [B, in, H, W]."""
     gelu = torch.nn.functional.gelu
     h = _lin(p["w_lift"], p["b_lift"], x)
-    B, Cw, H, W = h.shape
     for l in range(layers):
-        X = torch.fft.rfft2(h)
-        out = torch.zeros(B, Cw, H, W // 2 + 1, dtype=torch.complex128)
-        Wc = torch.complex(p[f"sr{l}"], p[f"si{l}"])
-        for kyi in range(2 * m1):
-            ky = kyi if kyi < m1 else H + kyi - 2 * m1
-            for kx in range(m2):
-                mu = kyi * m2 + kx
-                out[:, :, ky, kx] = X[:, :, ky, kx] @ Wc[mu]
-        S = torch.fft.irfft2(out, s=(H, W))
+        S = spectral_conv(h, p[f"sr{l}"], p[f"si{l}"], m1, m2)
         h = _lin(p[f"w{l}"], p[f"b{l}"], h) + S
         if l < layers - 1:
             h = gelu(h)

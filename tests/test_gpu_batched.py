@@ -89,15 +89,14 @@ def test_autotuner_measures_caches_and_applies(tmp_path, monkeypatch):
             autotune.set_tiles(torch.float32, k, ty, zc)
 
 
-def _batched_run(T, K, monkeypatch, env):
+def _batched_run(T, K, opts):
+    from diffnano_amd import _lib as L
     from diffnano_amd.batched import BatchedFDTD3D
     shapes, eps, srcs, mons = _cases()
-    for k, v in env.items():
-        monkeypatch.setenv(k, v)
-    b = BatchedFDTD3D(shapes, [dict(s) for s in srcs], mons, dl=25.0, pml_layers=3, n_steps=T,
-                      courant=0.4, checkpoint_every=K, device="cuda", use_graph=True)
-    for k in env:
-        monkeypatch.delenv(k)
+    with L.options(**opts):  # the plan picks its kernels when it is built
+        b = BatchedFDTD3D(shapes, [dict(s) for s in srcs], mons, dl=25.0, pml_layers=3,
+                          n_steps=T, courant=0.4, checkpoint_every=K, device="cuda",
+                          use_graph=True)
     es = [torch.tensor(e, device="cuda", requires_grad=True) for e in eps]
     outs = b.forward(es)
     fields = [f.clone() for f in b.final_fields()]
@@ -106,14 +105,14 @@ def _batched_run(T, K, monkeypatch, env):
 
 
 @pytest.mark.parametrize("T,K", [(50, 8), (37, 6), (21, 4), (9, 2)])
-def test_batched_two_step_matches_one_step_bitwise(T, K, monkeypatch):
+def test_batched_two_step_matches_one_step_bitwise(T, K):
     """The batched two-step kernels (pairs of steps per persistent launch,
     their own buffer order, single steps at odd tails) reproduce the
     one-step batched path bitwise: monitors, final fields and gradients, for
     step counts and checkpoint intervals that leave odd segments."""
-    p2, F2, E2, G2 = _batched_run(T, K, monkeypatch, {})
-    pm, Fm, Em, Gm = _batched_run(T, K, monkeypatch, {"DNX_FDTD3D_ADJ2": "0"})
-    p1, F1, E1, G1 = _batched_run(T, K, monkeypatch, {"DNX_FDTD3D_STEPS": "1"})
+    p2, F2, E2, G2 = _batched_run(T, K, {})
+    pm, Fm, Em, Gm = _batched_run(T, K, {"adj2": 0})
+    p1, F1, E1, G1 = _batched_run(T, K, {"one_step": 1})
     assert p2 == (True, True) and pm == (True, False) and p1 == (False, False)
     for run in ((F2, E2, G2), (Fm, Em, Gm)):
         for a, b in zip(run[0], F1):
@@ -124,7 +123,7 @@ def test_batched_two_step_matches_one_step_bitwise(T, K, monkeypatch):
             assert torch.equal(a, b), (a - b).abs().max()
 
 
-def test_batched_two_step_switched_off_by_tiles(monkeypatch):
+def test_batched_two_step_switched_off_by_tiles():
     """Tile height 0 for the two-step kernels (the autotuner's choice when
     the one-step kernels are faster) gives a plan on the one-step path."""
     from diffnano_amd import autotune
@@ -132,11 +131,11 @@ def test_batched_two_step_switched_off_by_tiles(monkeypatch):
     try:
         autotune.set_tiles(torch.float32, 2, 0, 1)
         autotune.set_tiles(torch.float32, 3, 0, 1)
-        p0, F0, E0, G0 = _batched_run(21, 4, monkeypatch, {})
+        p0, F0, E0, G0 = _batched_run(21, 4, {})
     finally:
         for k, (ty, zc) in zip((2, 3), saved):
             autotune.set_tiles(torch.float32, k, ty, zc)
-    p1, F1, E1, G1 = _batched_run(21, 4, monkeypatch, {"DNX_FDTD3D_STEPS": "1"})
+    p1, F1, E1, G1 = _batched_run(21, 4, {"one_step": 1})
     assert p0 == (False, False)
     for a, b in zip(F0 + E0 + G0, F1 + E1 + G1):
         assert torch.equal(a, b)

@@ -59,7 +59,7 @@ def test_gemm_bf16_operands_gelu_and_pre():
 @pytest.mark.parametrize("M,K", [(32, 9), (32, 32), (128, 32), (6, 128), (128, 6), (32, 128)])
 @pytest.mark.parametrize("N", [4096, 1000])
 @pytest.mark.parametrize("xbf", [False, True])
-def test_pointwise_kernels(M, K, N, xbf, monkeypatch):
+def test_pointwise_kernels(M, K, N, xbf, lib_opts):
     """The FNO's channel-mixing shapes (K <= 32 on the matrix-core kernel,
     K = 128 on the vector-ALU one): every epilogue (alpha, beta Cin, row
     bias, pre, GELU), a ragged cell count, a weight read through a
@@ -76,7 +76,7 @@ def test_pointwise_kernels(M, K, N, xbf, monkeypatch):
     outs = []
     for valu in (False, True):
         if valu:
-            monkeypatch.setenv("DNX_GEMM_PW_VALU", "1")
+            lib_opts.set("gemm_pw_valu", 1)
         C = torch.full((M, N), float("nan"), device="cuda")
         pre = torch.full_like(C, float("nan"))
         gemm(M, N, K, Wt, (1, M, 0), X, (N, 1, 0), C, (N, 1, 0), alpha=0.75, beta=-0.5, Cin=Cin,
@@ -93,7 +93,7 @@ def test_pointwise_kernels(M, K, N, xbf, monkeypatch):
 
 @pytest.mark.parametrize("abf,bbf", [(False, False), (True, False), (False, True), (True, True)])
 @pytest.mark.parametrize("M,N,K,batch", [(128, 128, 64, 1), (200, 72, 640, 2), (9, 300, 128, 3)])
-def test_gemm_nt_fast_path_matches_generic_and_fp32(M, N, K, batch, abf, bbf, monkeypatch):
+def test_gemm_nt_fast_path_matches_generic_and_fp32(M, N, K, batch, abf, bbf, lib_opts):
     """Both operands contiguous along K (the fast path), BF16 or FP32 in
     memory, ragged M/N, several K steps and batches."""
     from diffnano_amd.fno import gemm
@@ -114,7 +114,7 @@ def test_gemm_nt_fast_path_matches_generic_and_fp32(M, N, K, batch, abf, bbf, mo
     torch.cuda.synchronize()
     assert rel_l2(pre.cpu().numpy(), ref.cpu().numpy()) < 2e-6
     C2 = torch.empty_like(C)
-    monkeypatch.setenv("DNX_GEMM_GENERIC", "1")
+    lib_opts.set("gemm_generic", 1)
     gemm(M, N, K, A, (K, 1, M * K), B, (1, K, N * K), C2, (N, 1, M * N), batch=batch,
          alpha=0.75, bias=bias, bias_mode=2, act=1)
     torch.cuda.synchronize()
@@ -124,7 +124,7 @@ def test_gemm_nt_fast_path_matches_generic_and_fp32(M, N, K, batch, abf, bbf, mo
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 @pytest.mark.parametrize("abf,bbf", [(False, False), (True, True), (False, True)])
 @pytest.mark.parametrize("M,N,K", [(32, 1000, 96), (130, 70, 100), (6, 257, 128)])
-def test_gemm_tile_layouts_match_generic(M, N, K, abf, bbf, ta, tb, monkeypatch):
+def test_gemm_tile_layouts_match_generic(M, N, K, abf, bbf, ta, tb, lib_opts):
     """Layout-specialised tile kernel (K- or row-contiguous operands, K tails)
     against the element-wise kernel, bitwise, and against FP32 products."""
     from diffnano_amd.fno import gemm
@@ -143,7 +143,7 @@ def test_gemm_tile_layouts_match_generic(M, N, K, abf, bbf, ta, tb, monkeypatch)
     C1 = torch.empty(M, N, device="cuda")
     C2 = torch.empty(M, N, device="cuda")
     gemm(M, N, K, A, sA, B, sB, C1, (N, 1, 0), beta=0.5, Cin=Cin)
-    monkeypatch.setenv("DNX_GEMM_GENERIC", "1")
+    lib_opts.set("gemm_generic", 1)
     gemm(M, N, K, A, sA, B, sB, C2, (N, 1, 0), beta=0.5, Cin=Cin)
     torch.cuda.synchronize()
     assert torch.equal(C1, C2)
@@ -207,7 +207,7 @@ def test_fno_backward_matches_fft_oracle():
 @pytest.mark.parametrize("abf,bbf", [(False, True), (True, True), (False, False)])
 @pytest.mark.parametrize("M,N,K,batch", [(256, 256, 2048, 1), (300, 522, 2112, 2),
                                          (512, 1024, 4096, 1)])
-def test_gemm_large_tile_kernel_matches_small_and_fp32(M, N, K, batch, abf, bbf, monkeypatch):
+def test_gemm_large_tile_kernel_matches_small_and_fp32(M, N, K, batch, abf, bbf, lib_opts):
     """The 256 x 256 MFMA 32x32x16 kernel (M, N >= 256, K >= 2048, K-contiguous operands):
     ragged M and N (vector and element epilogue paths), both bias modes,
     GELU with the pre-activation, beta * Cin, against FP32 products of the
@@ -235,10 +235,10 @@ def test_gemm_large_tile_kernel_matches_small_and_fp32(M, N, K, batch, abf, bbf,
         assert rel_l2(pre.cpu().numpy(), ref.cpu().numpy()) < 2e-6, mode
         assert rel_l2(C.cpu().numpy(), torch.nn.functional.gelu(ref).cpu().numpy()) < 2e-6
         C2 = torch.empty_like(C)
-        monkeypatch.setenv("DNX_GEMM_SMALL", "1")
+        lib_opts.set("gemm_small", 1)
         gemm(M, N, K, A, (K, 1, M * K), B, (1, K, N * K), C2, (N, 1, M * N), batch=batch,
              alpha=0.75, beta=0.5, Cin=Cin, bias=bias, bias_mode=mode, act=1)
-        monkeypatch.delenv("DNX_GEMM_SMALL")
+        lib_opts.reset("gemm_small")
         torch.cuda.synchronize()
         assert rel_l2(C.cpu().numpy(), C2.cpu().numpy()) < 1e-6
 

@@ -318,12 +318,12 @@ def _run_fwd_grad(shape, dtype, T=30, src=None, L=3):
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-13), (torch.float32, 1e-6)])
-def test_fused_kernels_match_simple_variant(dtype, tol, monkeypatch):
+def test_fused_kernels_match_simple_variant(dtype, tol, lib_opts):
     """The fused single-pass kernels against the two-kernel variant, on a
     ragged grid (partial y tiles, partial waves, several z chunks)."""
     shape = (37, 21, 70)
     f1, g1 = _run_fwd_grad(shape, dtype)
-    monkeypatch.setenv("DNX_FDTD3D_VARIANT", "simple")
+    lib_opts.set("simple", 1)
     f2, g2 = _run_fwd_grad(shape, dtype)
     assert rel_l2(f1.numpy(), f2.numpy()) < tol
     assert rel_l2(g1.numpy(), g2.numpy()) < tol
@@ -374,7 +374,7 @@ def _run_monitored(shape, T, ckpt, n_mon):
 @pytest.mark.parametrize("ty,zc,T,ckpt,n_mon", [(4, 32, 37, 6, 1), (2, 3, 24, 5, 1),
                                                 (4, 5, 25, 64, 1), (2, 4, 20, 7, 0),
                                                 (2, 6, 21, 6, 2)])
-def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, monkeypatch):
+def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, lib_opts):
     """The temporally blocked (two steps per launch) kernels with fused
     source and monitor against the one-step kernels: ragged grid, several
     z chunks, odd step counts and checkpoint intervals (single steps at the
@@ -390,14 +390,14 @@ def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, monkeyp
     try:
         for k in (2, 3):
             L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, ty, zc))
-        monkeypatch.setenv("DNX_FDTD3D_ADJ2", "0")
+        lib_opts.set("adj2", 0)
         two = _run_monitored(shape, T, ckpt, n_mon)            # two-step forward
-        monkeypatch.delenv("DNX_FDTD3D_ADJ2")
+        lib_opts.reset("adj2")
         two_adj = _run_monitored(shape, T, ckpt, n_mon)        # and two-step adjoint
     finally:
         for k, t, z in saved:
             L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, k, t, z))
-    monkeypatch.setenv("DNX_FDTD3D_STEPS", "1")
+    lib_opts.set("one_step", 1)
     one = _run_monitored(shape, T, ckpt, n_mon)
     for a, b, c in zip(two, two_adj, one):
         assert torch.equal(a, c), (a - c).abs().max()
@@ -409,7 +409,7 @@ def test_two_step_kernels_match_one_step_bitwise(ty, zc, T, ckpt, n_mon, monkeyp
                                        ((5, 9, 0, 1, 3, 37), "z"),         # y = 0 rows
                                        ((13, 14, 10, 11, 39, 40), "xz"),   # far corner
                                        ((2, 12, 2, 9, 19, 22), "y")])      # holds the source
-def test_two_step_monitor_at_edges_and_source_bitwise(box, comps, monkeypatch):
+def test_two_step_monitor_at_edges_and_source_bitwise(box, comps, lib_opts):
     """The forward pair's monitor sample recomputed after the launch (cells
     on every grid edge, every component mask, a box holding the source)
     against the one-step kernels, with the gradient through the monitor."""
@@ -429,7 +429,7 @@ def test_two_step_monitor_at_edges_and_source_bitwise(box, comps, monkeypatch):
         return r.field.detach().clone(), F.detach().clone(), eps.grad.clone()
 
     two = run()
-    monkeypatch.setenv("DNX_FDTD3D_STEPS", "1")
+    lib_opts.set("one_step", 1)
     one = run()
     for a, b in zip(two, one):
         assert torch.equal(a, b), (a - b).abs().max()
@@ -437,7 +437,7 @@ def test_two_step_monitor_at_edges_and_source_bitwise(box, comps, monkeypatch):
 
 
 @pytest.mark.parametrize("ty,zc,occ", [(1, 5, "1"), (1, 64, "2"), (2, 7, "2"), (4, 3, "1")])
-def test_two_step_adjoint_tiles_bitwise(ty, zc, occ, monkeypatch):
+def test_two_step_adjoint_tiles_bitwise(ty, zc, occ, lib_opts):
     """Every tile height / occupancy variant of the two-step adjoint gives
     the one-step gradient bitwise."""
     import diffnano_amd._lib as L
@@ -445,11 +445,11 @@ def test_two_step_adjoint_tiles_bitwise(ty, zc, occ, monkeypatch):
     shape = (17, 11, 66)
     t0, z0 = C.c_int32(), C.c_int32()
     L.check(L.lib().dnx_fdtd3d_get_tiles(L.DNX_F32, 3, C.byref(t0), C.byref(z0)))
-    monkeypatch.setenv("DNX_FDTD3D_STEPS", "1")
+    lib_opts.set("one_step", 1)
     one = _run_monitored(shape, 23, 8, 1)
-    monkeypatch.delenv("DNX_FDTD3D_STEPS")
-    monkeypatch.setenv("DNX_FDTD3D_ADJ2", "1")
-    monkeypatch.setenv("DNX_FDTD3D_ADJ2_OCC", occ)
+    lib_opts.reset("one_step")
+    lib_opts.set("adj2", 1)
+    lib_opts.set("adj2_occ", int(occ))
     try:
         L.check(L.lib().dnx_fdtd3d_set_tiles(L.DNX_F32, 3, ty, zc))
         two = _run_monitored(shape, 23, 8, 1)

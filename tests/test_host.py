@@ -173,3 +173,119 @@ def test_blob_batch_generator_matches_scipy_blobs():
         e = W.blob_eps_batch(z, H, H, lo).numpy()
         for i, s in enumerate(seeds):
             np.testing.assert_array_equal(e[i], W.blob_eps_2d(s, H=H, W=H, lo=lo))
+
+
+# ---------------------------------------------------------------------------
+# Allocation statement, options, workspace query, non-finite eps guard
+# ---------------------------------------------------------------------------
+
+
+def test_library_sources_allocate_no_device_memory():
+    """The C ABI allocates nothing: no hipMalloc / hipMallocAsync /
+    hipFree / hipHostMalloc call anywhere in the HIP sources (buffers and
+    workspaces come from the caller; torch's caching allocator owns them)."""
+    csrc = os.path.join(ROOT, "diffnano_amd", "csrc")
+    pat = re.compile(r"\bhip(Malloc\w*|Free\w*|HostMalloc|HostAlloc|ExtMalloc\w*)\s*\(")
+    for f in sorted(os.listdir(csrc)):
+        text = open(os.path.join(csrc, f)).read()
+        code = re.sub(r"//[^\n]*|/\*.*?\*/", "", text, flags=re.S)
+        assert not pat.search(code), f"{f}: {pat.search(code).group(0)}"
+
+
+def test_options_round_trip_and_reject_unknown_ids():
+    lib = L.load()
+    for name, oid in L.OPTIONS.items():
+        v = C.c_int32(-7)
+        assert lib.dnx_get_option(oid, C.byref(v)) == 0 and v.value >= 0, name
+        assert lib.dnx_set_option(oid, v.value + 1) == 0
+        w = C.c_int32()
+        lib.dnx_get_option(oid, C.byref(w))
+        assert w.value == v.value + 1
+        assert lib.dnx_set_option(oid, v.value) == 0
+    assert lib.dnx_set_option(99, 1) == -1 and b"unknown option" in lib.dnx_last_error()
+    assert lib.dnx_get_option(-1, C.byref(C.c_int32())) == -1
+
+
+def _fake_grid(shape, code):
+    # host-side queries never dereference the coefficient pointers
+    return L.Grid3D(shape[0], shape[1], shape[2], code, 0.2, 16, 16, 16, 16, 16)
+
+
+def test_adjoint_workspace_query():
+    lib = L.load()
+    g = _fake_grid((256, 256, 256), L.DNX_F32)
+    m = L.Monitor(L.Box(0, 256, 0, 256, 236, 237), 2, 5, 16, None, 16)
+    assert lib.dnx_fdtd3d_adjoint_workspace_bytes(C.byref(g), C.byref(m), 1) == 256 * 256 * 4
+    m3 = L.Monitor(L.Box(0, 4, 0, 5, 6, 9), 7, 2, 16, None, 16)
+    assert lib.dnx_fdtd3d_adjoint_workspace_bytes(C.byref(g), C.byref(m3), 1) == 3 * 60 * 4
+    assert lib.dnx_fdtd3d_adjoint_workspace_bytes(C.byref(g), None, 0) == 0
+    pair = (L.Monitor * 2)(m, m3)   # more monitors than the two-step kernels take
+    assert lib.dnx_fdtd3d_adjoint_workspace_bytes(C.byref(g), pair, 2) == 0
+    g64 = _fake_grid((256, 256, 256), L.DNX_F64)  # the float64 path has no two-step kernels
+    assert lib.dnx_fdtd3d_adjoint_workspace_bytes(C.byref(g64), C.byref(m), 1) == 0
+    bad = _fake_grid((0, 4, 4), L.DNX_F32)
+    assert lib.dnx_fdtd3d_adjoint_workspace_bytes(C.byref(bad), None, 0) < 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_non_finite_eps_is_flagged_before_any_launch(dim):
+    from diffnano_amd import FDTDSolver2D, FDTDSolver3D
+    shape = (6, 7, 8) if dim == 3 else (7, 8)
+    eps = torch.full(shape, 2.0, dtype=torch.float64)
+    eps[(1,) * dim] = float("nan")
+    mk = (lambda **kw: FDTDSolver3D(grid_shape=shape, pml_layers=1, n_steps=4, **kw)) if dim == 3 \
+        else (lambda **kw: FDTDSolver2D(grid_shape=shape, pml_layers=1, n_steps=4, **kw))
+    with pytest.raises(ValueError, match="non-finite"):
+        mk(check_eps="raise").forward(eps)
+    eps[(1,) * dim] = float("inf")
+    with pytest.raises(ValueError, match="1 non-finite"):
+        mk(check_eps="raise").forward(eps)
+    if not torch.cuda.is_available():
+        # "warn" warns, then reaches the device selection (no GPU here)
+        with pytest.warns(RuntimeWarning, match="non-finite"):
+            with pytest.raises(RuntimeError, match="no CPU fallback"):
+                mk().forward(eps)
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            mk(check_eps="off").forward(eps)
+    with pytest.raises(ValueError, match="check_eps"):
+        mk(check_eps="sometimes")
+
+
+def test_batched_backward_refuses_a_stale_forward():
+    """The batched plan keeps one set of checkpoints: a backward after a
+    later forward on the same batch raises instead of returning a gradient
+    of the wrong forward (host logic of _BatchedFn, with a stand-in batch)."""
+    from diffnano_amd.batched import BatchedFDTD3D
+
+    class FakeBatch:
+        out_device = torch.device("cpu")
+        forward = BatchedFDTD3D.forward
+        _forward_detached = BatchedFDTD3D._forward_detached
+
+        def __init__(self):
+            self._gen = 0
+            self.accs = [torch.zeros(1, 1, 1, 1, 2, 2, dtype=torch.float64)]
+            self.calls = []
+
+        def _set_eps(self, eps_list):
+            self.accs[0].fill_(float(eps_list[0].sum()))
+
+        def _run_forward(self, record):
+            self.calls.append(("fwd", record))
+
+        def _run_backward(self, grads):
+            self.calls.append(("bwd",))
+            return [torch.ones(2, 2, 2, dtype=torch.float64)]
+
+    b = FakeBatch()
+    e = torch.ones(2, 2, 2, dtype=torch.float64, requires_grad=True)
+    out = b.forward([e])
+    out[0].abs().sum().backward()           # fresh: fine
+    assert torch.equal(e.grad, torch.ones(2, 2, 2, dtype=torch.float64))
+    out1 = b.forward([e])
+    with torch.no_grad():
+        b.forward([e])                       # an evaluation forward in between
+    with pytest.raises(RuntimeError, match="another forward ran"):
+        out1[0].abs().sum().backward()
+    assert b.calls.count(("bwd",)) == 1
+    assert ("fwd", False) in b.calls and ("fwd", True) in b.calls

@@ -11,6 +11,7 @@ import json
 import math
 
 import numpy as np
+import torch
 import pytest
 
 from oracle import fdtd_oracle as O
@@ -369,3 +370,47 @@ def test_adjoint2d_central_fd(pol):
         h = 1e-5
         fd = (loss(eps + h * v) - loss(eps - h * v)) / (2 * h)
         assert abs(np.sum(grad * v) - fd) < 1e-6 * max(1.0, abs(fd))
+
+
+# ---------------------------------------------------------------------------
+# FNO oracle: the spectral layer against a brute-force DFT
+# ---------------------------------------------------------------------------
+
+
+def _spectral_brute(h, sr, si, m1, m2):
+    """Direct sums, no FFT: forward DFT X(fy, fx) = sum_{y,x} h e^{-2 pi i (fy y/H + fx x/W)}
+    at the kept frequencies fy in {0..m1-1} U {-m1..-1} (row index kyi, fy = kyi for
+    kyi < m1, else kyi - 2 m1) and fx in {0..m2-1}; channel mixing per mode; real
+    inverse over the kept half-spectrum: y = (1/HW) sum c(fx) Re[Y e^{+2 pi i (...)}],
+    c(0) = 1, c(fx) = 2 for 0 < fx < W/2 (the Hermitian partner of the column)."""
+    B, I, H, W = h.shape
+    O = sr.shape[2]
+    yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    out = np.zeros((B, O, H, W))
+    for kyi in range(2 * m1):
+        fy = kyi if kyi < m1 else kyi - 2 * m1
+        for fx in range(m2):
+            ph = np.exp(-2j * np.pi * (fy * yy / H + fx * xx / W))
+            X = (h * ph[None, None]).sum(axis=(2, 3))             # [B, I]
+            Wm = sr[kyi * m2 + fx] + 1j * si[kyi * m2 + fx]       # [I, O]
+            Y = X @ Wm                                            # [B, O]
+            c = 1.0 if fx == 0 else 2.0
+            out += c * np.real(Y[:, :, None, None] * np.conj(ph)[None, None]) / (H * W)
+    return out
+
+
+@pytest.mark.parametrize("H,W,m1,m2", [(8, 8, 2, 3), (6, 10, 3, 2), (9, 7, 1, 3)])
+def test_fno_oracle_spectral_layer_matches_brute_force_dft(H, W, m1, m2):
+    from oracle import fno_oracle as FO
+    rng = np.random.default_rng(H * 100 + W * 10 + m1)
+    B, I, Oc = 2, 3, 2
+    h = rng.standard_normal((B, I, H, W))
+    sr = rng.standard_normal((2 * m1 * m2, I, Oc))
+    si = rng.standard_normal((2 * m1 * m2, I, Oc))
+    got = FO.spectral_conv(torch.from_numpy(h), torch.from_numpy(sr), torch.from_numpy(si),
+                           m1, m2).numpy()
+    ref = _spectral_brute(h, sr, si, m1, m2)
+    assert np.abs(got - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+    # a transposed mode map (negative rows taken as H - kyi) must not pass
+    bad = _spectral_brute(h, sr[::-1].copy(), si[::-1].copy(), m1, m2)
+    assert np.abs(got - bad).max() > 1e-3
