@@ -55,25 +55,156 @@ METRIC = "Mcell-updates/s fwd+adjoint, %HBM roofline; designs/s batched vs plain
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fdtd3d_256_hbm_traffic.md")
 
 
-def _cpu_baseline(eps, solver_kw, steps):
-    """FP64 oracle fwd+adj over `steps` steps of the full grid (host cores)."""
-    from oracle import fdtd_oracle as O
+def _cpu_threads():
+    """Host threads for the CPU baseline: the process's CPU affinity, capped
+    by OMP_NUM_THREADS when that is set (the host share of this GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() and int(env) > 0 else n
 
-    shape = solver_kw["grid_shape"]
-    p = O.Params3D(shape, dl=solver_kw["dl"], pml_layers=solver_kw["pml_layers"],
-                   courant=solver_kw["courant"])
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_baseline(eps, solver_kw, fwd_steps, adj_steps):
+    """The reference's CPU path (torch FP64 eager, restated op for op in
+    oracle/torch_path.py) on the full grid, on the host's cores: forward
+    steps (one warm-up step first), then the time-reversal procedure
+    (detached forward with per-step E snapshots + the autograd recompute
+    backward) over ``adj_steps`` steps.  value: fwd+adj Mcell-updates/s."""
+    import torch
+
+    from oracle import torch_path as TP
+
+    threads = _cpu_threads()
+    torch.set_num_threads(threads)
+    shape = tuple(solver_kw["grid_shape"])
+    p = TP.RefPath3D(shape, dl=solver_kw["dl"], wavelength_nm=solver_kw["wavelength_nm"],
+                     pml_layers=solver_kw["pml_layers"], courant=solver_kw["courant"])
+    # the reference injects Ez through a 0/1 mask: the same yz sheet position
     src = {"type": "gaussian_pulse", "plane": "yz", "x": solver_kw["pml_layers"] + 4}
-    wave = O.waveform_table(steps, p.dt, p.omega, src)
-    mask = O.source_mask_3d(shape, src)
-    lam = [np.ones(shape) for _ in range(3)]
-    t0 = time.perf_counter()
-    O.adjoint3d(eps, p, steps, lam, 1, mask, wave)
-    t = time.perf_counter() - t0
+    mask = TP.source_mask(shape, src)
+    e = torch.from_numpy(np.ascontiguousarray(eps))
     n = int(np.prod(shape))
-    return {"value": round(n * steps / t / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"FP64 NumPy oracle fwd+adj, {steps} steps of the full {shape} grid, "
-                      f"{t:.1f} s"}
+    TP.time_reversal_forward(p, e, src, mask, 1)                 # warm-up
+    t0 = time.perf_counter()
+    TP.time_reversal_forward(p, e, src, mask, fwd_steps)
+    t_f = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    field, snaps = TP.time_reversal_forward(p, e, src, mask, adj_steps)
+    t_trf = time.perf_counter() - t0
+    del snaps
+    t0 = time.perf_counter()
+    TP.time_reversal_backward(p, e, src, mask, adj_steps, torch.ones_like(field))
+    t_trb = time.perf_counter() - t0
+    return {"value": round(n * adj_steps / (t_trf + t_trb) / 1e6, 3), "unit": "Mcell-updates/s",
+            "cores": threads, "kind": "port",
+            "fwd_mcells": round(n * fwd_steps / t_f / 1e6, 3), "T_cpu": adj_steps,
+            "cpu_model": _cpu_model(),
+            "sample": f"torch FP64 restatement of the reference CPU path on the full {shape} "
+                      f"grid: {fwd_steps} forward steps ({t_f:.1f} s), time-reversal forward "
+                      f"+ autograd-recompute backward over T_cpu = {adj_steps} steps "
+                      f"({t_trf + t_trb:.1f} s), {threads} threads"}
+
+
+def _cpu_baseline_c0(eps, solver_kw, src, line):
+    """The reference's 2D TM CPU path (oracle/torch_path.py) at full size
+    and full T: the forward with the DFT line, then plain autograd of
+    sum |DFT|^2 (the reference's 2D solver has no other gradient mode).
+    One thread: 200 x 200 arrays are too small for intra-op threads."""
+    import math
+
+    import torch
+
+    from oracle import fdtd_oracle as O
+    from oracle import torch_path as TP
+
+    torch.set_num_threads(1)
+    shape = tuple(solver_kw["grid_shape"])
+    T = solver_kw["n_steps"]
+    p = TP.RefPath2D(shape, dl=solver_kw["dl"], wavelength_nm=solver_kw["wavelength_nm"],
+                     pml_layers=solver_kw["pml_layers"], courant=solver_kw["courant"])
+    mask = torch.as_tensor(O.source_mask_2d(shape, src))
+    e = torch.from_numpy(np.ascontiguousarray(eps)).clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    _, (fr, fi) = TP.tm_run(p, e, src, mask, T, line=line,
+                            omega=2 * math.pi / solver_kw["wavelength_nm"])
+    t_f = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    (fr ** 2 + fi ** 2).sum().backward()
+    t_b = time.perf_counter() - t0
+    n = int(np.prod(shape))
+    return {"value": round(n * T / (t_f + t_b) / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "port", "fwd_mcells": round(n * T / t_f / 1e6, 3), "T_cpu": T,
+            "cpu_model": _cpu_model(),
+            "sample": f"torch FP64 restatement of the reference 2D TM CPU path, full {shape} "
+                      f"grid and {T} steps: forward with the DFT line {t_f:.1f} s, autograd "
+                      f"backward {t_b:.1f} s, 1 thread"}
+
+
+def _bench_c0(args):
+    """2D TM 200 x 200, 2000 steps: forward + adjoint of sum |DFT Ez|^2 on the
+    1 x 60 line (configuration 0).  The 640 KB state is L2-resident, so the
+    rate is reported in Mcell-updates/s only (no HBM roofline)."""
+    import torch
+
+    from diffnano_amd import FDTDSolver2D, workloads
+
+    dev = torch.device("cuda", 0)
+    eps_np, solver_kw, src, mons = workloads.config_c0()
+    solver = FDTDSolver2D(**solver_kw, device=dev)
+    eps0 = torch.tensor(eps_np, dtype=torch.float64, device=dev)
+
+    def one_step():
+        e = eps0.clone().requires_grad_(True)
+        F = solver.forward(e, source=src, monitors=mons).metadata["monitors"][0]
+        loss = (F.abs() ** 2).sum()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = one_step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    H, W = solver_kw["grid_shape"]
+    T = solver_kw["n_steps"]
+    y0, y1, x0, x1 = mons[0]["box"]
+    out = {"metric": "Mcell-updates/s fwd+adjoint (2D TM, L2-resident)",
+           "value": round(H * W * T / wall / 1e6, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded Si/SiO2 blob design)",
+           "config": {"workload": f"2D TM FDTD {H}x{W}, {T} steps, "
+                                  f"{solver_kw['pml_layers']}-cell CPML, fwd+adjoint of the "
+                                  f"1x60 DFT line", "grid": [H, W], "n_steps": T},
+           "loss": float(loss)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = _cpu_baseline_c0(eps_np, solver_kw, src, (y0, x0, x1))
+    print(json.dumps(out), flush=True)
+
+
+def _reference_loss(args, solver_kw):
+    """The reference's own value of this workload's loss (the committed
+    full-size golden of the 8000-step forward), when the run is that one."""
+    path = os.path.join(ROOT, "tests", "golden", f"{args.config}.npz")
+    if args.config not in ("c1", "c2") or not os.path.exists(path):
+        return None
+    g = np.load(path, allow_pickle=False)
+    meta = json.loads(str(g["meta"]))
+    if meta.get("T") != solver_kw["n_steps"]:
+        return None
+    return float(g["loss"])
 
 
 def _copy_bandwidth(torch, dev, mib=1024):
@@ -207,6 +338,40 @@ def _bench_batched(args):
         "shapes": sorted({str(p.shape) for p in probs_b}),
     }
     print(json.dumps(out), flush=True)
+
+
+def _designs_sample(n_designs, n_steps, iters, dev):
+    """Reduced, stated form of the batched configuration for the default
+    line: the seeded draw of ``n_designs`` designs from the four grid
+    classes, ``iters`` Adam iteration(s) of ``n_steps`` steps, batched (one
+    persistent launch per step over all designs, HIP-graph replay) and one
+    design at a time, default tiles.  Equal-FoM check: the FoMs of every
+    iteration and the updated densities of both loops."""
+    import torch
+
+    from diffnano_amd import inverse_design
+
+    res = {}
+    for mode in ("batched", "plain"):
+        probs = inverse_design.batched_problems(n_designs, seed=0)
+        inverse_design.adam_loop(probs[:2], iterations=1, n_steps=8, mode=mode, device=dev)
+        torch.cuda.synchronize()
+        res[mode] = inverse_design.adam_loop(probs, iterations=iters, n_steps=n_steps, mode=mode,
+                                             device=dev)
+    fb, fp = res["batched"]["fom"], res["plain"]["fom"]
+    drho = max(float((a - b).abs().max()) for a, b in zip(res["batched"]["rho"],
+                                                          res["plain"]["rho"]))
+    cells = sum(int(np.prod(p.shape)) for p in inverse_design.batched_problems(n_designs, seed=0))
+    b, pl = res["batched"]["design_iters_per_s"], res["plain"]["design_iters_per_s"]
+    return {"unit": "design-iterations/s", "batched": round(b, 4), "plain": round(pl, 4),
+            "batched_over_plain": round(b / pl, 3), "designs": n_designs, "iterations": iters,
+            "n_steps": n_steps, "total_cells": cells,
+            "batched_mcells_fwd_adj": round(cells * n_steps * iters / res["batched"]["wall_s"] / 1e6,
+                                            1),
+            "fom_max_rel_diff": float(np.max(np.abs(fb - fp) / np.abs(fp))),
+            "rho_max_abs_diff": drho,
+            "sample": f"{n_designs} designs drawn from 96^3, 128^3, 160x160x96, 192x128x128, "
+                      f"{iters} Adam iteration(s) x {n_steps} steps fwd+adj per mode"}
 
 
 def fno_flops_per_sample(m):
@@ -356,11 +521,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--tune", type=int, default=None,
                     help="c1/c2: autotune the tile shapes first (default: on for c1)")
     ap.add_argument("--n-steps", type=int, default=None, help="time steps (default: config's)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="CPU baseline: forward steps timed and T_cpu of the fwd+adj sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--designs-sample", type=int, default=48,
+                    help="c2: designs of the batched-vs-plain sub-record (0: skip)")
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--designs", type=int, default=48, help="c3: number of designs")
     ap.add_argument("--iters", type=int, default=None,
@@ -380,6 +549,9 @@ def main():
         args.iters = 10 if args.config == "c4" else 2
     if args.tune is None:
         args.tune = 1 if args.config == "c1" else 0
+    if args.config == "c0":
+        _bench_c0(args)
+        return
     if args.config == "c3":
         _bench_batched(args)
         return
@@ -469,7 +641,11 @@ def main():
     # measured bytes per launch exist for the 256^3 workload (the profiled one)
     traffic = _read_traffic().get(dom_k["kernel"]) if args.config == "c2" else None
 
-    cpu = _cpu_baseline(eps_np, solver_kw, args.cpu_steps)
+    designs = None
+    if args.config == "c2" and args.designs_sample > 0:
+        designs = _designs_sample(args.designs_sample, 3000, 1, dev)
+    cpu = None if args.no_cpu else _cpu_baseline(eps_np, solver_kw, args.cpu_steps,
+                                                 args.cpu_steps)
 
     out = {
         "metric": METRIC,
@@ -506,6 +682,11 @@ def main():
         "grad_l2": float(grad.norm().item()),
         "cpu_baseline": cpu,
     }
+    if designs is not None:
+        out["designs_per_s"] = designs
+    ref = _reference_loss(args, solver_kw)
+    if ref is not None:
+        out["loss_rel_err_vs_reference"] = abs(float(loss.item()) - ref) / abs(ref)
     print(json.dumps(out), flush=True)
 
 

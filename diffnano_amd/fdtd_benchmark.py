@@ -1,42 +1,45 @@
-"""FDTD benchmark suite and external cross-validation, drop-in for DiffNano's
-``diffnano/solvers/fdtd_benchmark.py``: the same result schemas, suite loop,
-summary table and JSON exchange format, driving ``FDTDSolver3D`` on the
-hand-written kernels.
+"""FDTD benchmark suite and external cross-validation for the MI355X solver.
 
-* ``BenchmarkConfig`` / ``BenchmarkResult`` (fdtd_benchmark.py:36-56).
-* ``FDTDBenchmarkSuite.run`` (:86-181): per grid size one seeded ε (1.5 + U[0, 1)
-  with a centred block of 4.0, seed 42, :66-73); per backward mode a solver
-  without CPML (courant 0.35, dl 20, 1550 nm), forward timed, ``field.sum()``
-  backpropagated and timed, peak memory, and the gradient's cosine against
-  the "autograd" mode's (None for that mode itself and for modes run before
-  it).  Mode "time_reversal" sets ``backward="time_reversal"``,
-  "checkpoint" sets ``use_checkpoint=True, checkpoint_segments=2``, any
-  other name runs the default.
-* ``summary_table`` (:183-209), ``_estimate_cpu_peak_mb`` (:212-221).
-* ``ExternalCrossValidator`` (:224-310): forward relative L2 error against
-  an external field, gradient cosine, the seeded test-case description and
-  the JSON loader (``{"field": {"shape", "data"}, "gradient": {...}, ...}``).
+Drop-in for the interface of DiffNano's ``diffnano/solvers/fdtd_benchmark.py``
+(the dataclasses ``BenchmarkConfig`` / ``BenchmarkResult`` at :36-56,
+``FDTDBenchmarkSuite.run`` / ``summary_table`` at :86-204 and
+``ExternalCrossValidator`` at :219-310): same class and method names,
+arguments, result fields and keys, the seed-42 test permittivity and the
+``{"shape": [...], "data": [...]}`` JSON exchange format.  The measurement
+itself is ``_harness.run_trial``: HIP-event phase times on the solver's
+stream and the allocator's memory peak (the kernels run asynchronously on
+the GPU, so host clocks without a synchronisation would time launches).
 
-Differences: the timed regions end with ``torch.cuda.synchronize()`` when a
-GPU is present (the kernels run asynchronously, so an unsynchronised clock
-would time the launch, not the solve); peak memory on the GPU counts
-torch's allocator (the solver's checkpoints and states are torch tensors).
-Every backward mode gives the same gradient here (the modes only choose
-the adjoint's checkpoint interval), so the cosines are 1 up to rounding.
+Semantics kept from the reference: every backward mode of a grid runs on
+the same seeded permittivity with ``field.sum()`` as the loss; mode
+"time_reversal" sets ``backward="time_reversal"``, "checkpoint" sets
+``use_checkpoint=True, checkpoint_segments=2``, any other name runs the
+solver's default; a result's gradient cosine is taken against the most
+recent "autograd" result of its grid, and is None for "autograd" itself and
+for modes listed before the first "autograd".  Here all modes return the
+same gradient (they only choose the adjoint's checkpoint interval), so the
+cosines are 1 up to rounding.
 """
 
 from __future__ import annotations
 
-import gc
 import json
-import time
 from dataclasses import dataclass, field as dc_field
 from pathlib import Path
 from typing import Any
 
 import torch
 
+from ._harness import agreement, resolve_device, run_trial, seeded_eps, storage_bytes
+
 __all__ = ["BenchmarkConfig", "BenchmarkResult", "FDTDBenchmarkSuite", "ExternalCrossValidator"]
+
+# backward-mode name -> solver keywords (other names: the solver's default)
+MODE_KWARGS: dict[str, dict[str, Any]] = {
+    "time_reversal": {"backward": "time_reversal"},
+    "checkpoint": {"use_checkpoint": True, "checkpoint_segments": 2},
+}
+REFERENCE_MODE = "autograd"
 
 
 @dataclass
@@ -60,171 +63,143 @@ class BenchmarkResult:
     device: str
 
 
-def _resolve_device(requested: str) -> tuple[str, str]:
-    if requested == "cuda":
-        if torch.cuda.is_available():
-            return "cuda", "GPU (ROCm)"
-        return "cpu", "CPU (GPU requested but not available)"
-    return "cpu", "CPU"
-
-
 def _make_eps_grid(grid_size: tuple[int, int, int], device: str) -> torch.Tensor:
-    """1.5 + U[0, 1) with a centred block of 4.0 (half of each extent), seed 42."""
-    D, H, W = grid_size
-    torch.manual_seed(42)
-    eps = 1.5 + torch.rand(D, H, W, dtype=torch.float64, device=device)
-    d, h, w = D // 4, H // 4, W // 4
-    if d > 0 and h > 0 and w > 0:
-        eps[D // 2 - d:D // 2 + d, H // 2 - h:H // 2 + h, W // 2 - w:W // 2 + w] = 4.0
-    return eps
+    return seeded_eps(grid_size, device)
 
 
-def _cosine_sim(a: torch.Tensor, b: torch.Tensor) -> float:
-    fa = a.flatten().to(torch.float64)
-    fb = b.flatten().to(torch.float64)
-    denom = (fa.norm() * fb.norm()).item()
-    if denom < 1e-30:
-        return 0.0
-    return torch.dot(fa, fb).item() / denom
+def model_peak_mb(grid_size: tuple[int, int, int], n_steps: int, extra_kw: dict) -> float:
+    """Device memory of one solve from the storage model (used where no
+    allocator peak exists), at the checkpoint interval the keywords select."""
+    if extra_kw.get("use_checkpoint") or extra_kw.get("backward") == "checkpoint":
+        K = -(-n_steps // max(1, extra_kw.get("checkpoint_segments", 4)))
+    else:
+        K = extra_kw.get("checkpoint_every", 64)
+    return sum(storage_bytes(grid_size, n_steps, K).values()) / 1e6
 
 
-def _sync():
-    if torch.cuda.is_available():
-        torch.cuda.synchronize()
+def _cosines(modes: list[str], grads: list[torch.Tensor]) -> list[float | None]:
+    """Cosine of every mode's gradient against the latest reference-mode
+    gradient listed before it (None for the reference mode itself and for
+    modes without an earlier reference)."""
+    out: list[float | None] = []
+    ref = None
+    for mode, g in zip(modes, grads):
+        if mode == REFERENCE_MODE:
+            ref = g
+            out.append(None)
+        else:
+            out.append(None if ref is None else round(agreement(g, ref).cosine, 6))
+    return out
 
 
-def _estimate_cpu_peak_mb(eps: torch.Tensor, n_steps: int,
-                          grid_size: tuple[int, int, int]) -> float:
-    """Six FP64 field arrays, times 4 when ε requires grad (:212-221)."""
-    D, H, W = grid_size
-    return 6 * D * H * W * 8 * (4 if eps.requires_grad else 1) / 1e6
+def _render_table(columns: list[str], rows: list[list[str]]) -> str:
+    """Right-aligned columns sized to their widest cell, ' | ' separated,
+    with a dashed rule under the header."""
+    widths = [max(len(c), *(len(r[i]) for r in rows)) for i, c in enumerate(columns)]
+    fmt = lambda cells: " | ".join(c.rjust(w) for c, w in zip(cells, widths))  # noqa: E731
+    head = fmt(columns)
+    return "\n".join([head, "-" * len(head)] + [fmt(r) for r in rows])
 
 
 class FDTDBenchmarkSuite:
-    """Forward/backward time, peak memory and gradient agreement of a 3D
-    FDTD class across grid sizes and backward modes."""
+    """Forward / backward time, device-memory peak and gradient agreement of
+    a 3D FDTD class over grid sizes and backward modes."""
+
+    COLUMNS = ["grid", "mode", "fwd_ms", "bwd_ms", "mem_MB", "cos_vs_AD", "device"]
 
     def __init__(self) -> None:
         self._results: list[BenchmarkResult] = []
 
     def run(self, fdtd3d_class: type, config: BenchmarkConfig) -> list[BenchmarkResult]:
-        device, _ = _resolve_device(config.device)
+        device = resolve_device(config.device)
+        modes = list(config.backward_modes)
         self._results = []
-        for grid_size in config.grid_sizes:
-            eps_template = _make_eps_grid(grid_size, device)
-            reference_grad: torch.Tensor | None = None
-            for mode in config.backward_modes:
-                gc.collect()
-                if device == "cuda":
-                    torch.cuda.empty_cache()
-                    torch.cuda.reset_peak_memory_stats()
-                    mem_before = torch.cuda.memory_allocated()
-                else:
-                    mem_before = 0
-                kw: dict[str, Any] = {}
-                if mode == "time_reversal":
-                    kw["backward"] = "time_reversal"
-                elif mode == "checkpoint":
-                    kw.update(use_checkpoint=True, checkpoint_segments=2)
-                solver = fdtd3d_class(grid_shape=grid_size, dl=20.0, wavelength_nm=1550.0,
-                                      pml_layers=0, n_steps=config.n_time_steps,
-                                      device=device, courant=0.35, **kw)
-                eps = eps_template.clone().detach().requires_grad_(True)
-
-                _sync()
-                t0 = time.perf_counter()
-                result = solver.forward(eps)
-                _sync()
-                fwd_ms = (time.perf_counter() - t0) * 1e3
-                loss = result.field.sum()
-                t0 = time.perf_counter()
-                loss.backward()
-                _sync()
-                bwd_ms = (time.perf_counter() - t0) * 1e3
-
-                grad = eps.grad.detach().clone() if eps.grad is not None else torch.zeros_like(eps)
-                if device == "cuda":
-                    peak_mb = (torch.cuda.max_memory_allocated() - mem_before) / 1e6
-                else:
-                    peak_mb = _estimate_cpu_peak_mb(eps, config.n_time_steps, grid_size)
-                cos = None
-                if mode == "autograd":
-                    reference_grad = grad
-                elif reference_grad is not None:
-                    cos = _cosine_sim(reference_grad, grad)
+        for grid in config.grid_sizes:
+            eps = seeded_eps(grid, device)
+            trials = [run_trial(fdtd3d_class, grid, config.n_time_steps, device,
+                                MODE_KWARGS.get(m, {}), eps) for m in modes]
+            for mode, t, cos in zip(modes, trials, _cosines(modes, [t.gradient for t in trials])):
+                peak = (t.peak_bytes / 1e6 if t.peak_bytes is not None else
+                        model_peak_mb(grid, config.n_time_steps, MODE_KWARGS.get(mode, {})))
                 self._results.append(BenchmarkResult(
-                    grid_size=grid_size, backward_mode=mode,
-                    forward_time_ms=round(fwd_ms, 3), backward_time_ms=round(bwd_ms, 3),
-                    peak_memory_mb=round(peak_mb, 3),
-                    gradient_cosine_vs_autograd=None if cos is None else round(cos, 6),
+                    grid_size=grid, backward_mode=mode,
+                    forward_time_ms=round(t.forward_ms, 3),
+                    backward_time_ms=round(t.backward_ms, 3),
+                    peak_memory_mb=round(peak, 3), gradient_cosine_vs_autograd=cos,
                     device=device))
         return self._results
 
     def summary_table(self) -> str:
         if not self._results:
             return "No benchmark results."
-        header = (f"{'grid':>12s} | {'mode':>14s} | {'fwd_ms':>8s} | {'bwd_ms':>8s} | "
-                  f"{'mem_MB':>8s} | {'cos_vs_AD':>10s} | {'device':>6s}")
-        lines = [header, "-" * len(header)]
-        for r in self._results:
-            g = "x".join(str(v) for v in r.grid_size)
-            cos = ("ref" if r.gradient_cosine_vs_autograd is None
-                   else f"{r.gradient_cosine_vs_autograd:.4f}")
-            lines.append(f"{g:>12s} | {r.backward_mode:>14s} | {r.forward_time_ms:8.1f} | "
-                         f"{r.backward_time_ms:8.1f} | {r.peak_memory_mb:8.2f} | "
-                         f"{cos:>10s} | {r.device:>6s}")
-        return "\n".join(lines)
+        rows = [["x".join(map(str, r.grid_size)), r.backward_mode, f"{r.forward_time_ms:.1f}",
+                 f"{r.backward_time_ms:.1f}", f"{r.peak_memory_mb:.2f}",
+                 "ref" if r.gradient_cosine_vs_autograd is None
+                 else f"{r.gradient_cosine_vs_autograd:.4f}", r.device]
+                for r in self._results]
+        return _render_table(self.COLUMNS, rows)
+
+
+def _decode_array(key: str, entry: Any) -> torch.Tensor:
+    """One ``{"shape": [...], "data": [...]}`` entry as a float64 tensor
+    (``data`` flat in row-major order, or already nested)."""
+    if not isinstance(entry, dict) or "shape" not in entry or "data" not in entry:
+        raise ValueError(f"external results: {key!r} must be an object with 'shape' and 'data'")
+    shape = tuple(int(v) for v in entry["shape"])
+    t = torch.tensor(entry["data"], dtype=torch.float64).reshape(-1)
+    if t.numel() != (1 if not shape else int(torch.tensor(shape).prod())):
+        raise ValueError(f"external results: {key!r} holds {t.numel()} values for shape {shape}")
+    return t.reshape(shape)
 
 
 class ExternalCrossValidator:
-    """Compare this solver's outputs with results computed elsewhere."""
+    """Compare this solver's field and gradient with results computed by an
+    external solver and exchanged as JSON."""
+
+    ARRAY_KEYS = ("field", "gradient")
 
     def validate_forward(self, our_result: torch.Tensor, external_result: torch.Tensor,
                          rtol: float = 1e-3) -> dict[str, Any]:
-        ours = our_result.detach().to(torch.float64)
-        ext = external_result.detach().to(torch.float64).to(ours.device)
-        diff = ours - ext
-        rel = diff.norm().item() / max(ext.norm().item(), 1e-30)
-        return {"relative_error": rel, "max_absolute_error": diff.abs().max().item(),
-                "passed": rel < rtol}
+        a = agreement(our_result, external_result)
+        return {"relative_error": a.relative_error, "max_absolute_error": a.max_absolute_error,
+                "passed": a.relative_error < rtol}
 
     def validate_gradient(self, our_grad: torch.Tensor, external_grad: torch.Tensor,
                           cosine_threshold: float = 0.99) -> dict[str, Any]:
-        ours = our_grad.detach().to(torch.float64)
-        ext = external_grad.detach().to(torch.float64).to(ours.device)
-        cos = _cosine_sim(ours, ext)
-        rel = (ours - ext).norm().item() / max(ext.norm().item(), 1e-30)
-        return {"cosine_similarity": cos, "relative_error": rel,
-                "passed": cos >= cosine_threshold}
+        a = agreement(our_grad, external_grad)
+        return {"cosine_similarity": a.cosine, "relative_error": a.relative_error,
+                "passed": a.cosine >= cosine_threshold}
 
     def generate_test_case(self, grid_size: tuple[int, int, int] = (16, 16, 16),
                            source_pos: tuple[int, int, int] | None = None,
                            freq: float = 1.0) -> dict[str, Any]:
-        """The case description an external solver is run on (seed 42)."""
-        D, H, W = grid_size
-        if source_pos is None:
-            source_pos = (D // 2, H // 2, W // 2)
+        """The case an external solver is run on: torch seed 42,
+        eps = 1.5 + U[0, 1) (no block), a unit Gaussian-pulse point source
+        (default: the centre cell), 50 steps, dl 20 nm, Courant 0.35."""
         seed = 42
         torch.manual_seed(seed)
-        eps = 1.5 + torch.rand(D, H, W, dtype=torch.float64)
-        return {
-            "grid_size": grid_size, "source_pos": source_pos, "freq": freq, "eps": eps,
-            "source_config": {"type": "gaussian_pulse", "pos": list(source_pos),
-                              "amplitude": 1.0},
-            "n_steps": 50, "dl": 20.0, "wavelength_nm": 1550.0, "courant": 0.35,
-            "seed": seed,
-        }
+        eps = 1.5 + torch.rand(*grid_size, dtype=torch.float64)
+        pos = tuple(source_pos) if source_pos is not None else tuple(n // 2 for n in grid_size)
+        return {"grid_size": grid_size, "source_pos": pos, "freq": freq, "eps": eps,
+                "source_config": {"type": "gaussian_pulse", "pos": list(pos), "amplitude": 1.0},
+                "n_steps": 50, "dl": 20.0, "wavelength_nm": 1550.0, "courant": 0.35,
+                "seed": seed}
 
     @staticmethod
     def load_external_results(path: str) -> dict[str, Any]:
-        """JSON with ``field`` / ``gradient`` as ``{"shape": [...], "data": [...]}``
-        (loaded as float64 tensors) and any other keys passed through."""
-        with open(Path(path)) as f:
-            raw = json.load(f)
-        out: dict[str, Any] = {}
-        for key in ("field", "gradient"):
-            if key in raw:
-                out[key] = torch.tensor(raw[key]["data"], dtype=torch.float64).reshape(
-                    tuple(raw[key]["shape"]))
-        out.update({k: v for k, v in raw.items() if k not in ("field", "gradient")})
-        return out
+        """JSON whose ``field`` / ``gradient`` are ``{"shape", "data"}``
+        objects (returned as float64 tensors); other keys pass through."""
+        raw = json.loads(Path(path).read_text())
+        return {k: (_decode_array(k, v) if k in ExternalCrossValidator.ARRAY_KEYS else v)
+                for k, v in raw.items()}
+
+    @staticmethod
+    def save_external_results(path: str, **entries: Any) -> None:
+        """Write results in the format ``load_external_results`` reads:
+        tensors and arrays become ``{"shape", "data"}`` objects."""
+        def enc(v):
+            if isinstance(v, torch.Tensor) or hasattr(v, "shape"):
+                t = torch.as_tensor(v, dtype=torch.float64).detach().cpu()
+                return {"shape": list(t.shape), "data": t.reshape(-1).tolist()}
+            return v
+        Path(path).write_text(json.dumps({k: enc(v) for k, v in entries.items()}))

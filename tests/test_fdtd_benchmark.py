@@ -7,9 +7,9 @@ import pytest
 import torch
 
 from diffnano_amd._result import SimResult
+from diffnano_amd._harness import agreement, storage_bytes
 from diffnano_amd.fdtd_benchmark import (BenchmarkConfig, BenchmarkResult, ExternalCrossValidator,
-                                         FDTDBenchmarkSuite, _estimate_cpu_peak_mb,
-                                         _make_eps_grid)
+                                         FDTDBenchmarkSuite, _make_eps_grid, model_peak_mb)
 
 
 def test_config_defaults_and_eps_grid():
@@ -25,10 +25,27 @@ def test_config_defaults_and_eps_grid():
     assert torch.equal(e, _make_eps_grid((8, 12, 16), "cpu"))     # seeded
 
 
-def test_cpu_peak_estimate_hand_value():
-    eps = torch.zeros(16, 16, 16, requires_grad=True)
-    assert _estimate_cpu_peak_mb(eps, 100, (16, 16, 16)) == pytest.approx(0.786432)
-    assert _estimate_cpu_peak_mb(eps.detach(), 100, (16, 16, 16)) == pytest.approx(0.196608)
+def test_storage_model_hand_values():
+    N = 16 ** 3
+    s = storage_bytes((16, 16, 16), 100, 64)       # float32, 3D
+    assert s["eps_and_coefficient"] == N * 12 and s["state"] == 2 * 6 * N * 4
+    assert s["checkpoints"] == 2 * 6 * N * 4       # ceil(100 / 64) = 2 slots
+    assert s["replay_store"] == 63 * 3 * N * 4 and s["adjoint"] == 25 * N * 4
+    assert s["gradient"] == 8 * N and s["field"] == 24 * N
+    s1 = storage_bytes((4, 5, 6), 7, 1, elem_bytes=8, n_components=3)   # float64, 2D-style
+    assert s1["checkpoints"] == 7 * 3 * 120 * 8 and s1["replay_store"] == 120 * 8
+    assert storage_bytes((4, 4, 4), 0, 8)["checkpoints"] == 0
+    # "checkpoint" mode: K = ceil(T / segments); others: checkpoint_every (default 64)
+    assert model_peak_mb((16, 16, 16), 100, {"use_checkpoint": True, "checkpoint_segments": 2}) \
+        == pytest.approx(sum(storage_bytes((16, 16, 16), 100, 50).values()) / 1e6)
+    assert model_peak_mb((16, 16, 16), 100, {}) == pytest.approx(sum(s.values()) / 1e6)
+
+
+def test_agreement_hand_values():
+    a = agreement(torch.tensor([3.0, 4.5]), torch.tensor([3.0, 4.0]))
+    assert a.relative_error == pytest.approx(0.1) and a.max_absolute_error == 0.5
+    assert agreement(torch.tensor([0.0, 1.0]), torch.tensor([1.0, 0.0])).cosine == 0.0
+    assert agreement(torch.zeros(3), torch.zeros(3)).cosine == 0.0
 
 
 class ToySolver:
@@ -61,21 +78,30 @@ def test_suite_loop_modes_and_cosines():
     assert res[1].gradient_cosine_vs_autograd is None
     assert res[2].gradient_cosine_vs_autograd == 1.0
     assert all(r.device == "cpu" and r.forward_time_ms >= 0 for r in res)
-    assert res[0].peak_memory_mb == pytest.approx(6 * 64 * 8 * 4 / 1e6, abs=1e-3)
+    assert res[0].peak_memory_mb == pytest.approx(
+        round(model_peak_mb((4, 4, 4), 3, {"backward": "time_reversal"}), 3))
     lines = suite.summary_table().splitlines()
     assert len(lines) == 2 + len(res)
     assert "4x4x4" in lines[2] and lines[2].rstrip().endswith("cpu") and "ref" in lines[2]
     assert "1.0000" in lines[4]
+    # the latest "autograd" result is the reference of the modes after it
+    cfg2 = BenchmarkConfig(grid_sizes=[(4, 4, 4)], n_time_steps=3,
+                           backward_modes=["autograd", "checkpoint", "autograd", "x"])
+    r2 = FDTDBenchmarkSuite().run(ToySolver, cfg2)
+    assert [r.gradient_cosine_vs_autograd for r in r2] == [None, 1.0, None, 1.0]
 
 
 def test_summary_table_empty_and_format():
     s = FDTDBenchmarkSuite()
     assert s.summary_table() == "No benchmark results."
     s._results = [BenchmarkResult((16, 16, 16), "checkpoint", 12.345, 67.891, 1.5, 0.999876,
-                                  "cuda")]
-    row = s.summary_table().splitlines()[2]
-    assert row == ("    16x16x16 |     checkpoint |     12.3 |     67.9 |     1.50 |"
-                   "     0.9999 |   cuda")
+                                  "cuda"),
+                  BenchmarkResult((8, 8, 8), "autograd", 1234.5, 2.0, 10.0, None, "cpu")]
+    head, rule, row, row2 = s.summary_table().splitlines()
+    assert head == "    grid |       mode | fwd_ms | bwd_ms | mem_MB | cos_vs_AD | device"
+    assert rule == "-" * len(head)
+    assert row == "16x16x16 | checkpoint |   12.3 |   67.9 |   1.50 |    0.9999 |   cuda"
+    assert row2 == "   8x8x8 |   autograd | 1234.5 |    2.0 |  10.00 |       ref |    cpu"
 
 
 def test_validate_forward_and_gradient_hand_values():
@@ -107,6 +133,19 @@ def test_test_case_and_json_round_trip(tmp_path):
     assert out["field"].dtype == torch.float64 and out["field"].shape == (2, 3)
     assert out["field"][1, 2] == 6.0 and out["gradient"][2] == 0.125
     assert out["solver"] == "external" and out["n_steps"] == 50
+    # writer -> reader round trip, and malformed entries are rejected
+    q = tmp_path / "ours.json"
+    f = torch.arange(24, dtype=torch.float64).reshape(2, 3, 4) / 7
+    ExternalCrossValidator.save_external_results(str(q), field=f, gradient=f[0], tag="x")
+    back = ExternalCrossValidator.load_external_results(str(q))
+    assert torch.equal(back["field"], f) and torch.equal(back["gradient"], f[0])
+    assert back["tag"] == "x"
+    p.write_text(json.dumps({"field": {"shape": [2, 2], "data": [1, 2, 3]}}))
+    with pytest.raises(ValueError, match="3 values for shape"):
+        ExternalCrossValidator.load_external_results(str(p))
+    p.write_text(json.dumps({"gradient": [1, 2]}))
+    with pytest.raises(ValueError, match="'shape' and 'data'"):
+        ExternalCrossValidator.load_external_results(str(p))
 
 
 # ---------------------------------------------------------------------------
@@ -124,11 +163,10 @@ def test_gpu_schemas_and_memory_model():
     r = G.StabilityReport()
     assert r.forward_errors == [] and r.memory_used_mb == {} and r.is_valid is False
     S = G.GPUMemoryStrategy
-    cells = 8 * 8 * 8
-    assert G._estimate_peak_mb((8, 8, 8), 10, S.FULL_AUTODIFF, "cpu") == 10 * 6 * 10 * cells * 8 / 1e6
-    assert G._estimate_peak_mb((8, 8, 8), 10, S.TIME_REVERSAL, "cpu") == 3 * 10 * cells * 8 / 1e6
-    assert G._estimate_peak_mb((8, 8, 8), 10, S.CHECKPOINT, "cpu") == 4 * 6 * cells * 8 / 1e6
-    assert G._estimate_peak_mb((8, 8, 8), 10, S.CHECKPOINT, "cuda", 2.5e6) == 2.5
+    tot = lambda K: sum(storage_bytes((8, 8, 8), 10, K).values()) / 1e6  # noqa: E731
+    assert G.storage_mb((8, 8, 8), 10, S.FULL_AUTODIFF) == pytest.approx(tot(64))
+    assert G.storage_mb((8, 8, 8), 10, S.TIME_REVERSAL) == pytest.approx(tot(64))
+    assert G.storage_mb((8, 8, 8), 10, S.CHECKPOINT) == pytest.approx(tot(3))   # ceil(10/4)
 
 
 def _stub(grads):
@@ -202,9 +240,47 @@ def test_fdtdx_cross_validation_logic(monkeypatch):
     rep = v.run_cross_validation(n_seeds=2, grid_size=(4, 4, 4), n_steps=10)
     assert rep.forward_errors == [0.0, 0.0] and rep.gradient_cosines == [pytest.approx(1.0)] * 2
     assert rep.is_valid
-    assert rep.memory_used_mb == {"full_autodiff": 10 * 6 * 10 * 64 * 8 / 1e6}
+    if not torch.cuda.is_available():
+        assert rep.memory_used_mb == {"full_autodiff": round(
+            G.storage_mb((4, 4, 4), 10, G.GPUMemoryStrategy.FULL_AUTODIFF), 3)}
     # an external reference 1 % off in the field fails the 1e-3 forward bar
     off = G.FDTDXCrossValidator(lambda g, n: {"field": 1.01 * ref["field"],
                                               "gradient": ref["gradient"]})
     rep = off.run_cross_validation(n_seeds=1, grid_size=(4, 4, 4), n_steps=10)
     assert rep.forward_errors[0] == pytest.approx(1 / 101) and not rep.is_valid
+
+
+# ---------------------------------------------------------------------------
+# MultiScaleMetalens host logic (tiling, coupling bands, tile boundary)
+# ---------------------------------------------------------------------------
+
+
+def test_multiscale_tiles_padding_and_order():
+    m = G.MultiScaleMetalens((6, 6, 6), (4, 4, 4), (2, 1, 2))
+    d = torch.arange(6 * 5 * 7, dtype=torch.float64).reshape(6, 5, 7) + 2.0
+    t = m.tiles(d)
+    assert [o for o, _ in t] == [(0, 0, 0), (0, 0, 4), (4, 0, 0), (4, 0, 4)]
+    o, last = t[3]
+    assert torch.equal(last[:2, :4, :3], d[4:6, 0:4, 4:7])
+    assert torch.all(last[2:] == 1.0) and torch.all(last[:, :, 3:] == 1.0)
+    assert torch.equal(t[0][1], d[:4, :4, :4])
+    with pytest.raises(ValueError, match="coupling"):
+        G.MultiScaleMetalens((6, 6, 6), (4, 4, 4), (1, 1, 1), coupling="edges")
+
+
+def test_multiscale_coupling_bands():
+    m = G.MultiScaleMetalens((4, 4, 4), (12, 12, 12), (1, 1, 1), fine_pml=2)
+    fine = torch.zeros(3, 12, 12, 12, dtype=torch.float64)
+    bnd = torch.ones(3, 4, 4, 4, dtype=torch.float64) * 7.0
+    out = m._couple(fine, bnd)
+    assert torch.all(out[:, :2] == 7.0) and torch.all(out[:, -2:] == 7.0)
+    assert torch.all(out[:, :, :2] == 7.0) and torch.all(out[:, :, -2:] == 7.0)
+    assert torch.all(out[:, 2:-2, 2:-2] == 0.0)          # axis 2 bands are not coupled
+    assert torch.all(fine == 0.0)                         # the input is not modified
+    same = torch.rand(3, 12, 12, 12, dtype=torch.float64)
+    assert m._couple(fine, same) is fine                  # equal shapes: no coupling
+    # the tile's share of the coarse domain: tile (6..12) of a 12-cell axis on 4 coarse cells
+    m2 = G.MultiScaleMetalens((4, 4, 4), (6, 6, 6), (2, 2, 2))
+    c = torch.arange(3 * 64, dtype=torch.float64).reshape(3, 4, 4, 4)
+    sub = m2._tile_boundary(c, (6, 0, 6), (12, 12, 12))
+    assert torch.equal(sub, c[:, 2:4, 0:2, 2:4])

@@ -414,3 +414,79 @@ def test_fno_oracle_spectral_layer_matches_brute_force_dft(H, W, m1, m2):
     # a transposed mode map (negative rows taken as H - kyi) must not pass
     bad = _spectral_brute(h, sr[::-1].copy(), si[::-1].copy(), m1, m2)
     assert np.abs(got - bad).max() > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# The torch restatement of the reference's CPU path (the CPU baseline)
+# ---------------------------------------------------------------------------
+
+
+def test_torch_path_step_matches_reference_step_golden():
+    from oracle import torch_path as TP
+    g = golden("step3d.npz")
+    p = TP.RefPath3D((10, 10, 10), dl=20.0, pml_layers=3, courant=0.4)
+    f = [torch.from_numpy(a) for a in g["fin"]]
+    eps = torch.from_numpy(g["eps"])
+    out = TP.ref_time_step(p, *f, eps, torch.ones_like(eps))
+    assert rel_l2(np.stack([o.numpy() for o in out]), g["fout"]) < 1e-15
+
+
+@pytest.mark.parametrize("name,src", [("point", {"type": "gaussian_pulse", "pos": [6, 5, 4],
+                                                 "amplitude": 2.0}),
+                                      ("xy", {"type": "gaussian_pulse", "plane": "xy", "z": 3}),
+                                      ("yz", {"type": "continuous", "plane": "yz", "x": 2}),
+                                      ("line", {"type": "gaussian_pulse"})])
+def test_torch_path_forward_matches_reference_golden(name, src):
+    from oracle import torch_path as TP
+    g = golden("fwd3d.npz")
+    p = TP.RefPath3D((12, 12, 12), dl=20.0, pml_layers=3, courant=0.4)
+    field, snaps = TP.time_reversal_forward(p, torch.from_numpy(g["eps"]), src,
+                                            TP.source_mask((12, 12, 12), src), 60)
+    assert rel_l2(field.numpy(), g[f"field_{name}"]) < 1e-14
+    assert len(snaps[0]) == 60 and len(snaps[3]) == 60
+    assert torch.equal(snaps[2][-1], field[0])
+
+
+@pytest.mark.parametrize("tag,N,L,T,cour", [("small", 8, 0, 6, 0.35), ("pml", 10, 2, 24, 0.4)])
+def test_torch_path_time_reversal_gradient_matches_reference_golden(tag, N, L, T, cour):
+    from oracle import torch_path as TP
+    g = golden("grad3d.npz")
+    p = TP.RefPath3D((N, N, N), dl=20.0, pml_layers=L, courant=cour)
+    src = {"type": "gaussian_pulse"}
+    eps = torch.from_numpy(g[f"{tag}_eps"])
+    mask = TP.source_mask((N, N, N), src)
+    field, _ = TP.time_reversal_forward(p, eps, src, mask, T)
+    assert rel_l2(field.numpy(), g[f"{tag}_field"]) < 1e-14
+    grad = TP.time_reversal_backward(p, eps, src, mask, T, torch.ones_like(field))
+    assert rel_l2(grad.numpy(), g[f"{tag}_grad_sum"]) < 1e-13
+    grad2 = TP.time_reversal_backward(p, eps, src, mask, T, 2 * field)
+    assert rel_l2(grad2.numpy(), g[f"{tag}_grad_energy"]) < 1e-13
+
+
+def test_torch_path_tm_matches_reference_goldens():
+    from oracle import fdtd_oracle as O
+    from oracle import torch_path as TP
+    g = golden("fdtd2d.npz")
+    p = TP.RefPath2D((24, 20), dl=20.0, pml_layers=4, courant=0.5)
+    for name, src in (("point", {"type": "gaussian_pulse", "pos": [9, 7]}),
+                      ("cw", {"type": "continuous"})):
+        mask = torch.as_tensor(O.source_mask_2d((24, 20), src))
+        eps = torch.from_numpy(g["eps"]).clone().requires_grad_(True)
+        Ez, _ = TP.tm_run(p, eps, src, mask, 80)
+        assert rel_l2(Ez.detach().numpy(), g[f"TM_{name}_field"][0]) < 1e-14
+        (Ez ** 2).sum().backward()
+        assert rel_l2(eps.grad.numpy(), g[f"TM_{name}_grad"]) < 1e-13
+    # the full 200 x 200 / 2000-step case: final Ez and the 1550 nm DFT line
+    c = golden("c0.npz")
+    meta = json.loads(str(c["meta"]))
+    p0 = TP.RefPath2D((200, 200), dl=20.0, pml_layers=10, courant=0.5)
+    mask = torch.as_tensor(O.source_mask_2d((200, 200), meta["src"]))
+    nt = torch.get_num_threads()
+    torch.set_num_threads(1)       # 200 x 200 arrays: threads only add overhead
+    try:
+        Ez, (fr, fi) = TP.tm_run(p0, torch.from_numpy(c["eps"]), meta["src"], mask, 2000,
+                                 line=(160, 70, 130), omega=2 * math.pi / 1550.0)
+    finally:
+        torch.set_num_threads(nt)
+    assert rel_l2(Ez.numpy(), c["Ez"]) < 1e-13
+    assert rel_l2(fr.numpy(), c["Fr"]) < 1e-13 and rel_l2(fi.numpy(), c["Fi"]) < 1e-13
