@@ -93,6 +93,10 @@ _SIGS = {
     "dnx_fdtd3d_adjoint_workspace_bytes": ([C.POINTER(Grid3D), C.POINTER(Monitor), _I], _L),
     "dnx_fdtd3d_adjoint_run": ([C.POINTER(Grid3D), _P, _P, _I, _I, _P, _P, C.POINTER(Source), _I,
                                 C.POINTER(Monitor), _I, _P, _P, _L, _P, C.POINTER(_I)], _I),
+    "dnx_fdtd3d_adjoint_static_scratch_bytes": ([], _L),
+    "dnx_fdtd3d_adjoint_run_noalloc": ([C.POINTER(Grid3D), _P, _P, _I, _I, _P, _P,
+                                        C.POINTER(Source), _I, C.POINTER(Monitor), _I, _P, _P,
+                                        C.POINTER(_I)], _I),
     "dnx_fdtd3d_set_tiles": ([_I, _I, _I, _I], _I),
     "dnx_fdtd3d_get_tiles": ([_I, _I, C.POINTER(_I), C.POINTER(_I)], _I),
     "dnx_fdtd2d_tm_batch": ([C.POINTER(Grid2D), _I, _P, _I, C.POINTER(Source), C.POINTER(Box),

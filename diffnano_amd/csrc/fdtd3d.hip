@@ -19,6 +19,7 @@
 // so every wave's loads are 256-byte coalesced rows.
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "fdtd3d_fused.h"
 
@@ -734,4 +735,65 @@ extern "C" int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* 
                           ws_bytes, s, &fin);
   if (final_in_work) *final_in_work = fin;
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint sweep without caller workspace: the two-step kernels' monitor
+// adjoint terms live in a fixed device array of this library
+// ---------------------------------------------------------------------------
+
+// Every byte of `scratch` is written by k_monitor_d before a two-step
+// adjoint launch reads it, so nothing has to start at zero.
+static constexpr int64_t kAdjScratchBytes = int64_t(4) << 20;  // 4 MiB
+static __device__ __align__(256) unsigned char scratch[kAdjScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).  The array's name
+// is only a host-side stand-in: kernels receive this address.
+int scratch_address(void** out) {
+  int dev = -1;
+  DNX_CHECK_HIP(hipGetDevice(&dev));
+  DNX_REQUIRE(dev >= 0 && dev < kMaxDevices, "adjoint scratch: device %d beyond %d", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    DNX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DNX_REQUIRE(p != nullptr, "adjoint scratch: no device address on device %d", dev);
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return DNX_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t dnx_fdtd3d_adjoint_static_scratch_bytes(void) { return kAdjScratchBytes; }
+
+extern "C" int32_t dnx_fdtd3d_adjoint_run_noalloc(const dnx_grid3d* g, void* lam, void* work,
+                                                  int32_t m_hi, int32_t m_lo, const void* e_lo,
+                                                  const void* estore, const dnx_source* srcs,
+                                                  int32_t n_src, const dnx_monitor* mons,
+                                                  int32_t n_mon, void* S, void* stream,
+                                                  int32_t* final_in_work) {
+  int rc = check_grid3(g);
+  if (rc) return rc;
+  DNX_REQUIRE(n_mon >= 0 && (n_mon == 0 || mons != nullptr), "adjoint_run_noalloc: monitors");
+  const int64_t need = adj3_workspace_bytes(g, mons, n_mon);
+  if (need > kAdjScratchBytes) {
+    set_error("fdtd3d_adjoint_run_noalloc: the monitors need %lld bytes of scratch, beyond the "
+              "library's fixed %lld (use dnx_fdtd3d_adjoint_run with a workspace)",
+              (long long)need, (long long)kAdjScratchBytes);
+    return DNX_ERANGE;
+  }
+  void* ws = nullptr;
+  if (need > 0 && (rc = scratch_address(&ws))) return rc;
+  return dnx_fdtd3d_adjoint_run(g, lam, work, m_hi, m_lo, e_lo, estore, srcs, n_src, mons, n_mon,
+                                S, ws, need, stream, final_in_work);
 }

@@ -207,6 +207,24 @@ int32_t dnx_fdtd3d_adjoint_run(const dnx_grid3d* g, void* lam, void* work, int32
                                void* workspace, int64_t ws_bytes, void* stream,
                                int32_t* final_in_work);
 
+/* dnx_fdtd3d_adjoint_run without a workspace argument.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * The scratch the two-step kernels need (the monitor adjoint terms, the
+ * bytes dnx_fdtd3d_adjoint_workspace_bytes reports) is a fixed device array
+ * of this library, dnx_fdtd3d_adjoint_static_scratch_bytes() = 4 MiB per
+ * device, held until the process ends; monitors needing more are rejected
+ * with DNX_ERANGE.  Calls on one device share that array, so calls that
+ * overlap on one device (other streams, other host threads) race; nothing
+ * in it has to start at zero.  Everything else as dnx_fdtd3d_adjoint_run;
+ * no hipMalloc, hipFree or allocating library routine is called. */
+int64_t dnx_fdtd3d_adjoint_static_scratch_bytes(void);
+int32_t dnx_fdtd3d_adjoint_run_noalloc(const dnx_grid3d* g, void* lam, void* work, int32_t m_hi,
+                                       int32_t m_lo, const void* e_lo, const void* estore,
+                                       const dnx_source* srcs, int32_t n_src,
+                                       const dnx_monitor* mons, int32_t n_mon, void* S,
+                                       void* stream, int32_t* final_in_work);
+
 /* Tile shape of the fused 3D kernels for one dtype and kernel (0 forward/
  * replay, 1 adjoint, 2 two-step forward/replay, 3 two-step adjoint; the
  * two-step kernels run for float32 only): ty rows (2, 4 or 8; 2 or 4 for
