@@ -107,7 +107,7 @@ _SIGS = {
     "dnx_fno_mode_mix": ([_P, _P, _P, _P, _I, _I, _I, _L, _I, _I, _P], _I),
     "dnx_fno_mode_wgrad": ([_P, _P, _P, _P, _I, _I, _L, _I, _I, _P], _I),
     "dnx_fno_wgrad": ([_P, _P, _P, _P, _I, _I, _L, _I, _P], _I),
-    "dnx_batch3d_workspace_bytes": ([_I], _L),
+    "dnx_batch3d_workspace_bytes": ([C.POINTER(Design3D), _I], _L),
     "dnx_batch3d_side_elems": ([C.POINTER(Design3D), _I], _L),
     "dnx_batch3d_create": ([C.POINTER(Design3D), _I, _I, _I, C.POINTER(Pools3D), _P, _L, _I, _P,
                             C.POINTER(_P)], _I),

@@ -251,7 +251,7 @@ def tune_batch(shapes, dtype=torch.float32, device=None, steps=8, force=False, a
                     for k in (0, 1):
                         set_tiles(dtype, k, *picked[k][0])
                     for k in (2, 3):
-                        for ty in TY2_BATCH:
+                        for ty in (TY2_BATCH if k == 2 else (2,)):
                             for zc in ZC_CHOICES:
                                 set_tiles(dtype, k, ty, zc)
                                 tf, tb = measure(0)

@@ -132,7 +132,9 @@ class BatchedFDTD3D:
         self.side = torch.empty(max(side_n, 1), dtype=dt, device=dev)
         pools.side = self.side.data_ptr()
         self._pools = pools
-        nbytes = lib.dnx_batch3d_workspace_bytes(self.n)
+        nbytes = lib.dnx_batch3d_workspace_bytes(arr, self.n)
+        if nbytes < 0:
+            raise RuntimeError("libdnx_fdtd: batched workspace query failed")
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
         plan = C.c_void_p()
         L.check(lib.dnx_batch3d_create(arr, self.n, self.n_steps, self.K, C.byref(pools),

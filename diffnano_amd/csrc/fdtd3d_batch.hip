@@ -37,18 +37,46 @@ struct dnx_batch3d_plan {
   std::vector<int64_t> side_off;  // element offset of design d's monitor buffer
   bool use_graph = false;
   hipGraphExec_t g_fwd = nullptr, g_rep = nullptr, g_adj = nullptr;
+  std::vector<int2> map_f2, map_a2;  // block maps of the two-step launches (host copies)
+  int2* d_map_f2 = nullptr;
+  int2* d_map_a2 = nullptr;
 };
 
 namespace dnx {
 
-// Descriptor table of one launch kind: kernel 0 / 1 (one-step forward /
-// adjoint) or 2 / 3 (two-step), whose tile shape sets the tiles per design.
+// Tile shape of a batched launch kind: kernel 0 / 1 (one-step forward /
+// adjoint) or 2 / 3 (two-step).  The batched two-step adjoint is built for
+// 2-row tiles only (with 4 rows its per-design arguments make it spill).
+static Fused3Tiles batch_tiles(int dtype, int kernel) {
+  Fused3Tiles t = fused3_tiles(dtype, kernel);
+  if (t.ty <= 0) t.ty = 2;  // two-step kernel switched off: table unused
+  if (kernel == 3 && t.ty > 2) t.ty = 2;
+  return t;
+}
+
+// Work-groups of a two-step launch kind over these designs (block map rows
+// x 8), for the workspace query and the plan.
+static void block_map_for(const dnx_design3d* ds, int n, int dtype, int kernel,
+                          std::vector<int2>& out) {
+  const Fused3Tiles t = batch_tiles(dtype, kernel);
+  std::vector<int> tiles(n);
+  std::vector<int64_t> cost(n);
+  for (int d = 0; d < n; ++d) {
+    const dnx_grid3d& g = ds[d].grid;
+    tiles[d] = int(cdiv(g.ny, t.ty)) * int(cdiv(g.nz, t.zc));
+    // lanes x rows x planes (plus the two restart planes of a z-chunk)
+    cost[d] = int64_t(cdiv(g.nx, 64) * 64) * t.ty * (std::min(t.zc, int(g.nz)) + 2);
+  }
+  batch_block_map(tiles.data(), cost.data(), n, out);
+}
+
+// Descriptor table of one launch kind, whose tile shape sets the tiles per
+// design.
 template <typename T>
 static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, int kernel,
                        Batch3Design<T>* out, int* total_tiles) {
   const bool adjoint = kernel == 1 || kernel == 3;
-  Fused3Tiles t = fused3_tiles(p->dtype, kernel);
-  if (t.ty <= 0) t.ty = 2;  // two-step kernel switched off: table unused
+  const Fused3Tiles t = batch_tiles(p->dtype, kernel);
   int start = 0;
   for (int d = 0; d < p->n; ++d) {
     const dnx_design3d& de = ds[d];
@@ -132,7 +160,7 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
     }
     const int tf2 = fused3_tiles(p->dtype, 2).ty, ta2 = fused3_tiles(p->dtype, 3).ty;
     p->fpairs = ok && (tf2 == 2 || tf2 == 4);
-    p->apairs = ok && (ta2 == 2 || ta2 == 4) && option(DNX_OPT_FDTD3D_ADJ2) != 0;
+    p->apairs = ok && ta2 > 0 && option(DNX_OPT_FDTD3D_ADJ2) != 0;
   }
   if ((p->fpairs || p->apairs) && p->max_mon_cells > 0) {
     if (!p->pools.side) {
@@ -156,6 +184,17 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
     DNX_CHECK_HIP(hipMemcpyAsync(*dst[k], tab[k], tb, hipMemcpyHostToDevice, s));
   }
   p->d_ctr = reinterpret_cast<int*>(base + 4 * tbr);
+  // block maps of the two-step launches, after the counter
+  unsigned char* maps = base + 4 * tbr + 256;
+  for (int k = 2; k < 4; ++k) {
+    std::vector<int2>& m = k == 2 ? p->map_f2 : p->map_a2;
+    block_map_for(ds, p->n, p->dtype, k, m);
+    void* dm = maps;
+    maps += ((m.size() * sizeof(int2) + 255) / 256) * 256;
+    DNX_CHECK_HIP(hipMemcpyAsync(dm, m.data(), m.size() * sizeof(int2), hipMemcpyHostToDevice,
+                                 s));
+    (k == 2 ? p->d_map_f2 : p->d_map_a2) = static_cast<int2*>(dm);
+  }
   int dev = 0, cus = 256;
   DNX_CHECK_HIP(hipGetDevice(&dev));
   DNX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -170,22 +209,24 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
   p->grid_f = std::min(tot[0], bf * cus);
   p->grid_a = std::min(tot[1], ba * cus);
   const int fl = p->fpairs ? 1 : 0, al = p->apairs ? 1 : 0;
-  p->Lf = Batch3Launch{p->d_fwd, p->n, tot[0], p->K, 0, p->d_ctr, fl};
-  p->La = Batch3Launch{p->d_adj, p->n, tot[1], p->K, 0, p->d_ctr, al};
+  p->Lf = Batch3Launch{p->d_fwd, p->n, tot[0], p->K, 0, p->d_ctr, fl, nullptr, 0};
+  p->La = Batch3Launch{p->d_adj, p->n, tot[1], p->K, 0, p->d_ctr, al, nullptr, 0};
   if constexpr (std::is_same<T, float>::value) {
     if (p->fpairs) {
-      p->ty_f2 = fused3_tiles(p->dtype, 2).ty;
+      p->ty_f2 = batch_tiles(p->dtype, 2).ty;
       int b2 = 1;
       if ((rc = prepare_batch3_pair<T>(p->ty_f2, false, p->max_w, &b2))) return rc;
-      p->grid_f2 = std::min(tot[2], b2 * cus);
-      p->Lf2 = Batch3Launch{p->d_fwd2, p->n, tot[2], p->K, 0, p->d_ctr, 1};
+      p->grid_f2 = int(p->map_f2.size());
+      p->Lf2 = Batch3Launch{p->d_fwd2, p->n, tot[2], p->K, 0, p->d_ctr, 1, p->d_map_f2,
+                            int(p->map_f2.size())};
     }
     if (p->apairs) {
-      p->ty_a2 = fused3_tiles(p->dtype, 3).ty;
+      p->ty_a2 = batch_tiles(p->dtype, 3).ty;
       int b2 = 1;
       if ((rc = prepare_batch3_pair<T>(p->ty_a2, true, p->max_w, &b2))) return rc;
-      p->grid_a2 = std::min(tot[3], b2 * cus);
-      p->La2 = Batch3Launch{p->d_adj2, p->n, tot[3], p->K, 0, p->d_ctr, 1};
+      p->grid_a2 = int(p->map_a2.size());
+      p->La2 = Batch3Launch{p->d_adj2, p->n, tot[3], p->K, 0, p->d_ctr, 1, p->d_map_a2,
+                            int(p->map_a2.size())};
     }
   }
   return DNX_OK;
@@ -401,9 +442,17 @@ extern "C" int64_t dnx_batch3d_side_elems(const dnx_design3d* designs, int32_t n
   return dnx::side_offset(designs, n);
 }
 
-extern "C" int64_t dnx_batch3d_workspace_bytes(int32_t n_designs) {
-  const size_t tb = sizeof(Batch3Design<double>) * size_t(std::max(n_designs, 1));
-  return int64_t(4 * ((tb + 255) / 256) * 256 + 256);  // four tables + counter
+extern "C" int64_t dnx_batch3d_workspace_bytes(const dnx_design3d* designs, int32_t n_designs) {
+  if (!designs || n_designs <= 0) return -1;
+  const size_t tb = sizeof(Batch3Design<double>) * size_t(n_designs);
+  int64_t bytes = int64_t(4 * ((tb + 255) / 256) * 256 + 256);  // four tables + counter
+  const int dtype = designs[0].grid.dtype;
+  for (int k = 2; k < 4; ++k) {  // the two-step block maps at the current tile shapes
+    std::vector<int2> m;
+    block_map_for(designs, n_designs, dtype, k, m);
+    bytes += int64_t(((m.size() * sizeof(int2) + 255) / 256) * 256);
+  }
+  return bytes;
 }
 
 extern "C" int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, int32_t n_steps,
@@ -416,7 +465,9 @@ extern "C" int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, in
   DNX_REQUIRE(n_steps >= 1, "batch3d_create: n_steps must be >= 1");
   DNX_REQUIRE(ckpt_every >= 2 && ckpt_every % 2 == 0,
               "batch3d_create: ckpt_every must be even and >= 2 (got %d)", ckpt_every);
-  DNX_REQUIRE(ws_bytes >= dnx_batch3d_workspace_bytes(n), "batch3d_create: workspace too small");
+  DNX_REQUIRE(ws_bytes >= dnx_batch3d_workspace_bytes(designs, n),
+              "batch3d_create: workspace too small (size it with dnx_batch3d_workspace_bytes "
+              "after setting the tile shapes)");
   DNX_REQUIRE(pools->state[0] && pools->state[1] && pools->lam[0] && pools->lam[1] &&
                   pools->ckpt && pools->S && (ckpt_every < 2 || pools->estore),
               "batch3d_create: null pool");

@@ -3,6 +3,8 @@
 // reads (E, H, ce) once and writes (E', H') once.
 #pragma once
 
+#include <vector>
+
 #include "dnx_aux.h"
 
 namespace dnx {
@@ -73,7 +75,15 @@ struct Batch3Launch {
   int store;            // forward: also store E (replay)
   const int* step;      // device step counter
   int layout;           // buffer layout (batch_in_slot)
+  // two-step launches: one work-group per entry, (design, tile of that
+  // design) or (-1, -1) for an idle slot; `blocks` entries (a multiple of 8)
+  const int2* block_tile;
+  int blocks;
 };
+
+// Block -> tile map of a two-step batched launch (see batch_block_map).
+void batch_block_map(const int* tiles, const int64_t* tile_cost, int n_designs,
+                     std::vector<int2>& out);
 
 template <typename T>
 int prepare_batch3(int ty, bool adjoint, int max_w, int* blocks_per_cu);
@@ -151,6 +161,7 @@ int launch_batch3_pair(const Batch3Launch& L, bool adjoint, int ty, int max_w, i
 template <typename T>
 int launch_batch_monitor_pair(const void* designs, int n_designs, int64_t max_cells,
                               const int* step, bool adjoint, hipStream_t s);
+
 
 template <typename T>
 int launch_fused3_fwd(const Fused3Args<T>& a, int ty, hipStream_t s);

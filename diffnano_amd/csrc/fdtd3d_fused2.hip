@@ -22,6 +22,8 @@
 // Per sub-step the cell arithmetic is exactly that of the one-step kernels
 // (fdtd3d_fused.hip), including the fused soft source and the DFT monitor
 // accumulation, so the two-step path reproduces the one-step path bitwise.
+#include <algorithm>
+
 #include "fdtd3d_fused.h"
 
 #include <cstdlib>
@@ -1323,20 +1325,16 @@ int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
 template int launch_fused2_adj<float>(const Fused2Args<float>&, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// Batched two-step launches: one persistent launch advances every design of
-// a batch by two steps, walking the global tile list (tiles of design d are
-// [tile_start[d], tile_start[d+1]), tile shapes of kernel 2 / 3) with
-// stride gridDim.x, XCD-remapped like the single-grid kernels.  Buffers
+// Batched two-step launches: one launch advances every design of a batch
+// by two steps, one work-group per tile (tile shapes of kernel 2 / 3), the
+// work-group -> (design, tile) assignment read from the plan's block map
+// (XCD-balanced, heavy tiles first; batch_block_map).  The tile bodies are
+// the single-grid kernels' with folded edge factors; the forward stages
+// its monitor samples (a recompute after the launch would cost more here:
+// the batch's monitor planes are many and strided).  Buffers
 // follow layout 1 of batch_in_slot; E^m of the adjoint comes from the
 // checkpoint (m % K == 0) or the replay store, as in k3_batch_adj.
 // ---------------------------------------------------------------------------
-
-template <typename T>
-__device__ inline int batch2_design(const Batch3Design<T>* D, int n_designs, int t) {
-  int d = 0;
-  while (d + 1 < n_designs && D[d + 1].tile_start <= t) ++d;
-  return d;
-}
 
 template <typename T>
 __device__ inline const T* batch2_em(const Batch3Design<T>& Dd, int m, int K) {
@@ -1357,58 +1355,46 @@ __device__ inline void batch2_mon(const Batch3Design<T>& Dd, bool on, Fused2Args
 template <typename T, int TY>
 __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
   extern __shared__ unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-  const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
+  const int2 bt = L.block_tile[blockIdx.x];
+  if (bt.x < 0) return;
+  const Batch3Design<T>& Dd = static_cast<const Batch3Design<T>*>(L.designs)[bt.x];
   const int n = *L.step;
   const int ib = batch_in_slot(1, n);
-  const int G = gridDim.x;
-  const int b = blockIdx.x;
-  const int lw = (b & 7) * (G >> 3) + (b >> 3);
-  for (int t = lw; t < L.total_tiles; t += G) {
-    const int d = batch2_design(D, L.n_designs, t);
-    const Batch3Design<T>& Dd = D[d];
-    Fused2Args<T> f{};
-    f.a = Dd.a;
-    f.a.in = Dd.buf[ib];
-    f.a.out = Dd.buf[ib ^ 1];
-    f.a.es = L.store ? Dd.estore + int64_t(n % L.K) * Dd.estore_stride : nullptr;
-    f.es2 = L.store ? Dd.estore + int64_t(n % L.K + 1) * Dd.estore_stride : nullptr;
-    f.a.src_wave = Dd.wave;
-    f.n = n;
-    batch2_mon(Dd, !L.store, f);
-    const int lt = t - Dd.tile_start;
-    fwd2_tile<T, TY, false, true, false>(f, (lt % f.a.ty_tiles) * TY,
-                                         (lt / f.a.ty_tiles) * f.a.zc, sm);
-    __syncthreads();  // LDS is reused by the next tile
-  }
+  Fused2Args<T> f{};
+  f.a = Dd.a;
+  f.a.in = Dd.buf[ib];
+  f.a.out = Dd.buf[ib ^ 1];
+  f.a.es = L.store ? Dd.estore + int64_t(n % L.K) * Dd.estore_stride : nullptr;
+  f.es2 = L.store ? Dd.estore + int64_t(n % L.K + 1) * Dd.estore_stride : nullptr;
+  f.a.src_wave = Dd.wave;
+  f.n = n;
+  const int lt = bt.y;
+  batch2_mon(Dd, !L.store, f);
+  fwd2_tile<T, TY, false, true, true>(f, (lt % f.a.ty_tiles) * TY,
+                                       (lt / f.a.ty_tiles) * f.a.zc,
+                                       reinterpret_cast<T*>(smem_raw));
 }
 
 template <typename T, int TY>
 __global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
   extern __shared__ unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-  const Batch3Design<T>* D = static_cast<const Batch3Design<T>*>(L.designs);
+  const int2 bt = L.block_tile[blockIdx.x];
+  if (bt.x < 0) return;
+  const Batch3Design<T>& Dd = static_cast<const Batch3Design<T>*>(L.designs)[bt.x];
   const int m = *L.step;  // steps m and m-1
   const int ib = batch_in_slot(1, m + 1);
-  const int G = gridDim.x;
-  const int b = blockIdx.x;
-  const int lw = (b & 7) * (G >> 3) + (b >> 3);
-  for (int t = lw; t < L.total_tiles; t += G) {
-    const int d = batch2_design(D, L.n_designs, t);
-    const Batch3Design<T>& Dd = D[d];
-    Fused2Args<T> f{};
-    f.a = Dd.a;
-    f.a.in = Dd.buf[ib];
-    f.a.out = Dd.buf[ib ^ 1];
-    f.a.em = batch2_em(Dd, m, L.K);
-    f.em2 = batch2_em(Dd, m - 1, L.K);
-    f.a.src_wave = Dd.wave;
-    f.n = m;
-    batch2_mon(Dd, true, f);
-    const int lt = t - Dd.tile_start;
-    adj2_tile<T, TY, false>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc, sm);
-    __syncthreads();
-  }
+  Fused2Args<T> f{};
+  f.a = Dd.a;
+  f.a.in = Dd.buf[ib];
+  f.a.out = Dd.buf[ib ^ 1];
+  f.a.em = batch2_em(Dd, m, L.K);
+  f.em2 = batch2_em(Dd, m - 1, L.K);
+  f.a.src_wave = Dd.wave;
+  f.n = m;
+  batch2_mon(Dd, true, f);
+  const int lt = bt.y;
+  adj2_tile<T, TY, true>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc,
+                         reinterpret_cast<T*>(smem_raw));
 }
 
 // After a forward pair at counter n: acc += (v1 at phase row n) then (v2 at
@@ -1456,10 +1442,20 @@ static size_t batch2_smem(int max_w) {
   return sizeof(T) * size_t(fwd2_lds_elems<TY>()) * size_t(max_w);
 }
 
+// The batched two-step adjoint exists for 2-row tiles only (with 4 rows
+// its per-design arguments make it spill to scratch).
+template <typename T, int TY>
+static const void* batch2_kernel(bool adjoint) {
+  if constexpr (TY == 2) {
+    if (adjoint) return reinterpret_cast<const void*>(&k3_batch2_adj<T, TY>);
+  }
+  return adjoint ? nullptr : reinterpret_cast<const void*>(&k3_batch2_fwd<T, TY>);
+}
+
 template <typename T, int TY>
 static int prepare_batch2_ty(bool adjoint, int max_w, int* blocks_per_cu) {
-  const void* f = adjoint ? reinterpret_cast<const void*>(&k3_batch2_adj<T, TY>)
-                          : reinterpret_cast<const void*>(&k3_batch2_fwd<T, TY>);
+  const void* f = batch2_kernel<T, TY>(adjoint);
+  DNX_REQUIRE(f != nullptr, "batch3 pair: no %d-row batched two-step adjoint", TY);
   DNX_CHECK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(batch2_smem<T, TY>(FUSED_MAX_W))));
   const int threads = ((max_w + 63) / 64) * 64;
@@ -1473,13 +1469,20 @@ static int prepare_batch2_ty(bool adjoint, int max_w, int* blocks_per_cu) {
 template <typename T, int TY>
 static int launch_batch2_ty(const Batch3Launch& L, bool adjoint, int max_w, int grid,
                             hipStream_t s) {
+  (void)grid;  // one work-group per block-map entry
   const int threads = ((max_w + 63) / 64) * 64;
-  const int nb = ((grid + 7) / 8) * 8;
+  const int nb = L.blocks;
   const size_t smem = batch2_smem<T, TY>(max_w);
-  if (adjoint)
-    hipLaunchKernelGGL((k3_batch2_adj<T, TY>), dim3(nb), dim3(threads), smem, s, L);
-  else
-    hipLaunchKernelGGL((k3_batch2_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, L);
+  const void* kern = batch2_kernel<T, TY>(adjoint);
+  DNX_REQUIRE(kern != nullptr, "batch3 pair: no %d-row batched two-step adjoint", TY);
+  if constexpr (TY == 2) {
+    if (adjoint) {
+      hipLaunchKernelGGL((k3_batch2_adj<T, TY>), dim3(nb), dim3(threads), smem, s, L);
+      DNX_CHECK_HIP(hipGetLastError());
+      return DNX_OK;
+    }
+  }
+  hipLaunchKernelGGL((k3_batch2_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, L);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
@@ -1509,6 +1512,40 @@ int launch_batch_monitor_pair(const void* designs, int n_designs, int64_t max_ce
                      s, static_cast<const Batch3Design<T>*>(designs), step, adjoint ? 1 : 0);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
+}
+
+// Designs are packed onto the 8 XCDs (work-group b runs on XCD b % 8)
+// longest-processing-time first by their total tile cost; on each XCD the
+// designs run in decreasing tile cost, their tiles contiguous (neighbouring
+// tiles share halo rows in that XCD's L2).  Entry s * 8 + k is the s-th tile
+// of XCD k; shorter lists are padded with idle entries.
+void batch_block_map(const int* tiles, const int64_t* tile_cost, int n, std::vector<int2>& out) {
+  std::vector<int> order(n);
+  for (int d = 0; d < n; ++d) order[d] = d;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return tile_cost[a] * tiles[a] > tile_cost[b] * tiles[b];
+  });
+  std::vector<std::vector<int>> bin(8);
+  int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int d : order) {
+    int k = 0;
+    for (int j = 1; j < 8; ++j)
+      if (load[j] < load[k]) k = j;
+    bin[k].push_back(d);
+    load[k] += tile_cost[d] * tiles[d];
+  }
+  size_t rows = 0;
+  std::vector<std::vector<int2>> lists(8);
+  for (int k = 0; k < 8; ++k) {
+    std::stable_sort(bin[k].begin(), bin[k].end(),
+                     [&](int a, int b) { return tile_cost[a] > tile_cost[b]; });
+    for (int d : bin[k])
+      for (int t = 0; t < tiles[d]; ++t) lists[k].push_back(make_int2(d, t));
+    rows = std::max(rows, lists[k].size());
+  }
+  out.assign(rows * 8, make_int2(-1, -1));
+  for (int k = 0; k < 8; ++k)
+    for (size_t r = 0; r < lists[k].size(); ++r) out[r * 8 + k] = lists[k][r];
 }
 
 template int prepare_batch3_pair<float>(int, bool, int, int*);

@@ -266,8 +266,12 @@ typedef struct dnx_batch3d_pools {
 
 typedef struct dnx_batch3d_plan dnx_batch3d_plan;  /* opaque host object */
 
-/* Device workspace the plan needs (descriptor tables + step counter). */
-int64_t dnx_batch3d_workspace_bytes(int32_t n_designs);
+/* Device workspace the plan needs: descriptor tables, the step counter and
+ * the block maps of the two-step launches (one work-group per tile, the
+ * designs packed onto the XCDs by cost), at the tile shapes set now (set
+ * them before this query; the plan takes them when it is created).
+ * Returns -1 on bad arguments. */
+int64_t dnx_batch3d_workspace_bytes(const dnx_design3d* designs, int32_t n_designs);
 /* Elements (of the designs' dtype) of the pools.side buffer: two samples of
  * every design's monitor-box components. */
 int64_t dnx_batch3d_side_elems(const dnx_design3d* designs, int32_t n_designs);
