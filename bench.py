@@ -314,11 +314,14 @@ def _bench_batched(args):
     classes = sorted({p.shape for p in probs_b})
     tiles_plain = {s: autotune.tune(s, apply=False) for s in classes}
     tiles_batch = autotune.tune_batch([p.shape for p in probs_b], apply=False)
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     res_b = inverse_design.adam_loop(probs_b, iterations=args.iters, n_steps=T, mode="batched",
-                                     device=dev, tiles=tiles_batch)
+                                     device=dev, tiles=tiles_batch, log=log)
     probs_p = inverse_design.batched_problems(args.designs, seed=0)
     res_p = inverse_design.adam_loop(probs_p, iterations=args.iters, n_steps=T, mode="plain",
-                                     device=dev, tiles=tiles_plain)
+                                     device=dev, tiles=tiles_plain, log=log)
     tiles = {"batched": tiles_batch, "plain": {str(k): v for k, v in tiles_plain.items()}}
     fb, fp = res_b["fom"][-1], res_p["fom"][-1]
     cells = sum(int(np.prod(p.shape)) for p in probs_b)
@@ -525,7 +528,7 @@ def main():
     ap.add_argument("--tune", type=int, default=None,
                     help="c1/c2: autotune the tile shapes first (default: on for c1)")
     ap.add_argument("--n-steps", type=int, default=None, help="time steps (default: config's)")
-    ap.add_argument("--cpu-steps", type=int, default=2,
+    ap.add_argument("--cpu-steps", type=int, default=4,
                     help="CPU baseline: forward steps timed and T_cpu of the fwd+adj sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--designs-sample", type=int, default=48,
@@ -533,7 +536,7 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--designs", type=int, default=48, help="c3: number of designs")
     ap.add_argument("--iters", type=int, default=None,
-                    help="c3: Adam iterations per mode (default 2); c4: design iterations of the "
+                    help="c3: Adam iterations per mode (default 20); c4: design iterations of the "
                          "reference loop (default 10)")
     ap.add_argument("--samples", type=int, default=40000, help="c4: surrogate training samples")
     ap.add_argument("--epochs", type=int, default=30, help="c4: surrogate training epochs")
@@ -546,7 +549,7 @@ def main():
                                    "domain is designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.iters is None:
-        args.iters = 10 if args.config == "c4" else 2
+        args.iters = 10 if args.config == "c4" else 20
     if args.tune is None:
         args.tune = 1 if args.config == "c1" else 0
     if args.config == "c0":

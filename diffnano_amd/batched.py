@@ -129,6 +129,9 @@ class BatchedFDTD3D:
         pools.estore = self.estore.data_ptr()
         pools.S = self.S.data_ptr()
         side_n = int(lib.dnx_batch3d_side_elems(arr, self.n))
+        if side_n < 0:
+            raise ValueError("BatchedFDTD3D: a grid (nx <= 256) or monitor box the batched "
+                             "kernels cannot take")
         self.side = torch.empty(max(side_n, 1), dtype=dt, device=dev)
         pools.side = self.side.data_ptr()
         self._pools = pools

@@ -437,13 +437,53 @@ static int batch_backward(dnx_batch3d_plan* p, hipStream_t s) {
 
 using namespace dnx;
 
+// Grid shapes the batched queries can size (the plan checks the rest):
+// positive extents, nx within one work-group, at most 2^31 cells per design
+// and 2^28 tiles of the smallest tile shape in the batch.
+static bool batch_shapes_ok(const dnx_design3d* ds, int n) {
+  int64_t tiles = 0;
+  for (int d = 0; d < n; ++d) {
+    const dnx_grid3d& g = ds[d].grid;
+    if (g.nz <= 0 || g.ny <= 0 || g.nx <= 0 || g.nx > FUSED_MAX_W) return false;
+    if (int64_t(g.nz) * g.ny * g.nx > (int64_t(1) << 31)) return false;
+    tiles += int64_t(g.ny) * cdiv(g.nz, 8);
+    if (tiles > (int64_t(1) << 28)) return false;
+  }
+  return true;
+}
+
 extern "C" int64_t dnx_batch3d_side_elems(const dnx_design3d* designs, int32_t n) {
   if (!designs || n <= 0) return 0;
+  if (!batch_shapes_ok(designs, n)) return -1;
+  for (int d = 0; d < n; ++d) {
+    const dnx_monitor& m = designs[d].mon;
+    if (m.n_freq > 0 && !box_ok(m.box, designs[d].grid.nz, designs[d].grid.ny,
+                                designs[d].grid.nx))
+      return -1;
+  }
   return dnx::side_offset(designs, n);
 }
 
+extern "C" int64_t dnx_batch3d_block_map(const dnx_design3d* designs, int32_t n_designs,
+                                         int32_t kernel, int32_t* out, int64_t capacity) {
+  if (!designs || n_designs <= 0 || (kernel != 2 && kernel != 3) || capacity < 0 ||
+      !batch_shapes_ok(designs, n_designs))
+    return -1;
+  std::vector<int2> m;
+  block_map_for(designs, n_designs, designs[0].grid.dtype, kernel, m);
+  const int64_t n = int64_t(m.size());
+  if (out) {
+    if (capacity < n) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+      out[2 * i] = m[i].x;
+      out[2 * i + 1] = m[i].y;
+    }
+  }
+  return n;
+}
+
 extern "C" int64_t dnx_batch3d_workspace_bytes(const dnx_design3d* designs, int32_t n_designs) {
-  if (!designs || n_designs <= 0) return -1;
+  if (!designs || n_designs <= 0 || !batch_shapes_ok(designs, n_designs)) return -1;
   const size_t tb = sizeof(Batch3Design<double>) * size_t(n_designs);
   int64_t bytes = int64_t(4 * ((tb + 255) / 256) * 256 + 256);  // four tables + counter
   const int dtype = designs[0].grid.dtype;
@@ -465,7 +505,11 @@ extern "C" int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, in
   DNX_REQUIRE(n_steps >= 1, "batch3d_create: n_steps must be >= 1");
   DNX_REQUIRE(ckpt_every >= 2 && ckpt_every % 2 == 0,
               "batch3d_create: ckpt_every must be even and >= 2 (got %d)", ckpt_every);
-  DNX_REQUIRE(ws_bytes >= dnx_batch3d_workspace_bytes(designs, n),
+  const int64_t ws_need = dnx_batch3d_workspace_bytes(designs, n);
+  DNX_REQUIRE(ws_need >= 0,
+              "batch3d_create: a design's grid is empty, wider than %d or too large",
+              FUSED_MAX_W);
+  DNX_REQUIRE(ws_bytes >= ws_need,
               "batch3d_create: workspace too small (size it with dnx_batch3d_workspace_bytes "
               "after setting the tile shapes)");
   DNX_REQUIRE(pools->state[0] && pools->state[1] && pools->lam[0] && pools->lam[1] &&

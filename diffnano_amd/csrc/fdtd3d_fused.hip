@@ -813,6 +813,8 @@ extern "C" int32_t dnx_fdtd3d_get_tiles(int32_t dtype, int32_t kernel, int32_t* 
                                         int32_t* zc) {
   using namespace dnx;
   DNX_REQUIRE(ty && zc, "get_tiles: null output");
+  DNX_REQUIRE(dtype == DNX_F32 || dtype == DNX_F64, "get_tiles: bad dtype %d", dtype);
+  DNX_REQUIRE(kernel >= 0 && kernel <= 3, "get_tiles: kernel must be 0-3 (got %d)", kernel);
   const Fused3Tiles t = fused3_tiles(dtype, kernel);
   *ty = t.ty;
   *zc = t.zc;

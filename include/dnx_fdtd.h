@@ -273,8 +273,17 @@ typedef struct dnx_batch3d_plan dnx_batch3d_plan;  /* opaque host object */
  * Returns -1 on bad arguments. */
 int64_t dnx_batch3d_workspace_bytes(const dnx_design3d* designs, int32_t n_designs);
 /* Elements (of the designs' dtype) of the pools.side buffer: two samples of
- * every design's monitor-box components. */
+ * every design's monitor-box components (-1: a grid or monitor box the
+ * batch cannot take). */
 int64_t dnx_batch3d_side_elems(const dnx_design3d* designs, int32_t n_designs);
+/* The work-group -> (design, tile) map of the two-step launches (kernel 2
+ * forward, 3 adjoint) at the current tile shapes, as the plan would build
+ * it: entry b (out[2b], out[2b+1]) is what work-group b runs, (-1, -1) idle;
+ * work-group b runs on XCD b % 8.  Returns the entry count (out may be
+ * null to query it), -1 on bad arguments or too small a capacity.  Host
+ * only.  No reference counterpart (introspection of the batched launch). */
+int64_t dnx_batch3d_block_map(const dnx_design3d* designs, int32_t n_designs, int32_t kernel,
+                              int32_t* out, int64_t capacity);
 /* Build a plan (uploads descriptor tables into `workspace` on `stream`).
  * ckpt_every must be even.  use_graph: capture one K-step segment of each
  * sequence (forward, replay, adjoint) into a HIP graph on first use and

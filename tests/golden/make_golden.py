@@ -387,10 +387,21 @@ def c2():
     D, H, W = shape
     Fz = torch.zeros(len(omegas), D, H, dtype=torch.float64)
     st, F = None, (Fz, Fz.clone())
+    # DNX_GOLDEN_STATE: a file that keeps the state every 500 steps, so an
+    # interrupted run resumes from the last one it holds
+    keep = os.environ.get("DNX_GOLDEN_STATE")
+    a0 = 0
+    if keep and os.path.exists(keep):
+        held = torch.load(keep, weights_only=True)
+        a0, st, F = held["step"], held["state"], tuple(held["F"])
+        print(f"c2 resumed at step {a0}", flush=True)
     with torch.no_grad():
-        for a in range(0, T, 500):
+        for a in range(a0, T, 500):
             st, F = _sheet_harness(s, eps, mu, mask, min(a + 500, T), omegas, xm, a, st, F)
             print(f"c2 forward step {min(a + 500, T)}/{T}", flush=True)
+            if keep:
+                torch.save({"step": min(a + 500, T), "state": st, "F": list(F)}, keep + ".tmp")
+                os.replace(keep + ".tmp", keep)
     Fr, Fi = (f.numpy() for f in F)
     pw = (Fr ** 2 + Fi ** 2)
     c0 = D // 2 - 32

@@ -289,3 +289,80 @@ def test_batched_backward_refuses_a_stale_forward():
         out1[0].abs().sum().backward()
     assert b.calls.count(("bwd",)) == 1
     assert ("fwd", False) in b.calls and ("fwd", True) in b.calls
+
+
+def _designs(shapes, dtype=L.DNX_F32):
+    arr = (L.Design3D * len(shapes))()
+    for d, (nz, ny, nx) in enumerate(shapes):
+        arr[d].grid.nz, arr[d].grid.ny, arr[d].grid.nx = nz, ny, nx
+        arr[d].grid.dtype = dtype
+        arr[d].src.comp = -1
+    return arr
+
+
+def _batch_tile_shape(lib, kernel):
+    ty, zc = C.c_int32(), C.c_int32()
+    assert lib.dnx_fdtd3d_get_tiles(L.DNX_F32, kernel, C.byref(ty), C.byref(zc)) == 0
+    t = ty.value if ty.value > 0 else 2
+    return (min(t, 2) if kernel == 3 else t), zc.value
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("kernel", [2, 3])
+def test_batched_block_map_covers_every_tile_once_and_balances_the_xcds(seed, kernel):
+    """The two-step batched launches' work-group map: every tile of every
+    design exactly once, idle entries (-1, -1) only, the tiles of a design
+    contiguous on its XCD, and the per-XCD cost within one design of the
+    longest-processing-time bound (work-group b runs on XCD b % 8)."""
+    lib = L.load()
+    rng = np.random.default_rng(seed)
+    classes = [(96, 96, 96), (128, 128, 128), (96, 160, 160), (128, 128, 192), (10, 12, 40),
+               (37, 21, 70)]
+    n = int(rng.integers(1, 60))
+    shapes = [classes[i] for i in rng.integers(0, len(classes), n)]
+    arr = _designs(shapes)
+    cnt = lib.dnx_batch3d_block_map(arr, n, kernel, None, 0)
+    assert cnt > 0 and cnt % 8 == 0
+    buf = (C.c_int32 * (2 * cnt))()
+    assert lib.dnx_batch3d_block_map(arr, n, kernel, buf, cnt - 1) == -1
+    assert lib.dnx_batch3d_block_map(arr, n, kernel, buf, cnt) == cnt
+    m = np.frombuffer(buf, dtype=np.int32).reshape(-1, 2)
+    ty, zc = _batch_tile_shape(lib, kernel)
+    tiles = [-(-ny // ty) * -(-nz // zc) for nz, ny, nx in shapes]
+    cost = [(-(-nx // 64) * 64) * ty * (min(zc, nz) + 2) for nz, ny, nx in shapes]
+    live = m[m[:, 0] >= 0]
+    assert np.all(m[m[:, 0] < 0] == -1)
+    seen = sorted(map(tuple, live.tolist()))
+    assert seen == sorted((d, t) for d in range(n) for t in range(tiles[d]))
+    load = np.zeros(8)
+    for k in range(8):
+        col = m[k::8]
+        col = col[col[:, 0] >= 0]
+        ds = col[:, 0]
+        # one run per design on its XCD, and its tiles in order
+        runs = [int(ds[0])] + [int(b) for a, b in zip(ds[:-1], ds[1:]) if a != b] if len(ds) else []
+        assert len(runs) == len(set(runs))
+        for d in set(runs):
+            assert np.array_equal(col[ds == d][:, 1], np.arange(tiles[d]))
+        load[k] = sum(cost[d] * tiles[d] for d in set(runs))
+    total = sum(c * t for c, t in zip(cost, tiles))
+    biggest = max(c * t for c, t in zip(cost, tiles))
+    assert load.max() <= total / 8 + biggest
+
+
+def test_batched_queries_reject_grids_the_batch_cannot_take():
+    lib = L.load()
+    for bad in [(8, 8, 300), (0, 8, 8), (8, -1, 8), (1 << 20, 1 << 12, 256)]:
+        arr = _designs([(16, 16, 16), bad])
+        assert lib.dnx_batch3d_workspace_bytes(arr, 2) == -1, bad
+        assert lib.dnx_batch3d_side_elems(arr, 2) == -1, bad
+        assert lib.dnx_batch3d_block_map(arr, 2, 2, None, 0) == -1, bad
+    arr = _designs([(16, 16, 16)])
+    assert lib.dnx_batch3d_block_map(arr, 1, 1, None, 0) == -1     # one-step kernels: no map
+    assert lib.dnx_batch3d_workspace_bytes(arr, 1) > 0
+    arr[0].mon.n_freq = 1
+    arr[0].mon.box = L.Box(0, 4, 0, 4, 0, 99)                      # box past the grid
+    assert lib.dnx_batch3d_side_elems(arr, 1) == -1
+    arr[0].mon.box = L.Box(0, 4, 0, 4, 0, 5)
+    arr[0].mon.comp_mask = 5
+    assert lib.dnx_batch3d_side_elems(arr, 1) == 2 * 2 * 4 * 4 * 5
