@@ -7,23 +7,33 @@ at 1550 nm; 5-wavelength DFT of Ey on the output plane; loss = sum |DFT|^2.
 One bench "step" = forward + loss.backward() (dL/d eps over the full grid).
 
 Prints one JSON line.  value = Mcell-updates/s of fwd+adj = N_cells * T /
-(t_fwd + t_adj).  roofline: the dominant phase's algorithmic HBM bytes per
-cell-update (forward 52 B, replay 64 B, adjoint 72 B, FP32) times the cell
-updates it performs, over its HIP-event time (events on the stream the
-kernels run on), against 8 TB/s; ``traffic`` is the measured HBM bytes per
-launch of that kernel from rocprofv3 counters (profiles/, scaled by a copy
-kernel of known size).  cpu_baseline: the FP64 NumPy oracle (a port of the
-reference's CPU path), fwd+adj over a few steps of the same grid.
+(t_fwd + t_adj).  roofline: the dominant kernel (the timed launch class
+with the largest total time) -- its algorithmic HBM bytes per launch
+(two-step launches: 104 B per cell forward, 128 B with the replay's E
+store, 144 B adjoint; one-step: 52 / 64 / 72 B, FP32) over its average
+launch time from HIP events the library records around each launch on the
+stream it launches on, against 8 TB/s; ``traffic`` is the measured HBM
+bytes per launch of that kernel from rocprofv3 counters (profiles/,
+scaled by a copy kernel of known size).  cpu_baseline: the reference's CPU
+path restated op for op in torch FP64 (oracle/torch_path.py) on the full
+grid, on all host cores of the process's affinity: forward steps, then
+the time-reversal forward + autograd-recompute backward over T_cpu steps
+(``--cpu-steps``).  designs_per_s: a reduced, stated form of the batched
+configuration (48 designs, one Adam iteration of 3000 steps, batched and
+one design at a time, FoMs compared; ``--designs-sample 0`` skips it).
 
 ``--pmc-probe``: run a short fixed sequence (a 256 MiB device copy three
 times, then 4 forward and 4 adjoint steps of the workload grid, i.e. two
 launches of each two-step kernel) for hardware-counter collection; prints
 what it ran and exits.
 
-``--config c3``: batched vs one-design-at-a-time Adam loop over 48 uneven 3D
-grids.  ``--config c4``: the surrogate-prescreened 2D design loop against
-the exhaustive batched and the plain loops (``--fno-probe``: the FNO
-kernels alone, for profiling).
+``--config c0``: the 2D TM case (200 x 200, 2000 steps, fwd + adjoint of
+a 1 x 60 DFT line) with its CPU baseline at full size and T.  ``--config
+c1``: 128^3 x 4000 steps (tile shapes autotuned first).  ``--config c3``:
+batched vs one-design-at-a-time Adam loop over 48 uneven 3D grids (20
+iterations of 3000 steps).  ``--config c4``: the surrogate-prescreened 2D
+design loop against the exhaustive batched and the plain loops
+(``--fno-probe``: the FNO kernels alone, for profiling).
 """
 
 from __future__ import annotations
