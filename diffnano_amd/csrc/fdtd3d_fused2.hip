@@ -183,6 +183,15 @@ __device__ __forceinline__ float vreg(float v) {
   return r;
 }
 
+// x-neighbour read from an LDS row at a clamped lane, issued unconditionally
+// and selected afterwards (a read under the lane condition becomes an
+// exec-masked branch per read).
+template <typename T>
+__device__ __forceinline__ T lds_nb(bool on, const T* row, int xn) {
+  const T v = row[xn];
+  return on ? v : T(0);
+}
+
 // MON: stage the monitor samples of both sub-steps from inside the tile
 // (the batched kernel).  The single-grid kernel leaves them to
 // k_monitor_pair_fwd, which recomputes the intermediate sample after the
@@ -200,19 +209,21 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
   const int W = a.nx;
+  constexpr int LW = FUSED_MAX_W;  // LDS row pitch (compile-time: immediate offsets)
   constexpr int R = TY + 4;  // rows r = 0 .. TY+3  <->  y = y0 - 2 + r
   T* sA = sm;                           // E0 (Ey, Ez), rows 0..TY+2, by parity
-  T* sB = sA + 4 * (TY + 3) * W;        // H1 (Hy, Hz), rows 1..TY+2
-  T* sC = sB + 2 * (TY + 2) * W;        // E1 (Ey, Ez), rows 1..TY+1, by parity
-  T* sD = sC + 4 * (TY + 1) * W;        // H2 (Hy, Hz), rows 2..TY+1
-#define LA(p, c, r) sA[(((p) * 2 + (c)) * (TY + 3) + (r)) * W]
-#define LB(c, r) sB[((c) * (TY + 2) + (r) - 1) * W]
-#define LC(p, c, r) sC[(((p) * 2 + (c)) * (TY + 1) + (r) - 1) * W]
-#define LD(c, r) sD[((c) * TY + (r) - 2) * W]
+  T* sB = sA + 4 * (TY + 3) * LW;        // H1 (Hy, Hz), rows 1..TY+2
+  T* sC = sB + 2 * (TY + 2) * LW;        // E1 (Ey, Ez), rows 1..TY+1, by parity
+  T* sD = sC + 4 * (TY + 1) * LW;        // H2 (Hy, Hz), rows 2..TY+1
+#define LA(p, c, r) sA[(((p) * 2 + (c)) * (TY + 3) + (r)) * LW]
+#define LB(c, r) sB[((c) * (TY + 2) + (r) - 1) * LW]
+#define LC(p, c, r) sC[(((p) * 2 + (c)) * (TY + 1) + (r) - 1) * LW]
+#define LD(c, r) sD[((c) * TY + (r) - 2) * LW]
 
   const int z1 = min(a.nz, z0 + a.zc);
   const int x = threadIdx.x;
   const bool xv = x < W;
+  const int xp1 = min(x + 1, W - 1), xm1 = max(x - 1, 0);  // clamped neighbour lanes
   const int64_t N = a.N, P = a.P;
 
   const T kh = a.kh;
@@ -280,8 +291,13 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   auto ld = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r) -> T {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, int(voff[r]), c * cbytes, 0));
   };
+  // Stores of lanes outside the grid go to an offset past every record count
+  // (bit 31 set; in-grid offsets stay below 2^31) and are dropped by the
+  // descriptor's range check, so no store needs a lane mask.
+  const uint32_t lane_oob = xv ? 0u : 0x80000000u;
   auto st = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r, T v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r]), c * cbytes, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r] | lane_oob),
+                                          c * cbytes, 0);
   };
 
   const bool has_src = a.src_comp >= 0;
@@ -310,7 +326,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int c = 0; c < 3; ++c) e0C[c][r] = ld(rsrc(a.in, zs, 6), c, r);
     }
-    if (xv) {
+    {
 #pragma unroll
       for (int r = 0; r <= TY + 2; ++r) {
         (&LA(zs & 1, 0, r))[x] = e0C[1][r];
@@ -408,8 +424,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       for (int r = 0; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
         const T ex = e0C[0][r], ey = e0C[1][r], ez = e0C[2][r];
-        const T ey_x1 = xl ? (&LA(pb, 0, r))[x + 1] : T(0);
-        const T ez_x1 = xl ? (&LA(pb, 1, r))[x + 1] : T(0);
+        const T ey_x1 = lds_nb<T>(xl, &LA(pb, 0, r), xp1);
+        const T ez_x1 = lds_nb<T>(xl, &LA(pb, 1, r), xp1);
         const T dEz_dy = e0C[2][r + 1] - ez;
         const T dEx_dy = e0C[0][r + 1] - ex;
         const T dEy_dz = zl ? e0N[1][r] - ey : T(0);
@@ -420,7 +436,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         h1N[1][r] = bxv * h0[1][r] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
         h1N[2][r] = bxv * h0[2][r] - kh * (axv * dEy_dx - fl(r) * dEx_dy);
       }
-      if (xv) {
+      {
 #pragma unroll
         for (int r = 1; r <= TY + 2; ++r) {
           (&LB(0, r))[x] = h1N[1][r];
@@ -443,8 +459,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       for (int r = 1; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
         const T hx = h1N[0][r], hy = h1N[1][r], hz = h1N[2][r];
-        const T hy_x0 = xf ? (&LB(0, r))[x - 1] : T(0);
-        const T hz_x0 = xf ? (&LB(1, r))[x - 1] : T(0);
+        const T hy_x0 = lds_nb<T>(xf, &LB(0, r), xm1);
+        const T hz_x0 = lds_nb<T>(xf, &LB(1, r), xm1);
         const T dHz_dy = hz - h1N[2][r - 1];
         const T dHx_dy = hx - h1N[0][r - 1];
         const T dHy_dz = zf ? hy - h1F[1][r - 1] : T(0);
@@ -460,12 +476,12 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         add_src_val<T>(a.src_comp, w1 * spP[r - 1], e);  // spP is 0 outside the box
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) e1P[cc][r - 1] = e[cc];
-        if (own && r >= 2 && r <= TY + 1 && rv(r) && xv) {
+        if (own && r >= 2 && r <= TY + 1 && rv(r)) {  // (uniform; st drops padding lanes)
           if (a.es) {
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) st(rsrc(a.es, p, 3), cc, r, e[cc]);
           }
-          if (mzA && mbx.row(r))
+          if (mzA && mbx.row(r))  // (the row test includes the lane's)
             mon_stage_at<T>(A2.mon[0], A2.side[0], 0, mbx.index(A2.mon[0].box, dzmA, r), e);
         }
       }
@@ -482,8 +498,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       for (int r = 1; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
         const T ex = e1Q[0][r - 1], ey = e1Q[1][r - 1], ez = e1Q[2][r - 1];
-        const T ey_x1 = xl ? (&LC(qb, 0, r))[x + 1] : T(0);
-        const T ez_x1 = xl ? (&LC(qb, 1, r))[x + 1] : T(0);
+        const T ey_x1 = lds_nb<T>(xl, &LC(qb, 0, r), xp1);
+        const T ez_x1 = lds_nb<T>(xl, &LC(qb, 1, r), xp1);
         const T dEz_dy = e1Q[2][r] - ez;
         const T dEx_dy = e1Q[0][r] - ex;
         const T dEy_dz = zlq ? e1P[1][r - 1] - ey : T(0);
@@ -494,7 +510,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         h2N[1][r - 1] = bxv * h1F[1][r - 1] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
         h2N[2][r - 1] = bxv * h1F[2][r - 1] - kh * (axv * dEy_dx - fl(r) * dEx_dy);
       }
-      if (xv) {
+      {
 #pragma unroll
         for (int r = 2; r <= TY + 1; ++r) {
           (&LD(0, r))[x] = h2N[1][r - 1];
@@ -502,7 +518,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         }
       }
     }
-    if (doA && xv) {
+    if (doA) {
 #pragma unroll
       for (int r = 1; r <= TY + 1; ++r) {
         (&LC(p & 1, 0, r))[x] = e1P[1][r - 1];
@@ -519,8 +535,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       for (int r = 2; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
         const T hx = h2N[0][r - 1], hy = h2N[1][r - 1], hz = h2N[2][r - 1];
-        const T hy_x0 = xf ? (&LD(0, r))[x - 1] : T(0);
-        const T hz_x0 = xf ? (&LD(1, r))[x - 1] : T(0);
+        const T hy_x0 = lds_nb<T>(xf, &LD(0, r), xm1);
+        const T hz_x0 = lds_nb<T>(xf, &LD(1, r), xm1);
         const T dHz_dy = hz - h2N[2][r - 2];
         const T dHx_dy = hx - h2N[0][r - 2];
         const T dHy_dz = zfq ? hy - h2P[1][r - 2] : T(0);
@@ -534,7 +550,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         e[1] = bxy * e1Q[1][r - 1] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
         e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ff(r) * dHx_dy);
         add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
-        if (own && rv(r) && xv) {
+        if (own && rv(r)) {  // (uniform; st drops padding lanes)
 #pragma unroll
           for (int cc = 0; cc < 3; ++cc) {
             st(rsrc(a.out, q, 6), cc, r, e[cc]);
@@ -612,7 +628,7 @@ template <typename T, int TY, bool PF>
 static int launch_fwd2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
-  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * a.a.nx;
+  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W;
   static bool attr_set = false;
   if (!attr_set) {
     DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF>),
@@ -825,19 +841,21 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
   const int W = a.nx;
+  constexpr int LW = FUSED_MAX_W;  // LDS row pitch (compile-time: immediate offsets)
   constexpr int R = TY + 4;  // rows r = 0 .. TY+3  <->  y = y0 - 2 + r
   T* sA = sm;                           // u0 (uy, uz), rows 0..TY+2, by parity
-  T* sB = sA + 4 * (TY + 3) * W;        // mu1 (y, z), rows 1..TY+2
-  T* sC = sB + 2 * (TY + 2) * W;        // u1 (uy, uz), rows 1..TY+1, by parity
-  T* sD = sC + 4 * (TY + 1) * W;        // mu2 (y, z), rows 2..TY+1
-#define LA(p, c, r) sA[(((p) * 2 + (c)) * (TY + 3) + (r)) * W]
-#define LB(c, r) sB[((c) * (TY + 2) + (r) - 1) * W]
-#define LC(p, c, r) sC[(((p) * 2 + (c)) * (TY + 1) + (r) - 1) * W]
-#define LD(c, r) sD[((c) * TY + (r) - 2) * W]
+  T* sB = sA + 4 * (TY + 3) * LW;        // mu1 (y, z), rows 1..TY+2
+  T* sC = sB + 2 * (TY + 2) * LW;        // u1 (uy, uz), rows 1..TY+1, by parity
+  T* sD = sC + 4 * (TY + 1) * LW;        // mu2 (y, z), rows 2..TY+1
+#define LA(p, c, r) sA[(((p) * 2 + (c)) * (TY + 3) + (r)) * LW]
+#define LB(c, r) sB[((c) * (TY + 2) + (r) - 1) * LW]
+#define LC(p, c, r) sC[(((p) * 2 + (c)) * (TY + 1) + (r) - 1) * LW]
+#define LD(c, r) sD[((c) * TY + (r) - 2) * LW]
 
   const int z1 = min(a.nz, z0 + a.zc);
   const int x = threadIdx.x;
   const bool xv = x < W;
+  const int xp1 = min(x + 1, W - 1), xm1 = max(x - 1, 0);  // clamped neighbour lanes
   const int64_t N = a.N, P = a.P;
 
   const T kh = a.kh;
@@ -897,8 +915,13 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
   auto ld = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r) -> T {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, int(voff[r]), c * cbytes, 0));
   };
+  // Stores of lanes outside the grid go to an offset past every record count
+  // (bit 31 set; in-grid offsets stay below 2^31) and are dropped by the
+  // descriptor's range check, so no store needs a lane mask.
+  const uint32_t lane_oob = xv ? 0u : 0x80000000u;
   auto st = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r, T v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r]), c * cbytes, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r] | lane_oob),
+                                          c * cbytes, 0);
   };
 
   const int m = A2.n;  // sub-steps m (stage A) and m-1 (stage B)
@@ -945,7 +968,7 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         if (r >= 1 && r <= TY + 2) lC[k][r - 1] = l;
       }
     }
-    if (xv) {
+    {
 #pragma unroll
       for (int r = 0; r <= TY + 2; ++r) {
         (&LA(zs & 1, 0, r))[x] = uC[1][r];
@@ -1042,8 +1065,8 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
       for (int r = 0; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
         const T ux = uC[0][r], uy = uC[1][r], uz = uC[2][r];
-        const T uy_x1 = xl ? (&LA(pb, 0, r))[x + 1] : T(0);
-        const T uz_x1 = xl ? (&LA(pb, 1, r))[x + 1] : T(0);
+        const T uy_x1 = lds_nb<T>(xl, &LA(pb, 0, r), xp1);
+        const T uz_x1 = lds_nb<T>(xl, &LA(pb, 1, r), xp1);
         const T tz_y = (zf ? uy : T(0)) - (zl ? uN[1][r] : T(0));
         const T tz_x = (zf ? ux : T(0)) - (zl ? uN[0][r] : T(0));
         T ty_z, ty_x;
@@ -1061,7 +1084,7 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         m1N[1][r] = bxv * m0[1][r] + tx_z - axv * tz_x;
         m1N[2][r] = bxv * m0[2][r] + ty_x - ayr[r] * tx_y;
       }
-      if (xv) {
+      {
 #pragma unroll
         for (int r = 1; r <= TY + 2; ++r) {
           (&LB(0, r))[x] = m1N[1][r];
@@ -1089,10 +1112,10 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 1; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
-        const bool mine = own && r >= 2 && r <= TY + 1 && rv(r) && xv;
+        const bool mine = own && r >= 2 && r <= TY + 1 && rv(r);  // uniform (padding lanes: stores dropped)
         const T mx = m1N[0][r], my = m1N[1][r], mz = m1N[2][r];
-        const T my_x0 = xf ? (&LB(0, r))[x - 1] : T(0);
-        const T mz_x0 = xf ? (&LB(1, r))[x - 1] : T(0);
+        const T my_x0 = lds_nb<T>(xf, &LB(0, r), xm1);
+        const T mz_x0 = lds_nb<T>(xf, &LB(1, r), xm1);
         T y_mz, y_mx;
         if constexpr (FOLD) {
           y_mz = afm1[r] * m1N[2][r - 1] - ain[r] * mz;
@@ -1154,8 +1177,8 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
       for (int r = 1; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
         const T ux = u1Q[0][r - 1], uy = u1Q[1][r - 1], uz = u1Q[2][r - 1];
-        const T uy_x1 = xl ? (&LC(qb, 0, r))[x + 1] : T(0);
-        const T uz_x1 = xl ? (&LC(qb, 1, r))[x + 1] : T(0);
+        const T uy_x1 = lds_nb<T>(xl, &LC(qb, 0, r), xp1);
+        const T uz_x1 = lds_nb<T>(xl, &LC(qb, 1, r), xp1);
         const T tz_y = (zfq ? uy : T(0)) - (zlq ? u1P[1][r - 1] : T(0));
         const T tz_x = (zfq ? ux : T(0)) - (zlq ? u1P[0][r - 1] : T(0));
         T ty_z, ty_x;
@@ -1173,7 +1196,7 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         m2N[1][r - 1] = bxv * m1F[1][r - 1] + tx_z - axv * tz_x;
         m2N[2][r - 1] = bxv * m1F[2][r - 1] + ty_x - ayr[r] * tx_y;
       }
-      if (xv) {
+      {
 #pragma unroll
         for (int r = 2; r <= TY + 1; ++r) {
           (&LD(0, r))[x] = m2N[1][r - 1];
@@ -1181,7 +1204,7 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
         }
       }
     }
-    if (doA && xv) {
+    if (doA) {
 #pragma unroll
       for (int r = 1; r <= TY + 1; ++r) {
         (&LC(p & 1, 0, r))[x] = u1P[1][r - 1];
@@ -1197,10 +1220,10 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #pragma unroll
       for (int r = 2; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
-        const bool mine = own && rv(r) && xv;
+        const bool mine = own && rv(r);  // uniform (padding lanes: stores dropped)
         const T mx = m2N[0][r - 1], my = m2N[1][r - 1], mz = m2N[2][r - 1];
-        const T my_x0 = xf ? (&LD(0, r))[x - 1] : T(0);
-        const T mz_x0 = xf ? (&LD(1, r))[x - 1] : T(0);
+        const T my_x0 = lds_nb<T>(xf, &LD(0, r), xm1);
+        const T mz_x0 = lds_nb<T>(xf, &LD(1, r), xm1);
         T y_mz, y_mx;
         if constexpr (FOLD) {
           y_mz = afm1[r] * m2N[2][r - 2] - ain[r] * mz;
@@ -1297,7 +1320,7 @@ template <typename T, int TY, int MINW>
 static int launch_adj2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
-  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * a.a.nx;
+  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W;
   static bool attr_set = false;
   if (!attr_set) {
     DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_adj<T, TY, MINW>),
@@ -1439,7 +1462,7 @@ __global__ void k_batch_monitor2(const Batch3Design<T>* D, const int* step, int 
 
 template <typename T, int TY>
 static size_t batch2_smem(int max_w) {
-  return sizeof(T) * size_t(fwd2_lds_elems<TY>()) * size_t(max_w);
+  return sizeof(T) * size_t(fwd2_lds_elems<TY>()) * size_t(FUSED_MAX_W);
 }
 
 // The batched two-step adjoint exists for 2-row tiles only (with 4 rows
