@@ -311,35 +311,46 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   if constexpr (!FOLD) sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
   mbx.init(A2.mon[0].box, x, y0 - 2, R, MON && A2.n_mon > 0 && xv);
 
-  T e0C[3][R], e0N[3][R];
-  T h1F[3][TY + 2];  // H1 at plane p-1, rows 1..TY+2 (index r-1)
-  T e1Q[3][TY + 2];  // E1 at plane p-1, rows 1..TY+2 (index r-1)
-  T h2P[2][TY];      // H2 (x, y) at plane p-2, rows 2..TY+1 (index r-2)
-  T ceQ[TY];         // ce at plane p-1, rows 2..TY+1 (index r-2)
-  T spQ[TY];         // source profile at plane p-1, rows 2..TY+1 (0 outside the box)
+  // The march keeps two register sets of every plane-carried value and the
+  // loop body runs twice per trip with their roles swapped (A: the older
+  // plane, B: the newer), so no value is copied from one plane's set to the
+  // next.  Index = tile row for e0 and h1, row - 1 for the others.
+  T eA[3][R], eB[3][R];            // E0 at planes p (C) and p+1 (N)
+  T h1A[3][TY + 3], h1B[3][TY + 3];  // H1 at planes p-1 (F) and p (N)
+  T e1A[3][TY + 2], e1B[3][TY + 2];  // E1 at planes p-1 (Q) and p (P)
+  T h2A[3][TY + 1], h2B[3][TY + 1];  // H2 at planes p-2 (P) and p-1 (N)
+  T ceA[TY + 2], ceB[TY + 2];        // ce at planes p-1 (Q) and p (P)
+  T spA[TY + 2], spB[TY + 2];        // source profile at planes p-1 (Q) and p (P)
   const int zs = z0 >= 2 ? z0 - 2 : 0;
   {
 #pragma unroll
-    for (int r = 0; r < TY; ++r) spQ[r] = T(0);
-#pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) e0C[c][r] = ld(rsrc(a.in, zs, 6), c, r);
+      for (int c = 0; c < 3; ++c) eA[c][r] = ld(rsrc(a.in, zs, 6), c, r);
     }
     {
 #pragma unroll
       for (int r = 0; r <= TY + 2; ++r) {
-        (&LA(zs & 1, 0, r))[x] = e0C[1][r];
-        (&LA(zs & 1, 1, r))[x] = e0C[2][r];
+        (&LA(zs & 1, 0, r))[x] = eA[1][r];
+        (&LA(zs & 1, 1, r))[x] = eA[2][r];
       }
     }
 #pragma unroll
-    for (int r = 0; r < TY + 2; ++r) {
+    for (int r = 0; r < TY + 3; ++r) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) h1F[c][r] = e1Q[c][r] = T(0);
+      for (int c = 0; c < 3; ++c) h1A[c][r] = T(0);
     }
 #pragma unroll
-    for (int r = 0; r < TY; ++r) h2P[0][r] = h2P[1][r] = ceQ[r] = T(0);
+    for (int r = 0; r < TY + 2; ++r) {
+      ceA[r] = spA[r] = T(0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) e1A[c][r] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < TY + 1; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) h2A[c][r] = T(0);
+    }
   }
   // PF: the loads of stage A (E at p+1, H and ce at p) are issued one plane
   // ahead, right after stage A of the previous plane, so that they are in
@@ -365,16 +376,16 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   }
   __syncthreads();
 
-#pragma unroll 1
-  for (int p = zs; p <= z1; ++p) {
+  // One plane of the march: reads the older set (e0C, h1F, e1Q, h2P, ceQ,
+  // spQ), writes the newer (e0N, h1N, e1P, h2N, ceP, spP).
+  auto plane = [&](const int p, T(&e0C)[3][R], T(&e0N)[3][R], T(&h1F)[3][TY + 3],
+                   T(&h1N)[3][TY + 3], T(&e1Q)[3][TY + 2], T(&e1P)[3][TY + 2],
+                   T(&h2P)[3][TY + 1], T(&h2N)[3][TY + 1], T(&ceQ)[TY + 2], T(&ceP)[TY + 2],
+                   T(&spQ)[TY + 2], T(&spP)[TY + 2]) {
     const bool doA = p < a.nz;       // stage A at plane p
     const bool doB = p - 1 >= zs;    // stage B at plane q = p - 1
     const int q = p - 1;
     const bool zl = p + 1 < a.nz, zf = p >= 1;
-    T h1N[3][TY + 3];  // H1 at plane p, rows 0..TY+2
-    T e1P[3][TY + 2];  // E1 at plane p, rows 1..TY+2 (index r-1)
-    T ceP[TY + 2];     // ce at plane p, rows 1..TY+2 (index r-1)
-    T spP[TY + 2];     // source profile at plane p, rows 1..TY+2 (0 outside the box)
     // ---------------- stage A, part 1: loads and H1(p)
     if (doA) {
       T h0[3][TY + 3];
@@ -463,8 +474,8 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T hz_x0 = lds_nb<T>(xf, &LB(1, r), xm1);
         const T dHz_dy = hz - h1N[2][r - 1];
         const T dHx_dy = hx - h1N[0][r - 1];
-        const T dHy_dz = zf ? hy - h1F[1][r - 1] : T(0);
-        const T dHx_dz = zf ? hx - h1F[0][r - 1] : T(0);
+        const T dHy_dz = zf ? hy - h1F[1][r] : T(0);
+        const T dHx_dz = zf ? hx - h1F[0][r] : T(0);
         const T dHz_dx = xf ? hz - hz_x0 : T(0);
         const T dHy_dx = xf ? hy - hy_x0 : T(0);
         const T c = ceP[r - 1];
@@ -490,7 +501,6 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       if (p + 1 < a.nz) prefetch(p + 1);
     }
     // ---------------- stage B, part 1: H2(q)
-    T h2N[3][TY + 1];  // H2 at plane q, rows 1..TY+1 (index r-1)
     const bool zlq = q + 1 < a.nz, zfq = q >= 1;
     if (doB) {
       const int qb = q & 1;
@@ -506,9 +516,9 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T dEx_dz = zlq ? e1P[0][r - 1] - ex : T(0);
         const T dEz_dx = xl ? ez_x1 - ez : T(0);
         const T dEy_dx = xl ? ey_x1 - ey : T(0);
-        h2N[0][r - 1] = byr[r] * h1F[0][r - 1] - kh * (fl(r) * dEz_dy - axv * dEy_dz);
-        h2N[1][r - 1] = bxv * h1F[1][r - 1] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
-        h2N[2][r - 1] = bxv * h1F[2][r - 1] - kh * (axv * dEy_dx - fl(r) * dEx_dy);
+        h2N[0][r - 1] = byr[r] * h1F[0][r] - kh * (fl(r) * dEz_dy - axv * dEy_dz);
+        h2N[1][r - 1] = bxv * h1F[1][r] - kh * (axv * dEx_dz - ayr[r] * dEz_dx);
+        h2N[2][r - 1] = bxv * h1F[2][r] - kh * (axv * dEy_dx - fl(r) * dEx_dy);
       }
       {
 #pragma unroll
@@ -539,17 +549,17 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
         const T hz_x0 = lds_nb<T>(xf, &LD(1, r), xm1);
         const T dHz_dy = hz - h2N[2][r - 2];
         const T dHx_dy = hx - h2N[0][r - 2];
-        const T dHy_dz = zfq ? hy - h2P[1][r - 2] : T(0);
-        const T dHx_dz = zfq ? hx - h2P[0][r - 2] : T(0);
+        const T dHy_dz = zfq ? hy - h2P[1][r - 1] : T(0);
+        const T dHx_dz = zfq ? hx - h2P[0][r - 1] : T(0);
         const T dHz_dx = xf ? hz - hz_x0 : T(0);
         const T dHy_dx = xf ? hy - hy_x0 : T(0);
-        const T c = ceQ[r - 2];
+        const T c = ceQ[r - 1];
         const T bxy = bxv * byr[r];
         T e[3];
         e[0] = bxy * e1Q[0][r - 1] + c * (ff(r) * dHz_dy - axv * dHy_dz);
         e[1] = bxy * e1Q[1][r - 1] + c * (axv * dHx_dz - ayr[r] * dHz_dx);
         e[2] = bxy * e1Q[2][r - 1] + c * (axv * dHy_dx - ff(r) * dHx_dy);
-        add_src_val<T>(a.src_comp, w2 * spQ[r - 2], e);
+        add_src_val<T>(a.src_comp, w2 * spQ[r - 1], e);
         if (own && rv(r)) {  // (uniform; st drops padding lanes)
 #pragma unroll
           for (int cc = 0; cc < 3; ++cc) {
@@ -564,34 +574,22 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
             mon_stage_at<T>(A2.mon[0], A2.side[0], 1, mbx.index(A2.mon[0].box, dzmB, r), e);
         }
       }
+    } else {
+      // (first plane of the march: H2 of plane zs - 1 reads as zero next)
 #pragma unroll
-      for (int r = 2; r <= TY + 1; ++r) {
-        h2P[0][r - 2] = h2N[0][r - 1];
-        h2P[1][r - 2] = h2N[1][r - 1];
+      for (int r = 0; r < TY + 1; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h2N[c][r] = T(0);
       }
     }
-    // ---------------- advance the march
-    if (doA) {
-#pragma unroll
-      for (int r = 1; r <= TY + 2; ++r) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          h1F[c][r - 1] = h1N[c][r];
-          e1Q[c][r - 1] = e1P[c][r - 1];
-        }
-      }
-#pragma unroll
-      for (int r = 2; r <= TY + 1; ++r) {
-        ceQ[r - 2] = ceP[r - 1];
-        spQ[r - 2] = spP[r - 1];
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) e0C[c][r] = e0N[c][r];
-      }
-    }
+  };
+  int p = zs;
+#pragma unroll 1
+  for (; p + 1 <= z1; p += 2) {
+    plane(p, eA, eB, h1A, h1B, e1A, e1B, h2A, h2B, ceA, ceB, spA, spB);
+    plane(p + 1, eB, eA, h1B, h1A, e1B, e1A, h2B, h2A, ceB, ceA, spB, spA);
   }
+  if (p <= z1) plane(p, eA, eB, h1A, h1B, e1A, e1B, h2A, h2B, ceA, ceB, spA, spB);
 #undef LA
 #undef LB
 #undef LC
