@@ -204,7 +204,10 @@ __device__ __forceinline__ T lds_nb(bool on, const T* row, int xn) {
 // kernel) or applied as selects where the factors are used (the batched
 // kernel, whose per-design arguments leave no scalar registers for them:
 // folded, it spills).  The forms differ at most in the sign of a zero.
-template <typename T, int TY, bool PF, bool MON, bool FOLD = true>
+// NT: non-temporal stores (the output state is not read again by this
+// launch), for grids whose state does not fit the last-level cache; on
+// cache-resident grids they cost the next launch its cache hits.
+template <typename T, int TY, bool PF, bool MON, bool FOLD = true, bool NT = false>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
@@ -297,7 +300,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   const uint32_t lane_oob = xv ? 0u : 0x80000000u;
   auto st = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r, T v) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r] | lane_oob),
-                                          c * cbytes, 0);
+                                          c * cbytes, NT ? 2 : 0);
   };
 
   const bool has_src = a.src_comp >= 0;
@@ -607,34 +610,38 @@ __device__ inline void block_tile2(int nblocks, int ty_tiles, int zc, int& y0, i
   z0 = (logical / ty_tiles) * zc;
 }
 
-template <typename T, int TY, bool PF>
+template <typename T, int TY, bool PF, bool NT>
 __global__ __launch_bounds__(256, 2) void k3_fused2_fwd(Fused2Args<T> a) {
   extern __shared__ unsigned char smem_raw[];
   int y0, z0;
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  fwd2_tile<T, TY, PF, false>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  fwd2_tile<T, TY, PF, false, true, NT>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
+
+// Whether a grid's stores bypass the cache: one state (6 N floats) above
+// half of the 256 MB last-level cache.
+inline bool stream_stores(int64_t N) { return N * 6 * 4 > (int64_t(128) << 20); }
 
 template <int TY>
 constexpr int fwd2_lds_elems() {
   return 4 * (TY + 3) + 2 * (TY + 2) + 4 * (TY + 1) + 2 * TY;
 }
 
-template <typename T, int TY, bool PF>
+template <typename T, int TY, bool PF, bool NT>
 static int launch_fwd2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
   const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W;
   static bool attr_set = false;
   if (!attr_set) {
-    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF>),
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF, NT>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k3_fused2_fwd<T, TY, PF>), dim3(nb), dim3(threads), smem, s, a);
+  hipLaunchKernelGGL((k3_fused2_fwd<T, TY, PF, NT>), dim3(nb), dim3(threads), smem, s, a);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
@@ -775,8 +782,13 @@ int launch_monitor_pair_fwd(const Fused2Args<T>& f, const dnx_monitor& m, int n,
 template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
   const bool pf = option(DNX_OPT_FDTD3D_FWD2_PF) != 0;
-  if (ty == 4) return pf ? launch_fwd2_ty<T, 4, true>(a, s) : launch_fwd2_ty<T, 4, false>(a, s);
-  if (ty == 2) return pf ? launch_fwd2_ty<T, 2, true>(a, s) : launch_fwd2_ty<T, 2, false>(a, s);
+  const bool nt = stream_stores(a.a.N);  // (the prefetch variant keeps cached stores)
+  if (ty == 4)
+    return pf ? launch_fwd2_ty<T, 4, true, false>(a, s)
+              : nt ? launch_fwd2_ty<T, 4, false, true>(a, s) : launch_fwd2_ty<T, 4, false, false>(a, s);
+  if (ty == 2)
+    return pf ? launch_fwd2_ty<T, 2, true, false>(a, s)
+              : nt ? launch_fwd2_ty<T, 2, false, true>(a, s) : launch_fwd2_ty<T, 2, false, false>(a, s);
   set_error("fused2: unsupported tile height %d", ty);
   return DNX_EINVAL;
 }
@@ -834,7 +846,7 @@ __device__ __forceinline__ T mon_apply(int mask, const T (&dv)[3], T (&lam)[3],
 // registers; the single-grid kernel), or as selects on each difference (the
 // batched kernel, whose per-design arguments leave no room for the factors).
 // The two forms differ at most in the sign of a zero.
-template <typename T, int TY, bool FOLD>
+template <typename T, int TY, bool FOLD, bool NT = false>
 __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
@@ -919,7 +931,7 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
   const uint32_t lane_oob = xv ? 0u : 0x80000000u;
   auto st = [&](const __amdgpu_buffer_rsrc_t& rs, int c, int r, T v) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(voff[r] | lane_oob),
-                                          c * cbytes, 0);
+                                          c * cbytes, NT ? 2 : 0);
   };
 
   const int m = A2.n;  // sub-steps m (stage A) and m-1 (stage B)
@@ -1304,29 +1316,29 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #undef LD
 }
 
-template <typename T, int TY, int MINW>
+template <typename T, int TY, int MINW, bool NT>
 __global__ __launch_bounds__(256, MINW) void k3_fused2_adj(Fused2Args<T> a) {
   extern __shared__ unsigned char smem_raw[];
   int y0, z0;
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  adj2_tile<T, TY, true>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  adj2_tile<T, TY, true, NT>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
 
-template <typename T, int TY, int MINW>
+template <typename T, int TY, int MINW, bool NT = false>
 static int launch_adj2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
   const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W;
   static bool attr_set = false;
   if (!attr_set) {
-    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_adj<T, TY, MINW>),
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_adj<T, TY, MINW, NT>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k3_fused2_adj<T, TY, MINW>), dim3(nb), dim3(threads), smem, s, a);
+  hipLaunchKernelGGL((k3_fused2_adj<T, TY, MINW, NT>), dim3(nb), dim3(threads), smem, s, a);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
@@ -1336,9 +1348,14 @@ int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
   // occupancy 2 (launch bounds for two work-groups per CU) unless
   // DNX_FDTD3D_ADJ2_OCC=1; TY = 4 fits the register file only at occupancy 1
   const bool two = option(DNX_OPT_FDTD3D_ADJ2_OCC) != 1;
+  const bool nt = stream_stores(a.a.N);  // (the default occupancy only)
   if (ty == 4) return launch_adj2_ty<T, 4, 1>(a, s);
-  if (ty == 2) return two ? launch_adj2_ty<T, 2, 2>(a, s) : launch_adj2_ty<T, 2, 1>(a, s);
-  if (ty == 1) return two ? launch_adj2_ty<T, 1, 2>(a, s) : launch_adj2_ty<T, 1, 1>(a, s);
+  if (ty == 2)
+    return !two ? launch_adj2_ty<T, 2, 1>(a, s)
+                : nt ? launch_adj2_ty<T, 2, 2, true>(a, s) : launch_adj2_ty<T, 2, 2>(a, s);
+  if (ty == 1)
+    return !two ? launch_adj2_ty<T, 1, 1>(a, s)
+                : nt ? launch_adj2_ty<T, 1, 2, true>(a, s) : launch_adj2_ty<T, 1, 2>(a, s);
   set_error("fused2: unsupported tile height %d", ty);
   return DNX_EINVAL;
 }
