@@ -431,11 +431,19 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
   const bool pairs = use_pairs<T>(gd, fused, n_src, n_mon, probe, 2);
   Fused2Args<T> f2{};
   int ty2 = 0;
+  bool monx = false;  // monitor 0's DFT inside the forward pair's tiles
   if (pairs) {
-    // the forward kernel stages no monitor samples: k_monitor_pair_fwd
-    // recomputes the intermediate one after each launch
+    // An x-normal monitor plane is accumulated by the tiles themselves;
+    // otherwise k_monitor_pair_fwd recomputes the intermediate sample after
+    // each launch.
     const Fused3Tiles t2 = grid_tiles(gd, 2);
-    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, 0, nullptr, 2);
+    monx = n_mon == 1 && monitor_in_tile(mons[0], t2.ty, t2.zc);
+    f2 = fused2_args<T>(gd, t2, srcs, n_src, mons, monx ? 1 : 0, nullptr, 2);
+    if (monx) {
+      f2.mph = mons[0].phase;
+      f2.macc = mons[0].acc;
+      f2.mnf = mons[0].n_freq;
+    }
     ty2 = t2.ty;
   }
   for (int k = 0; k < n_steps; ++k) {
@@ -457,7 +465,7 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
         int rc = launch_fused2_fwd<T>(f2, ty2, s);
         timed_end(slot ? "fdtd3d_fwd2_store" : "fdtd3d_fwd2", s, tl);
         if (rc) return rc;
-        for (int j = 0; j < n_mon; ++j)
+        for (int j = monx ? 1 : 0; j < n_mon; ++j)
           if ((rc = launch_monitor_pair_fwd<T>(f2, mons[j], n, s))) return rc;
       }
       advance(out);

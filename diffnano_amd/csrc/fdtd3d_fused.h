@@ -133,6 +133,11 @@ struct Fused2Args {
   int n_mon;
   FusedMon mon[FUSED2_MAX_MON];
   T* side[FUSED2_MAX_MON];  // compact monitor buffers (see above)
+  // single-grid forward with the DFT of monitor 0 done in the tile (an
+  // x-normal monitor plane; mnf = 0: not used): phase table, accumulators
+  const double* mph;
+  double* macc;
+  int mnf;
 };
 
 // Two-step tile configuration (kernel 2 forward, 3 adjoint); float32 only.
@@ -140,6 +145,9 @@ template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s);
 template <typename T>
 int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s);
+// Whether the forward pair kernel can run monitor m's DFT in its tiles (an
+// x-normal plane whose per-tile samples fit the staging space).
+bool monitor_in_tile(const dnx_monitor& m, int ty, int zc);
 // DFT monitor m of the forward pair just launched with f (steps n, n+1):
 // the intermediate sample recomputed from f.a.in, the second read from
 // f.a.out (the single-grid forward kernel stages no monitor samples).

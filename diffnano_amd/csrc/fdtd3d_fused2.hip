@@ -207,7 +207,13 @@ __device__ __forceinline__ T lds_nb(bool on, const T* row, int xn) {
 // NT: non-temporal stores (the output state is not read again by this
 // launch), for grids whose state does not fit the last-level cache; on
 // cache-resident grids they cost the next launch its cache hits.
-template <typename T, int TY, bool PF, bool MON, bool FOLD = true, bool NT = false>
+// MONX: the DFT of monitor 0, an x-normal plane, inside the tile: the lane
+// of the plane's column writes both samples of its owned cells to LDS
+// during the march, and after it the work-group accumulates them (FP64, in
+// the order of k_monitor_pair_fwd) into the monitor's accumulators; each
+// cell belongs to exactly one tile.  The samples are the tile's own values.
+template <typename T, int TY, bool PF, bool MON, bool FOLD = true, bool NT = false,
+          bool MONX = false>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
@@ -218,6 +224,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   T* sB = sA + 4 * (TY + 3) * LW;        // H1 (Hy, Hz), rows 1..TY+2
   T* sC = sB + 2 * (TY + 2) * LW;        // E1 (Ey, Ez), rows 1..TY+1, by parity
   T* sD = sC + 4 * (TY + 1) * LW;        // H2 (Hy, Hz), rows 2..TY+1
+  T* sM = sD + 2 * TY * LW;              // MONX: samples [zc][TY][2][3]
 #define LA(p, c, r) sA[(((p) * 2 + (c)) * (TY + 3) + (r)) * LW]
 #define LB(c, r) sB[((c) * (TY + 2) + (r) - 1) * LW]
 #define LC(p, c, r) sC[(((p) * 2 + (c)) * (TY + 1) + (r) - 1) * LW]
@@ -313,6 +320,29 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
   RowBits sbx;  // the batched kernel's source test (FOLD = false)
   if constexpr (!FOLD) sbx.init(a.src_box, x, y0 - 2, R, has_src && xv);
   mbx.init(A2.mon[0].box, x, y0 - 2, R, MON && A2.n_mon > 0 && xv);
+  // MONX: the owned cells of the monitor plane in this tile: planes
+  // [mzlo, mzhi), tile rows in the bit mask mrows (uniform; 0 when the tile
+  // holds none), the plane's column in lane mon_lane.  All three
+  // components of both samples of a cell are staged, [zc][TY][2][3], so
+  // that a staging store is one per-plane offset plus an immediate.
+  int mzlo = 0, mzhi = 0;
+  uint32_t mrows = 0;
+  bool mon_lane = false;
+  if constexpr (MONX) {
+    const Box& mxb = A2.mon[0].box;
+    mzlo = max(z0, mxb.z0);
+    mzhi = min(z1, mxb.z0 + mxb.nz);
+    const int mylo = max(y0, mxb.y0), myhi = min(min(y0 + TY, a.ny), mxb.y0 + mxb.ny);
+#pragma unroll
+    for (int r = 2; r <= TY + 1; ++r)
+      if (mzlo < mzhi && y0 - 2 + r >= mylo && y0 - 2 + r < myhi) mrows |= 1u << r;
+    mon_lane = x == mxb.x0;
+  }
+  auto mstage = [&](int sub, int z, int r, const T (&e)[3]) {
+    T* cell = sM + (z - z0) * (TY * 6) + (r - 2) * 6 + sub * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cell[c] = e[c];
+  };
 
   // The march keeps two register sets of every plane-carried value and the
   // loop body runs twice per trip with their roles swapped (A: the older
@@ -469,6 +499,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       const bool own = p >= z0 && p < z1;
       int dzmA;
       const bool mzA = MON && RowBox<R>::zin(A2.mon[0].box, p, dzmA);
+      const uint32_t mxA = (MONX && p >= mzlo && p < mzhi) ? mrows : 0u;  // (uniform)
 #pragma unroll
       for (int r = 1; r <= TY + 2; ++r) {
         const int y = y0 - 2 + r;
@@ -497,6 +528,11 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
           }
           if (mzA && mbx.row(r))  // (the row test includes the lane's)
             mon_stage_at<T>(A2.mon[0], A2.side[0], 0, mbx.index(A2.mon[0].box, dzmA, r), e);
+          if constexpr (MONX) {
+            if ((mxA >> r) & 1u) {
+              if (mon_lane) mstage(0, p, r, e);
+            }
+          }
         }
       }
     }
@@ -544,6 +580,7 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
       const bool own = q >= z0;
       int dzmB;
       const bool mzB = MON && RowBox<R>::zin(A2.mon[0].box, q, dzmB);
+      const uint32_t mxB = (MONX && q >= mzlo && q < mzhi) ? mrows : 0u;  // (uniform)
 #pragma unroll
       for (int r = 2; r <= TY + 1; ++r) {
         const int y = y0 - 2 + r;
@@ -575,6 +612,11 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
           }
           if (mzB && mbx.row(r))
             mon_stage_at<T>(A2.mon[0], A2.side[0], 1, mbx.index(A2.mon[0].box, dzmB, r), e);
+          if constexpr (MONX) {
+            if ((mxB >> r) & 1u) {
+              if (mon_lane) mstage(1, q, r, e);
+            }
+          }
         }
       }
     } else {
@@ -593,6 +635,53 @@ __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0,
     plane(p + 1, eB, eA, h1B, h1A, e1B, e1A, h2B, h2A, ceB, ceA, spB, spA);
   }
   if (p <= z1) plane(p, eA, eB, h1A, h1B, e1A, e1B, h2A, h2B, ceA, ceB, spA, spB);
+  if constexpr (MONX) {
+    if (mrows != 0u) {  // (uniform)
+      __syncthreads();
+      // (the monitor's parameters are read here, not kept live through the
+      // march: volatile reads are neither hoisted nor merged with the ones
+      // above)
+      auto vget = [](const auto& v) {
+        return *reinterpret_cast<const volatile std::remove_reference_t<decltype(v)>*>(&v);
+      };
+      const FusedMon& mmv = A2.mon[0];
+      const Box mxb{vget(mmv.box.z0), vget(mmv.box.y0), vget(mmv.box.x0),
+                    vget(mmv.box.nz), vget(mmv.box.ny), vget(mmv.box.nx)};
+      const int mmask = vget(mmv.mask), ncomp = vget(mmv.ncomp), nf = vget(A2.mnf);
+      const double* ph = vget(A2.mph) + int64_t(A2.n) * nf * 2;
+      double* const macc = vget(A2.macc);
+      const int mylo = max(y0, mxb.y0), myhi = min(min(y0 + TY, a.ny), mxb.y0 + mxb.ny);
+      const int nzm = mzhi - mzlo, nym = myhi - mylo;
+      const int items = nzm * nym * ncomp * nf;
+      const int64_t cells = mxb.cells();
+      for (int t = int(threadIdx.x); t < items; t += int(blockDim.x)) {
+        int u = t;
+        const int dy = u % nym;
+        u /= nym;
+        const int dz = u % nzm;
+        u /= nzm;
+        const int ci = u % ncomp, f = u / ncomp;
+        int c = 0;  // the ci-th monitored component
+        for (int k = 0, seen = 0; k < 3; ++k)
+          if ((mmask >> k) & 1) {
+            if (seen == ci) c = k;
+            ++seen;
+          }
+        const int z = mzlo + dz, y = mylo + dy;
+        const T* cell = sM + (z - z0) * (TY * 6) + (y - y0) * 6;
+        const double v1 = double(cell[c]), v2 = double(cell[3 + c]);
+        const int64_t i = (int64_t(z - mxb.z0) * mxb.ny + (y - mxb.y0)) * mxb.nx;
+        double* qa = macc + ((int64_t(f) * ncomp + ci) * cells + i) * 2;
+        double re = qa[0], im = qa[1];
+        re += v1 * ph[2 * f];
+        im -= v1 * ph[2 * f + 1];
+        re += v2 * ph[2 * nf + 2 * f];
+        im -= v2 * ph[2 * nf + 2 * f + 1];
+        qa[0] = re;
+        qa[1] = im;
+      }
+    }
+  }
 #undef LA
 #undef LB
 #undef LC
@@ -610,14 +699,24 @@ __device__ inline void block_tile2(int nblocks, int ty_tiles, int zc, int& y0, i
   z0 = (logical / ty_tiles) * zc;
 }
 
-template <typename T, int TY, bool PF, bool NT>
+template <typename T, int TY, bool PF, bool NT, bool MONX>
 __global__ __launch_bounds__(256, 2) void k3_fused2_fwd(Fused2Args<T> a) {
   extern __shared__ unsigned char smem_raw[];
   int y0, z0;
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  fwd2_tile<T, TY, PF, false, true, NT>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  fwd2_tile<T, TY, PF, false, true, NT, MONX>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+}
+
+// MONX staging space: both samples of the 3 components per owned cell of
+// an x-normal plane (zc planes x TY rows per tile), at most 8 KB.
+constexpr int kMonxBytes = 8192;
+bool monitor_in_tile(const dnx_monitor& m, int ty, int zc) {
+  const Box b = make_box(m.box);
+  return m.n_freq > 0 && b.nx == 1 && (m.comp_mask & 7) != 0 &&
+         int64_t(2) * 3 * zc * ty * int64_t(sizeof(float)) <= kMonxBytes &&
+         b.cells() < (int64_t(1) << 31);
 }
 
 // Whether a grid's stores bypass the cache: one state (6 N floats) above
@@ -629,19 +728,21 @@ constexpr int fwd2_lds_elems() {
   return 4 * (TY + 3) + 2 * (TY + 2) + 4 * (TY + 1) + 2 * TY;
 }
 
-template <typename T, int TY, bool PF, bool NT>
+template <typename T, int TY, bool PF, bool NT, bool MONX = false>
 static int launch_fwd2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
-  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W;
+  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W +
+                      (MONX ? sizeof(T) * 2 * 3 * a.a.zc * TY : 0);
   static bool attr_set = false;
   if (!attr_set) {
-    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF, NT>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
+    DNX_CHECK_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF, NT, MONX>),
+        hipFuncAttributeMaxDynamicSharedMemorySize,
+        int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W + (MONX ? kMonxBytes : 0))));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k3_fused2_fwd<T, TY, PF, NT>), dim3(nb), dim3(threads), smem, s, a);
+  hipLaunchKernelGGL((k3_fused2_fwd<T, TY, PF, NT, MONX>), dim3(nb), dim3(threads), smem, s, a);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
@@ -783,6 +884,15 @@ template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
   const bool pf = option(DNX_OPT_FDTD3D_FWD2_PF) != 0;
   const bool nt = stream_stores(a.a.N);  // (the prefetch variant keeps cached stores)
+  if (a.mnf > 0) {  // monitor DFT in the tiles (the caller checked monitor_in_tile)
+    DNX_REQUIRE(!pf && a.n_mon == 1, "fused2: in-tile monitor needs one monitor, no prefetch");
+    if (ty == 4)
+      return nt ? launch_fwd2_ty<T, 4, false, true, true>(a, s)
+                : launch_fwd2_ty<T, 4, false, false, true>(a, s);
+    if (ty == 2)
+      return nt ? launch_fwd2_ty<T, 2, false, true, true>(a, s)
+                : launch_fwd2_ty<T, 2, false, false, true>(a, s);
+  }
   if (ty == 4)
     return pf ? launch_fwd2_ty<T, 4, true, false>(a, s)
               : nt ? launch_fwd2_ty<T, 4, false, true>(a, s) : launch_fwd2_ty<T, 4, false, false>(a, s);
