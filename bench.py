@@ -233,6 +233,34 @@ def _copy_bandwidth(torch, dev, mib=1024):
     return 2 * a.numel() * 4 / (e0.elapsed_time(e1) / 10) / 1e6
 
 
+def _stream_ceiling(torch, dev, n_cells):
+    """Measured streaming ceilings (GB/s, read + written bytes / time) from
+    the library's probe kernels on the current stream: a 1 GiB non-temporal
+    16-byte copy, and the streams of one FDTD step on the workload's cells
+    (seven arrays in, six out, one dword per lane and array)."""
+    from diffnano_amd import _lib as L
+
+    lib = L.lib()
+    h = L.stream_handle(dev)
+    out = {}
+    for kind, (n_in, n_out, n) in {0: (1, 1, 256 * 1024 * 1024), 1: (7, 6, n_cells)}.items():
+        src = torch.zeros(n_in * n, dtype=torch.float32, device=dev)
+        dst = torch.empty(n_out * n, dtype=torch.float32, device=dev)
+        for _ in range(3):
+            L.check(lib.dnx_stream_probe(kind, src.data_ptr(), dst.data_ptr(), n, h))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            L.check(lib.dnx_stream_probe(kind, src.data_ptr(), dst.data_ptr(), n, h))
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        out["copy_nt_gbs" if kind == 0 else "step_streams_gbs"] = round(
+            (n_in + n_out) * n * 4 / (ms / 1e3) / 1e9, 1)
+        del src, dst
+    return out
+
+
 def _read_traffic():
     """Measured HBM bytes per launch per kernel from the committed counter
     summary (lines `kernel_name: <bytes> bytes per launch`), or {}."""
@@ -648,6 +676,7 @@ def main():
     total_bytes = sum(BYTES[k] * updates[k] for k in BYTES)
     overall_gbs = total_bytes / (sum(phases.values()) / 1e3) / 1e9
     copy_gbs = _copy_bandwidth(torch, dev)
+    ceiling = _stream_ceiling(torch, dev, N)
     # Dominant kernel: the timed launch class with the largest total time.
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     dom_k = kern[dom]
@@ -683,13 +712,16 @@ def main():
                      "frac": round(dom_k["gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "alg_bytes_per_launch": TIMED[dom][2] * N,
-                     "avg_launch_us": dom_k["avg_us"]},
+                     "avg_launch_us": dom_k["avg_us"],
+                     "measured_hbm_gbs": (round(traffic / (dom_k["avg_us"] * 1e-6) / 1e9, 1)
+                                          if traffic else None)},
         "kernels": kern,
         "phases_ms_per_step": {k: round(v / args.steps, 2) for k, v in phases.items()},
         "phase_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "fwd_adj_roofline_frac": round(overall_gbs / HBM_PEAK_GBS, 4),
         "copy_gbs": round(copy_gbs, 1),
         "fwd_adj_frac_of_copy": round(overall_gbs / copy_gbs, 4),
+        "stream_ceiling": ceiling,
         "fwd_mcells": round(N * T / (phases["forward"] / args.steps / 1e3) / 1e6, 2),
         "loss": float(loss.item()),
         "grad_l2": float(grad.norm().item()),

@@ -109,6 +109,7 @@ _SIGS = {
     "dnx_fno_wgrad": ([_P, _P, _P, _P, _I, _I, _L, _I, _P], _I),
     "dnx_batch3d_workspace_bytes": ([C.POINTER(Design3D), _I], _L),
     "dnx_batch3d_side_elems": ([C.POINTER(Design3D), _I], _L),
+    "dnx_stream_probe": ([_I, _P, _P, _L, _P], _I),
     "dnx_batch3d_block_map": ([C.POINTER(Design3D), _I, _I, C.POINTER(C.c_int32), _L], _L),
     "dnx_batch3d_create": ([C.POINTER(Design3D), _I, _I, _I, C.POINTER(Pools3D), _P, _L, _I, _P,
                             C.POINTER(_P)], _I),

@@ -384,6 +384,48 @@ extern "C" int32_t dnx_get_option(int32_t id, int32_t* value) {
   return DNX_OK;
 }
 
+// ---- streaming-bandwidth probes (the ceiling the FDTD kernels are held to) ----
+namespace dnx {
+typedef float v4f __attribute__((ext_vector_type(4)));
+// kind 0: a copy of n floats, 16 bytes per lane, non-temporal loads and stores
+__global__ void k_stream_copy(const v4f* __restrict__ a, v4f* __restrict__ b, int64_t n4) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+// kind 1: the streams of one FDTD step on n cells, one dword per lane and
+// array: six fields and a coefficient in (7 n floats), six fields out
+__global__ void k_stream_step(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float k = in[6 * n + i];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) out[f * n + i] = in[f * n + i] * k;
+}
+}  // namespace dnx
+
+extern "C" int32_t dnx_stream_probe(int32_t kind, const void* src, void* dst, int64_t n,
+                                    void* stream) {
+  using namespace dnx;
+  DNX_REQUIRE(src && dst && n > 0, "stream_probe: null buffer or empty size");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (kind == 0) {
+    DNX_REQUIRE(n % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(dst) & 15) == 0,
+                "stream_probe: the copy needs 16-byte aligned buffers of a multiple of 4 floats");
+    const int64_t n4 = n / 4;
+    hipLaunchKernelGGL(k_stream_copy, dim3(unsigned(cdiv(n4, 256))), dim3(256), 0, s,
+                       static_cast<const v4f*>(src), static_cast<v4f*>(dst), n4);
+  } else if (kind == 1) {
+    hipLaunchKernelGGL(k_stream_step, dim3(unsigned(cdiv(n, 256))), dim3(256), 0, s,
+                       static_cast<const float*>(src), static_cast<float*>(dst), n);
+  } else {
+    set_error("stream_probe: kind must be 0 (copy) or 1 (FDTD step streams), got %d", kind);
+    return DNX_EINVAL;
+  }
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
 extern "C" int32_t dnx_timing_enable(int32_t on) {
   dnx::g_timing_on = on != 0;
   return DNX_OK;

@@ -207,18 +207,22 @@ __device__ __forceinline__ T lds_nb(bool on, const T* row, int xn) {
 // NT: non-temporal stores (the output state is not read again by this
 // launch), for grids whose state does not fit the last-level cache; on
 // cache-resident grids they cost the next launch its cache hits.
+// LW: the LDS row pitch, a compile-time constant (immediate LDS offsets) at
+// least the work-group width (every lane writes its column): 128 for grids
+// up to 128 columns, 256 otherwise (the smaller pitch keeps the LDS of
+// narrow grids from limiting the work-groups per CU).
 // MONX: the DFT of monitor 0, an x-normal plane, inside the tile: the lane
 // of the plane's column writes both samples of its owned cells to LDS
 // during the march, and after it the work-group accumulates them (FP64, in
 // the order of k_monitor_pair_fwd) into the monitor's accumulators; each
 // cell belongs to exactly one tile.  The samples are the tile's own values.
 template <typename T, int TY, bool PF, bool MON, bool FOLD = true, bool NT = false,
-          bool MONX = false>
+          bool MONX = false, int LW = FUSED_MAX_W>
 __device__ __forceinline__ void fwd2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
   const int W = a.nx;
-  constexpr int LW = FUSED_MAX_W;  // LDS row pitch (compile-time: immediate offsets)
+  static_assert(LW % 64 == 0 && LW <= FUSED_MAX_W, "LDS row pitch");  // >= the work-group width
   constexpr int R = TY + 4;  // rows r = 0 .. TY+3  <->  y = y0 - 2 + r
   T* sA = sm;                           // E0 (Ey, Ez), rows 0..TY+2, by parity
   T* sB = sA + 4 * (TY + 3) * LW;        // H1 (Hy, Hz), rows 1..TY+2
@@ -699,14 +703,14 @@ __device__ inline void block_tile2(int nblocks, int ty_tiles, int zc, int& y0, i
   z0 = (logical / ty_tiles) * zc;
 }
 
-template <typename T, int TY, bool PF, bool NT, bool MONX>
+template <typename T, int TY, bool PF, bool NT, bool MONX, int LW>
 __global__ __launch_bounds__(256, 2) void k3_fused2_fwd(Fused2Args<T> a) {
   extern __shared__ unsigned char smem_raw[];
   int y0, z0;
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  fwd2_tile<T, TY, PF, false, true, NT, MONX>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  fwd2_tile<T, TY, PF, false, true, NT, MONX, LW>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
 
 // MONX staging space: both samples of the 3 components per owned cell of
@@ -728,21 +732,23 @@ constexpr int fwd2_lds_elems() {
   return 4 * (TY + 3) + 2 * (TY + 2) + 4 * (TY + 1) + 2 * TY;
 }
 
-template <typename T, int TY, bool PF, bool NT, bool MONX = false>
+template <typename T, int TY, bool PF, bool NT, bool MONX, int LW>
 static int launch_fwd2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
-  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W +
+  DNX_REQUIRE(threads <= LW, "fused2: LDS pitch %d below the work-group width %d", LW, threads);
+  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * LW +
                       (MONX ? sizeof(T) * 2 * 3 * a.a.zc * TY : 0);
   static bool attr_set = false;
   if (!attr_set) {
     DNX_CHECK_HIP(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF, NT, MONX>),
+        reinterpret_cast<const void*>(&k3_fused2_fwd<T, TY, PF, NT, MONX, LW>),
         hipFuncAttributeMaxDynamicSharedMemorySize,
-        int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W + (MONX ? kMonxBytes : 0))));
+        int(sizeof(T) * fwd2_lds_elems<TY>() * LW + (MONX ? kMonxBytes : 0))));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k3_fused2_fwd<T, TY, PF, NT, MONX>), dim3(nb), dim3(threads), smem, s, a);
+  hipLaunchKernelGGL((k3_fused2_fwd<T, TY, PF, NT, MONX, LW>), dim3(nb), dim3(threads), smem, s,
+                     a);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
@@ -880,25 +886,32 @@ int launch_monitor_pair_fwd(const Fused2Args<T>& f, const dnx_monitor& m, int n,
   return DNX_OK;
 }
 
+// Variant choice: the prefetch variant (opt-in) at pitch 256 with cached
+// stores; otherwise pitch 128 for grids up to 128 columns (whose state is
+// cache-resident, cached stores), pitch 256 with non-temporal stores when
+// the state exceeds the cache; the in-tile monitor where the caller set it.
+template <typename T, int TY>
+static int launch_fwd2_pick(const Fused2Args<T>& a, hipStream_t s) {
+  const bool monx = a.mnf > 0;
+  if (option(DNX_OPT_FDTD3D_FWD2_PF) != 0) {
+    DNX_REQUIRE(!monx, "fused2: the prefetch variant has no in-tile monitor");
+    return launch_fwd2_ty<T, TY, true, false, false, 256>(a, s);
+  }
+  if (a.a.nx <= 128)
+    return monx ? launch_fwd2_ty<T, TY, false, false, true, 128>(a, s)
+                : launch_fwd2_ty<T, TY, false, false, false, 128>(a, s);
+  if (stream_stores(a.a.N))
+    return monx ? launch_fwd2_ty<T, TY, false, true, true, 256>(a, s)
+                : launch_fwd2_ty<T, TY, false, true, false, 256>(a, s);
+  return monx ? launch_fwd2_ty<T, TY, false, false, true, 256>(a, s)
+              : launch_fwd2_ty<T, TY, false, false, false, 256>(a, s);
+}
+
 template <typename T>
 int launch_fused2_fwd(const Fused2Args<T>& a, int ty, hipStream_t s) {
-  const bool pf = option(DNX_OPT_FDTD3D_FWD2_PF) != 0;
-  const bool nt = stream_stores(a.a.N);  // (the prefetch variant keeps cached stores)
-  if (a.mnf > 0) {  // monitor DFT in the tiles (the caller checked monitor_in_tile)
-    DNX_REQUIRE(!pf && a.n_mon == 1, "fused2: in-tile monitor needs one monitor, no prefetch");
-    if (ty == 4)
-      return nt ? launch_fwd2_ty<T, 4, false, true, true>(a, s)
-                : launch_fwd2_ty<T, 4, false, false, true>(a, s);
-    if (ty == 2)
-      return nt ? launch_fwd2_ty<T, 2, false, true, true>(a, s)
-                : launch_fwd2_ty<T, 2, false, false, true>(a, s);
-  }
-  if (ty == 4)
-    return pf ? launch_fwd2_ty<T, 4, true, false>(a, s)
-              : nt ? launch_fwd2_ty<T, 4, false, true>(a, s) : launch_fwd2_ty<T, 4, false, false>(a, s);
-  if (ty == 2)
-    return pf ? launch_fwd2_ty<T, 2, true, false>(a, s)
-              : nt ? launch_fwd2_ty<T, 2, false, true>(a, s) : launch_fwd2_ty<T, 2, false, false>(a, s);
+  DNX_REQUIRE(a.mnf == 0 || a.n_mon == 1, "fused2: in-tile monitor needs exactly one monitor");
+  if (ty == 4) return launch_fwd2_pick<T, 4>(a, s);
+  if (ty == 2) return launch_fwd2_pick<T, 2>(a, s);
   set_error("fused2: unsupported tile height %d", ty);
   return DNX_EINVAL;
 }
@@ -956,12 +969,12 @@ __device__ __forceinline__ T mon_apply(int mask, const T (&dv)[3], T (&lam)[3],
 // registers; the single-grid kernel), or as selects on each difference (the
 // batched kernel, whose per-design arguments leave no room for the factors).
 // The two forms differ at most in the sign of a zero.
-template <typename T, int TY, bool FOLD, bool NT = false>
+template <typename T, int TY, bool FOLD, bool NT = false, int LW = FUSED_MAX_W>
 __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0, const int z0,
                                           T* sm) {
   const Fused3Args<T>& a = A2.a;
   const int W = a.nx;
-  constexpr int LW = FUSED_MAX_W;  // LDS row pitch (compile-time: immediate offsets)
+  static_assert(LW % 64 == 0 && LW <= FUSED_MAX_W, "LDS row pitch");  // >= the work-group width
   constexpr int R = TY + 4;  // rows r = 0 .. TY+3  <->  y = y0 - 2 + r
   T* sA = sm;                           // u0 (uy, uz), rows 0..TY+2, by parity
   T* sB = sA + 4 * (TY + 3) * LW;        // mu1 (y, z), rows 1..TY+2
@@ -1426,29 +1439,30 @@ __device__ __forceinline__ void adj2_tile(const Fused2Args<T>& A2, const int y0,
 #undef LD
 }
 
-template <typename T, int TY, int MINW, bool NT>
+template <typename T, int TY, int MINW, bool NT, int LW>
 __global__ __launch_bounds__(256, MINW) void k3_fused2_adj(Fused2Args<T> a) {
   extern __shared__ unsigned char smem_raw[];
   int y0, z0;
   bool live;
   block_tile2<TY>(a.a.nblocks, a.a.ty_tiles, a.a.zc, y0, z0, live);
   if (!live) return;
-  adj2_tile<T, TY, true, NT>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
+  adj2_tile<T, TY, true, NT, LW>(a, y0, z0, reinterpret_cast<T*>(smem_raw));
 }
 
-template <typename T, int TY, int MINW, bool NT = false>
+template <typename T, int TY, int MINW, bool NT = false, int LW = 256>
 static int launch_adj2_ty(const Fused2Args<T>& a, hipStream_t s) {
   const int threads = ((a.a.nx + 63) / 64) * 64;
   const int nb = ((a.a.nblocks + 7) / 8) * 8;
-  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * FUSED_MAX_W;
+  DNX_REQUIRE(threads <= LW, "fused2: LDS pitch %d below the work-group width %d", LW, threads);
+  const size_t smem = sizeof(T) * size_t(fwd2_lds_elems<TY>()) * LW;
   static bool attr_set = false;
   if (!attr_set) {
-    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k3_fused2_adj<T, TY, MINW, NT>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      int(sizeof(T) * fwd2_lds_elems<TY>() * FUSED_MAX_W)));
+    DNX_CHECK_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k3_fused2_adj<T, TY, MINW, NT, LW>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(T) * fwd2_lds_elems<TY>() * LW)));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k3_fused2_adj<T, TY, MINW, NT>), dim3(nb), dim3(threads), smem, s, a);
+  hipLaunchKernelGGL((k3_fused2_adj<T, TY, MINW, NT, LW>), dim3(nb), dim3(threads), smem, s, a);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }
@@ -1458,14 +1472,20 @@ int launch_fused2_adj(const Fused2Args<T>& a, int ty, hipStream_t s) {
   // occupancy 2 (launch bounds for two work-groups per CU) unless
   // DNX_FDTD3D_ADJ2_OCC=1; TY = 4 fits the register file only at occupancy 1
   const bool two = option(DNX_OPT_FDTD3D_ADJ2_OCC) != 1;
-  const bool nt = stream_stores(a.a.N);  // (the default occupancy only)
+  // (at the default occupancy: pitch 128 up to 128 columns, non-temporal
+  // stores beyond the cache)
+  const bool nt = stream_stores(a.a.N), narrow = a.a.nx <= 128;
   if (ty == 4) return launch_adj2_ty<T, 4, 1>(a, s);
   if (ty == 2)
     return !two ? launch_adj2_ty<T, 2, 1>(a, s)
-                : nt ? launch_adj2_ty<T, 2, 2, true>(a, s) : launch_adj2_ty<T, 2, 2>(a, s);
+                : narrow ? launch_adj2_ty<T, 2, 2, false, 128>(a, s)
+                : nt     ? launch_adj2_ty<T, 2, 2, true>(a, s)
+                         : launch_adj2_ty<T, 2, 2>(a, s);
   if (ty == 1)
     return !two ? launch_adj2_ty<T, 1, 1>(a, s)
-                : nt ? launch_adj2_ty<T, 1, 2, true>(a, s) : launch_adj2_ty<T, 1, 2>(a, s);
+                : narrow ? launch_adj2_ty<T, 1, 2, false, 128>(a, s)
+                : nt     ? launch_adj2_ty<T, 1, 2, true>(a, s)
+                         : launch_adj2_ty<T, 1, 2>(a, s);
   set_error("fused2: unsupported tile height %d", ty);
   return DNX_EINVAL;
 }
@@ -1500,7 +1520,7 @@ __device__ inline void batch2_mon(const Batch3Design<T>& Dd, bool on, Fused2Args
   f.side[0] = Dd.m_side;
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int LW>
 __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
   extern __shared__ unsigned char smem_raw[];
   const int2 bt = L.block_tile[blockIdx.x];
@@ -1518,12 +1538,12 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
   f.n = n;
   const int lt = bt.y;
   batch2_mon(Dd, !L.store, f);
-  fwd2_tile<T, TY, false, true, true>(f, (lt % f.a.ty_tiles) * TY,
+  fwd2_tile<T, TY, false, true, true, false, false, LW>(f, (lt % f.a.ty_tiles) * TY,
                                        (lt / f.a.ty_tiles) * f.a.zc,
                                        reinterpret_cast<T*>(smem_raw));
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int LW>
 __global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
   extern __shared__ unsigned char smem_raw[];
   const int2 bt = L.block_tile[blockIdx.x];
@@ -1541,7 +1561,7 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
   f.n = m;
   batch2_mon(Dd, true, f);
   const int lt = bt.y;
-  adj2_tile<T, TY, true>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc,
+  adj2_tile<T, TY, true, false, LW>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc,
                          reinterpret_cast<T*>(smem_raw));
 }
 
@@ -1585,27 +1605,36 @@ __global__ void k_batch_monitor2(const Batch3Design<T>* D, const int* step, int 
   }
 }
 
+// LDS pitch of a batch: 128 when every design has at most 128 columns.
+inline int batch2_pitch(int max_w) { return max_w <= 128 ? 128 : 256; }
+
 template <typename T, int TY>
 static size_t batch2_smem(int max_w) {
-  return sizeof(T) * size_t(fwd2_lds_elems<TY>()) * size_t(FUSED_MAX_W);
+  return sizeof(T) * size_t(fwd2_lds_elems<TY>()) * size_t(batch2_pitch(max_w));
 }
 
 // The batched two-step adjoint exists for 2-row tiles only (with 4 rows
 // its per-design arguments make it spill to scratch).
-template <typename T, int TY>
+template <typename T, int TY, int LW>
 static const void* batch2_kernel(bool adjoint) {
   if constexpr (TY == 2) {
-    if (adjoint) return reinterpret_cast<const void*>(&k3_batch2_adj<T, TY>);
+    if (adjoint) return reinterpret_cast<const void*>(&k3_batch2_adj<T, TY, LW>);
   }
-  return adjoint ? nullptr : reinterpret_cast<const void*>(&k3_batch2_fwd<T, TY>);
+  return adjoint ? nullptr : reinterpret_cast<const void*>(&k3_batch2_fwd<T, TY, LW>);
+}
+
+template <typename T, int TY>
+static const void* batch2_kernel_w(bool adjoint, int max_w) {
+  return batch2_pitch(max_w) == 128 ? batch2_kernel<T, TY, 128>(adjoint)
+                                    : batch2_kernel<T, TY, 256>(adjoint);
 }
 
 template <typename T, int TY>
 static int prepare_batch2_ty(bool adjoint, int max_w, int* blocks_per_cu) {
-  const void* f = batch2_kernel<T, TY>(adjoint);
+  const void* f = batch2_kernel_w<T, TY>(adjoint, max_w);
   DNX_REQUIRE(f != nullptr, "batch3 pair: no %d-row batched two-step adjoint", TY);
   DNX_CHECK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(batch2_smem<T, TY>(FUSED_MAX_W))));
+                                    int(batch2_smem<T, TY>(max_w))));
   const int threads = ((max_w + 63) / 64) * 64;
   int nb = 0;
   DNX_CHECK_HIP(
@@ -1614,23 +1643,30 @@ static int prepare_batch2_ty(bool adjoint, int max_w, int* blocks_per_cu) {
   return DNX_OK;
 }
 
+template <typename T, int TY, int LW>
+static void launch_batch2_lw(const Batch3Launch& L, bool adjoint, int threads, size_t smem,
+                             hipStream_t s) {
+  if constexpr (TY == 2) {
+    if (adjoint) {
+      hipLaunchKernelGGL((k3_batch2_adj<T, TY, LW>), dim3(L.blocks), dim3(threads), smem, s, L);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k3_batch2_fwd<T, TY, LW>), dim3(L.blocks), dim3(threads), smem, s, L);
+}
+
 template <typename T, int TY>
 static int launch_batch2_ty(const Batch3Launch& L, bool adjoint, int max_w, int grid,
                             hipStream_t s) {
   (void)grid;  // one work-group per block-map entry
   const int threads = ((max_w + 63) / 64) * 64;
-  const int nb = L.blocks;
   const size_t smem = batch2_smem<T, TY>(max_w);
-  const void* kern = batch2_kernel<T, TY>(adjoint);
+  const void* kern = batch2_kernel_w<T, TY>(adjoint, max_w);
   DNX_REQUIRE(kern != nullptr, "batch3 pair: no %d-row batched two-step adjoint", TY);
-  if constexpr (TY == 2) {
-    if (adjoint) {
-      hipLaunchKernelGGL((k3_batch2_adj<T, TY>), dim3(nb), dim3(threads), smem, s, L);
-      DNX_CHECK_HIP(hipGetLastError());
-      return DNX_OK;
-    }
-  }
-  hipLaunchKernelGGL((k3_batch2_fwd<T, TY>), dim3(nb), dim3(threads), smem, s, L);
+  if (batch2_pitch(max_w) == 128)
+    launch_batch2_lw<T, TY, 128>(L, adjoint, threads, smem, s);
+  else
+    launch_batch2_lw<T, TY, 256>(L, adjoint, threads, smem, s);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }

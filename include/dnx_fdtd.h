@@ -139,6 +139,13 @@ int32_t dnx_get_option(int32_t id, int32_t* value);
  * dnx_timing_read waits for the recorded events, returns the summed
  * duration and the launch count, and clears that name. */
 int32_t dnx_timing_enable(int32_t on);
+/* Streaming-bandwidth probe on `stream` (no reference counterpart: the
+ * measured ceiling the kernels are compared with).  kind 0: copy n floats
+ * src -> dst (16-byte lanes, non-temporal; n a multiple of 4, buffers
+ * 16-byte aligned); kind 1: the streams of one FDTD step on n cells, src =
+ * 7 n floats (six fields and a coefficient), dst = 6 n floats, one dword
+ * per lane and array. */
+int32_t dnx_stream_probe(int32_t kind, const void* src, void* dst, int64_t n, void* stream);
 int32_t dnx_timing_read(const char* name, double* total_ms, int64_t* launches);
 
 /* ce[i] = dt / (eps[i] * dl) computed in float64, stored in dtype.
