@@ -265,6 +265,27 @@ class Plan:
         self.out_unsqueeze = out_unsqueeze
         self.probe = int(probe)
         self.probe_values = None
+        # device bytes the time-reversal forward may spend on E of the final
+        # steps (3D): None = half the free device memory, keeping 16 GiB free
+        self.e_store_bytes = None
+
+
+_KEEP_FREE = 16 << 30
+
+
+def _tail_segments(plan, nseg, K, seg_bytes):
+    """Trailing checkpoint segments whose E (every step) the forward keeps,
+    so that the backward runs their adjoint without replaying them: as many
+    as ``plan.e_store_bytes`` (or, by default, half of the free device
+    memory above 16 GiB) holds, at most all but the first."""
+    eng = plan.engine
+    if eng.dim != 3 or plan.probe >= 0 or nseg < 2 or seg_bytes <= 0:
+        return 0
+    budget = plan.e_store_bytes
+    if budget is None:
+        free, _ = torch.cuda.mem_get_info(eng.dev)
+        budget = max(0, (free - _KEEP_FREE) // 2)
+    return int(min(nseg - 1, int(budget) // seg_bytes))
 
 
 def _field_out(plan, state):
@@ -297,10 +318,13 @@ def forward_only(plan, eps64):
 class _TimeReversalFDTD(torch.autograd.Function):
     """FDTD forward with a checkpoint-and-replay discrete adjoint.
 
-    Forward: state checkpoints every K steps (no per-step snapshots).
+    Forward: state checkpoints every K steps, and E of every step of the
+    last M segments (as many as the plan's E budget holds; the reference's
+    time-reversal forward keeps E of every step, fdtd3d.py:147-150).
     Backward: for each segment in reverse, replay from its checkpoint storing
-    E, then run the adjoint kernels over the segment.  Memory: T/K full
-    states + (K-1) E fields.  The input eps may live on any device (it is
+    E (not needed for the last M), then run the adjoint kernels over the
+    segment.  Memory: T/K full states + (K-1) E fields + M K E fields.  The
+    input eps may live on any device (it is
     moved to the GPU inside); outputs are returned on ``plan.out_device``
     and the gradient on eps's device, so this Function is the grad_fn of
     the solver's field.
@@ -326,13 +350,28 @@ class _TimeReversalFDTD(torch.autograd.Function):
         probe_out = None
         if plan.probe >= 0:
             probe_out = torch.zeros(max(T, 1), dtype=eng.dtype, device=eng.dev)
+        ne = eng.ne
+        elem = torch.tensor([], dtype=eng.dtype).element_size()
+        n_tail = _tail_segments(plan, nseg, K, K * ne * N * elem) if nseg else 0
+        a_tail = (nseg - n_tail) * K  # first step whose segment keeps its E
+        tail = (torch.empty((T - a_tail) * ne * N, dtype=eng.dtype, device=eng.dev)
+                if n_tail else None)
         _begin("forward")
-        state = eng.run(state, 0, T, srcs[:2], mons[:2], ckpt, K, probe=plan.probe,
-                        probe_out=probe_out)
+        if n_tail:
+            s0, w0 = state, torch.empty_like(state)
+            mid = eng.run(s0, 0, a_tail, srcs[:2], mons[:2], ckpt, K, work=w0)
+            state = eng.run(mid, a_tail, T - a_tail, srcs[:2], mons[:2],
+                            ckpt[(a_tail // K) * ns * N:], K, estore=tail,
+                            work=w0 if mid is s0 else s0)
+        else:
+            state = eng.run(state, 0, T, srcs[:2], mons[:2], ckpt, K, probe=plan.probe,
+                            probe_out=probe_out)
         _end("forward")
         plan.probe_values = probe_out[:T] if probe_out is not None else None
         ctx.plan = plan
         ctx.ckpt = ckpt
+        ctx.tail = tail
+        ctx.a_tail = a_tail if n_tail else T
         ctx.state = state
         ctx.srcs = srcs
         outs = [_field_out(plan, state)]
@@ -368,13 +407,16 @@ class _TimeReversalFDTD(torch.autograd.Function):
         for k in range(nseg - 1, -1, -1):
             a, b = k * K, min(T, (k + 1) * K)
             slot = ctx.ckpt[k * ns * N:(k + 1) * ns * N]
-            if b - a > 1:
+            es = estore
+            if a >= ctx.a_tail:  # E^{a+1} .. E^{b-1} kept by the forward
+                es = ctx.tail[(a - ctx.a_tail) * ne * N:]
+            elif b - a > 1:
                 _begin("replay")
                 eng.run(work, a, b - a - 1, srcs[:2], (None, 0), estore=estore, work=work2,
                         init=slot)
                 _end("replay")
             _begin("adjoint")
-            lam, lam_spare = eng.adjoint_run(lam, b, a, slot[:ne * N], estore, srcs[:2], mons[:2],
+            lam, lam_spare = eng.adjoint_run(lam, b, a, slot[:ne * N], es, srcs[:2], mons[:2],
                                              S, lam_spare, ws)
             _end("adjoint")
         grad = eng.finalize_grad(S)

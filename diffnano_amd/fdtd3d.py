@@ -10,7 +10,11 @@ kernels).  ``device`` is where results are returned; the kernels always run
 on the GPU, and there is no CPU path.
 
 Extensions (keyword-only, absent from the reference): ``dtype``,
-``checkpoint_every`` (adjoint checkpoint interval), ``forward(...,
+``checkpoint_every`` (adjoint checkpoint interval), ``check_eps``
+(NaN/Inf permittivity: "warn", "raise" or "off"), ``e_store_gb`` (device
+memory the gradient forward may use to keep E of the final steps, which
+the backward then does not replay; None: half of the free device memory
+above 16 GiB, 0: none; never changes results), ``forward(...,
 monitors=[...])`` running DFT monitors, and source keys ``component``
 ("Ex" / "Ey" / "Ez") and ``profile`` (amplitude over the source box).
 Non-cubic grids run with each CPML profile evaluated along the axis it
@@ -60,6 +64,7 @@ class FDTDSolver3D:
         dtype: torch.dtype = torch.float32,
         checkpoint_every: int = 64,
         check_eps: str = "warn",
+        e_store_gb: float | None = None,
     ):
         self.grid_shape = grid_shape
         self.dl = dl
@@ -73,6 +78,9 @@ class FDTDSolver3D:
         self.backward = backward
         self.dtype = dtype
         self.checkpoint_every = checkpoint_every
+        if e_store_gb is not None and not e_store_gb >= 0:
+            raise ValueError(f"e_store_gb must be None or >= 0, got {e_store_gb!r}")
+        self.e_store_gb = e_store_gb
         if check_eps not in EPS_CHECKS:
             raise ValueError(f"check_eps must be one of {EPS_CHECKS}, got {check_eps!r}")
         self.check_eps = check_eps
@@ -191,6 +199,8 @@ class FDTDSolver3D:
         mon_specs = self._monitor_specs(monitors)
         plan = Plan(eng, self.n_steps, self._ckpt_every(), self._source_specs(src), mon_specs,
                     out_order=[2, 0, 1], out_unsqueeze=False, out_device=self._device)
+        if self.e_store_gb is not None:
+            plan.e_store_bytes = int(self.e_store_gb * 1e9)
         if torch.is_grad_enabled() and eps_r.requires_grad:
             outs = _TimeReversalFDTD.apply(eps_r, plan)
             field, mon_raw = outs[0], list(outs[1:])

@@ -469,3 +469,60 @@ def test_determinism_bitwise():
         (f ** 2).sum().backward()
         outs.append((f.detach().clone(), eps.grad.clone()))
     assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("T,K", [(50, 8), (45, 8), (33, 6)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_e_store_budget_does_not_change_results(T, K, dtype):
+    """The forward keeping E of the final segments (so the backward skips
+    their replay) gives bitwise the gradient, field and monitors of the pure
+    checkpoint-and-replay adjoint, for budgets holding none, some and all but
+    the first segment (odd step counts, a partial last segment, monitors and
+    a profiled source)."""
+    shape = (20, 24, 70)
+    rng = np.random.default_rng(4)
+    eps0 = torch.tensor(1.5 + 2.0 * rng.random(shape))
+    src = {"type": "gaussian_pulse", "plane": "yz", "x": 6, "component": "Ey"}
+    mons = [{"box": (0, 20, 0, 24, 60, 61), "wavelengths": [1500.0, 1600.0],
+             "components": ["Ey"]}]
+    N = int(np.prod(shape))
+    ebytes = 3 * N * (4 if dtype == torch.float32 else 8)
+    out = []
+    for budget in (0.0, 2 * K * ebytes / 1e9, None, 1e3):
+        s = _solver3d(grid_shape=shape, dl=20.0, pml_layers=4, n_steps=T, courant=0.4,
+                      backward="time_reversal", checkpoint_every=K, dtype=dtype,
+                      e_store_gb=budget)
+        eps = eps0.clone().requires_grad_(True)
+        res = s.forward(eps, source=src, monitors=mons)
+        F = res.metadata["monitors"][0]
+        ((F.abs() ** 2).sum() + (res.field ** 2).sum()).backward()
+        out.append((res.field.detach().clone(), F.detach().clone(), eps.grad.clone()))
+    assert out[0][2].abs().max() > 0
+    for f, F, g in out[1:]:
+        assert torch.equal(f, out[0][0]) and torch.equal(F, out[0][1])
+        assert torch.equal(g, out[0][2])
+
+
+def test_e_store_budget_keeps_the_final_segments():
+    """With a budget the forward allocates E of the final segments (peak
+    memory grows by that much) and without one it does not."""
+    shape, T, K = (16, 16, 64), 40, 8
+    eps0 = torch.full(shape, 2.0, dtype=torch.float64)
+    N = int(np.prod(shape))
+    seg = K * 3 * N * 4
+    peaks = []
+    for budget in (0.0, 3 * seg / 1e9):
+        s = _solver3d(grid_shape=shape, dl=20.0, pml_layers=2, n_steps=T, courant=0.4,
+                      backward="time_reversal", checkpoint_every=K, e_store_gb=budget)
+        eps = eps0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        res = s.forward(eps, source={"type": "gaussian_pulse", "pos": [8, 8, 32]})
+        torch.cuda.synchronize()
+        peaks.append(torch.cuda.max_memory_allocated() - base)
+        (res.field ** 2).sum().backward()
+    # 3 segments of E kept: the last 3 segments' steps (3 K E fields)
+    assert peaks[1] - peaks[0] >= 3 * K * 3 * N * 4 - 3 * N * 4
+    with pytest.raises(ValueError):
+        _solver3d(grid_shape=shape, n_steps=T, e_store_gb=-1.0)
