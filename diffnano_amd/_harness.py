@@ -133,21 +133,36 @@ def run_trial(solver_cls: type, grid: tuple[int, int, int], n_steps: int, device
 
 
 def storage_bytes(grid: tuple[int, int, int], n_steps: int, checkpoint_every: int,
-                  elem_bytes: int = 4, n_components: int = 6) -> dict[str, int]:
+                  elem_bytes: int = 4, n_components: int = 6,
+                  e_store_bytes: int | None = None) -> dict[str, int]:
     """Device buffers of one forward + adjoint of ``FDTDSolver3D`` (or 2D with
     ``n_components=3``), by name: the permittivity and coefficient, the
     forward state and its ping-pong twin, ceil(T/K) checkpoints, the replay
-    store of K-1 E fields, four adjoint / replay state buffers, the gradient
-    accumulator, the float64 gradient and the float64 output field."""
+    store of K-1 E fields, E of every step of the final segments that the
+    3D time-reversal forward keeps (``final_segments``), four adjoint /
+    replay state buffers, the gradient accumulator, the float64 gradient
+    and the float64 output field.
+
+    ``e_store_bytes`` is the solver's E budget (its ``e_store_gb`` keyword);
+    None models the default on a device with room to spare (free memory of
+    at least 16 GiB plus twice the store), where every segment but the
+    first keeps its E."""
     N = math.prod(grid)
     K = max(1, int(checkpoint_every))
     ne = 3 if n_components == 6 else 1
     s = elem_bytes
+    nseg = math.ceil(n_steps / K) if n_steps else 0
+    tail = 0
+    if n_components == 6 and nseg >= 2:
+        seg = K * ne * N * s
+        m = nseg - 1 if e_store_bytes is None else min(nseg - 1, int(e_store_bytes) // seg)
+        tail = (n_steps - (nseg - m) * K) * ne * N * s if m > 0 else 0
     return {
         "eps_and_coefficient": N * (8 + s),
         "state": 2 * n_components * N * s,
-        "checkpoints": math.ceil(n_steps / K) * n_components * N * s if n_steps else 0,
+        "checkpoints": nseg * n_components * N * s,
         "replay_store": max(K - 1, 1) * ne * N * s,
+        "final_segments": tail,
         "adjoint": 4 * n_components * N * s + N * s,
         "gradient": N * 8,
         "field": ne * N * 8,

@@ -74,7 +74,9 @@ def model_peak_mb(grid_size: tuple[int, int, int], n_steps: int, extra_kw: dict)
         K = -(-n_steps // max(1, extra_kw.get("checkpoint_segments", 4)))
     else:
         K = extra_kw.get("checkpoint_every", 64)
-    return sum(storage_bytes(grid_size, n_steps, K).values()) / 1e6
+    eb = extra_kw.get("e_store_gb")
+    return sum(storage_bytes(grid_size, n_steps, K, e_store_bytes=None if eb is None
+                             else int(eb * 1e9)).values()) / 1e6
 
 
 def _cosines(modes: list[str], grads: list[torch.Tensor]) -> list[float | None]:

@@ -32,8 +32,19 @@ def test_storage_model_hand_values():
     assert s["checkpoints"] == 2 * 6 * N * 4       # ceil(100 / 64) = 2 slots
     assert s["replay_store"] == 63 * 3 * N * 4 and s["adjoint"] == 25 * N * 4
     assert s["gradient"] == 8 * N and s["field"] == 24 * N
+    # E of steps 64..99 (the second, last segment) kept by the forward
+    assert s["final_segments"] == 36 * 3 * N * 4
+    assert storage_bytes((16, 16, 16), 100, 64, e_store_bytes=0)["final_segments"] == 0
+    # a budget of two segments of 10 steps keeps segments 8 and 9 of 10: steps 80..99
+    seg = 10 * 3 * N * 4
+    assert storage_bytes((16, 16, 16), 100, 10, e_store_bytes=2 * seg + 1)["final_segments"] \
+        == 20 * 3 * N * 4
+    # ... and never the first segment, however large the budget
+    assert storage_bytes((16, 16, 16), 100, 10, e_store_bytes=99 * seg)["final_segments"] \
+        == 90 * 3 * N * 4
     s1 = storage_bytes((4, 5, 6), 7, 1, elem_bytes=8, n_components=3)   # float64, 2D-style
     assert s1["checkpoints"] == 7 * 3 * 120 * 8 and s1["replay_store"] == 120 * 8
+    assert s1["final_segments"] == 0   # the 2D solvers keep no such store
     assert storage_bytes((4, 4, 4), 0, 8)["checkpoints"] == 0
     # "checkpoint" mode: K = ceil(T / segments); others: checkpoint_every (default 64)
     assert model_peak_mb((16, 16, 16), 100, {"use_checkpoint": True, "checkpoint_segments": 2}) \
