@@ -334,9 +334,6 @@ class _TimeReversalFDTD(torch.autograd.Function):
     def forward(ctx, eps, plan):
         eng = plan.engine
         ctx.eps_device = eps.device
-        # saved so that an in-place change of eps before backward raises (the
-        # engine's coefficients and -S/eps are taken from this eps)
-        ctx.save_for_backward(eps)
         eng.set_eps(eps)
         T = plan.n_steps
         K = plan.ckpt_every
@@ -369,10 +366,12 @@ class _TimeReversalFDTD(torch.autograd.Function):
         _end("forward")
         plan.probe_values = probe_out[:T] if probe_out is not None else None
         ctx.plan = plan
-        ctx.ckpt = ckpt
-        ctx.tail = tail
+        # Saved through autograd: an in-place change of eps before backward
+        # raises (the coefficients and -S/eps come from this eps), and the
+        # checkpoints, the E store and the final state are released once a
+        # backward without retain_graph has run, not when the graph dies.
+        ctx.save_for_backward(eps, ckpt, tail, state)
         ctx.a_tail = a_tail if n_tail else T
-        ctx.state = state
         ctx.srcs = srcs
         outs = [_field_out(plan, state)]
         for acc in mons[2]:
@@ -381,7 +380,7 @@ class _TimeReversalFDTD(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_field, *g_mons):
-        ctx.saved_tensors  # noqa: B018 -- raises if eps was modified in place
+        _, ckpt, tail, state = ctx.saved_tensors  # raises if eps was modified in place
         plan = ctx.plan
         eng = plan.engine
         T, K = plan.n_steps, plan.ckpt_every
@@ -397,7 +396,7 @@ class _TimeReversalFDTD(torch.autograd.Function):
         S = torch.empty(N, dtype=dt, device=dev)
         srcs = ctx.srcs
         mons = eng.monitors(plan.monitors, T, grads=list(g_mons))
-        eng.adjoint_init(lam, ctx.state[:ne * N], mons[:2], T, S)
+        eng.adjoint_init(lam, state[:ne * N], mons[:2], T, S)
         nseg = math.ceil(T / K)
         estore = torch.empty(max(K - 1, 1) * ne * N, dtype=dt, device=dev)
         work = torch.empty(ns * N, dtype=dt, device=dev)
@@ -406,10 +405,10 @@ class _TimeReversalFDTD(torch.autograd.Function):
         ws = eng.adjoint_workspace(mons[:2])
         for k in range(nseg - 1, -1, -1):
             a, b = k * K, min(T, (k + 1) * K)
-            slot = ctx.ckpt[k * ns * N:(k + 1) * ns * N]
+            slot = ckpt[k * ns * N:(k + 1) * ns * N]
             es = estore
             if a >= ctx.a_tail:  # E^{a+1} .. E^{b-1} kept by the forward
-                es = ctx.tail[(a - ctx.a_tail) * ne * N:]
+                es = tail[(a - ctx.a_tail) * ne * N:]
             elif b - a > 1:
                 _begin("replay")
                 eng.run(work, a, b - a - 1, srcs[:2], (None, 0), estore=estore, work=work2,

@@ -526,3 +526,38 @@ def test_e_store_budget_keeps_the_final_segments():
     assert peaks[1] - peaks[0] >= 3 * K * 3 * N * 4 - 3 * N * 4
     with pytest.raises(ValueError):
         _solver3d(grid_shape=shape, n_steps=T, e_store_gb=-1.0)
+
+
+def test_backward_releases_checkpoints_while_the_graph_lives():
+    """The checkpoints, the E store of the final segments and the final
+    state are autograd-saved tensors: a backward without retain_graph frees
+    them even though the loss (and with it the graph) is still referenced;
+    with retain_graph a second backward gives the same gradient."""
+    shape, T, K = (16, 16, 64), 40, 8
+    N = int(np.prod(shape))
+    eps0 = torch.full(shape, 2.0, dtype=torch.float64)
+    src = {"type": "gaussian_pulse", "pos": [8, 8, 32]}
+    s = _solver3d(grid_shape=shape, dl=20.0, pml_layers=2, n_steps=T, courant=0.4,
+                  backward="time_reversal", checkpoint_every=K, e_store_gb=1.0)
+    eps = eps0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    loss = (s.forward(eps, source=src).field ** 2).sum()
+    torch.cuda.synchronize()
+    held = torch.cuda.memory_allocated() - base
+    # 5 checkpoints of 6 fields and E of the last 4 segments (32 steps) at least
+    assert held >= (5 * 6 + 32 * 3) * N * 4
+    loss.backward()
+    torch.cuda.synchronize()
+    after = torch.cuda.memory_allocated() - base
+    assert after < 8 * N * 8, (held, after)   # the gradient and small host-side tensors
+    with pytest.raises(RuntimeError):
+        loss.backward()   # the saved buffers are gone: autograd says so
+    g1 = eps.grad.clone()
+    eps2 = eps0.clone().requires_grad_(True)
+    loss2 = (s.forward(eps2, source=src).field ** 2).sum()
+    loss2.backward(retain_graph=True)
+    g2 = eps2.grad.clone()
+    eps2.grad = None
+    loss2.backward()
+    assert torch.equal(g1, g2) and torch.equal(eps2.grad, g2)
