@@ -15,7 +15,7 @@ import threading
 import torch
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdnx_fdtd.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 DNX_F32 = 0
 DNX_F64 = 1
@@ -68,7 +68,7 @@ class GemmArgs(C.Structure):
                 ("Cin", C.c_void_p), ("sIm", C.c_int64), ("sIn", C.c_int64), ("sIb", C.c_int64),
                 ("alpha", C.c_float), ("beta", C.c_float), ("bias", C.c_void_p),
                 ("bias_mode", C.c_int32), ("act", C.c_int32), ("pre", C.c_void_p),
-                ("a_bf16", C.c_int32), ("b_bf16", C.c_int32)]
+                ("a_bf16", C.c_int32), ("b_bf16", C.c_int32), ("c_bf16", C.c_void_p)]
 
 
 _P = C.c_void_p
@@ -102,7 +102,7 @@ _SIGS = {
     "dnx_fdtd2d_tm_batch": ([C.POINTER(Grid2D), _I, _P, _I, C.POINTER(Source), C.POINTER(Box),
                              _I, _P, _P, _P], _I),
     "dnx_gemm_bf16": ([C.POINTER(GemmArgs), _P], _I),
-    "dnx_gelu_bwd": ([_P, _P, _P, _L, _P], _I),
+    "dnx_gelu_bwd": ([_P, _P, _P, _P, _L, _P], _I),
     "dnx_sum_partials": ([_P, _I, _L, _P, C.c_float, _P], _I),
     "dnx_fno_mode_mix": ([_P, _P, _P, _P, _I, _I, _I, _L, _I, _I, _P], _I),
     "dnx_fno_mode_wgrad": ([_P, _P, _P, _P, _I, _I, _L, _I, _I, _P], _I),
@@ -159,7 +159,7 @@ def lib():
 # Kernel-selection options (DNX_OPT_* in include/dnx_fdtd.h).  They pick
 # kernel variants, never results.
 OPTIONS = {"simple": 0, "one_step": 1, "adj2": 2, "fwd2_pf": 3, "adj2_occ": 4,
-           "gemm_generic": 5, "gemm_pw_valu": 6, "gemm_small": 7}
+           "gemm_generic": 5, "gemm_pw_valu": 6, "gemm_small": 7, "gemm_big_reg": 8}
 
 
 def get_option(name):

@@ -49,6 +49,7 @@ static void init_options() {
   g_opt[DNX_OPT_GEMM_GENERIC] = env_int("DNX_GEMM_GENERIC", 0) != 0 ? 1 : 0;
   g_opt[DNX_OPT_GEMM_PW_VALU] = env_int("DNX_GEMM_PW_VALU", 0) != 0 ? 1 : 0;
   g_opt[DNX_OPT_GEMM_SMALL] = env_int("DNX_GEMM_SMALL", 0) != 0 ? 1 : 0;
+  g_opt[DNX_OPT_GEMM_BIG_REG] = env_int("DNX_GEMM_BIG_REG", 0) != 0 ? 1 : 0;
 }
 
 int option(int id) {

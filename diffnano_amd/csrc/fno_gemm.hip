@@ -147,6 +147,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         if (g.pre) g.pre[ci] = v;
         if (g.act == 1) v = gelu(v);
         g.C[ci] = v;
+        if (g.c_bf16) g.c_bf16[ci] = f2bf(v);
       }
     }
   }
@@ -565,6 +566,10 @@ __global__ __launch_bounds__(256) void k_cmix(GemmArgs g) {
       for (int j = 0; j < 4; ++j) v[j] = gelu(v[j]);
     }
     *reinterpret_cast<float4*>(g.C + ci) = float4{v[0], v[1], v[2], v[3]};
+    if (g.c_bf16)
+      *reinterpret_cast<uint2*>(g.c_bf16 + ci) =
+          uint2{uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16),
+                uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16)};
   };
   // Small K: all K input rows are loaded first (K loads in flight per
   // thread), then each output row is summed and stored.  Large K with few
@@ -625,6 +630,7 @@ static bool launch_cmix(const GemmArgs& g, hipStream_t s) {
   if (g.sBk % 4 != 0 || g.sCm % 4 != 0 || !al(g.B) || !al(g.C)) return false;
   if (g.beta != 0.f && (g.sIn != 1 || g.sIm % 4 != 0 || !al(g.Cin))) return false;
   if (g.pre && !al(g.pre)) return false;
+  if (g.c_bf16 && (reinterpret_cast<uintptr_t>(g.c_bf16) & 7u) != 0) return false;
   if (g.b_bf16 && g.sBk % 8 != 0) return false;
   return cmix_mk<32, 9>(g, s) || cmix_mk<32, 32>(g, s) || cmix_mk<128, 32>(g, s) ||
          cmix_mk<6, 128>(g, s) || cmix_mk<128, 6>(g, s) || cmix_mk<32, 128>(g, s);
@@ -862,6 +868,141 @@ static int launch_big(const GemmArgs& g, hipStream_t s) {
   return DNX_OK;
 }
 
+// ---- Long-K BF16 kernel with operands loaded straight into LDS ------------
+//
+// The products of k_gemm_big for BF16 A and B (the truncated DFTs with
+// K = 2048-16384): the same 256 x 256 x 64 tile, XCD remap, operand swap
+// and MFMA sequence per accumulator (so bitwise the same results), but
+//  * 8 waves (4 along M x 2 along N, 64 x 128 of C each, 128 accumulators
+//    per lane): two waves per SIMD, so one wave's LDS reads and barrier
+//    waits overlap the other's MFMAs;
+//  * global -> LDS without registers: global_load_lds_dwordx4 writes 16
+//    bytes per lane to LDS at a wave-uniform base + 16 * lane, a 1 KB run per
+//    instruction (8 rows of 64 k); two stages of 64 KB, the next K step in
+//    flight while the current one is multiplied, one barrier per K step;
+//  * unpadded rows (128 bytes) with the 16-byte chunk c of row r stored at
+//    chunk c ^ ((r ^ (r >> 3)) & 7): the lanes of an MFMA operand read
+//    (32 rows, one chunk) fall on distinct banks, and the DMA writes stay
+//    contiguous (each lane loads the global chunk its LDS slot holds).
+// Measured on 8192 x 1024 x 4096 (x4 split) against k_gemm_big: 311-330 vs
+// 472-499 us with FP32 A, 397-404 us with BF16 A.
+
+constexpr int LBK = 64;                      // K per stage
+constexpr int LDS_MIN_K = 1024;              // shortest reduction it takes
+constexpr int LSTAGE = 256 * LBK;            // BF16 elements of one operand's stage
+constexpr size_t LSMEM = size_t(2) * 2 * LSTAGE * sizeof(unsigned short);  // 128 KiB
+
+// 16 bytes per lane from global memory into LDS at lds + 16 * lane (lds
+// wave-uniform).  A plain function, so the host pass never instantiates the
+// device builtin inside a template.
+__device__ __forceinline__ void lds_dma16(const void* gp, void* lds) {
+  __builtin_amdgcn_global_load_lds(gp, lds, 16, 0, 0);
+}
+
+__device__ __forceinline__ int lds_swz(int r) { return (r ^ (r >> 3)) & 7; }
+
+__global__ __launch_bounds__(512, 1) void k_gemm_lds(GemmArgs g, int tiles_n, int tiles) {
+  constexpr int IPO = 4;  // load instructions per wave per operand per stage (32 / 8 waves)
+  extern __shared__ unsigned short lsm[];
+  const int nb = gridDim.x, bx = blockIdx.x;
+  const int logical = (bx & 7) * (nb >> 3) + (bx >> 3);
+  if (logical >= tiles) return;
+  const int m0 = (logical / tiles_n) * 256;
+  const int n0 = (logical % tiles_n) * 256;
+  const int b = blockIdx.z;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 128;
+  const unsigned short* A = static_cast<const unsigned short*>(g.A) + int64_t(b) * g.sAb;
+  const unsigned short* B = static_cast<const unsigned short*>(g.B) + int64_t(b) * g.sBb;
+  // Instruction j of this wave fills rows 8q .. 8q+7 (q = wave + 8 j) of a
+  // stage; lane i writes slot (row 8q + i/8, chunk i%8), i.e. the global
+  // chunk (i%8) ^ swz(row) of that row.  Rows past the matrix are clamped
+  // (their products are never stored).
+  int64_t ga[IPO], gb[IPO];
+#pragma unroll
+  for (int j = 0; j < IPO; ++j) {
+    const int row = (wave + 8 * j) * 8 + (lane >> 3);
+    const int c = (lane & 7) ^ lds_swz(row);
+    ga[j] = int64_t(min(m0 + row, g.M - 1)) * g.sAm + c * 8;
+    gb[j] = int64_t(min(n0 + row, g.N - 1)) * g.sBn + c * 8;
+  }
+  const int nk = g.K / LBK;
+  auto issue = [&](int kt) {
+    unsigned short* sa = lsm + (kt & 1) * 2 * LSTAGE;
+    unsigned short* sb = sa + LSTAGE;
+#pragma unroll
+    for (int j = 0; j < IPO; ++j) {
+      const int q = wave + 8 * j;
+      lds_dma16(A + ga[j] + int64_t(kt) * LBK, sa + q * 512);
+      lds_dma16(B + gb[j] + int64_t(kt) * LBK, sb + q * 512);
+    }
+  };
+  f32x16 acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+  const int lr = lane & 31, kh = lane >> 5;
+  issue(0);
+  for (int kt = 0; kt < nk; ++kt) {
+    // this wave's loads of step kt have landed (vmcnt counts the DMA loads;
+    // expcnt and lgkmcnt are left at their maxima), then every wave's have,
+    // and every wave is done with the stage that step kt + 1 overwrites
+    __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
+    __builtin_amdgcn_s_barrier();
+    if (kt + 1 < nk) issue(kt + 1);
+    const unsigned short* Ac = lsm + (kt & 1) * 2 * LSTAGE;
+    const unsigned short* Bc = Ac + LSTAGE;
+#pragma unroll
+    for (int ks = 0; ks < LBK; ks += 16) {
+      const int c = ks / 8 + kh;
+      bf16x8 af[2], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int r = wm + i * 32 + lr;
+        af[i] = *reinterpret_cast<const bf16x8*>(&Ac[r * LBK + ((c ^ lds_swz(r)) << 3)]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = wn + j * 32 + lr;
+        bfr[j] = *reinterpret_cast<const bf16x8*>(&Bc[r * LBK + ((c ^ lds_swz(r)) << 3)]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[j], af[i], acc[i][j], 0, 0, 0);
+    }
+  }
+  const int64_t coff = int64_t(b) * g.sCb;
+  const int64_t ioff = int64_t(b) * g.sIb;
+  auto tile = [&](const f32x16& a, int i, int j) {
+    const int m = m0 + wm + i * 32 + lr;
+    const int n = n0 + wn + j * 32 + 4 * kh;
+    big_epilogue(g, m, n, coff, ioff, f32x4{a[0], a[1], a[2], a[3]});
+    big_epilogue(g, m, n + 8, coff, ioff, f32x4{a[4], a[5], a[6], a[7]});
+    big_epilogue(g, m, n + 16, coff, ioff, f32x4{a[8], a[9], a[10], a[11]});
+    big_epilogue(g, m, n + 24, coff, ioff, f32x4{a[12], a[13], a[14], a[15]});
+  };
+  tile(acc[0][0], 0, 0); tile(acc[0][1], 0, 1); tile(acc[0][2], 0, 2); tile(acc[0][3], 0, 3);
+  tile(acc[1][0], 1, 0); tile(acc[1][1], 1, 1); tile(acc[1][2], 1, 2); tile(acc[1][3], 1, 3);
+}
+
+static int launch_lds(const GemmArgs& g, hipStream_t s) {
+  static std::atomic<bool> attr{false};
+  if (!attr.load()) {
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gemm_lds),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(LSMEM)));
+    attr = true;
+  }
+  const int tn = int(cdiv(g.N, 256)), tiles = int(cdiv(g.M, 256)) * tn;
+  const int nb = ((tiles + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_gemm_lds, dim3(nb, 1, g.batch), dim3(512), LSMEM, s, g, tn, tiles);
+  return DNX_OK;
+}
+
 // out[i] = sum_j part[j*n + i] + beta * out[i]
 __global__ void k_sum_partials(const float* __restrict__ part, int s, int64_t n,
                                float* __restrict__ out, float beta) {
@@ -897,19 +1038,27 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // four elements per thread (16-byte accesses when all three arrays are
 // 16-byte aligned and n % 4 == 0; element by element otherwise)
-__global__ void k_gelu_bwd(const float* dy, const float* __restrict__ pre, float* dx, int64_t n,
+__global__ void k_gelu_bwd(const float* dy, const float* __restrict__ pre, float* dx,
+                           unsigned short* __restrict__ dx16, int64_t n,
                            int vec) {  // dx may be dy (in place)
   const int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
   if (i >= n) return;
   if (vec) {
     const float4 x = *reinterpret_cast<const float4*>(pre + i);
     const float4 d = *reinterpret_cast<const float4*>(dy + i);
-    *reinterpret_cast<float4*>(dx + i) =
-        float4{d.x * gelu_grad(x.x), d.y * gelu_grad(x.y), d.z * gelu_grad(x.z),
-               d.w * gelu_grad(x.w)};
+    const float4 r = float4{d.x * gelu_grad(x.x), d.y * gelu_grad(x.y), d.z * gelu_grad(x.z),
+                            d.w * gelu_grad(x.w)};
+    *reinterpret_cast<float4*>(dx + i) = r;
+    if (dx16)
+      *reinterpret_cast<uint2*>(dx16 + i) = uint2{uint32_t(f2bf(r.x)) | (uint32_t(f2bf(r.y)) << 16),
+                                                  uint32_t(f2bf(r.z)) | (uint32_t(f2bf(r.w)) << 16)};
     return;
   }
-  for (int64_t q = i; q < i + 4 && q < n; ++q) dx[q] = dy[q] * gelu_grad(pre[q]);
+  for (int64_t q = i; q < i + 4 && q < n; ++q) {
+    const float r = dy[q] * gelu_grad(pre[q]);
+    dx[q] = r;
+    if (dx16) dx16[q] = f2bf(r);
+  }
 }
 
 // ---- Channel-mixing products on the matrix cores ------------------------
@@ -1021,6 +1170,9 @@ __global__ __launch_bounds__(256) void k_pw(GemmArgs g) {
       const int vo = n * 4 + 4 * half * sCm;
       const auto rc = rsrc(g.C, int64_t(mt * 32) * sCm);
       const auto rp = rsrc(g.pre ? g.pre : g.C, int64_t(mt * 32) * sCm);
+      // BF16 copy: the same element offsets at half the bytes
+      const auto r16 = rsrc(g.c_bf16 ? static_cast<void*>(g.c_bf16) : static_cast<void*>(g.C),
+                            int64_t(mt * 32) * (sCm / 2));
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int m = mb + 8 * (v >> 2) + (v & 3);
@@ -1033,6 +1185,7 @@ __global__ __launch_bounds__(256) void k_pw(GemmArgs g) {
         if (g.pre) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rp, vo, so, 0);
         if (g.act == 1) r = gelu(r);
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rc, vo, so, 0);
+        if (g.c_bf16) __builtin_amdgcn_raw_buffer_store_b16(f2bf(r), r16, vo / 2, so / 2, 0);
       }
     }
   }
@@ -1058,6 +1211,7 @@ static bool launch_pw(const GemmArgs& g, hipStream_t s) {
   if (g.batch != 1 || g.sBn != 1 || g.sCn != 1) return false;
   if (g.beta != 0.f && g.sIn != 1) return false;
   if (g.M > 128 || g.K > 128 || g.N < 32) return false;
+  if (g.c_bf16 && (reinterpret_cast<uintptr_t>(g.c_bf16) & 1u) != 0) return false;
   // byte offsets within a block of 16 rows of X or 32 rows of C, pre, Cin
   // fit in 31 bits
   const int64_t lim = int64_t(1) << 31;
@@ -1078,6 +1232,8 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
   DNX_REQUIRE(g->beta == 0.f || g->Cin, "gemm: beta needs Cin");
   DNX_REQUIRE(g->bias_mode == 0 || g->bias, "gemm: bias missing");
   DNX_REQUIRE(g->batch <= 65535, "gemm: batch > 65535");
+  DNX_REQUIRE(!g->c_bf16 || (g->M <= 128 && g->K <= 128),
+              "gemm: a BF16 copy of C is written by the channel-mixing products only");
   dim3 grid(cdiv(g->N, GBN), cdiv(g->M, GBM), g->batch);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool generic = option(DNX_OPT_GEMM_GENERIC) != 0;
@@ -1086,10 +1242,16 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
     done = true;
   } else if (!generic && launch_cmix(*g, s)) {
     done = true;
-  } else if (!generic && nt_ok(*g) && g->M >= XBM && g->N >= XBN && g->K >= XBIG_K &&
+  } else if (g->c_bf16) {
+    // a layout the mixing kernels do not take: the element-wise kernel
+    // below writes the BF16 copy too
+  } else if (!generic && nt_ok(*g) && g->M >= XBM && g->N >= XBN &&
+             g->K >= ((g->a_bf16 && g->b_bf16 && option(DNX_OPT_GEMM_BIG_REG) == 0) ? LDS_MIN_K
+                                                                                 : XBIG_K) &&
              option(DNX_OPT_GEMM_SMALL) == 0) {
     int rc;
-    if (g->a_bf16 && g->b_bf16) rc = launch_big<true, true>(*g, s);
+    if (g->a_bf16 && g->b_bf16 && option(DNX_OPT_GEMM_BIG_REG) == 0) rc = launch_lds(*g, s);
+    else if (g->a_bf16 && g->b_bf16) rc = launch_big<true, true>(*g, s);
     else if (g->a_bf16) rc = launch_big<true, false>(*g, s);
     else if (g->b_bf16) rc = launch_big<false, true>(*g, s);
     else rc = launch_big<false, false>(*g, s);
@@ -1140,13 +1302,14 @@ extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, flo
   return DNX_OK;
 }
 
-extern "C" int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, int64_t n,
-                                void* stream) {
+extern "C" int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx,
+                                unsigned short* dx_bf16, int64_t n, void* stream) {
   DNX_REQUIRE(dy && pre && dx && n > 0, "gelu_bwd: bad arguments");
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
-  const int vec = (n % 4 == 0 && al(dy) && al(pre) && al(dx)) ? 1 : 0;
+  const int vec = (n % 4 == 0 && al(dy) && al(pre) && al(dx) &&
+                   (reinterpret_cast<uintptr_t>(dx_bf16) & 7u) == 0) ? 1 : 0;
   hipLaunchKernelGGL(k_gelu_bwd, dim3(cdiv(cdiv(n, 4), 256)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), dy, pre, dx, n, vec);
+                     static_cast<hipStream_t>(stream), dy, pre, dx, dx_bf16, n, vec);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }

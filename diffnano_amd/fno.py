@@ -43,8 +43,11 @@ def _off(t, elems):
 
 
 def gemm(M, N, K, A, sA, B, sB, Cout, sC, *, batch=1, a_off=0, b_off=0, c_off=0, alpha=1.0,
-         beta=0.0, Cin=None, sI=None, i_off=0, bias=None, bias_mode=0, act=0, pre=None):
-    """C = act(alpha A B + beta Cin + bias); sA = (sAm, sAk, sAb) etc. (elements)."""
+         beta=0.0, Cin=None, sI=None, i_off=0, bias=None, bias_mode=0, act=0, pre=None,
+         c16=None):
+    """C = act(alpha A B + beta Cin + bias); sA = (sAm, sAk, sAb) etc. (elements).
+    ``c16`` (BF16, C's layout) also receives C rounded to BF16 (channel-mixing
+    products, M and K <= 128)."""
     g = GemmArgs()
     g.M, g.N, g.K, g.batch = int(M), int(N), int(K), int(batch)
     g.A, (g.sAm, g.sAk, g.sAb) = _off(A, a_off), sA
@@ -59,6 +62,10 @@ def gemm(M, N, K, A, sA, B, sB, Cout, sC, *, batch=1, a_off=0, b_off=0, c_off=0,
     g.bias = bias.data_ptr() if bias is not None else None
     g.bias_mode, g.act = int(bias_mode), int(act)
     g.pre = _off(pre, c_off) if pre is not None else None
+    if c16 is not None:
+        if c16.dtype != torch.bfloat16:
+            raise TypeError("c16 must be bfloat16")
+        g.c_bf16 = _off(c16, c_off)
     g.a_bf16 = 1 if A.dtype == torch.bfloat16 else 0
     g.b_bf16 = 1 if B.dtype == torch.bfloat16 else 0
     for t in (A, B):
@@ -87,8 +94,9 @@ def mode_wgrad(X, dY, dWr, dWi, B, M, ld, c, nsplit):
                                        int(nsplit), L.stream_handle(dY.device)))
 
 
-def gelu_bwd(dy, pre, dx):
-    L.check(L.lib().dnx_gelu_bwd(dy.data_ptr(), pre.data_ptr(), dx.data_ptr(), dy.numel(),
+def gelu_bwd(dy, pre, dx, dx16=None):
+    L.check(L.lib().dnx_gelu_bwd(dy.data_ptr(), pre.data_ptr(), dx.data_ptr(),
+                                 dx16.data_ptr() if dx16 is not None else None, dy.numel(),
                                  L.stream_handle(dy.device)))
     return dx
 
@@ -125,12 +133,16 @@ def sum_partials(part, s, n, out, beta=0.0):
     return out
 
 
-def gemm_nt(M, N, K, A, B, out, *, target_blocks=None, beta=0.0, Cin=None):
+def gemm_nt(M, N, K, A, B, out, *, target_blocks=None, beta=0.0, Cin=None, A16=None):
     """out[M x N] = A[M x K] B[N x K]^T (+ beta Cin), both operands row-major
     with K contiguous, split along K into a batch of partial products (then
     summed) when the output alone has too few tiles to fill the GPU (256 x
     256 tiles, one work-group per CU, for M, N >= 256 and long K; 128 x 128
-    tiles otherwise)."""
+    tiles otherwise).  ``A16``: A already rounded to BF16 (what the product
+    does to an FP32 A on load), read in its place -- same result, half the
+    bytes, and the direct-to-LDS kernel for BF16 operands."""
+    if A16 is not None:
+        A = A16
     big = M >= 256 and N >= 256 and K >= 4096  # split chunks keep K >= 2048
     tb = 256 if big else 128
     if target_blocks is None:
@@ -188,14 +200,19 @@ class FNO2d:
         self.names = list(self.params)
 
     # -- building blocks ------------------------------------------------------
-    def _pointwise(self, Wt, x, out, bias=None, act=0, pre=None, Cin=None, beta=0.0):
-        """out[o][n] = act(sum_i Wt[o][i] x[i][n] + bias[o] + beta Cin[o][n])."""
+    def _pointwise(self, Wt, x, out, bias=None, act=0, pre=None, Cin=None, beta=0.0, c16=None):
+        """out[o][n] = act(sum_i Wt[o][i] x[i][n] + bias[o] + beta Cin[o][n])
+        (and out rounded to BF16 into ``c16``)."""
         O, I = Wt.shape
         n = x.shape[1]
         return gemm(O, n, I, Wt, (I, 1, 0), x, (n, 1, 0), out, (n, 1, 0), bias=bias,
-                    bias_mode=2 if bias is not None else 0, act=act, pre=pre, Cin=Cin, beta=beta)
+                    bias_mode=2 if bias is not None else 0, act=act, pre=pre, Cin=Cin, beta=beta,
+                    c16=c16)
 
     fast_mix = True  # spectral mixing on the dedicated kernels where shapes allow
+    # the forward-DFT products read BF16 copies of their FP32 operand, written
+    # by the kernels that produce it (bitwise the same products)
+    bf16_copies = True
 
     def _fast_mix_ok(self, B):
         return self.fast_mix and self.width == 32 and B % 8 == 0 and self.nmodes % 16 == 0
@@ -229,11 +246,15 @@ class FNO2d:
         dev = self.dev
         x2 = x.reshape(self.in_ch, n)
         h = torch.empty(c, n, device=dev)
-        self._pointwise(p["w_lift"], x2, h, bias=p["b_lift"])
+        # h rounded to BF16 for the forward-DFT product, written by the layer
+        # that produces h (one buffer: each layer's product reads it before
+        # the layer's pointwise part overwrites it)
+        h16 = torch.empty(c, n, device=dev, dtype=torch.bfloat16) if self.bf16_copies else None
+        self._pointwise(p["w_lift"], x2, h, bias=p["b_lift"], c16=h16)
         saved = {"x": x2, "h": [], "Z": [], "pre": []}
         for l in range(self.layers):
             Z = torch.empty(c * B, M2, device=dev)
-            gemm_nt(c * B, M2, P, h, self.FT, Z)
+            gemm_nt(c * B, M2, P, h, self.FT, Z, A16=h16)
             # the inverse-DFT product rounds Y to BF16 on load: the mixing
             # kernel writes it rounded (same values, half the bytes)
             Y = torch.empty(c * B, M2, device=dev,
@@ -245,7 +266,7 @@ class FNO2d:
             hn = torch.empty(c, n, device=dev)
             pre = None if (last or not save) else torch.empty(c, n, device=dev)
             self._pointwise(p[f"w{l}"], h, hn, bias=p[f"b{l}"], act=0 if last else 1, pre=pre,
-                            Cin=S, beta=1.0)
+                            Cin=S, beta=1.0, c16=None if last else h16)
             if save:
                 saved["h"].append(h)
                 saved["Z"].append(Z)
@@ -332,17 +353,19 @@ class FNO2d:
         gelu_bwd(dq, saved["qpre"], dq)
         self._wgrad_bias(dq, saved["hL"], g["w_p1"], g["b_p1"])
         dh = torch.empty(c, n, device=dev)
-        gemm(c, n, self.hidden, p["w_p1"], (1, c, 0), dq, (n, 1, 0), dh, (n, 1, 0))
+        # d(pre) rounded to BF16 for the spectral branch's forward-DFT product
+        d16 = torch.empty(c, n, device=dev, dtype=torch.bfloat16) if self.bf16_copies else None
+        gemm(c, n, self.hidden, p["w_p1"], (1, c, 0), dq, (n, 1, 0), dh, (n, 1, 0), c16=d16)
         del dq
         for l in range(self.layers - 1, -1, -1):
             h, Z, pre = saved["h"][l], saved["Z"][l], saved["pre"][l]
-            dpre = dh if pre is None else gelu_bwd(dh, pre, torch.empty_like(dh))
+            dpre = dh if pre is None else gelu_bwd(dh, pre, torch.empty_like(dh), d16)
             self._wgrad_bias(dpre, h, g[f"w{l}"], g[f"b{l}"])
             dhn = torch.empty(c, n, device=dev)
             gemm(c, n, c, p[f"w{l}"], (1, c, 0), dpre, (n, 1, 0), dhn, (n, 1, 0))
             # spectral branch: dY = dS G^T
             dY = torch.empty(c * B, M2, device=dev)
-            gemm_nt(c * B, M2, P, dpre, self.G, dY)
+            gemm_nt(c * B, M2, P, dpre, self.G, dY, A16=d16)
             sr, si = p[f"sr{l}"], p[f"si{l}"]
             ld = M2
             sZ = (ld, B * ld, 1)       # Z/dZ/dY as [b][i] per mode

@@ -39,7 +39,7 @@ extern "C" {
 #pragma GCC visibility push(default)
 #endif
 
-#define DNX_ABI_VERSION 3
+#define DNX_ABI_VERSION 4
 
 enum { DNX_F32 = 0, DNX_F64 = 1 };
 enum {
@@ -128,7 +128,9 @@ enum {
   DNX_OPT_GEMM_GENERIC = 5,    /* 1: FNO products on the element-wise kernel [DNX_GEMM_GENERIC] */
   DNX_OPT_GEMM_PW_VALU = 6,    /* 1: FNO 1x1 layers on the vector ALU [DNX_GEMM_PW_VALU] */
   DNX_OPT_GEMM_SMALL = 7,      /* 1: FNO products on the 128x128 tile kernel [DNX_GEMM_SMALL] */
-  DNX_OPT_COUNT = 8
+  DNX_OPT_GEMM_BIG_REG = 8,    /* 1: long-K BF16 products stage operands through registers instead
+                                  of loading them straight into LDS [DNX_GEMM_BIG_REG] */
+  DNX_OPT_COUNT = 9
 };
 int32_t dnx_set_option(int32_t id, int32_t value);
 int32_t dnx_get_option(int32_t id, int32_t* value);

@@ -27,7 +27,11 @@ extern "C" {
  * FP32 (a_bf16/b_bf16 = 0) or BF16 (= 1) in memory and are multiplied in
  * BF16 on MFMA with FP32 accumulation; C, Cin, bias, pre are FP32.
  * bias_mode: 0 none, 1 per column, 2 per row.  act: 0 none, 1 GELU (erf).
- * pre: if not NULL, receives the value before the activation. */
+ * pre: if not NULL, receives the value before the activation.
+ * c_bf16: if not NULL, also receives C rounded to BF16 (nearest even), at
+ * C's element offsets -- the operand a later BF16 product reads instead of
+ * rounding C itself (same values, half the bytes); channel-mixing
+ * products only (M, K <= 128), others are rejected. */
 typedef struct dnx_gemm_args {
   int32_t M, N, K, batch;
   const void* A;
@@ -44,6 +48,7 @@ typedef struct dnx_gemm_args {
   int32_t act;
   float* pre;
   int32_t a_bf16, b_bf16;
+  unsigned short* c_bf16;
 } dnx_gemm_args;
 
 int32_t dnx_gemm_bf16(const dnx_gemm_args* args, void* stream);
@@ -53,8 +58,10 @@ int32_t dnx_gemm_bf16(const dnx_gemm_args* args, void* stream);
 int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out, float beta,
                          void* stream);
 
-/* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32. */
-int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, int64_t n, void* stream);
+/* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32; dx_bf16, if not NULL,
+ * also receives dx rounded to BF16 (nearest even). */
+int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, unsigned short* dx_bf16,
+                     int64_t n, void* stream);
 
 /* Spectral channel mixing of an FNO layer, per Fourier mode, complex, FP32:
  *   Y[q][b][r*M + mu] (r = 0 real, 1 imaginary part)

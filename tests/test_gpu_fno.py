@@ -206,7 +206,7 @@ def test_fno_backward_matches_fft_oracle():
 
 @pytest.mark.parametrize("abf,bbf", [(False, True), (True, True), (False, False)])
 @pytest.mark.parametrize("M,N,K,batch", [(256, 256, 2048, 1), (300, 522, 2112, 2),
-                                         (512, 1024, 4096, 1)])
+                                         (512, 1024, 4096, 1), (520, 768, 1024, 1)])
 def test_gemm_large_tile_kernel_matches_small_and_fp32(M, N, K, batch, abf, bbf, lib_opts):
     """The 256 x 256 MFMA 32x32x16 kernel (M, N >= 256, K >= 2048, K-contiguous operands):
     ragged M and N (vector and element epilogue paths), both bias modes,
@@ -241,6 +241,110 @@ def test_gemm_large_tile_kernel_matches_small_and_fp32(M, N, K, batch, abf, bbf,
         lib_opts.reset("gemm_small")
         torch.cuda.synchronize()
         assert rel_l2(C.cpu().numpy(), C2.cpu().numpy()) < 1e-6
+        if abf and bbf and K >= 2048:
+            # BF16 operands run on the direct-to-LDS kernel (K >= 1024); the
+            # register-staged kernel (K >= 2048) does the same MFMA sequence
+            # per accumulator: bitwise equal
+            C3, pre3 = torch.empty_like(C), torch.empty_like(C)
+            lib_opts.set("gemm_big_reg", 1)
+            gemm(M, N, K, A, (K, 1, M * K), B, (1, K, N * K), C3, (N, 1, M * N), batch=batch,
+                 alpha=0.75, beta=0.5, Cin=Cin, bias=bias, bias_mode=mode, act=1, pre=pre3)
+            lib_opts.reset("gemm_big_reg")
+            torch.cuda.synchronize()
+            assert torch.equal(C, C3) and torch.equal(pre, pre3)
+
+
+@pytest.mark.parametrize("M,N,K,batch", [(256, 256, 2048, 1), (300, 522, 2112, 2),
+                                         (8192, 1024, 4096, 2)])
+def test_lds_kernel_bf16_operand_equals_fp32_operand(M, N, K, batch, lib_opts):
+    """The long-K product with A given as BF16 (the rounded copy the FNO's
+    producers write) on the direct-to-LDS kernel equals, bit for bit, the
+    product with the FP32 A that the register-staged kernel rounds on load:
+    ragged tiles, a split-K batch and the spectral layer's own shape."""
+    from diffnano_amd.fno import gemm
+    g = torch.Generator(device="cuda").manual_seed(M + N + K + batch)
+    A = torch.randn(batch, M, K, device="cuda", generator=g)
+    B = torch.randn(batch, N, K, device="cuda", generator=g).to(torch.bfloat16)
+    outs = []
+    for a in (A, A.to(torch.bfloat16)):
+        C = torch.full((batch, M, N), float("nan"), device="cuda")
+        gemm(M, N, K, a, (K, 1, M * K), B, (1, K, N * K), C, (N, 1, M * N), batch=batch)
+        outs.append(C)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    ref = torch.bmm(_bf(A)[:1], B[:1].float().transpose(1, 2).contiguous())
+    assert rel_l2(outs[1][:1].cpu().numpy(), ref.cpu().numpy()) < 2e-6
+
+
+@pytest.mark.parametrize("M,K", [(32, 9), (32, 32), (128, 32), (6, 128), (32, 128)])
+@pytest.mark.parametrize("path", ["default", "valu", "generic"])
+def test_bf16_copy_of_the_mixing_output(M, K, path, lib_opts):
+    """c16 receives C rounded to nearest even (torch's float -> bfloat16),
+    on every kernel that takes the mixing products; other products reject it."""
+    from diffnano_amd.fno import gemm
+    N = 3000
+    g = torch.Generator(device="cuda").manual_seed(M + 17 * K)
+    W = torch.randn(M, K, device="cuda", generator=g)
+    X = torch.randn(K, N, device="cuda", generator=g)
+    Cin = torch.randn(M, N, device="cuda", generator=g)
+    bias = torch.randn(M, device="cuda", generator=g)
+    if path == "valu":
+        lib_opts.set("gemm_pw_valu", 1)
+    if path == "generic":
+        lib_opts.set("gemm_generic", 1)
+    C = torch.empty(M, N, device="cuda")
+    c16 = torch.zeros(M, N, device="cuda", dtype=torch.bfloat16)
+    gemm(M, N, K, W, (K, 1, 0), X, (N, 1, 0), C, (N, 1, 0), beta=1.0, Cin=Cin, bias=bias,
+         bias_mode=2, act=1, c16=c16)
+    torch.cuda.synchronize()
+    assert torch.equal(c16, C.to(torch.bfloat16))
+    big = torch.randn(256, 2048, device="cuda")
+    with pytest.raises(RuntimeError, match="BF16 copy"):
+        gemm(256, 256, 2048, big, (2048, 1, 0), big, (1, 2048, 0),
+             torch.empty(256, 256, device="cuda"), (256, 1, 0),
+             c16=torch.empty(256, 256, device="cuda", dtype=torch.bfloat16))
+
+
+@pytest.mark.parametrize("n", [4096, 1001])
+def test_gelu_backward_bf16_copy(n):
+    from diffnano_amd.fno import gelu_bwd
+    g = torch.Generator(device="cuda").manual_seed(n)
+    dy = torch.randn(n, device="cuda", generator=g)
+    pre = torch.randn(n, device="cuda", generator=g)
+    dx, d16 = torch.empty_like(dy), torch.empty(n, device="cuda", dtype=torch.bfloat16)
+    gelu_bwd(dy, pre, dx, d16)
+    x = pre.double()
+    ref = dy.double() * (0.5 * (1 + torch.erf(x / 2 ** 0.5)) + x * torch.exp(-x * x / 2) /
+                         (2 * torch.pi) ** 0.5)
+    torch.cuda.synchronize()
+    assert rel_l2(dx.cpu().numpy(), ref.cpu().numpy()) < 1e-6
+    assert torch.equal(d16, dx.to(torch.bfloat16))
+
+
+def test_fno_bf16_copies_change_nothing():
+    """The forward-DFT products read BF16 copies of h and d(pre) written by
+    their producers (the direct-to-LDS kernel at this size: 128 x 128 grid,
+    K = 16384): output and every parameter gradient bitwise equal to the
+    products rounding the FP32 operands themselves."""
+    from diffnano_amd.fno import FNO2d
+    m = FNO2d(128, 128, in_ch=3, out_ch=6, width=32, modes=(16, 16), layers=2, hidden=32,
+              device="cuda", seed=4)
+    torch.manual_seed(5)
+    x = torch.randn(8, 3, 128, 128, device="cuda")
+    w = torch.randn(8, 6, 128, 128, device="cuda")
+    res = []
+    for copies in (True, False):
+        m.bf16_copies = copies
+        for v in m.params.values():
+            v.grad = None
+        out = m(x)
+        (out * w).sum().backward()
+        torch.cuda.synchronize()
+        res.append((out.detach().clone(), {k: v.grad.clone() for k, v in m.params.items()}))
+    m.bf16_copies = True
+    assert torch.equal(res[0][0], res[1][0])
+    for k in m.names:
+        assert torch.equal(res[0][1][k], res[1][1][k]), k
 
 
 @pytest.mark.parametrize("conj", [False, True])
