@@ -205,3 +205,12 @@ def check_eps_finite(eps: torch.Tensor, mode: str) -> bool:
             raise ValueError(msg)
         warnings.warn(msg, RuntimeWarning, stacklevel=3)
     return finite
+
+
+def spare_device_bytes(dev):
+    """Device memory a solve may take: what the driver reports free plus what
+    torch's caching allocator holds without using it (its cached blocks are
+    handed out before the driver is asked for more)."""
+    import torch
+    free, _ = torch.cuda.mem_get_info(dev)
+    return int(free + torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev))

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from ._common import phase_table
+from ._common import phase_table, spare_device_bytes
 
 
 class PhaseTimer:
@@ -283,8 +283,7 @@ def _tail_segments(plan, nseg, K, seg_bytes):
         return 0
     budget = plan.e_store_bytes
     if budget is None:
-        free, _ = torch.cuda.mem_get_info(eng.dev)
-        budget = max(0, (free - _KEEP_FREE) // 2)
+        budget = max(0, (spare_device_bytes(eng.dev) - _KEEP_FREE) // 2)
     return int(min(nseg - 1, int(budget) // seg_bytes))
 
 

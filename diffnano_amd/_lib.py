@@ -56,7 +56,8 @@ class Design3D(C.Structure):
 
 class Pools3D(C.Structure):
     _fields_ = [("state", C.c_void_p * 2), ("lam", C.c_void_p * 2), ("ckpt", C.c_void_p),
-                ("estore", C.c_void_p), ("S", C.c_void_p), ("side", C.c_void_p)]
+                ("estore", C.c_void_p), ("S", C.c_void_p), ("side", C.c_void_p),
+                ("etail", C.c_void_p), ("a_tail", C.c_int32)]
 
 
 class GemmArgs(C.Structure):

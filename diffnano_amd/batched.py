@@ -28,7 +28,7 @@ import torch
 
 from . import _lib as L
 from ._common import (COMPONENTS_3D, comp_mask_3d, cpml_profile, monitor_box, phase_table,
-                      source_box_3d, source_profile, waveform_table)
+                      source_box_3d, source_profile, spare_device_bytes, waveform_table)
 from ._engine import _begin, _end
 
 __all__ = ["BatchedFDTD3D"]
@@ -44,7 +44,7 @@ class BatchedFDTD3D:
 
     def __init__(self, grid_shapes, sources, monitors, dl=25.0, wavelength_nm=1550.0,
                  pml_layers=12, n_steps=3000, courant=0.4, checkpoint_every=64, device=None,
-                 dtype=torch.float32, use_graph=True):
+                 dtype=torch.float32, use_graph=True, e_store_gb=None):
         if len(sources) != len(grid_shapes) or len(monitors) != len(grid_shapes):
             raise ValueError("one source and one monitor per design")
         self.shapes = [tuple(int(v) for v in s) for s in grid_shapes]
@@ -70,6 +70,22 @@ class BatchedFDTD3D:
         self.ckpt = torch.empty(nseg * 6 * tot, dtype=dt, device=dev)
         self.estore = torch.empty(max(self.K - 1, 1) * 3 * tot, dtype=dt, device=dev)
         self.S = torch.zeros(tot, dtype=dt, device=dev)
+        # E of every step of the final segments, kept by a recording forward
+        # so that the backward skips their replay: as many segments as
+        # e_store_gb (default: half the free device memory above 16 GiB)
+        # holds, at most all but the first
+        if e_store_gb is not None and not e_store_gb >= 0:
+            raise ValueError(f"e_store_gb must be None or >= 0, got {e_store_gb!r}")
+        elem = torch.tensor([], dtype=dt).element_size()
+        seg_bytes = self.K * 3 * tot * elem
+        if e_store_gb is None:
+            budget = max(0, (spare_device_bytes(dev) - (16 << 30)) // 2)
+        else:
+            budget = int(e_store_gb * 1e9)
+        self.tail_segments = int(min(max(nseg - 1, 0), budget // seg_bytes))
+        self.a_tail = (nseg - self.tail_segments) * self.K if self.tail_segments else self.n_steps
+        self.etail = (torch.empty((self.n_steps - self.a_tail) * 3 * tot, dtype=dt, device=dev)
+                      if self.tail_segments else None)
         self.ce = torch.ones(tot, dtype=dt, device=dev)
         self.eps = torch.ones(tot, dtype=torch.float64, device=dev)
         self._keep = []
@@ -128,6 +144,8 @@ class BatchedFDTD3D:
         pools.ckpt = self.ckpt.data_ptr()
         pools.estore = self.estore.data_ptr()
         pools.S = self.S.data_ptr()
+        pools.etail = self.etail.data_ptr() if self.etail is not None else None
+        pools.a_tail = self.a_tail if self.etail is not None else 0
         side_n = int(lib.dnx_batch3d_side_elems(arr, self.n))
         if side_n < 0:
             raise ValueError("BatchedFDTD3D: a grid (nx <= 256) or monitor box the batched "

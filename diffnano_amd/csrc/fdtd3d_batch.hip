@@ -36,7 +36,7 @@ struct dnx_batch3d_plan {
   std::vector<int64_t> off;  // sum_{e<d} N_e
   std::vector<int64_t> side_off;  // element offset of design d's monitor buffer
   bool use_graph = false;
-  hipGraphExec_t g_fwd = nullptr, g_rep = nullptr, g_adj = nullptr;
+  hipGraphExec_t g_fwd = nullptr, g_rep = nullptr, g_adj = nullptr, g_ftail = nullptr;
   std::vector<int2> map_f2, map_a2;  // block maps of the two-step launches (host copies)
   int2* d_map_f2 = nullptr;
   int2* d_map_a2 = nullptr;
@@ -108,6 +108,8 @@ static void fill_table(dnx_batch3d_plan* p, const dnx_design3d* ds, int kernel,
     b.buf[1] = static_cast<T*>(bufs[1]) + 6 * p->off[d];
     b.ckpt = static_cast<T*>(p->pools.ckpt) + 6 * p->off[d];
     b.estore = p->pools.estore ? static_cast<T*>(p->pools.estore) + 3 * p->off[d] : nullptr;
+    b.etail = p->pools.etail ? static_cast<T*>(p->pools.etail) + 3 * p->off[d] : nullptr;
+    b.a_tail = p->pools.etail ? p->pools.a_tail : p->T;
     b.ckpt_stride = p->sum6;
     b.estore_stride = p->sum3;
     b.tile_start = start;
@@ -234,39 +236,44 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
 
 // ---- step sequences (each reads the device counter) ----------------------
 
-// Forward (store = replay with E storage): step pairs on the two-step
-// path, a single step when one remains (only at the end of a sequence, so
-// every pair starts at an even step).
+// Forward: step pairs on the two-step path, a single step when one
+// remains (only at the end of a sequence, so every pair starts at an even
+// step).  store: 0 none, 1 the replay (E into the replay store, no
+// monitors), 2 a recording forward over the final segments (E into their
+// store, monitors accumulated).
 template <typename T>
-static int seq_forward(dnx_batch3d_plan* p, int steps, bool store, hipStream_t s,
+static int seq_forward(dnx_batch3d_plan* p, int steps, int store, hipStream_t s,
                        bool timed = true) {
+  static const char* names2[3] = {"batch3d_fwd2", "batch3d_fwd2_store", "batch3d_fwd2_etail"};
+  static const char* names1[3] = {"batch3d_fwd", "batch3d_fwd_store", "batch3d_fwd_etail"};
+  const bool mon = store != 1;
   int k = 0;
   int rc;
   if constexpr (std::is_same<T, float>::value) {
     if (p->fpairs) {
       Batch3Launch L = p->Lf2;
-      L.store = store ? 1 : 0;
+      L.store = store;
       for (; k + 1 < steps; k += 2) {
         TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
         rc = launch_batch3_pair<T>(L, false, p->ty_f2, p->max_w, p->grid_f2, s);
-        timed_end(store ? "batch3d_fwd2_store" : "batch3d_fwd2", s, tl);
+        timed_end(names2[store], s, tl);
         if (rc) return rc;
-        if (!store && (rc = launch_batch_monitor_pair<T>(p->d_fwd2, p->n, p->max_mon_cells,
-                                                          p->d_ctr, false, s)))
+        if (mon && (rc = launch_batch_monitor_pair<T>(p->d_fwd2, p->n, p->max_mon_cells,
+                                                       p->d_ctr, false, s)))
           return rc;
         if ((rc = launch_counter_add(p->d_ctr, 2, s))) return rc;
       }
     }
   }
   Batch3Launch L = p->Lf;
-  L.store = store ? 1 : 0;
+  L.store = store;
   for (; k < steps; ++k) {
     TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
     rc = launch_batch3<T>(L, false, p->ty_f, p->max_w, p->grid_f, s);
-    timed_end(store ? "batch3d_fwd_store" : "batch3d_fwd", s, tl);
+    timed_end(names1[store], s, tl);
     if (rc) return rc;
-    if (!store && (rc = launch_batch_monitor<T>(p->d_fwd, p->n, p->max_mon_cells, p->K,
-                                                 p->d_ctr, false, 0, L.layout, s)))
+    if (mon && (rc = launch_batch_monitor<T>(p->d_fwd, p->n, p->max_mon_cells, p->K,
+                                              p->d_ctr, false, 0, L.layout, s)))
       return rc;
     if ((rc = launch_counter_add(p->d_ctr, 1, s))) return rc;
   }
@@ -319,7 +326,7 @@ static int capture(dnx_batch3d_plan* p, int which, int steps, hipGraphExec_t* ou
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     rc = which == 2 ? seq_adjoint<T>(p, steps, cs, false)
-                    : seq_forward<T>(p, steps, which == 1, cs, false);
+                    : seq_forward<T>(p, steps, which == 3 ? 2 : which, cs, false);
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (rc == DNX_OK && e2 != hipSuccess) {
       set_error("batch3d: graph capture failed: %s", hipGetErrorString(e2));
@@ -342,12 +349,14 @@ static int capture(dnx_batch3d_plan* p, int which, int steps, hipGraphExec_t* ou
   return rc;
 }
 
-// Run `steps` steps of a sequence: from the graph when the count matches a
-// captured full segment, else by direct launches.
+// Run `steps` steps of a sequence (which: 0 forward, 1 replay, 2 adjoint,
+// 3 recording forward over the final segments): from the graph when the
+// count matches a captured full segment, else by direct launches.
 template <typename T>
 static int run_seq(dnx_batch3d_plan* p, int which, int steps, hipStream_t s) {
   if (steps <= 0) return DNX_OK;
-  hipGraphExec_t* slot = which == 0 ? &p->g_fwd : which == 1 ? &p->g_rep : &p->g_adj;
+  hipGraphExec_t* slot = which == 0 ? &p->g_fwd : which == 1 ? &p->g_rep
+                         : which == 2 ? &p->g_adj : &p->g_ftail;
   const int full = which == 1 ? p->K - 1 : p->K;
   if (p->use_graph && steps == full) {
     if (*slot == nullptr) {
@@ -362,7 +371,8 @@ static int run_seq(dnx_batch3d_plan* p, int which, int steps, hipStream_t s) {
       return DNX_OK;
     }
   }
-  return which == 2 ? seq_adjoint<T>(p, steps, s) : seq_forward<T>(p, steps, which == 1, s);
+  return which == 2 ? seq_adjoint<T>(p, steps, s)
+                    : seq_forward<T>(p, steps, which == 3 ? 2 : which, s);
 }
 
 template <typename T>
@@ -376,7 +386,8 @@ static int batch_forward(dnx_batch3d_plan* p, bool record, hipStream_t s) {
                                    p->pools.state[batch_in_slot(p->fpairs, k0)],
                                    sizeof(T) * p->sum6, hipMemcpyDeviceToDevice, s));
     }
-    if ((rc = run_seq<T>(p, 0, len, s))) return rc;
+    const bool tail = record && p->pools.etail && k0 >= p->pools.a_tail;
+    if ((rc = run_seq<T>(p, tail ? 3 : 0, len, s))) return rc;
   }
   return DNX_OK;
 }
@@ -403,7 +414,8 @@ static int batch_backward(dnx_batch3d_plan* p, hipStream_t s) {
   for (int k = nseg - 1; k >= 0; --k) {
     const int a = k * p->K;
     const int len = std::min(p->K, T_ - a);
-    if (len > 1) {
+    // segments after a_tail: E of every step was kept by the forward
+    if (len > 1 && !(p->pools.etail && a >= p->pools.a_tail)) {
       DNX_CHECK_HIP(hipMemcpyAsync(p->pools.state[batch_in_slot(p->fpairs, a)],
                                    static_cast<const T*>(p->pools.ckpt) + int64_t(k) * p->sum6,
                                    sizeof(T) * p->sum6, hipMemcpyDeviceToDevice, s));
@@ -515,6 +527,10 @@ extern "C" int32_t dnx_batch3d_create(const dnx_design3d* designs, int32_t n, in
   DNX_REQUIRE(pools->state[0] && pools->state[1] && pools->lam[0] && pools->lam[1] &&
                   pools->ckpt && pools->S && (ckpt_every < 2 || pools->estore),
               "batch3d_create: null pool");
+  DNX_REQUIRE(!pools->etail || (pools->a_tail >= 0 && pools->a_tail < n_steps &&
+                                pools->a_tail % ckpt_every == 0),
+              "batch3d_create: a_tail must be a multiple of ckpt_every in [0, n_steps) (got %d)",
+              pools->a_tail);
   const int dtype = designs[0].grid.dtype;
   auto* p = new dnx_batch3d_plan();
   p->dtype = dtype;
@@ -591,6 +607,7 @@ extern "C" int32_t dnx_batch3d_destroy(dnx_batch3d_plan* p) {
   if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
   if (p->g_rep) (void)hipGraphExecDestroy(p->g_rep);
   if (p->g_adj) (void)hipGraphExecDestroy(p->g_adj);
+  if (p->g_ftail) (void)hipGraphExecDestroy(p->g_ftail);
   delete p;
   return DNX_OK;
 }

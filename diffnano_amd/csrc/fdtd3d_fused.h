@@ -43,6 +43,8 @@ struct Batch3Design {
   T* buf[2];              // ping-pong buffers (forward state or adjoint), 6N each
   T* ckpt;                // checkpoint of step k*K at ckpt + k*ckpt_stride (6N)
   T* estore;              // replay E of step kK+1+j at estore + j*estore_stride (3N)
+  T* etail;               // E of step a_tail+1+j at etail + j*estore_stride (3N), kept by
+  int a_tail;             //   the forward for the final segments (a_tail = T: none)
   int64_t ckpt_stride;
   int64_t estore_stride;
   const T* wave;          // source waveform table
@@ -67,12 +69,32 @@ __host__ __device__ inline int batch_in_slot(int layout, int j) {
   return layout ? ((j + 1) >> 1) & 1 : j & 1;
 }
 
+// E^m for the adjoint at step m: the checkpoint (m % K == 0), the final
+// segments' store (m > a_tail) or the replay store of the current segment.
+template <typename T>
+__device__ inline const T* batch_em(const Batch3Design<T>& D, int m, int K) {
+  const int r = m % K;
+  if (r == 0) return D.ckpt + int64_t(m / K) * D.ckpt_stride;
+  if (m > D.a_tail) return D.etail + int64_t(m - D.a_tail - 1) * D.estore_stride;
+  return D.estore + int64_t(r - 1) * D.estore_stride;
+}
+
+// Where the forward step from n stores E^{n+1}: store 1 = the replay store
+// (slot n % K), 2 = the final segments' store (slot n - a_tail), 0 = nowhere.
+template <typename T>
+__device__ inline T* batch_es(const Batch3Design<T>& D, int store, int n, int K) {
+  if (store == 2) return D.etail + int64_t(n - D.a_tail) * D.estore_stride;
+  if (store == 1) return D.estore + int64_t(n % K) * D.estore_stride;
+  return nullptr;
+}
+
 struct Batch3Launch {
   const void* designs;  // device array of Batch3Design<T>
   int n_designs;
   int total_tiles;
   int K;                // checkpoint interval
-  int store;            // forward: also store E (replay)
+  int store;            // forward: also store E (1: replay store, 2: final segments' store,
+                        // monitors accumulated as without a store)
   const int* step;      // device step counter
   int layout;           // buffer layout (batch_in_slot)
   // two-step launches: one work-group per entry, (design, tile of that

@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k3_batch_fwd(Batch3Launch L) {
     Fused3Args<T> a = D[d].a;
     a.in = D[d].buf[ib];
     a.out = D[d].buf[ib ^ 1];
-    a.es = L.store ? D[d].estore + int64_t(n % L.K) * D[d].estore_stride : nullptr;
+    a.es = batch_es(D[d], L.store, n, L.K);
     if (a.src_comp >= 0) a.src_wave = D[d].wave + n;
     const int lt = t - D[d].tile_start;
     fwd_tile<T, TY>(a, (lt % a.ty_tiles) * TY, (lt / a.ty_tiles) * a.zc, sm);
@@ -560,9 +560,7 @@ __global__ __launch_bounds__(256) void k3_batch_adj(Batch3Launch L) {
     Fused3Args<T> a = D[d].a;
     a.in = D[d].buf[ib];
     a.out = D[d].buf[ib ^ 1];
-    const int r = m % L.K;
-    a.em = r == 0 ? D[d].ckpt + int64_t(m / L.K) * D[d].ckpt_stride
-                  : D[d].estore + int64_t(r - 1) * D[d].estore_stride;
+    a.em = batch_em(D[d], m, L.K);
     if (a.src_comp >= 0) a.src_wave = D[d].wave + m;
     const int lt = t - D[d].tile_start;
     adj_tile<T, TY>(a, (lt % a.ty_tiles) * TY, (lt / a.ty_tiles) * a.zc, sm);
@@ -621,9 +619,7 @@ __global__ void k_batch_monitor(const Batch3Design<T>* D, int K, const int* step
     }
   } else {
     T* Lm = dz.buf[obuf];
-    const int r = k % K;
-    const T* Em = r == 0 ? dz.ckpt + int64_t(k / K) * dz.ckpt_stride
-                         : dz.estore + int64_t(r - 1) * dz.estore_stride;
+    const T* Em = batch_em(dz, k, K);
     const double* ph = dz.m_phase + int64_t(k - 1) * nf * 2;
     int ci = 0;
     T s = T(0);

@@ -1501,15 +1501,9 @@ template int launch_fused2_adj<float>(const Fused2Args<float>&, int, hipStream_t
 // its monitor samples (a recompute after the launch would cost more here:
 // the batch's monitor planes are many and strided).  Buffers
 // follow layout 1 of batch_in_slot; E^m of the adjoint comes from the
-// checkpoint (m % K == 0) or the replay store, as in k3_batch_adj.
+// checkpoint (m % K == 0), the final segments' store or the replay store
+// (batch_em), as in k3_batch_adj.
 // ---------------------------------------------------------------------------
-
-template <typename T>
-__device__ inline const T* batch2_em(const Batch3Design<T>& Dd, int m, int K) {
-  const int r = m % K;
-  return r == 0 ? Dd.ckpt + int64_t(m / K) * Dd.ckpt_stride
-                : Dd.estore + int64_t(r - 1) * Dd.estore_stride;
-}
 
 template <typename T>
 __device__ inline void batch2_mon(const Batch3Design<T>& Dd, bool on, Fused2Args<T>& f) {
@@ -1532,12 +1526,12 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
   f.a = Dd.a;
   f.a.in = Dd.buf[ib];
   f.a.out = Dd.buf[ib ^ 1];
-  f.a.es = L.store ? Dd.estore + int64_t(n % L.K) * Dd.estore_stride : nullptr;
-  f.es2 = L.store ? Dd.estore + int64_t(n % L.K + 1) * Dd.estore_stride : nullptr;
+  f.a.es = batch_es(Dd, L.store, n, L.K);
+  f.es2 = batch_es(Dd, L.store, n + 1, L.K);
   f.a.src_wave = Dd.wave;
   f.n = n;
   const int lt = bt.y;
-  batch2_mon(Dd, !L.store, f);
+  batch2_mon(Dd, L.store != 1, f);
   fwd2_tile<T, TY, false, true, true, false, false, LW>(f, (lt % f.a.ty_tiles) * TY,
                                        (lt / f.a.ty_tiles) * f.a.zc,
                                        reinterpret_cast<T*>(smem_raw));
@@ -1555,8 +1549,8 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
   f.a = Dd.a;
   f.a.in = Dd.buf[ib];
   f.a.out = Dd.buf[ib ^ 1];
-  f.a.em = batch2_em(Dd, m, L.K);
-  f.em2 = batch2_em(Dd, m - 1, L.K);
+  f.a.em = batch_em(Dd, m, L.K);
+  f.em2 = batch_em(Dd, m - 1, L.K);
   f.a.src_wave = Dd.wave;
   f.n = m;
   batch2_mon(Dd, true, f);

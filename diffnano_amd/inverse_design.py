@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import workloads
+from ._common import spare_device_bytes
 from .batched import BatchedFDTD3D
 from .fdtd3d import FDTDSolver3D
 
@@ -58,15 +59,41 @@ def batched_problems(n_designs=48, seed=0, dl=25.0, pml_layers=12, classes=GRID_
             for d, c in enumerate(picks)]
 
 
+def _group_plans(problems, group, dl, pml_layers, n_steps, checkpoint_every, dev, use_graph):
+    """Batched plans over consecutive groups of ``group`` designs (all in
+    one plan when None), the device memory left after their buffers split
+    evenly into their stores of the final segments' E (16 GiB kept free)."""
+    n = len(problems)
+    g = n if not group else max(1, min(int(group), n))
+    chunks = [list(range(i, min(n, i + g))) for i in range(0, n, g)]
+    K = max(2, int(checkpoint_every) + (int(checkpoint_every) & 1))
+    nseg = -(-n_steps // K)
+    cells = [sum(int(np.prod(problems[i].shape)) for i in c) for c in chunks]
+    base = sum((nseg * 6 + (K - 1) * 3 + 12 * 2 + 3) * c * 4 for c in cells)
+    budget = max(0, spare_device_bytes(dev) - (16 << 30) - base) * 0.9 / len(chunks)
+    plans = []
+    for c in chunks:
+        ps = [problems[i] for i in c]
+        plans.append((c, BatchedFDTD3D([p.shape for p in ps], [p.source for p in ps],
+                                       [p.monitor for p in ps], dl=dl, pml_layers=pml_layers,
+                                       n_steps=n_steps, courant=0.4,
+                                       checkpoint_every=checkpoint_every, device=dev,
+                                       use_graph=use_graph, e_store_gb=budget / 1e9)))
+    return plans
+
+
 def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layers=12,
               checkpoint_every=64, mode="batched", device="cuda", use_graph=True, log=None,
-              tiles=None):
+              tiles=None, group=None):
     """Run `iterations` Adam steps on every design.  Returns dict with the
     FoM history (iterations x designs), final rho, wall time (s) and
     design-iterations per second.  ``tiles``: batched mode, one autotuner
     result ({"forward": (ty, zc), "adjoint": ..., "forward2": ...,
     "adjoint2": ...}) applied before the plan is built; plain mode, a dict
-    shape -> such a result applied before each design's solve."""
+    shape -> such a result applied before each design's solve.  ``group``:
+    batched mode, designs per batched plan (None: one plan for all); smaller
+    groups leave device memory for E of more final segments, whose replay
+    the backward then skips."""
     from . import autotune
 
     dev = torch.device(device)
@@ -78,10 +105,8 @@ def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layer
     opt = torch.optim.Adam(rhos, lr=lr)
     hist = []
     if mode == "batched":
-        solver = BatchedFDTD3D([p.shape for p in problems], [p.source for p in problems],
-                               [p.monitor for p in problems], dl=dl, pml_layers=pml_layers,
-                               n_steps=n_steps, courant=0.4, checkpoint_every=checkpoint_every,
-                               device=dev, use_graph=use_graph)
+        plans = _group_plans(problems, group, dl, pml_layers, n_steps, checkpoint_every, dev,
+                             use_graph)
     else:
         solvers = {}
         for p in problems:
@@ -95,9 +120,13 @@ def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layer
     for it in range(iterations):
         opt.zero_grad()
         if mode == "batched":
-            outs = solver.forward([p.eps(r, bg) for p, r, bg in zip(problems, rhos, bgs)])
-            foms = [(F.abs() ** 2).sum() for F in outs]
-            (-sum(foms)).backward()
+            foms = [None] * len(problems)
+            for idx, solver in plans:
+                outs = solver.forward([problems[i].eps(rhos[i], bgs[i]) for i in idx])
+                fg = [(F.abs() ** 2).sum() for F in outs]
+                (-sum(fg)).backward()
+                for i, f in zip(idx, fg):
+                    foms[i] = f
         else:
             foms = []
             for p, r, bg in zip(problems, rhos, bgs):

@@ -263,7 +263,10 @@ typedef struct dnx_design3d {
  * ckpt: ceil(T/K) slots of sum 6N; estore: K-1 slots of sum 3N.  side:
  * dnx_batch3d_side_elems() elements (monitor staging of the float32
  * two-step kernels); null runs the one-step kernels when a design has a
- * monitor. */
+ * monitor.  etail (may be null): T - a_tail slots of sum 3N in which the
+ * recording forward keeps E of every step after a_tail (a multiple of K,
+ * 0 <= a_tail < T), so that the backward runs those segments' adjoint
+ * without replaying them. */
 typedef struct dnx_batch3d_pools {
   void* state[2];
   void* lam[2];
@@ -271,6 +274,8 @@ typedef struct dnx_batch3d_pools {
   void* estore;
   void* S;
   void* side;
+  void* etail;
+  int32_t a_tail;
 } dnx_batch3d_pools;
 
 typedef struct dnx_batch3d_plan dnx_batch3d_plan;  /* opaque host object */

@@ -89,19 +89,44 @@ def test_autotuner_measures_caches_and_applies(tmp_path, monkeypatch):
             autotune.set_tiles(torch.float32, k, ty, zc)
 
 
-def _batched_run(T, K, opts):
+def _batched_run(T, K, opts, e_store_gb=None, dtype=torch.float32, tails=None):
     from diffnano_amd import _lib as L
     from diffnano_amd.batched import BatchedFDTD3D
     shapes, eps, srcs, mons = _cases()
     with L.options(**opts):  # the plan picks its kernels when it is built
         b = BatchedFDTD3D(shapes, [dict(s) for s in srcs], mons, dl=25.0, pml_layers=3,
                           n_steps=T, courant=0.4, checkpoint_every=K, device="cuda",
-                          use_graph=True)
+                          use_graph=True, e_store_gb=e_store_gb, dtype=dtype)
+    if tails is not None:
+        tails.append(b.tail_segments)
     es = [torch.tensor(e, device="cuda", requires_grad=True) for e in eps]
     outs = b.forward(es)
     fields = [f.clone() for f in b.final_fields()]
     sum((F.abs() ** 2).sum() for F in outs).backward()
     return b.uses_pairs, [o.detach().clone() for o in outs], fields, [e.grad.clone() for e in es]
+
+
+@pytest.mark.parametrize("T,K", [(50, 8), (37, 6), (9, 2)])
+@pytest.mark.parametrize("opts,dtype", [({}, torch.float32), ({"one_step": 1}, torch.float32),
+                                        ({}, torch.float64)])
+def test_batched_e_store_of_final_segments_changes_nothing(T, K, opts, dtype):
+    """A recording forward that keeps E of the final segments (so the
+    backward skips their replay) gives bitwise the monitors, final fields
+    and gradients of the plan that replays every segment: no store, a
+    two-segment store, and the default (every segment but the first)."""
+    shapes = _cases()[0]
+    tot = sum(int(np.prod(s)) for s in shapes)
+    seg_gb = K * 3 * tot * (4 if dtype == torch.float32 else 8) / 1e9
+    tails = []
+    runs = [_batched_run(T, K, opts, e_store_gb=g, dtype=dtype, tails=tails)
+            for g in (0.0, 2.5 * seg_gb, None)]
+    nseg = -(-T // K)
+    assert tails == [0, min(2, nseg - 1), nseg - 1]
+    ref = runs[0]
+    for run in runs[1:]:
+        assert run[0] == ref[0]
+        for a, b in zip(run[1] + run[2] + run[3], ref[1] + ref[2] + ref[3]):
+            assert torch.equal(a, b)
 
 
 @pytest.mark.parametrize("T,K", [(50, 8), (37, 6), (21, 4), (9, 2)])
