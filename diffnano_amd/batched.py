@@ -31,11 +31,48 @@ from ._common import (COMPONENTS_3D, comp_mask_3d, cpml_profile, monitor_box, ph
                       source_box_3d, source_profile, spare_device_bytes, waveform_table)
 from ._engine import _begin, _end
 
-__all__ = ["BatchedFDTD3D"]
+__all__ = ["BatchedFDTD3D", "SharedStores"]
 
 
 Design3D = L.Design3D
 Pools3D = L.Pools3D
+
+
+def _tail_count(nseg, seg_bytes, e_store_gb, dev):
+    """Final segments whose E fits the budget (``e_store_gb``, or half the
+    spare device memory above 16 GiB), at most all but the first."""
+    if e_store_gb is not None and not e_store_gb >= 0:
+        raise ValueError(f"e_store_gb must be None or >= 0, got {e_store_gb!r}")
+    if e_store_gb is None:
+        budget = max(0, (spare_device_bytes(dev) - (16 << 30)) // 2)
+    else:
+        budget = int(e_store_gb * 1e9)
+    return int(min(max(nseg - 1, 0), budget // max(seg_bytes, 1)))
+
+
+class SharedStores:
+    """Checkpoints, replay store and final-segment E store for several
+    batched plans that run one forward + backward at a time (the design
+    loop's groups): sized for the largest plan (``max_cells``), allocated
+    once.  The plans share a forward counter, so a backward after another
+    sharing plan's forward raises (its checkpoints are gone)."""
+
+    def __init__(self, max_cells, n_steps, checkpoint_every=64, device=None,
+                 dtype=torch.float32, e_store_gb=None):
+        self.max_cells, self.n_steps = int(max_cells), int(n_steps)
+        self.K = max(2, int(checkpoint_every) + (int(checkpoint_every) & 1))
+        self.dtype = dtype
+        self.dev = L.compute_device(device)
+        nseg = math.ceil(self.n_steps / self.K)
+        elem = torch.tensor([], dtype=dtype).element_size()
+        c = self.max_cells
+        self.ckpt = torch.empty(nseg * 6 * c, dtype=dtype, device=self.dev)
+        self.estore = torch.empty(max(self.K - 1, 1) * 3 * c, dtype=dtype, device=self.dev)
+        self.tail_segments = _tail_count(nseg, self.K * 3 * c * elem, e_store_gb, self.dev)
+        self.a_tail = (nseg - self.tail_segments) * self.K if self.tail_segments else self.n_steps
+        self.etail = (torch.empty((self.n_steps - self.a_tail) * 3 * c, dtype=dtype,
+                                  device=self.dev) if self.tail_segments else None)
+        self._gen = 0
 
 
 class BatchedFDTD3D:
@@ -44,7 +81,7 @@ class BatchedFDTD3D:
 
     def __init__(self, grid_shapes, sources, monitors, dl=25.0, wavelength_nm=1550.0,
                  pml_layers=12, n_steps=3000, courant=0.4, checkpoint_every=64, device=None,
-                 dtype=torch.float32, use_graph=True, e_store_gb=None):
+                 dtype=torch.float32, use_graph=True, e_store_gb=None, stores=None):
         if len(sources) != len(grid_shapes) or len(monitors) != len(grid_shapes):
             raise ValueError("one source and one monitor per design")
         self.shapes = [tuple(int(v) for v in s) for s in grid_shapes]
@@ -67,31 +104,38 @@ class BatchedFDTD3D:
         nseg = math.ceil(self.n_steps / self.K)
         self.state = [torch.zeros(6 * tot, dtype=dt, device=dev) for _ in range(2)]
         self.lam = [torch.zeros(6 * tot, dtype=dt, device=dev) for _ in range(2)]
-        self.ckpt = torch.empty(nseg * 6 * tot, dtype=dt, device=dev)
-        self.estore = torch.empty(max(self.K - 1, 1) * 3 * tot, dtype=dt, device=dev)
         self.S = torch.zeros(tot, dtype=dt, device=dev)
-        # E of every step of the final segments, kept by a recording forward
-        # so that the backward skips their replay: as many segments as
-        # e_store_gb (default: half the free device memory above 16 GiB)
-        # holds, at most all but the first
-        if e_store_gb is not None and not e_store_gb >= 0:
-            raise ValueError(f"e_store_gb must be None or >= 0, got {e_store_gb!r}")
-        elem = torch.tensor([], dtype=dt).element_size()
-        seg_bytes = self.K * 3 * tot * elem
-        if e_store_gb is None:
-            budget = max(0, (spare_device_bytes(dev) - (16 << 30)) // 2)
+        if stores is not None:
+            # another owner's checkpoints and E stores (sized for at least
+            # this batch), used by one plan at a time
+            if (stores.max_cells < tot or stores.n_steps != self.n_steps or stores.K != self.K
+                    or stores.dtype != dt or stores.dev != dev):
+                raise ValueError("BatchedFDTD3D: the shared stores do not fit this batch "
+                                 "(cells, n_steps, checkpoint interval, dtype or device)")
+            self._gen_owner = stores
+            self.ckpt, self.estore, self.etail = stores.ckpt, stores.estore, stores.etail
+            self.tail_segments, self.a_tail = stores.tail_segments, stores.a_tail
         else:
-            budget = int(e_store_gb * 1e9)
-        self.tail_segments = int(min(max(nseg - 1, 0), budget // seg_bytes))
-        self.a_tail = (nseg - self.tail_segments) * self.K if self.tail_segments else self.n_steps
-        self.etail = (torch.empty((self.n_steps - self.a_tail) * 3 * tot, dtype=dt, device=dev)
-                      if self.tail_segments else None)
+            self._gen_owner = self
+            self.ckpt = torch.empty(nseg * 6 * tot, dtype=dt, device=dev)
+            self.estore = torch.empty(max(self.K - 1, 1) * 3 * tot, dtype=dt, device=dev)
+            # E of every step of the final segments, kept by a recording
+            # forward so that the backward skips their replay: as many
+            # segments as e_store_gb (default: half the spare device memory
+            # above 16 GiB) holds, at most all but the first
+            elem = torch.tensor([], dtype=dt).element_size()
+            self.tail_segments = _tail_count(nseg, self.K * 3 * tot * elem, e_store_gb, dev)
+            self.a_tail = ((nseg - self.tail_segments) * self.K if self.tail_segments
+                           else self.n_steps)
+            self.etail = (torch.empty((self.n_steps - self.a_tail) * 3 * tot, dtype=dt,
+                                      device=dev) if self.tail_segments else None)
         self.ce = torch.ones(tot, dtype=dt, device=dev)
         self.eps = torch.ones(tot, dtype=torch.float64, device=dev)
         self._keep = []
         # forward generation: the backward of a forward is valid only while no
         # other forward has overwritten the plan's checkpoints, coefficients
-        # and eps (they live in plan-wide buffers)
+        # and eps (they live in plan-wide buffers, the checkpoints possibly in
+        # stores shared with other plans: then the counter is the stores')
         self._gen = 0
         arr = (Design3D * self.n)()
         self.accs, self.grads, self.mon_shapes = [], [], []
@@ -239,8 +283,15 @@ class BatchedFDTD3D:
             outs = self._forward_detached(eps_list)
         return [torch.view_as_complex(o) for o in outs]
 
+    def _next_gen(self):
+        self._gen_owner._gen += 1
+        return self._gen_owner._gen
+
+    def _current_gen(self):
+        return self._gen_owner._gen
+
     def _forward_detached(self, eps_list):
-        self._gen += 1
+        self._next_gen()
         self._set_eps(eps_list)
         self._run_forward(record=False)
         return tuple(a.clone().to(self.out_device) for a in self.accs)
@@ -252,8 +303,7 @@ class _BatchedFn(torch.autograd.Function):
         ctx.batch = batch
         ctx.devices = [e.device for e in eps_list]
         ctx.recorded = True  # BatchedFDTD3D.forward applies this only to record
-        batch._gen += 1
-        ctx.gen = batch._gen
+        ctx.gen = batch._next_gen()
         batch._set_eps(eps_list)
         batch._run_forward(record=ctx.recorded)
         return tuple(a.clone().to(batch.out_device) for a in batch.accs)
@@ -261,11 +311,12 @@ class _BatchedFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, *grads):
         b = ctx.batch
-        if ctx.gen != b._gen:
+        if ctx.gen != b._current_gen():
             raise RuntimeError(
-                "BatchedFDTD3D: another forward ran on this batch after the one being "
-                "differentiated and overwrote its checkpoints and coefficients; call backward "
-                "before the next forward (or use a separate BatchedFDTD3D)")
+                "BatchedFDTD3D: another forward ran on this batch (or on one sharing its "
+                "stores) after the one being differentiated and overwrote its checkpoints and "
+                "coefficients; call backward before the next forward (or use a separate "
+                "BatchedFDTD3D)")
         if not ctx.recorded:
             raise RuntimeError("BatchedFDTD3D: this forward stored no checkpoints")
         gl = b._run_backward(list(grads))

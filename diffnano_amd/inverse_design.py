@@ -22,7 +22,7 @@ import torch
 
 from . import workloads
 from ._common import spare_device_bytes
-from .batched import BatchedFDTD3D
+from .batched import BatchedFDTD3D, SharedStores
 from .fdtd3d import FDTDSolver3D
 
 GRID_CLASSES = [(96, 96, 96), (128, 128, 128), (160, 160, 96), (192, 128, 128)]
@@ -60,17 +60,29 @@ def batched_problems(n_designs=48, seed=0, dl=25.0, pml_layers=12, classes=GRID_
 
 
 def _group_plans(problems, group, dl, pml_layers, n_steps, checkpoint_every, dev, use_graph):
-    """Batched plans over consecutive groups of ``group`` designs (all in
-    one plan when None), the device memory left after their buffers split
-    evenly into their stores of the final segments' E (16 GiB kept free)."""
+    """Batched plans over consecutive groups of designs (one plan for all
+    when ``group`` is None).  Grouped plans run one forward + backward at a
+    time and share one set of checkpoint and E stores (``SharedStores``),
+    sized for the largest group, with the device memory left after every
+    plan's own buffers given to E of the final segments."""
     n = len(problems)
-    g = n if not group else max(1, min(int(group), n))
-    chunks = [list(range(i, min(n, i + g))) for i in range(0, n, g)]
+    if not group:
+        return [(list(range(n)), BatchedFDTD3D(
+            [p.shape for p in problems], [p.source for p in problems],
+            [p.monitor for p in problems], dl=dl, pml_layers=pml_layers, n_steps=n_steps,
+            courant=0.4, checkpoint_every=checkpoint_every, device=dev, use_graph=use_graph))]
+    cells = [int(np.prod(p.shape)) for p in problems]
     K = max(2, int(checkpoint_every) + (int(checkpoint_every) & 1))
     nseg = -(-n_steps // K)
-    cells = [sum(int(np.prod(problems[i].shape)) for i in c) for c in chunks]
-    base = sum((nseg * 6 + (K - 1) * 3 + 12 * 2 + 3) * c * 4 for c in cells)
-    budget = max(0, spare_device_bytes(dev) - (16 << 30) - base) * 0.9 / len(chunks)
+    own = 4 * 28 * sum(cells)          # every plan's states, adjoints, S, ce, eps (bytes)
+    shared_per_cell = 4 * (nseg * 6 + max(K - 1, 1) * 3)
+    spare = max(0, spare_device_bytes(dev) - (16 << 30) - own)
+    g = max(1, min(int(group), n))
+    chunks = [list(range(i, min(n, i + g))) for i in range(0, n, g)]
+    max_cells = max(sum(cells[i] for i in c) for c in chunks)
+    budget = max(0, spare - shared_per_cell * max_cells) * 0.95
+    stores = SharedStores(max_cells, n_steps, checkpoint_every=checkpoint_every, device=dev,
+                          e_store_gb=budget / 1e9)
     plans = []
     for c in chunks:
         ps = [problems[i] for i in c]
@@ -78,7 +90,7 @@ def _group_plans(problems, group, dl, pml_layers, n_steps, checkpoint_every, dev
                                        [p.monitor for p in ps], dl=dl, pml_layers=pml_layers,
                                        n_steps=n_steps, courant=0.4,
                                        checkpoint_every=checkpoint_every, device=dev,
-                                       use_graph=use_graph, e_store_gb=budget / 1e9)))
+                                       use_graph=use_graph, stores=stores)))
     return plans
 
 
@@ -91,9 +103,11 @@ def adam_loop(problems, iterations=20, lr=0.05, n_steps=3000, dl=25.0, pml_layer
     result ({"forward": (ty, zc), "adjoint": ..., "forward2": ...,
     "adjoint2": ...}) applied before the plan is built; plain mode, a dict
     shape -> such a result applied before each design's solve.  ``group``:
-    batched mode, designs per batched plan (None: one plan for all); smaller
-    groups leave device memory for E of more final segments, whose replay
-    the backward then skips."""
+    batched mode, designs per batched plan (None: one plan for all); the
+    plans share their checkpoint and E stores and run in turn, so the
+    checkpoints of a batch too large for the device fit, and smaller groups
+    leave room for E of more final segments, whose replay the backward then
+    skips."""
     from . import autotune
 
     dev = torch.device(device)

@@ -164,3 +164,45 @@ def test_batched_two_step_switched_off_by_tiles():
     assert p0 == (False, False)
     for a, b in zip(F0 + E0 + G0, F1 + E1 + G1):
         assert torch.equal(a, b)
+
+
+def test_plans_sharing_stores_match_one_plan():
+    """Two plans that share one set of checkpoint and E stores (sized for
+    the larger, run one after the other) give bitwise the monitors and
+    gradients of one plan over all designs; a backward after the other
+    plan's forward raises."""
+    from diffnano_amd.batched import BatchedFDTD3D, SharedStores
+    shapes, eps, srcs, mons = _cases()
+    T, K = 37, 6
+    kw = dict(dl=25.0, pml_layers=3, n_steps=T, courant=0.4, checkpoint_every=K, device="cuda")
+
+    def run(plan, idx):
+        es = [torch.tensor(eps[i], device="cuda", requires_grad=True) for i in idx]
+        outs = plan.forward(es)
+        sum((F.abs() ** 2).sum() for F in outs).backward()
+        return [o.detach().clone() for o in outs], [e.grad.clone() for e in es]
+
+    whole = BatchedFDTD3D(shapes, [dict(s) for s in srcs], mons, **kw)
+    F0, G0 = run(whole, [0, 1, 2])
+    groups = [[0, 2], [1]]
+    cells = max(sum(int(np.prod(shapes[i])) for i in g) for g in groups)
+    for gb in (0.0, None):   # replay everything; E of all segments but the first
+        st = SharedStores(cells, T, checkpoint_every=K, device="cuda", e_store_gb=gb)
+        assert st.tail_segments == (0 if gb == 0.0 else -(-T // K) - 1)
+        plans = [BatchedFDTD3D([shapes[i] for i in g], [dict(srcs[i]) for i in g],
+                               [mons[i] for i in g], stores=st, **kw) for g in groups]
+        F, G = [None] * 3, [None] * 3
+        for g, p in zip(groups, plans):
+            f, gr = run(p, g)
+            for j, i in enumerate(g):
+                F[i], G[i] = f[j], gr[j]
+        for a, b in zip(F + G, F0 + G0):
+            assert torch.equal(a, b)
+        # plan 0's backward after plan 1's forward: its checkpoints are gone
+        e0 = [torch.tensor(eps[i], device="cuda", requires_grad=True) for i in groups[0]]
+        out0 = plans[0].forward(e0)
+        run(plans[1], groups[1])
+        with pytest.raises(RuntimeError, match="another forward ran"):
+            sum((F.abs() ** 2).sum() for F in out0).backward()
+    with pytest.raises(ValueError, match="shared stores"):
+        BatchedFDTD3D(shapes, [dict(s) for s in srcs], mons, stores=st, **kw)

@@ -261,9 +261,12 @@ def test_batched_backward_refuses_a_stale_forward():
         out_device = torch.device("cpu")
         forward = BatchedFDTD3D.forward
         _forward_detached = BatchedFDTD3D._forward_detached
+        _next_gen = BatchedFDTD3D._next_gen
+        _current_gen = BatchedFDTD3D._current_gen
 
-        def __init__(self):
+        def __init__(self, owner=None):
             self._gen = 0
+            self._gen_owner = owner if owner is not None else self
             self.accs = [torch.zeros(1, 1, 1, 1, 2, 2, dtype=torch.float64)]
             self.calls = []
 
@@ -289,6 +292,17 @@ def test_batched_backward_refuses_a_stale_forward():
         out1[0].abs().sum().backward()
     assert b.calls.count(("bwd",)) == 1
     assert ("fwd", False) in b.calls and ("fwd", True) in b.calls
+    # plans sharing their checkpoint stores share the counter: a forward of
+    # the other plan invalidates this one's backward, its own backward is fine
+    class Owner:
+        _gen = 0
+    own = Owner()
+    b1, b2 = FakeBatch(own), FakeBatch(own)
+    o1 = b1.forward([e])
+    o2 = b2.forward([e])
+    o2[0].abs().sum().backward()
+    with pytest.raises(RuntimeError, match="another forward ran"):
+        o1[0].abs().sum().backward()
 
 
 def _designs(shapes, dtype=L.DNX_F32):
