@@ -214,13 +214,16 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
   p->Lf = Batch3Launch{p->d_fwd, p->n, tot[0], p->K, 0, p->d_ctr, fl, nullptr, 0};
   p->La = Batch3Launch{p->d_adj, p->n, tot[1], p->K, 0, p->d_ctr, al, nullptr, 0};
   if constexpr (std::is_same<T, float>::value) {
+    // non-temporal stores when one state of the whole batch exceeds half of
+    // the 256 MB last-level cache (the single-grid rule)
+    const int nt = p->sum6 * int64_t(sizeof(T)) > (int64_t(128) << 20) ? 1 : 0;
     if (p->fpairs) {
       p->ty_f2 = batch_tiles(p->dtype, 2).ty;
       int b2 = 1;
       if ((rc = prepare_batch3_pair<T>(p->ty_f2, false, p->max_w, &b2))) return rc;
       p->grid_f2 = int(p->map_f2.size());
       p->Lf2 = Batch3Launch{p->d_fwd2, p->n, tot[2], p->K, 0, p->d_ctr, 1, p->d_map_f2,
-                            int(p->map_f2.size())};
+                            int(p->map_f2.size()), nt};
     }
     if (p->apairs) {
       p->ty_a2 = batch_tiles(p->dtype, 3).ty;
@@ -228,7 +231,7 @@ static int batch_setup(dnx_batch3d_plan* p, const dnx_design3d* ds, void* ws, hi
       if ((rc = prepare_batch3_pair<T>(p->ty_a2, true, p->max_w, &b2))) return rc;
       p->grid_a2 = int(p->map_a2.size());
       p->La2 = Batch3Launch{p->d_adj2, p->n, tot[3], p->K, 0, p->d_ctr, 1, p->d_map_a2,
-                            int(p->map_a2.size())};
+                            int(p->map_a2.size()), nt};
     }
   }
   return DNX_OK;

@@ -101,6 +101,7 @@ struct Batch3Launch {
   // design) or (-1, -1) for an idle slot; `blocks` entries (a multiple of 8)
   const int2* block_tile;
   int blocks;
+  int nt;               // two-step launches: non-temporal output stores
 };
 
 // Block -> tile map of a two-step batched launch (see batch_block_map).

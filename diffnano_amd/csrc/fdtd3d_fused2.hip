@@ -1514,7 +1514,7 @@ __device__ inline void batch2_mon(const Batch3Design<T>& Dd, bool on, Fused2Args
   f.side[0] = Dd.m_side;
 }
 
-template <typename T, int TY, int LW>
+template <typename T, int TY, int LW, bool NT>
 __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
   extern __shared__ unsigned char smem_raw[];
   const int2 bt = L.block_tile[blockIdx.x];
@@ -1532,12 +1532,12 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_fwd(Batch3Launch L) {
   f.n = n;
   const int lt = bt.y;
   batch2_mon(Dd, L.store != 1, f);
-  fwd2_tile<T, TY, false, true, true, false, false, LW>(f, (lt % f.a.ty_tiles) * TY,
+  fwd2_tile<T, TY, false, true, true, NT, false, LW>(f, (lt % f.a.ty_tiles) * TY,
                                        (lt / f.a.ty_tiles) * f.a.zc,
                                        reinterpret_cast<T*>(smem_raw));
 }
 
-template <typename T, int TY, int LW>
+template <typename T, int TY, int LW, bool NT>
 __global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
   extern __shared__ unsigned char smem_raw[];
   const int2 bt = L.block_tile[blockIdx.x];
@@ -1555,7 +1555,7 @@ __global__ __launch_bounds__(256, 2) void k3_batch2_adj(Batch3Launch L) {
   f.n = m;
   batch2_mon(Dd, true, f);
   const int lt = bt.y;
-  adj2_tile<T, TY, true, false, LW>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc,
+  adj2_tile<T, TY, true, NT, LW>(f, (lt % f.a.ty_tiles) * TY, (lt / f.a.ty_tiles) * f.a.zc,
                          reinterpret_cast<T*>(smem_raw));
 }
 
@@ -1609,25 +1609,28 @@ static size_t batch2_smem(int max_w) {
 
 // The batched two-step adjoint exists for 2-row tiles only (with 4 rows
 // its per-design arguments make it spill to scratch).
-template <typename T, int TY, int LW>
+template <typename T, int TY, int LW, bool NT>
 static const void* batch2_kernel(bool adjoint) {
   if constexpr (TY == 2) {
-    if (adjoint) return reinterpret_cast<const void*>(&k3_batch2_adj<T, TY, LW>);
+    if (adjoint) return reinterpret_cast<const void*>(&k3_batch2_adj<T, TY, LW, NT>);
   }
-  return adjoint ? nullptr : reinterpret_cast<const void*>(&k3_batch2_fwd<T, TY, LW>);
+  return adjoint ? nullptr : reinterpret_cast<const void*>(&k3_batch2_fwd<T, TY, LW, NT>);
 }
 
-template <typename T, int TY>
+template <typename T, int TY, bool NT>
 static const void* batch2_kernel_w(bool adjoint, int max_w) {
-  return batch2_pitch(max_w) == 128 ? batch2_kernel<T, TY, 128>(adjoint)
-                                    : batch2_kernel<T, TY, 256>(adjoint);
+  return batch2_pitch(max_w) == 128 ? batch2_kernel<T, TY, 128, NT>(adjoint)
+                                    : batch2_kernel<T, TY, 256, NT>(adjoint);
 }
 
 template <typename T, int TY>
 static int prepare_batch2_ty(bool adjoint, int max_w, int* blocks_per_cu) {
-  const void* f = batch2_kernel_w<T, TY>(adjoint, max_w);
+  const void* f = batch2_kernel_w<T, TY, false>(adjoint, max_w);
   DNX_REQUIRE(f != nullptr, "batch3 pair: no %d-row batched two-step adjoint", TY);
   DNX_CHECK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(batch2_smem<T, TY>(max_w))));
+  DNX_CHECK_HIP(hipFuncSetAttribute(batch2_kernel_w<T, TY, true>(adjoint, max_w),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(batch2_smem<T, TY>(max_w))));
   const int threads = ((max_w + 63) / 64) * 64;
   int nb = 0;
@@ -1637,16 +1640,28 @@ static int prepare_batch2_ty(bool adjoint, int max_w, int* blocks_per_cu) {
   return DNX_OK;
 }
 
-template <typename T, int TY, int LW>
-static void launch_batch2_lw(const Batch3Launch& L, bool adjoint, int threads, size_t smem,
+template <typename T, int TY, int LW, bool NT>
+static void launch_batch2_nt(const Batch3Launch& L, bool adjoint, int threads, size_t smem,
                              hipStream_t s) {
   if constexpr (TY == 2) {
     if (adjoint) {
-      hipLaunchKernelGGL((k3_batch2_adj<T, TY, LW>), dim3(L.blocks), dim3(threads), smem, s, L);
+      hipLaunchKernelGGL((k3_batch2_adj<T, TY, LW, NT>), dim3(L.blocks), dim3(threads), smem, s,
+                         L);
       return;
     }
   }
-  hipLaunchKernelGGL((k3_batch2_fwd<T, TY, LW>), dim3(L.blocks), dim3(threads), smem, s, L);
+  hipLaunchKernelGGL((k3_batch2_fwd<T, TY, LW, NT>), dim3(L.blocks), dim3(threads), smem, s, L);
+}
+
+// Non-temporal output stores when the batch's state exceeds half the
+// last-level cache (as for a single grid, stream_stores).
+template <typename T, int TY, int LW>
+static void launch_batch2_lw(const Batch3Launch& L, bool adjoint, int threads, size_t smem,
+                             hipStream_t s) {
+  if (L.nt)
+    launch_batch2_nt<T, TY, LW, true>(L, adjoint, threads, smem, s);
+  else
+    launch_batch2_nt<T, TY, LW, false>(L, adjoint, threads, smem, s);
 }
 
 template <typename T, int TY>
@@ -1655,7 +1670,7 @@ static int launch_batch2_ty(const Batch3Launch& L, bool adjoint, int max_w, int 
   (void)grid;  // one work-group per block-map entry
   const int threads = ((max_w + 63) / 64) * 64;
   const size_t smem = batch2_smem<T, TY>(max_w);
-  const void* kern = batch2_kernel_w<T, TY>(adjoint, max_w);
+  const void* kern = batch2_kernel_w<T, TY, false>(adjoint, max_w);
   DNX_REQUIRE(kern != nullptr, "batch3 pair: no %d-row batched two-step adjoint", TY);
   if (batch2_pitch(max_w) == 128)
     launch_batch2_lw<T, TY, 128>(L, adjoint, threads, smem, s);
