@@ -12,6 +12,8 @@
 //   Ex' = bx Ex + ce ax D-c Hz
 //   Ey' = by Ey - ce ay D-r Hz
 //   Hz' = bxby Hz + kh (ax D+r Ey' - ay D+c Ex')
+#include <atomic>
+
 #include "dnx_aux.h"
 
 // No multiply-add contraction: the batched and single-design kernels round
@@ -382,6 +384,141 @@ __global__ __launch_bounds__(BX2* BY2) void k2tm_batch_e(Tm2Batch a, int step) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Resident batched TM kernel for grids up to 128 x 128: one work-group of
+// 1024 threads per design advances it through every step of the call (thread
+// = one column c < 128 and a block of 16 consecutive rows), so a step costs
+// two barriers instead of two kernel launches over the whole batch.  Ez (A)
+// and Hx (B) of the whole design live in LDS, Hy and the coefficient in
+// registers; the last Hy row of every row block (C) is exchanged through
+// LDS.  H phase: read Ez and its x+1 / y+1 neighbours from A, update B and
+// Hy in place; E phase: read Hx and its x-1 neighbour from B, Hy of the row
+// below (register or C), update A in place.  The arithmetic and its order
+// are k2tm_batch_h / k2tm_batch_e's (bitwise their results), including the
+// FP32 DFT accumulation.
+// ---------------------------------------------------------------------------
+
+constexpr int RES_NX = 128;              // LDS row pitch and largest nx
+constexpr int RES_R = 16;                // rows per thread
+constexpr int RES_RB = 1024 / RES_NX;    // row blocks (8): largest ny = 128
+constexpr int RES_NY = RES_RB * RES_R;
+constexpr size_t RES_SMEM =
+    sizeof(float) * (size_t(2) * RES_NY * RES_NX + RES_RB * RES_NX + 2 * RES_NY);
+
+__global__ __launch_bounds__(1024) void k2tm_batch_res(Tm2Batch a, int n_steps) {
+  extern __shared__ float rsm[];
+  float* A = rsm;                            // Ez [row][RES_NX]
+  float* Bh = A + RES_NY * RES_NX;           // Hx [row][RES_NX]
+  float* Ch = Bh + RES_NY * RES_NX;          // Hy of each row block's last row
+  float* Py = Ch + RES_RB * RES_NX;          // (by, ay) per row
+  const int ny = a.ny, nx = a.nx;
+  const int64_t n = a.n;
+  const int b = blockIdx.x;
+  const int c = threadIdx.x % RES_NX;
+  const int rb = threadIdx.x / RES_NX;
+  const int r0 = rb * RES_R;
+  const bool cv = c < nx;
+  const int cr = min(c + 1, RES_NX - 1), cl = max(c - 1, 0);  // clamped LDS columns
+  float* st = a.st + int64_t(b) * 3 * n;
+  const float* ce = a.ce + int64_t(b) * n;
+  for (int i = threadIdx.x; i < RES_NY; i += 1024) {
+    Py[2 * i] = i < ny ? a.by[i] : 0.f;
+    Py[2 * i + 1] = i < ny ? a.ay[i] : 0.f;
+  }
+  const float bxv = cv ? a.bx[c] : 0.f, axv = cv ? a.ax[c] : 0.f;
+  const float kh = a.kh;
+  float hy[RES_R], cer[RES_R];
+#pragma unroll
+  for (int j = 0; j < RES_R; ++j) {
+    const int r = r0 + j;
+    const bool v = cv && r < ny;
+    const int64_t i = int64_t(r) * nx + c;
+    A[r * RES_NX + c] = v ? st[i] : 0.f;
+    Bh[r * RES_NX + c] = v ? st[n + i] : 0.f;
+    hy[j] = v ? st[2 * n + i] : 0.f;
+    cer[j] = v ? ce[i] : 0.f;
+  }
+  const bool src = a.has_src != 0;
+  const Box sb = a.sbox, mb = a.mbox;
+  const int64_t mcells = int64_t(mb.ny) * mb.nx;
+  float* acc = a.nf > 0 ? a.acc + int64_t(b) * a.nf * 2 * mcells : nullptr;
+  // whether this thread owns any source or monitor cell
+  const bool own_src = src && cv && c >= sb.x0 && c < sb.x0 + sb.nx && r0 < sb.y0 + sb.ny &&
+                       r0 + RES_R > sb.y0;
+  const bool own_mon = a.nf > 0 && cv && c >= mb.x0 && c < mb.x0 + mb.nx &&
+                       r0 < mb.y0 + mb.ny && r0 + RES_R > mb.y0;
+  // rows of this thread that hold source / monitor cells (empty ranges
+  // when the thread owns none)
+  const int sj0 = own_src ? max(sb.y0 - r0, 0) : 0;
+  const int sj1 = own_src ? min(sb.y0 + sb.ny - r0, RES_R) : 0;
+  const int mj0 = own_mon ? max(mb.y0 - r0, 0) : 0;
+  const int mj1 = own_mon ? min(mb.y0 + mb.ny - r0, RES_R) : 0;
+  float* const Ar = A + r0 * RES_NX + c;    // this thread's column, row r0
+  float* const Br = Bh + r0 * RES_NX + c;
+  const int dx = cr - c, dxl = cl - c;      // 1 / -1, or 0 at a clamped edge
+  // row below the block (clamped into the array; only used where r + 1 < ny)
+  const float* const Anext = A + min(r0 + RES_R, RES_NY - 1) * RES_NX + c;
+  __syncthreads();
+  for (int k = 0; k < n_steps; ++k) {
+    // H from E (k2tm_batch_h)
+#pragma unroll
+    for (int j = 0; j < RES_R; ++j) {
+      const int r = r0 + j;
+      const float by = Py[2 * r], ay = Py[2 * r + 1];
+      const float e = Ar[j * RES_NX];
+      const float er = Ar[j * RES_NX + dx];
+      const float eu = j + 1 < RES_R ? Ar[(j + 1) * RES_NX] : *Anext;
+      const float dc = (c + 1 < nx) ? er - e : 0.f;
+      const float dr = (r + 1 < ny) ? eu - e : 0.f;
+      Br[j * RES_NX] = by * Br[j * RES_NX] - kh * (ay * dc);
+      hy[j] = bxv * hy[j] + kh * (axv * dr);
+    }
+    Ch[rb * RES_NX + c] = hy[RES_R - 1];
+    __syncthreads();
+    // E from H (k2tm_batch_e)
+    const float hbelow = Ch[max(rb - 1, 0) * RES_NX + c];
+#pragma unroll
+    for (int j = 0; j < RES_R; ++j) {
+      const int r = r0 + j;
+      const float by = Py[2 * r], ay = Py[2 * r + 1];
+      const float hb = j > 0 ? hy[j - 1] : hbelow;
+      const float hxo = Br[j * RES_NX];
+      const float hl = Br[j * RES_NX + dxl];
+      const float dHy = r > 0 ? hy[j] - hb : 0.f;
+      const float dHx = c > 0 ? hxo - hl : 0.f;
+      Ar[j * RES_NX] = (bxv * by) * Ar[j * RES_NX] + cer[j] * (axv * dHy - ay * dHx);
+    }
+    // the soft source, then the DFT of the new Ez (own cells only)
+    if (sj0 < sj1) {
+      const float w = a.wave[k];
+      for (int j = sj0; j < sj1; ++j)
+        Ar[j * RES_NX] += w * a.sprof[(r0 + j - sb.y0) * sb.nx + (c - sb.x0)];
+    }
+    if (mj0 < mj1) {
+      const float* ph = a.phase + int64_t(k) * a.nf * 2;
+      for (int j = mj0; j < mj1; ++j) {
+        const float e = Ar[j * RES_NX];
+        const int64_t q = int64_t(r0 + j - mb.y0) * mb.nx + (c - mb.x0);
+        for (int f = 0; f < a.nf; ++f) {
+          acc[(2 * f) * mcells + q] += e * ph[2 * f];
+          acc[(2 * f + 1) * mcells + q] -= e * ph[2 * f + 1];
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < RES_R; ++j) {
+    const int r = r0 + j;
+    if (cv && r < ny) {
+      const int64_t i = int64_t(r) * nx + c;
+      st[i] = A[r * RES_NX + c];
+      st[n + i] = Bh[r * RES_NX + c];
+      st[2 * n + i] = hy[j];
+    }
+  }
+}
+
 }  // namespace dnx
 
 using namespace dnx;
@@ -492,6 +629,20 @@ extern "C" int32_t dnx_fdtd2d_tm_batch(const dnx_grid2d* g, int32_t batch, void*
     a.acc = acc;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->nx <= RES_NX && g->ny <= RES_RB * RES_R && option(DNX_OPT_FDTD2D_LAUNCH_PER_STEP) == 0) {
+    // the whole call in one launch, each design resident in one work-group
+    static std::atomic<bool> attr{false};
+    if (!attr.load()) {
+      DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k2tm_batch_res),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        int(RES_SMEM)));
+      attr = true;
+    }
+    if (n_steps > 0)
+      hipLaunchKernelGGL(k2tm_batch_res, dim3(batch), dim3(1024), RES_SMEM, s, a, n_steps);
+    DNX_CHECK_HIP(hipGetLastError());
+    return DNX_OK;
+  }
   const dim3 grid(cdiv(g->nx, BX2), cdiv(g->ny, BY2), batch), block(BX2, BY2);
   for (int k = 0; k < n_steps; ++k) {
     hipLaunchKernelGGL(k2tm_batch_h, grid, block, 0, s, a);

@@ -107,3 +107,32 @@ def test_ablation_network_ranking_beats_random_selection():
     assert np.mean(lifts) > 1.0
     assert np.mean(best_pct) > 0.5
     assert np.mean(lifts_a) > 1.0
+
+
+@pytest.mark.parametrize("shape,src,mon", [
+    ((128, 128), {"type": "gaussian_pulse"}, None),
+    ((100, 70), {"type": "gaussian_pulse", "pos": [37, 11]}, {"box": (10, 90, 60, 61)}),
+    ((33, 128), {"type": "continuous", "pos": [16, 100]}, {"box": (0, 33, 0, 128),
+                                                          "wavelengths": [1500.0, 1600.0]}),
+    ((128, 5), {"type": "gaussian_pulse", "pos": [64, 2]}, {"box": (120, 128, 0, 5)}),
+    ((17, 19), {"type": "gaussian_pulse", "pos": [16, 18]}, {"box": (0, 17, 18, 19)}),
+])
+def test_resident_batched_tm_matches_per_step_launches(shape, src, mon, lib_opts):
+    """The resident kernel (one work-group per design for the whole call,
+    grids up to 128 x 128) gives bitwise the final fields and FP32 DFT of
+    the two-launches-per-step kernels: full-grid and line monitors, sources
+    at the grid's last row and column, ragged rows and columns."""
+    from diffnano_amd.batched2d import BatchedTM2D
+    g = torch.Generator().manual_seed(shape[0] * 1000 + shape[1])
+    eps = 1.5 + 10.0 * torch.rand(6, *shape, generator=g, dtype=torch.float64)
+    outs = []
+    for variant in (1, 0):   # per-step launches, resident
+        lib_opts.set("fdtd2d_per_step", variant)
+        bt = BatchedTM2D(grid_shape=shape, dl=20.0, pml_layers=4, n_steps=300, courant=0.5,
+                         source=src, monitor=mon, device="cuda")
+        outs.append(bt.forward(eps, return_field=True))
+        torch.cuda.synchronize()
+    (F0, E0) = outs[0]
+    assert float(E0.abs().max()) > 0.0
+    for F1, E1 in outs[1:]:
+        assert torch.equal(F0, F1) and torch.equal(E0, E1)
