@@ -21,6 +21,9 @@ the time-reversal forward + autograd-recompute backward over T_cpu steps
 (``--cpu-steps``).  designs_per_s: a reduced, stated form of the batched
 configuration (48 designs, one Adam iteration of 3000 steps, batched and
 one design at a time, FoMs compared; ``--designs-sample 0`` skips it).
+designs_per_s_2d: the surrogate configuration's 2D design loop (256
+candidates per step judged by the batched FDTD) against the plain loop
+(one FDTDSolver2D solve per candidate), same decisions checked.
 
 ``--pmc-probe``: run a short fixed sequence (a 256 MiB device copy three
 times, then 4 forward and 4 adjoint steps of the workload grid, i.e. two
@@ -415,6 +418,41 @@ def _designs_sample(n_designs, n_steps, iters, dev):
                       f"{iters} Adam iteration(s) x {n_steps} steps fwd+adj per mode"}
 
 
+def _designs_sample_2d(dev, iters_batched=5, iters_plain=1):
+    """The 2D design loop of the surrogate configuration (128 x 128 TM, 2000
+    steps, 256 candidate designs per step, the best kept): every candidate
+    judged by the batched FDTD (one resident launch for all 256) against the
+    plain loop (FDTDSolver2D, one design at a time), from the same start and
+    candidate stream; the plain loop's FoMs must follow the batched loop's."""
+    import torch
+
+    from diffnano_amd import surrogate as SG
+    from diffnano_amd.workloads import SurrogateConfig
+
+    cfg = SurrogateConfig()
+    warm = SG.PrescreenDesigner(cfg, mode="exhaustive", seed=7, device=dev)
+    warm.step()
+    torch.cuda.synchronize()
+    ex = SG.PrescreenDesigner(cfg, mode="exhaustive", seed=7, device=dev)
+    t0 = time.perf_counter()
+    ex.run(iters_batched)
+    torch.cuda.synchronize()
+    t_ex = time.perf_counter() - t0
+    pl = SG.PrescreenDesigner(cfg, mode="plain", seed=7, device=dev)
+    t0 = time.perf_counter()
+    pl.run(iters_plain)
+    torch.cuda.synchronize()
+    t_pl = time.perf_counter() - t0
+    same = all(abs(a - b) <= 1e-4 * abs(b) for a, b in zip(pl.history, ex.history))
+    b, p = iters_batched / t_ex, iters_plain / t_pl
+    return {"unit": "design-iterations/s", "batched": round(b, 3), "plain": round(p, 4),
+            "batched_over_plain": round(b / p, 1), "candidates_per_iteration": 256,
+            "iterations": {"batched": iters_batched, "plain": iters_plain},
+            "fom_matches_within_1e-4": bool(same),
+            "sample": "2D TM 128x128, 2000 steps, 256 candidates per step judged by FDTD "
+                      "(batched resident kernel vs FDTDSolver2D one at a time)"}
+
+
 def fno_flops_per_sample(m):
     """Multiply-add FLOPs of one FNO2d forward sample (every product runs on
     the BF16 MFMA GEMM kernels)."""
@@ -698,6 +736,9 @@ def main():
     designs = None
     if args.config == "c2" and args.designs_sample > 0:
         designs = _designs_sample(args.designs_sample, 3000, 1, dev)
+    designs_2d = None
+    if args.config == "c2" and args.designs_sample > 0:
+        designs_2d = _designs_sample_2d(dev)
     cpu = None if args.no_cpu else _cpu_baseline(eps_np, solver_kw, args.cpu_steps,
                                                  args.cpu_steps)
 
@@ -741,6 +782,8 @@ def main():
     }
     if designs is not None:
         out["designs_per_s"] = designs
+    if designs_2d is not None:
+        out["designs_per_s_2d"] = designs_2d
     ref = _reference_loss(args, solver_kw)
     if ref is not None:
         out["loss_rel_err_vs_reference"] = abs(loss_val - ref) / abs(ref)
