@@ -1,5 +1,6 @@
 // libdnx_fdtd: error state, launch timing, coefficient preparation, and the
 // per-step side kernels (soft sources, running DFT monitors and adjoints).
+#include <algorithm>
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -50,7 +51,7 @@ static void init_options() {
   g_opt[DNX_OPT_GEMM_PW_VALU] = env_int("DNX_GEMM_PW_VALU", 0) != 0 ? 1 : 0;
   g_opt[DNX_OPT_GEMM_SMALL] = env_int("DNX_GEMM_SMALL", 0) != 0 ? 1 : 0;
   g_opt[DNX_OPT_GEMM_BIG_REG] = env_int("DNX_GEMM_BIG_REG", 0) != 0 ? 1 : 0;
-  g_opt[DNX_OPT_FDTD2D_LAUNCH_PER_STEP] = env_int("DNX_FDTD2D_PER_STEP", 0) != 0 ? 1 : 0;
+  g_opt[DNX_OPT_FDTD2D_LAUNCH_PER_STEP] = std::min(2, std::max(0, env_int("DNX_FDTD2D_PER_STEP", 0)));
 }
 
 int option(int id) {

@@ -629,7 +629,16 @@ extern "C" int32_t dnx_fdtd2d_tm_batch(const dnx_grid2d* g, int32_t batch, void*
     a.acc = acc;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (g->nx <= RES_NX && g->ny <= RES_RB * RES_R && option(DNX_OPT_FDTD2D_LAUNCH_PER_STEP) == 0) {
+  // The resident kernel runs each design on one CU: with few designs and a
+  // large DFT box its per-step accumulator traffic (per cell and frequency
+  // 16 bytes) is bound by that CU's memory bandwidth (128 x 128 x 3 DFT:
+  // 53 ms resident against 15 ms spread over the chip by per-step launches
+  // for 8 designs; equal near 256 designs), so it takes small DFT boxes
+  // (line monitors) at any batch size and large ones from 256 designs on.
+  const int opt = option(DNX_OPT_FDTD2D_LAUNCH_PER_STEP);
+  const int64_t dft = n_freq > 0 ? a.mbox.cells() * n_freq : 0;
+  const bool fits = g->nx <= RES_NX && g->ny <= RES_RB * RES_R;
+  if (fits && (opt == 2 || (opt == 0 && (dft <= 8192 || batch >= 256)))) {
     // the whole call in one launch, each design resident in one work-group
     static std::atomic<bool> attr{false};
     if (!attr.load()) {

@@ -130,8 +130,9 @@ enum {
   DNX_OPT_GEMM_SMALL = 7,      /* 1: FNO products on the 128x128 tile kernel [DNX_GEMM_SMALL] */
   DNX_OPT_GEMM_BIG_REG = 8,    /* 1: long-K BF16 products stage operands through registers instead
                                   of loading them straight into LDS [DNX_GEMM_BIG_REG] */
-  DNX_OPT_FDTD2D_LAUNCH_PER_STEP = 9, /* 1: batched 2D TM as two launches per step instead of one
-                                  resident launch per call [DNX_FDTD2D_PER_STEP] */
+  DNX_OPT_FDTD2D_LAUNCH_PER_STEP = 9, /* batched 2D TM: 0 picks, 1 two launches per step, 2 one
+                                  resident launch per call where the grid fits
+                                  [DNX_FDTD2D_PER_STEP] */
   DNX_OPT_COUNT = 10
 };
 int32_t dnx_set_option(int32_t id, int32_t value);

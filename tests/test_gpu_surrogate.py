@@ -120,13 +120,14 @@ def test_ablation_network_ranking_beats_random_selection():
 def test_resident_batched_tm_matches_per_step_launches(shape, src, mon, lib_opts):
     """The resident kernel (one work-group per design for the whole call,
     grids up to 128 x 128) gives bitwise the final fields and FP32 DFT of
-    the two-launches-per-step kernels: full-grid and line monitors, sources
-    at the grid's last row and column, ragged rows and columns."""
+    the two-launches-per-step kernels, and so does the library's pick
+    between them: full-grid and line monitors, sources at the grid's last
+    row and column, ragged rows and columns."""
     from diffnano_amd.batched2d import BatchedTM2D
     g = torch.Generator().manual_seed(shape[0] * 1000 + shape[1])
     eps = 1.5 + 10.0 * torch.rand(6, *shape, generator=g, dtype=torch.float64)
     outs = []
-    for variant in (1, 0):   # per-step launches, resident
+    for variant in (1, 2, 0):   # per-step launches, resident, the library's pick
         lib_opts.set("fdtd2d_per_step", variant)
         bt = BatchedTM2D(grid_shape=shape, dl=20.0, pml_layers=4, n_steps=300, courant=0.5,
                          source=src, monitor=mon, device="cuda")
