@@ -22,7 +22,7 @@ the time-reversal forward + autograd-recompute backward over T_cpu steps
 configuration (48 designs, one Adam iteration of 3000 steps, batched and
 one design at a time, FoMs compared; ``--designs-sample 0`` skips it).
 designs_per_s_2d: the surrogate configuration's 2D design loop (256
-candidates per step judged by the batched FDTD) against the plain loop
+candidates per step evaluated by the batched FDTD) against the plain loop
 (one FDTDSolver2D solve per candidate), same decisions checked.
 
 ``--pmc-probe``: run a short fixed sequence (a 256 MiB device copy three
@@ -421,7 +421,7 @@ def _designs_sample(n_designs, n_steps, iters, dev):
 def _designs_sample_2d(dev, iters_batched=5, iters_plain=1):
     """The 2D design loop of the surrogate configuration (128 x 128 TM, 2000
     steps, 256 candidate designs per step, the best kept): every candidate
-    judged by the batched FDTD (one resident launch for all 256) against the
+    evaluated by the batched FDTD (one resident launch for all 256) against the
     plain loop (FDTDSolver2D, one design at a time), from the same start and
     candidate stream; the plain loop's FoMs must follow the batched loop's."""
     import torch
@@ -449,7 +449,7 @@ def _designs_sample_2d(dev, iters_batched=5, iters_plain=1):
             "batched_over_plain": round(b / p, 1), "candidates_per_iteration": 256,
             "iterations": {"batched": iters_batched, "plain": iters_plain},
             "fom_matches_within_1e-4": bool(same),
-            "sample": "2D TM 128x128, 2000 steps, 256 candidates per step judged by FDTD "
+            "sample": "2D TM 128x128, 2000 steps, 256 candidates per step evaluated by FDTD "
                       "(batched resident kernel vs FDTDSolver2D one at a time)"}
 
 

@@ -1,4 +1,4 @@
-"""Torch FP64 restatement of the reference's CPU path (TEST / BASELINE INFRASTRUCTURE).
+"""Torch FP64 restatement of the reference's CPU path (test and CPU-baseline infrastructure).
 
 Only ``tests/`` and ``bench.py``'s CPU-baseline leg import this module; the
 product never does.  It performs, in PyTorch float64 on the host, the same
