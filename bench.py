@@ -36,7 +36,8 @@ c1``: 128^3 x 4000 steps (tile shapes autotuned first).  ``--config c3``:
 batched vs one-design-at-a-time Adam loop over 48 uneven 3D grids (20
 iterations of 3000 steps).  ``--config c4``: the surrogate-prescreened 2D
 design loop against the exhaustive batched and the plain loops
-(``--fno-probe``: the FNO kernels alone, for profiling).
+(``--fno-probe``: the FNO kernels alone; ``--tm2d-probe``: the batched
+2D TM kernels alone; for profiling).
 """
 
 from __future__ import annotations
@@ -343,6 +344,45 @@ def _fno_probe():
                                     "flops_per_sample_fwd": fl}}), flush=True)
 
 
+def _tm2d_probe():
+    """The batched 2D TM kernels alone, for profiling: the surrogate
+    configuration's 256 seeded designs (128 x 128, 2000 steps) with the
+    FoM-line monitor and with the full-grid monitor, each through two
+    launches per step and through the library's pick (resident where it
+    pays), three timed calls after a warm-up; prints the times."""
+    import torch
+
+    from diffnano_amd import _lib as L
+    from diffnano_amd.batched2d import BatchedTM2D
+    from diffnano_amd.workloads import SurrogateConfig, blob_noise
+
+    cfg = SurrogateConfig()
+    eps = cfg.eps(torch.stack([blob_noise(s) for s in range(256)]))
+    y0, y1, x0, x1 = cfg.line
+    mons = {"line": {"box": (y0, y1, x0, x1), "wavelengths": list(cfg.wavelengths)},
+            "full": {"box": (0, cfg.H, 0, cfg.W), "wavelengths": list(cfg.wavelengths)}}
+    out = {}
+    saved = L.get_option("fdtd2d_per_step")
+    try:
+        for name, mon in mons.items():
+            for label, opt in (("per_step", 1), ("library_pick", 0)):
+                L.set_option("fdtd2d_per_step", opt)
+                bt = BatchedTM2D(**cfg.solver, source=cfg.source, monitor=mon, device="cuda")
+                bt.forward(eps[:8])
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    bt.forward(eps)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) / 3 * 1e3
+                out[f"{name}_{label}_ms"] = round(ms, 2)
+                out[f"{name}_{label}_gcells"] = round(256 * cfg.H * cfg.W * cfg.solver["n_steps"]
+                                                      / ms / 1e6, 1)
+    finally:
+        L.set_option("fdtd2d_per_step", saved)
+    print(json.dumps({"tm2d_probe": out}), flush=True)
+
+
 def _bench_batched(args):
     """Batched vs one-at-a-time Adam design loop over uneven 3D grids."""
     import torch
@@ -628,6 +668,8 @@ def main():
     ap.add_argument("--wd", type=float, default=0.0, help="c4: AdamW weight decay")
     ap.add_argument("--finetune", type=int, default=4, help="c4: fine-tune steps per iteration")
     ap.add_argument("--fno-probe", action="store_true", help="c4: FNO kernels only (profiling)")
+    ap.add_argument("--tm2d-probe", action="store_true",
+                    help="c4: batched 2D TM kernels only (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the FDTD "
@@ -646,6 +688,8 @@ def main():
     if args.config == "c4":
         if args.fno_probe:
             _fno_probe()
+        elif args.tm2d_probe:
+            _tm2d_probe()
         else:
             _bench_surrogate(args)
         return
