@@ -181,7 +181,7 @@ def _bench_c0(args):
         F = solver.forward(e, source=src, monitors=mons).metadata["monitors"][0]
         loss = (F.abs() ** 2).sum()
         loss.backward()
-        return loss
+        return loss.detach()
 
     for _ in range(args.warmup):
         one_step()

@@ -88,6 +88,145 @@ __global__ __launch_bounds__(BX2* BY2) void k2tm_e(G2<T> g, T* __restrict__ st,
   if (es) es[i] = ez;
 }
 
+// ---- TM with the source and monitors fused ------------------------------
+//
+// The single-design TM step with its (at most one) soft source and (at most
+// kMaxMon2) Ez DFT monitors done by the E kernel, and the adjoint step with
+// the source's adjoint term in the H kernel and the monitors' adjoint terms
+// in the E kernel: two launches per step instead of four.  Per element the
+// operations and their order are those of the separate kernels (k_source,
+// k_monitor, k_source_adj, k_monitor_adj), so results are bitwise theirs.
+
+constexpr int kMaxMon2 = 4;
+
+template <typename T>
+struct Side2 {
+  int has_src;
+  Box sb;
+  const T* sprof;
+  const T* swave;  // waveform table (indexed by the step)
+  int n_mon;
+  Box mb[kMaxMon2];
+  int nf[kMaxMon2];
+  const double* ph[kMaxMon2];  // phase tables [step][f][2]
+  double* acc[kMaxMon2];       // forward: accumulators [f][1][cells][2]
+  const double* grd[kMaxMon2];  // adjoint: their gradients, same layout
+};
+
+__device__ __forceinline__ bool in_box2(const Box& b, int r, int c, int64_t& q) {
+  const int dr = r - b.y0, dc = c - b.x0;
+  if (dr < 0 || dr >= b.ny || dc < 0 || dc >= b.nx) return false;
+  q = int64_t(dr) * b.nx + dc;
+  return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2tm_e_side(G2<T> g, T* __restrict__ st,
+                                                        T* __restrict__ es, Side2<T> f, int n) {
+  DNX_2D_PROLOGUE
+  T* Ez = st;
+  const T* Hx = st + N;
+  const T* Hy = st + 2 * N;
+  const T dHy = r > 0 ? Hy[i] - Hy[i - nx] : T(0);
+  const T dHx = c > 0 ? Hx[i] - Hx[i - 1] : T(0);
+  T ez = (g.bx[c] * g.by[r]) * Ez[i] + g.ce[i] * (g.ax[c] * dHy - g.ay[r] * dHx);
+  int64_t q;
+  if (f.has_src && in_box2(f.sb, r, c, q)) ez += f.swave[n] * f.sprof[q];
+  Ez[i] = ez;
+  if (es) es[i] = ez;
+  for (int j = 0; j < f.n_mon; ++j) {
+    if (!in_box2(f.mb[j], r, c, q)) continue;
+    const int64_t cells = f.mb[j].cells();
+    const double* ph = f.ph[j] + int64_t(n) * f.nf[j] * 2;
+    const double v = double(ez);
+    for (int k = 0; k < f.nf[j]; ++k) {
+      double* a = f.acc[j] + (int64_t(k) * cells + q) * 2;
+      a[0] += v * ph[2 * k];
+      a[1] -= v * ph[2 * k + 1];
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2tm_adj_h_side(G2<T> g, T* __restrict__ lam,
+                                                            T* __restrict__ S, Side2<T> f,
+                                                            int m) {
+  DNX_2D_PROLOGUE
+  const T* L = lam;
+  T* Mx = lam + N;
+  T* My = lam + 2 * N;
+  int64_t q;
+  if (f.has_src && in_box2(f.sb, r, c, q)) S[i] -= f.swave[m] * f.sprof[q] * L[i];
+  const T u = g.ce[i] * L[i];
+  const T tc = (c >= 1 ? u : T(0)) - (c + 1 < g.nx ? g.ce[i + 1] * L[i + 1] : T(0));
+  const T tr = (r >= 1 ? u : T(0)) - (r + 1 < g.ny ? g.ce[i + nx] * L[i + nx] : T(0));
+  Mx[i] = g.by[r] * Mx[i] - g.ay[r] * tc;
+  My[i] = g.bx[c] * My[i] + g.ax[c] * tr;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX2* BY2) void k2tm_adj_e_side(G2<T> g, T* __restrict__ lam,
+                                                            const T* __restrict__ em,
+                                                            T* __restrict__ S, Side2<T> f,
+                                                            int m) {
+  DNX_2D_PROLOGUE
+  T* L = lam;
+  const T* Mx = lam + N;
+  const T* My = lam + 2 * N;
+  const T r_my = (r >= 1 ? My[i - nx] : T(0)) - (r <= g.ny - 2 ? My[i] : T(0));
+  const T c_mx = (c >= 1 ? Mx[i - 1] : T(0)) - (c <= g.nx - 2 ? Mx[i] : T(0));
+  const T t = g.kh * (g.ax[c] * r_my - g.ay[r] * c_mx);
+  L[i] = (g.bx[c] * g.by[r]) * L[i] + t;
+  S[i] += em[i] * t;
+  for (int j = 0; j < f.n_mon; ++j) {
+    int64_t q;
+    if (!in_box2(f.mb[j], r, c, q)) continue;
+    const int64_t cells = f.mb[j].cells();
+    const double* ph = f.ph[j] + int64_t(m - 1) * f.nf[j] * 2;
+    double d = 0.0;
+    for (int k = 0; k < f.nf[j]; ++k) {
+      const double* gg = f.grd[j] + (int64_t(k) * cells + q) * 2;
+      d += gg[0] * ph[2 * k] - gg[1] * ph[2 * k + 1];
+    }
+    const T dv = T(d);
+    L[i] += dv;
+    T sacc = T(0);
+    sacc += em[i] * dv;
+    S[i] += sacc;
+  }
+}
+
+// Whether the TM step takes the fused kernels: TM, at most one source, at
+// most kMaxMon2 monitors, each of Ez only, and not switched off.
+static bool side2_ok(const dnx_grid2d* gd, int n_src, const dnx_monitor* mons, int n_mon) {
+  if (gd->te || n_src > 1 || n_mon > kMaxMon2 || option(DNX_OPT_FDTD2D_SEPARATE_SIDE) != 0)
+    return false;
+  for (int j = 0; j < n_mon; ++j)
+    if (mons[j].comp_mask != 1) return false;
+  return true;
+}
+
+template <typename T>
+static Side2<T> make_side2(const dnx_source* srcs, int n_src, const dnx_monitor* mons,
+                           int n_mon, bool adjoint) {
+  Side2<T> f{};
+  f.has_src = n_src > 0 ? 1 : 0;
+  if (n_src > 0) {
+    f.sb = make_box(srcs[0].box);
+    f.sprof = static_cast<const T*>(srcs[0].profile);
+    f.swave = static_cast<const T*>(srcs[0].wave);
+  }
+  f.n_mon = n_mon;
+  for (int j = 0; j < n_mon; ++j) {
+    f.mb[j] = make_box(mons[j].box);
+    f.nf[j] = mons[j].n_freq;
+    f.ph[j] = mons[j].phase;
+    f.acc[j] = adjoint ? nullptr : mons[j].acc;
+    f.grd[j] = adjoint ? mons[j].grad : nullptr;
+  }
+  return f;
+}
+
 // ---- TE forward ----------------------------------------------------------
 
 template <typename T>
@@ -216,6 +355,8 @@ static int run2(const dnx_grid2d* gd, T* st, int n0, int n_steps, const dnx_sour
   const dim3 grid = grid2_launch(gd), block(BX2, BY2);
   const int64_t N = g.n;
   Comps<T> F{{st, nullptr, nullptr}};
+  const bool fuse = side2_ok(gd, n_src, mons, n_mon);
+  const Side2<T> side = fuse ? make_side2<T>(srcs, n_src, mons, n_mon, false) : Side2<T>{};
   for (int k = 0; k < n_steps; ++k) {
     const int n = n0 + k;
     if (ckpt && k % ckpt_every == 0) {
@@ -223,6 +364,16 @@ static int run2(const dnx_grid2d* gd, T* st, int n0, int n_steps, const dnx_sour
                                    sizeof(T) * 3 * N, hipMemcpyDeviceToDevice, s));
     }
     T* slot = es ? es + int64_t(k) * N : nullptr;
+    if (fuse) {
+      hipLaunchKernelGGL(k2tm_h<T>, grid, block, 0, s, g, st);
+      hipLaunchKernelGGL(k2tm_e_side<T>, grid, block, 0, s, g, st, slot, side, n);
+      DNX_CHECK_HIP(hipGetLastError());
+      if (probe >= 0) {
+        DNX_CHECK_HIP(hipMemcpyAsync(probe_out + k, st + probe, sizeof(T),
+                                     hipMemcpyDeviceToDevice, s));
+      }
+      continue;
+    }
     if (gd->te) {
       hipLaunchKernelGGL(k2te_e<T>, grid, block, 0, s, g, st);
       hipLaunchKernelGGL(k2te_h<T>, grid, block, 0, s, g, st, slot);
@@ -274,9 +425,17 @@ static int adj_run2(const dnx_grid2d* gd, T* lam, int m_hi, int m_lo, const T* e
   const dim3 grid = grid2_launch(gd), block(BX2, BY2);
   const int64_t N = g.n;
   Comps<T> L{{lam, nullptr, nullptr}};
+  const bool fuse = side2_ok(gd, n_src, mons, n_mon);
+  const Side2<T> side = fuse ? make_side2<T>(srcs, n_src, mons, n_mon, true) : Side2<T>{};
   for (int m = m_hi - 1; m >= m_lo && m >= 1; --m) {
     const T* em = (m == m_lo) ? e_lo : es + int64_t(m - m_lo - 1) * N;
     Comps<T> Em{{const_cast<T*>(em), nullptr, nullptr}};
+    if (fuse) {
+      hipLaunchKernelGGL(k2tm_adj_h_side<T>, grid, block, 0, s, g, lam, S, side, m);
+      hipLaunchKernelGGL(k2tm_adj_e_side<T>, grid, block, 0, s, g, lam, em, S, side, m);
+      DNX_CHECK_HIP(hipGetLastError());
+      continue;
+    }
     if (gd->te) {
       hipLaunchKernelGGL(k2te_adj_e<T>, grid, block, 0, s, g, lam, em, S);
       hipLaunchKernelGGL(k2te_adj_h<T>, grid, block, 0, s, g, lam);

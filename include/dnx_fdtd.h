@@ -133,7 +133,9 @@ enum {
   DNX_OPT_FDTD2D_LAUNCH_PER_STEP = 9, /* batched 2D TM: 0 picks, 1 two launches per step, 2 one
                                   resident launch per call where the grid fits
                                   [DNX_FDTD2D_PER_STEP] */
-  DNX_OPT_COUNT = 10
+  DNX_OPT_FDTD2D_SEPARATE_SIDE = 10, /* 1: 2D TM source and monitors as separate kernels
+                                  instead of inside the step kernels [DNX_FDTD2D_SEPARATE] */
+  DNX_OPT_COUNT = 11
 };
 int32_t dnx_set_option(int32_t id, int32_t value);
 int32_t dnx_get_option(int32_t id, int32_t* value);

@@ -259,6 +259,39 @@ def test_c0_full_case_vs_reference():
     assert ef < F32_FIELD and em < F32_FIELD and eg < F32_GRAD
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("src", [{"type": "gaussian_pulse", "pos": [20, 13], "probe": (25, 17)},
+                                 {"type": "continuous", "row": 9, "amplitude": 0.7}])
+def test_fdtd2d_tm_fused_source_and_monitors_bitwise(src, dtype, lib_opts):
+    """The TM step kernels that apply the source and accumulate the DFT
+    monitors themselves give the bits of the separate source, monitor and
+    adjoint kernels: final field, two monitors (one at the grid's edge, three
+    wavelengths) and dL/d eps."""
+    s = _solver2d(grid_shape=(45, 38), dl=20.0, pml_layers=6, n_steps=150, dtype=dtype,
+                  checkpoint_every=32)
+    gen = torch.Generator().manual_seed(11)
+    eps0 = 1.0 + 3.0 * torch.rand(45, 38, generator=gen, dtype=torch.float64)
+    mons = [{"box": (30, 31, 5, 33), "wavelengths": [1300.0, 1550.0, 1700.0]},
+            {"box": (0, 8, 30, 38), "wavelengths": [1550.0]}]
+
+    def run():
+        eps = eps0.clone().to(dtype).requires_grad_(True)
+        res = s.forward(eps, source=dict(src), monitors=[dict(m) for m in mons])
+        Fs = list(res.metadata["monitors"])
+        loss = (res.field ** 2).sum() + sum((F.abs() ** 2).sum() for F in Fs)
+        loss.backward()
+        probe = [torch.stack(res.metadata["snapshots"])] if res.metadata["snapshots"] else []
+        return [res.field.detach()] + [F.detach() for F in Fs] + probe + [eps.grad]
+
+    lib_opts.set("fdtd2d_separate", 0)
+    fused = run()
+    lib_opts.set("fdtd2d_separate", 1)
+    separate = run()
+    for a, b in zip(fused, separate):
+        assert torch.equal(a, b)
+    assert float(fused[-1].abs().max()) > 0
+
+
 # ---------------------------------------------------------------------------
 # C ABI error paths and API behaviour on the GPU
 # ---------------------------------------------------------------------------
