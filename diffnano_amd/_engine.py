@@ -266,7 +266,7 @@ class Plan:
         self.probe = int(probe)
         self.probe_values = None
         # device bytes the time-reversal forward may spend on E of the final
-        # steps (3D): None = half the free device memory, keeping 16 GiB free
+        # steps: None = half the spare device memory above 16 GiB
         self.e_store_bytes = None
 
 
@@ -279,7 +279,7 @@ def _tail_segments(plan, nseg, K, seg_bytes):
     as ``plan.e_store_bytes`` (or, by default, half of the free device
     memory above 16 GiB) holds, at most all but the first."""
     eng = plan.engine
-    if eng.dim != 3 or plan.probe >= 0 or nseg < 2 or seg_bytes <= 0:
+    if plan.probe >= 0 or nseg < 2 or seg_bytes <= 0:
         return 0
     budget = plan.e_store_bytes
     if budget is None:

@@ -139,7 +139,7 @@ def storage_bytes(grid: tuple[int, int, int], n_steps: int, checkpoint_every: in
     ``n_components=3``), by name: the permittivity and coefficient, the
     forward state and its ping-pong twin, ceil(T/K) checkpoints, the replay
     store of K-1 E fields, E of every step of the final segments that the
-    3D time-reversal forward keeps (``final_segments``), four adjoint /
+    time-reversal forward keeps (``final_segments``), four adjoint /
     replay state buffers, the gradient accumulator, the float64 gradient
     and the float64 output field.
 
@@ -153,7 +153,7 @@ def storage_bytes(grid: tuple[int, int, int], n_steps: int, checkpoint_every: in
     s = elem_bytes
     nseg = math.ceil(n_steps / K) if n_steps else 0
     tail = 0
-    if n_components == 6 and nseg >= 2:
+    if nseg >= 2:
         seg = K * ne * N * s
         m = nseg - 1 if e_store_bytes is None else min(nseg - 1, int(e_store_bytes) // seg)
         tail = (n_steps - (nseg - m) * K) * ne * N * s if m > 0 else 0

@@ -12,7 +12,9 @@
 //   Ex' = bx Ex + ce ax D-c Hz
 //   Ey' = by Ey - ce ay D-r Hz
 //   Hz' = bxby Hz + kh (ax D+r Ey' - ay D+c Ex')
+#include <algorithm>
 #include <atomic>
+#include <utility>
 
 #include "dnx_aux.h"
 
@@ -227,6 +229,437 @@ static Side2<T> make_side2(const dnx_source* srcs, int n_src, const dnx_monitor*
   return f;
 }
 
+// ---- TM, several steps per launch (temporal blocking) --------------------
+//
+// One 1024-thread work-group per tile of TBI x TBI cells.  It loads the tile
+// with a halo of TBS cells on each side (a region of TBH x TBW cells; Ez and
+// Hx in LDS, Hy, the coefficients and the source profile in registers, TBR
+// rows per thread) and advances the whole region by ks <= TBS steps.  A
+// region cell whose neighbour lies outside the region (but inside the grid)
+// takes its own value in the neighbour's place; the error that starts there
+// moves one cell per step (the H update reads the +1 neighbours, the E update
+// the -1 neighbours), so after ks steps every cell of the tile is exact.  The
+// work-group writes the tile's cells only, into the other state buffer: the
+// neighbouring tiles read their halos from the input buffer.  The E store
+// and the monitors take the tile's cells at every step; the source is
+// applied to every region cell it covers.  Per cell and step the arithmetic
+// and its order are those of k2tm_h + k2tm_e_side (forward) and
+// k2tm_adj_h_side + k2tm_adj_e_side (adjoint), so results are bitwise theirs.
+//
+// The step loop reads global memory only for the adjoint's E^m (loaded one
+// step ahead): the waveform values and the DFT phases of the launch's steps
+// are staged in LDS, and so are the DFT accumulators of the tile's monitor
+// cells (forward) and the monitor gradients of the region's cells (adjoint)
+// when they fit in TB_SLOTS complex values (otherwise those go to global
+// memory at every step, as in the step kernels).
+
+constexpr int TBW = 64;               // region columns, one thread column each
+constexpr int TBRB = 1024 / TBW;      // row blocks (16)
+constexpr int TBR = 4;                // rows per thread
+constexpr int TBH = TBRB * TBR;       // region rows (64)
+constexpr int TBS = 8;                // halo = most steps per launch
+constexpr int TBI = TBW - 2 * TBS;    // tile edge (48)
+constexpr int TB_NF = 8;              // most frequencies per monitor on this path
+constexpr int TB_SLOTS = 1024;        // LDS slots for accumulators / gradients
+static_assert(TBH == TBW, "square regions");
+
+struct TbTile {
+  int ri0, ci0;            // region origin (row, column)
+  int or0, or1, oc0, oc1;  // the tile: rows [or0, or1), columns [oc0, oc1)
+};
+
+__device__ __forceinline__ TbTile tb_tile(int ny, int nx) {
+  TbTile t;
+  const int a = blockIdx.y * TBI, b = blockIdx.x * TBI;
+  t.ri0 = max(0, min(a - TBS, ny - TBH));
+  t.ci0 = max(0, min(b - TBS, nx - TBW));
+  t.or0 = a;
+  t.or1 = min(a + TBI, ny);
+  t.oc0 = b;
+  t.oc1 = min(b + TBI, nx);
+  return t;
+}
+
+// Monitor m's box clipped to a rectangle, with the first of its LDS slots:
+// slot(k, r, c) = base + (k * (r1 - r0) + r - r0) * (c1 - c0) + c - c0.
+struct TbClip {
+  int r0, r1, c0, c1, base;
+};
+
+template <typename T>
+__device__ __forceinline__ int tb_clip(const Side2<T>& f, int rr0, int rr1, int cc0, int cc1,
+                                       TbClip* cl) {
+  int n = 0;
+  for (int m = 0; m < f.n_mon; ++m) {
+    const Box& b = f.mb[m];
+    TbClip k;
+    k.r0 = max(b.y0, rr0);
+    k.r1 = max(k.r0, min(b.y0 + b.ny, rr1));
+    k.c0 = max(b.x0, cc0);
+    k.c1 = max(k.c0, min(b.x0 + b.nx, cc1));
+    k.base = n;
+    cl[m] = k;
+    n += f.nf[m] * (k.r1 - k.r0) * (k.c1 - k.c0);
+  }
+  return n;
+}
+
+// Global offset (in doubles) of LDS slot `sl` in monitor m's [f][cells][2]
+// array; m is found from the bases.
+template <typename T>
+__device__ __forceinline__ int64_t tb_slot_global(const Side2<T>& f, const TbClip* cl, int sl,
+                                                  int& m) {
+  m = 0;
+  while (m + 1 < f.n_mon && sl >= cl[m + 1].base) ++m;
+  const TbClip& k = cl[m];
+  const int w = k.c1 - k.c0, h = k.r1 - k.r0;
+  const int o = sl - k.base;
+  const int kf = o / (w * h), rem = o - kf * w * h;
+  const int r = k.r0 + rem / w, c = k.c0 + rem % w;
+  const Box& b = f.mb[m];
+  const int64_t q = int64_t(r - b.y0) * b.nx + (c - b.x0);
+  return (int64_t(kf) * b.cells() + q) * 2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict__ sin,
+                                                    T* __restrict__ sout, T* __restrict__ es,
+                                                    Side2<T> f, int n0, int ks) {
+  __shared__ T A[TBH * TBW];     // Ez
+  __shared__ T B[TBH * TBW];     // Hx
+  __shared__ T Ch[TBRB * TBW];   // Hy of each row block's last row
+  __shared__ double PH[kMaxMon2 * TBS * TB_NF * 2];
+  __shared__ double LA[TB_SLOTS * 2];
+  __shared__ T WV[TBS];
+  const int ny = g.ny, nx = g.nx;
+  const int64_t N = g.n;
+  const int tid = threadIdx.x;
+  const TbTile t = tb_tile(ny, nx);
+  const int cl = tid % TBW, rb = tid / TBW;
+  const int rl0 = rb * TBR;
+  const int c = t.ci0 + cl;
+  const bool cv = c < nx;
+  const bool cown = c >= t.oc0 && c < t.oc1;
+  const int dxr = cl + 1 < TBW ? 1 : 0, dxl = cl > 0 ? -1 : 0;
+  const T bxv = cv ? g.bx[c] : T(0), axv = cv ? g.ax[c] : T(0);
+  const T kh = g.kh;
+  // staging: waveform, phases, the tile's accumulators
+  if (tid < ks) WV[tid] = f.has_src ? f.swave[n0 + tid] : T(0);
+  for (int m = 0; m < f.n_mon; ++m) {
+    const int nf = f.nf[m];
+    for (int e = tid; e < ks * nf * 2; e += 1024) {
+      const int sidx = e / (nf * 2), o = e - sidx * nf * 2;
+      PH[(m * TBS + sidx) * TB_NF * 2 + o] = f.ph[m][int64_t(n0 + sidx) * nf * 2 + o];
+    }
+  }
+  __shared__ TbClip clip[kMaxMon2];
+  __shared__ int nslots_s;
+  if (tid == 0) nslots_s = tb_clip(f, t.or0, t.or1, t.oc0, t.oc1, clip);
+  __syncthreads();
+  const int nslots = nslots_s;
+  const bool lds_acc = nslots <= TB_SLOTS;
+  if (lds_acc) {
+    for (int sl = tid; sl < nslots; sl += 1024) {
+      int m;
+      const int64_t o = tb_slot_global(f, clip, sl, m);
+      LA[2 * sl] = f.acc[m][o];
+      LA[2 * sl + 1] = f.acc[m][o + 1];
+    }
+  }
+  T hy[TBR], cer[TBR], byr[TBR], ayr[TBR], spr[TBR];
+  unsigned own = 0, srcm = 0;
+  T* const Ar = A + rl0 * TBW + cl;
+  T* const Br = B + rl0 * TBW + cl;
+#pragma unroll
+  for (int j = 0; j < TBR; ++j) {
+    const int r = t.ri0 + rl0 + j;
+    const bool v = cv && r < ny;
+    const int64_t i = int64_t(r) * nx + c;
+    Ar[j * TBW] = v ? sin[i] : T(0);
+    Br[j * TBW] = v ? sin[N + i] : T(0);
+    hy[j] = v ? sin[2 * N + i] : T(0);
+    cer[j] = v ? g.ce[i] : T(0);
+    byr[j] = r < ny ? g.by[r] : T(0);
+    ayr[j] = r < ny ? g.ay[r] : T(0);
+    own |= unsigned(v && cown && r >= t.or0 && r < t.or1) << j;
+    int64_t q;
+    spr[j] = T(0);
+    if (f.has_src && v && in_box2(f.sb, r, c, q)) {
+      srcm |= 1u << j;
+      spr[j] = f.sprof[q];
+    }
+  }
+  const T* const Anext = A + min(rl0 + TBR, TBH - 1) * TBW + cl;
+  // every load of the prologue lands here: the step loop then waits on no
+  // memory counter (a wait there would also wait for the E-store writes)
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int s = 0; s < ks; ++s) {
+    // H from E
+#pragma unroll
+    for (int j = 0; j < TBR; ++j) {
+      const int r = t.ri0 + rl0 + j;
+      const T e = Ar[j * TBW];
+      const T er = Ar[j * TBW + dxr];
+      const T eu = j + 1 < TBR ? Ar[(j + 1) * TBW] : *Anext;
+      const T dc = (c + 1 < nx) ? er - e : T(0);
+      const T dr = (r + 1 < ny) ? eu - e : T(0);
+      Br[j * TBW] = byr[j] * Br[j * TBW] - kh * (ayr[j] * dc);
+      hy[j] = bxv * hy[j] + kh * (axv * dr);
+    }
+    Ch[rb * TBW + cl] = hy[TBR - 1];
+    __syncthreads();
+    // E from H, the source, the E store and the monitors
+    const T hbelow = Ch[max(rb - 1, 0) * TBW + cl];
+    const T w = WV[s];
+#pragma unroll
+    for (int j = 0; j < TBR; ++j) {
+      const int r = t.ri0 + rl0 + j;
+      const T hb = j > 0 ? hy[j - 1] : hbelow;
+      const T hxo = Br[j * TBW];
+      const T hl = Br[j * TBW + dxl];
+      const T dHy = r > 0 ? hy[j] - hb : T(0);
+      const T dHx = c > 0 ? hxo - hl : T(0);
+      T ez = (bxv * byr[j]) * Ar[j * TBW] + cer[j] * (axv * dHy - ayr[j] * dHx);
+      if ((srcm >> j) & 1u) ez += w * spr[j];
+      Ar[j * TBW] = ez;
+      if ((own >> j) & 1u) {
+        const int64_t i = int64_t(r) * nx + c;
+        if (es) es[int64_t(s) * N + i] = ez;
+        for (int m = 0; m < f.n_mon; ++m) {
+          const TbClip& k = clip[m];
+          if (r < k.r0 || r >= k.r1 || c < k.c0 || c >= k.c1) continue;
+          const double* ph = PH + (m * TBS + s) * TB_NF * 2;
+          const double v = double(ez);
+          const int w_ = k.c1 - k.c0, h_ = k.r1 - k.r0;
+          const Box& b = f.mb[m];
+          const int64_t cells = b.cells();
+          const int64_t q = int64_t(r - b.y0) * b.nx + (c - b.x0);
+          // two loops, so that the LDS one compiles to LDS (not flat) accesses
+          if (lds_acc) {
+            for (int kf = 0; kf < f.nf[m]; ++kf) {
+              double* a = LA + 2 * (k.base + (kf * h_ + r - k.r0) * w_ + c - k.c0);
+              a[0] += v * ph[2 * kf];
+              a[1] -= v * ph[2 * kf + 1];
+            }
+          } else {
+            for (int kf = 0; kf < f.nf[m]; ++kf) {
+              double* a = f.acc[m] + (int64_t(kf) * cells + q) * 2;
+              a[0] += v * ph[2 * kf];
+              a[1] -= v * ph[2 * kf + 1];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < TBR; ++j) {
+    if (!((own >> j) & 1u)) continue;
+    const int r = t.ri0 + rl0 + j;
+    const int64_t i = int64_t(r) * nx + c;
+    sout[i] = Ar[j * TBW];
+    sout[N + i] = Br[j * TBW];
+    sout[2 * N + i] = hy[j];
+  }
+  if (lds_acc) {
+    for (int sl = tid; sl < nslots; sl += 1024) {
+      int m;
+      const int64_t o = tb_slot_global(f, clip, sl, m);
+      f.acc[m][o] = LA[2 * sl];
+      f.acc[m][o + 1] = LA[2 * sl + 1];
+    }
+  }
+}
+
+// Adjoint steps m_top, m_top - 1, ..., m_top - ks + 1 (all >= 1).  E^m of
+// step m comes from e_lo (m == m_lo) or es slot m - m_lo - 1.
+template <typename T>
+__global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict__ lin,
+                                                    T* __restrict__ lout, T* __restrict__ S,
+                                                    const T* __restrict__ e_lo,
+                                                    const T* __restrict__ es, int m_lo,
+                                                    Side2<T> f, int m_top, int ks) {
+  __shared__ T U[TBH * TBW];     // ce * lambda_E
+  __shared__ T X[TBH * TBW];     // mu_x
+  __shared__ T Cm[TBRB * TBW];   // mu_y of each row block's last row
+  __shared__ double PH[kMaxMon2 * TBS * TB_NF * 2];
+  __shared__ double GR[TB_SLOTS * 2];
+  __shared__ T WV[TBS];
+  const int ny = g.ny, nx = g.nx;
+  const int64_t N = g.n;
+  const int tid = threadIdx.x;
+  const TbTile t = tb_tile(ny, nx);
+  const int cl = tid % TBW, rb = tid / TBW;
+  const int rl0 = rb * TBR;
+  const int c = t.ci0 + cl;
+  const bool cv = c < nx;
+  const bool cown = c >= t.oc0 && c < t.oc1;
+  const int dxr = cl + 1 < TBW ? 1 : 0, dxl = cl > 0 ? -1 : 0;
+  const T bxv = cv ? g.bx[c] : T(0), axv = cv ? g.ax[c] : T(0);
+  const T kh = g.kh;
+  // staging: waveform, phases (row m - 1 of step m), the region's gradients
+  if (tid < ks) WV[tid] = f.has_src ? f.swave[m_top - tid] : T(0);
+  for (int m = 0; m < f.n_mon; ++m) {
+    const int nf = f.nf[m];
+    for (int e = tid; e < ks * nf * 2; e += 1024) {
+      const int sidx = e / (nf * 2), o = e - sidx * nf * 2;
+      PH[(m * TBS + sidx) * TB_NF * 2 + o] = f.ph[m][int64_t(m_top - sidx - 1) * nf * 2 + o];
+    }
+  }
+  __shared__ TbClip clip[kMaxMon2];
+  __shared__ int nslots_s;
+  if (tid == 0)
+    nslots_s = tb_clip(f, t.ri0, min(t.ri0 + TBH, ny), t.ci0, min(t.ci0 + TBW, nx), clip);
+  __syncthreads();
+  const int nslots = nslots_s;
+  const bool lds_grd = nslots <= TB_SLOTS;
+  if (lds_grd) {
+    for (int sl = tid; sl < nslots; sl += 1024) {
+      int m;
+      const int64_t o = tb_slot_global(f, clip, sl, m);
+      GR[2 * sl] = f.grd[m][o];
+      GR[2 * sl + 1] = f.grd[m][o + 1];
+    }
+  }
+  T L[TBR], my[TBR], Sr[TBR], cer[TBR], byr[TBR], ayr[TBR], spr[TBR], emr[TBR];
+  unsigned own = 0, srcm = 0, valid = 0;
+  T* const Ur = U + rl0 * TBW + cl;
+  T* const Xr = X + rl0 * TBW + cl;
+  {
+    const T* em0 = (m_top == m_lo) ? e_lo : es + int64_t(m_top - m_lo - 1) * N;
+#pragma unroll
+    for (int j = 0; j < TBR; ++j) {
+      const int r = t.ri0 + rl0 + j;
+      const bool v = cv && r < ny;
+      const int64_t i = int64_t(r) * nx + c;
+      L[j] = v ? lin[i] : T(0);
+      Xr[j * TBW] = v ? lin[N + i] : T(0);
+      my[j] = v ? lin[2 * N + i] : T(0);
+      cer[j] = v ? g.ce[i] : T(0);
+      byr[j] = r < ny ? g.by[r] : T(0);
+      ayr[j] = r < ny ? g.ay[r] : T(0);
+      const bool o = v && cown && r >= t.or0 && r < t.or1;
+      own |= unsigned(o) << j;
+      valid |= unsigned(v) << j;
+      Sr[j] = o ? S[i] : T(0);
+      emr[j] = o ? em0[i] : T(0);
+      int64_t q;
+      spr[j] = T(0);
+      if (f.has_src && v && in_box2(f.sb, r, c, q)) {
+        srcm |= 1u << j;
+        spr[j] = f.sprof[q];
+      }
+    }
+  }
+  const T* const Unext = U + min(rl0 + TBR, TBH - 1) * TBW + cl;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int s = 0; s < ks; ++s) {
+    const int m = m_top - s;
+    // the source's adjoint term (tile cells), then ce * lambda for the H update
+    const T w = WV[s];
+#pragma unroll
+    for (int j = 0; j < TBR; ++j) {
+      if (((srcm & own) >> j) & 1u) Sr[j] -= w * spr[j] * L[j];
+      Ur[j * TBW] = cer[j] * L[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < TBR; ++j) {
+      const int r = t.ri0 + rl0 + j;
+      const T u = Ur[j * TBW];
+      const T uc = Ur[j * TBW + dxr];
+      const T uu = j + 1 < TBR ? Ur[(j + 1) * TBW] : *Unext;
+      const T tc = (c >= 1 ? u : T(0)) - (c + 1 < nx ? uc : T(0));
+      const T tr = (r >= 1 ? u : T(0)) - (r + 1 < ny ? uu : T(0));
+      Xr[j * TBW] = byr[j] * Xr[j * TBW] - ayr[j] * tc;
+      my[j] = bxv * my[j] + axv * tr;
+    }
+    Cm[rb * TBW + cl] = my[TBR - 1];
+    __syncthreads();
+    const T mbelow = Cm[max(rb - 1, 0) * TBW + cl];
+#pragma unroll
+    for (int j = 0; j < TBR; ++j) {
+      const int r = t.ri0 + rl0 + j;
+      const T mb = j > 0 ? my[j - 1] : mbelow;
+      const T r_my = (r >= 1 ? mb : T(0)) - (r <= ny - 2 ? my[j] : T(0));
+      const T c_mx = (c >= 1 ? Xr[j * TBW + dxl] : T(0)) - (c <= nx - 2 ? Xr[j * TBW] : T(0));
+      const T tt = kh * (axv * r_my - ayr[j] * c_mx);
+      L[j] = (bxv * byr[j]) * L[j] + tt;
+      const bool o = (own >> j) & 1u;
+      const T emv = emr[j];
+      if (o) Sr[j] += emv * tt;
+      if (!((valid >> j) & 1u)) continue;
+      for (int mm = 0; mm < f.n_mon; ++mm) {
+        const TbClip& k = clip[mm];
+        if (r < k.r0 || r >= k.r1 || c < k.c0 || c >= k.c1) continue;
+        const double* ph = PH + (mm * TBS + s) * TB_NF * 2;
+        const int w_ = k.c1 - k.c0, h_ = k.r1 - k.r0;
+        const Box& b = f.mb[mm];
+        const int64_t cells = b.cells();
+        const int64_t q = int64_t(r - b.y0) * b.nx + (c - b.x0);
+        double d = 0.0;
+        if (lds_grd) {
+          for (int kf = 0; kf < f.nf[mm]; ++kf) {
+            const double* gg = GR + 2 * (k.base + (kf * h_ + r - k.r0) * w_ + c - k.c0);
+            d += gg[0] * ph[2 * kf] - gg[1] * ph[2 * kf + 1];
+          }
+        } else {
+          for (int kf = 0; kf < f.nf[mm]; ++kf) {
+            const double* gg = f.grd[mm] + (int64_t(kf) * cells + q) * 2;
+            d += gg[0] * ph[2 * kf] - gg[1] * ph[2 * kf + 1];
+          }
+        }
+        const T dv = T(d);
+        L[j] += dv;
+        if (o) {
+          T sacc = T(0);
+          sacc += emv * dv;
+          Sr[j] += sacc;
+        }
+      }
+    }
+    // E^m of the next step, ahead of its use
+    if (s + 1 < ks) {
+      const int mn = m - 1;
+      const T* emn = (mn == m_lo) ? e_lo : es + int64_t(mn - m_lo - 1) * N;
+#pragma unroll
+      for (int j = 0; j < TBR; ++j) {
+        const int r = t.ri0 + rl0 + j;
+        emr[j] = ((own >> j) & 1u) ? emn[int64_t(r) * nx + c] : T(0);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TBR; ++j) {
+    if (!((own >> j) & 1u)) continue;
+    const int r = t.ri0 + rl0 + j;
+    const int64_t i = int64_t(r) * nx + c;
+    lout[i] = L[j];
+    lout[N + i] = Xr[j * TBW];
+    lout[2 * N + i] = my[j];
+    S[i] = Sr[j];
+  }
+}
+
+// Steps per launch of the temporally blocked TM kernels: 0 when the step
+// kernels run instead (TE, more than one source or kMaxMon2 monitors, a
+// monitor of more than TB_NF frequencies, a probe, no second buffer, or the
+// option set to 1).
+static int tb_steps(const dnx_grid2d* gd, int n_src, const dnx_monitor* mons, int n_mon,
+                    int64_t probe, const void* a, const void* b) {
+  const int opt = option(DNX_OPT_FDTD2D_TB_STEPS);
+  if (opt == 1 || probe >= 0 || b == nullptr || a == b || !side2_ok(gd, n_src, mons, n_mon))
+    return 0;
+  for (int j = 0; j < n_mon; ++j)
+    if (mons[j].n_freq > TB_NF) return 0;
+  return opt <= 0 ? TBS : std::min(opt, TBS);
+}
+
+static dim3 tb_grid(const dnx_grid2d* g) { return dim3(cdiv(g->nx, TBI), cdiv(g->ny, TBI)); }
+
 // ---- TE forward ----------------------------------------------------------
 
 template <typename T>
@@ -348,15 +781,37 @@ static int check_grid2(const dnx_grid2d* g) {
 static dim3 grid2_launch(const dnx_grid2d* g) { return dim3(cdiv(g->nx, BX2), cdiv(g->ny, BY2)); }
 
 template <typename T>
-static int run2(const dnx_grid2d* gd, T* st, int n0, int n_steps, const dnx_source* srcs,
-                int n_src, const dnx_monitor* mons, int n_mon, T* ckpt, int ckpt_every, T* es,
-                int64_t probe, T* probe_out, hipStream_t s) {
+static int run2(const dnx_grid2d* gd, T* st, T* work, int n0, int n_steps,
+                const dnx_source* srcs, int n_src, const dnx_monitor* mons, int n_mon, T* ckpt,
+                int ckpt_every, T* es, int64_t probe, T* probe_out, hipStream_t s,
+                int32_t* final_in_work) {
   G2<T> g = make_g2<T>(gd);
   const dim3 grid = grid2_launch(gd), block(BX2, BY2);
   const int64_t N = g.n;
   Comps<T> F{{st, nullptr, nullptr}};
   const bool fuse = side2_ok(gd, n_src, mons, n_mon);
   const Side2<T> side = fuse ? make_side2<T>(srcs, n_src, mons, n_mon, false) : Side2<T>{};
+  if (const int tb = tb_steps(gd, n_src, mons, n_mon, probe, st, work)) {
+    // temporally blocked: launches of up to tb steps, ping-ponging between
+    // st and work, none crossing a checkpoint
+    T* cur = st;
+    T* nxt = work;
+    for (int k = 0; k < n_steps;) {
+      if (ckpt && k % ckpt_every == 0) {
+        DNX_CHECK_HIP(hipMemcpyAsync(ckpt + int64_t(k / ckpt_every) * 3 * N, cur,
+                                     sizeof(T) * 3 * N, hipMemcpyDeviceToDevice, s));
+      }
+      int ks = std::min(tb, n_steps - k);
+      if (ckpt) ks = std::min(ks, ckpt_every - k % ckpt_every);
+      hipLaunchKernelGGL(k2tm_tb_fwd<T>, tb_grid(gd), dim3(1024), 0, s, g, cur, nxt,
+                         es ? es + int64_t(k) * N : nullptr, side, n0 + k, ks);
+      DNX_CHECK_HIP(hipGetLastError());
+      std::swap(cur, nxt);
+      k += ks;
+    }
+    if (final_in_work) *final_in_work = cur != st ? 1 : 0;
+    return DNX_OK;
+  }
   for (int k = 0; k < n_steps; ++k) {
     const int n = n0 + k;
     if (ckpt && k % ckpt_every == 0) {
@@ -418,15 +873,36 @@ static int adj_init2(const dnx_grid2d* gd, T* lam, const T* eT, const dnx_monito
 }
 
 template <typename T>
-static int adj_run2(const dnx_grid2d* gd, T* lam, int m_hi, int m_lo, const T* e_lo,
+static int adj_run2(const dnx_grid2d* gd, T* lam, T* work, int m_hi, int m_lo, const T* e_lo,
                     const T* es, const dnx_source* srcs, int n_src, const dnx_monitor* mons,
-                    int n_mon, T* S, hipStream_t s) {
+                    int n_mon, T* S, hipStream_t s, int32_t* final_in_work) {
   G2<T> g = make_g2<T>(gd);
   const dim3 grid = grid2_launch(gd), block(BX2, BY2);
   const int64_t N = g.n;
-  Comps<T> L{{lam, nullptr, nullptr}};
   const bool fuse = side2_ok(gd, n_src, mons, n_mon);
   const Side2<T> side = fuse ? make_side2<T>(srcs, n_src, mons, n_mon, true) : Side2<T>{};
+  if (const int tb = tb_steps(gd, n_src, mons, n_mon, -1, lam, work)) {
+    T* cur = lam;
+    T* nxt = work;
+    const int m_stop = std::max(m_lo, 1);
+    for (int m = m_hi - 1; m >= m_stop;) {
+      const int ks = std::min(tb, m - m_stop + 1);
+      hipLaunchKernelGGL(k2tm_tb_adj<T>, tb_grid(gd), dim3(1024), 0, s, g, cur, nxt, S, e_lo,
+                         es, m_lo, side, m, ks);
+      DNX_CHECK_HIP(hipGetLastError());
+      std::swap(cur, nxt);
+      m -= ks;
+    }
+    if (m_lo == 0) {
+      for (int j = 0; j < n_src; ++j) {
+        int rc = launch_source_adj<T>(S, cur, g.ny, g.nx, srcs[j], 0, s);
+        if (rc) return rc;
+      }
+    }
+    if (final_in_work) *final_in_work = cur != lam ? 1 : 0;
+    return DNX_OK;
+  }
+  Comps<T> L{{lam, nullptr, nullptr}};
   for (int m = m_hi - 1; m >= m_lo && m >= 1; --m) {
     const T* em = (m == m_lo) ? e_lo : es + int64_t(m - m_lo - 1) * N;
     Comps<T> Em{{const_cast<T*>(em), nullptr, nullptr}};
@@ -687,7 +1163,6 @@ extern "C" int32_t dnx_fdtd2d_run(const dnx_grid2d* g, void* state, void* work, 
                                   const dnx_monitor* mons, int32_t n_mon, void* ckpt,
                                   int32_t ckpt_every, void* estore, int64_t probe_index,
                                   void* probe_out, void* stream, int32_t* final_in_work) {
-  (void)work;
   int rc = check_grid2(g);
   if (rc) return rc;
   DNX_REQUIRE(state != nullptr, "fdtd2d_run: null state");
@@ -701,12 +1176,14 @@ extern "C" int32_t dnx_fdtd2d_run(const dnx_grid2d* g, void* state, void* work, 
   if (final_in_work) *final_in_work = 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (g->dtype == DNX_F32)
-    return run2<float>(g, static_cast<float*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
-                       static_cast<float*>(ckpt), ckpt_every, static_cast<float*>(estore),
-                       probe_index, static_cast<float*>(probe_out), s);
-  return run2<double>(g, static_cast<double*>(state), n0, n_steps, srcs, n_src, mons, n_mon,
-                      static_cast<double*>(ckpt), ckpt_every, static_cast<double*>(estore),
-                      probe_index, static_cast<double*>(probe_out), s);
+    return run2<float>(g, static_cast<float*>(state), static_cast<float*>(work), n0, n_steps,
+                       srcs, n_src, mons, n_mon, static_cast<float*>(ckpt), ckpt_every,
+                       static_cast<float*>(estore), probe_index,
+                       static_cast<float*>(probe_out), s, final_in_work);
+  return run2<double>(g, static_cast<double*>(state), static_cast<double*>(work), n0, n_steps,
+                      srcs, n_src, mons, n_mon, static_cast<double*>(ckpt), ckpt_every,
+                      static_cast<double*>(estore), probe_index,
+                      static_cast<double*>(probe_out), s, final_in_work);
 }
 
 extern "C" int32_t dnx_fdtd2d_adjoint_init(const dnx_grid2d* g, void* lam, const void* e_T,
@@ -730,7 +1207,6 @@ extern "C" int32_t dnx_fdtd2d_adjoint_run(const dnx_grid2d* g, void* lam, void* 
                                           const void* estore, const dnx_source* srcs,
                                           int32_t n_src, const dnx_monitor* mons, int32_t n_mon,
                                           void* S, void* stream, int32_t* final_in_work) {
-  (void)work;
   int rc = check_grid2(g);
   if (rc) return rc;
   DNX_REQUIRE(lam && S, "fdtd2d_adjoint_run: null pointer");
@@ -744,12 +1220,12 @@ extern "C" int32_t dnx_fdtd2d_adjoint_run(const dnx_grid2d* g, void* lam, void* 
   if (final_in_work) *final_in_work = 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (g->dtype == DNX_F32)
-    return adj_run2<float>(g, static_cast<float*>(lam), m_hi, m_lo,
+    return adj_run2<float>(g, static_cast<float*>(lam), static_cast<float*>(work), m_hi, m_lo,
                            static_cast<const float*>(e_lo), static_cast<const float*>(estore),
-                           srcs, n_src, mons, n_mon, static_cast<float*>(S), s);
-  return adj_run2<double>(g, static_cast<double*>(lam), m_hi, m_lo,
+                           srcs, n_src, mons, n_mon, static_cast<float*>(S), s, final_in_work);
+  return adj_run2<double>(g, static_cast<double*>(lam), static_cast<double*>(work), m_hi, m_lo,
                           static_cast<const double*>(e_lo), static_cast<const double*>(estore),
-                          srcs, n_src, mons, n_mon, static_cast<double*>(S), s);
+                          srcs, n_src, mons, n_mon, static_cast<double*>(S), s, final_in_work);
 }
 
 extern "C" int32_t dnx_fdtd2d_tm_batch(const dnx_grid2d* g, int32_t batch, void* state,

@@ -8,7 +8,9 @@ TM, Hz for TE) and the reference's metadata.  The gradient (plain autograd
 in the reference) comes from the hand-written adjoint kernels.
 
 Extensions (keyword-only): ``backward`` (accepted for symmetry with 3D; all
-modes give the same gradient), ``dtype``, ``checkpoint_every``,
+modes give the same gradient), ``dtype``, ``checkpoint_every``, ``e_store_gb``
+(device memory for E of the final checkpoint segments, which the backward
+then does not replay; default: half the spare device memory above 16 GiB),
 ``forward(..., monitors=[{"box": (y0, y1, x0, x1), "wavelengths": [...]}])``
 and source key ``profile``.
 """
@@ -49,6 +51,7 @@ class FDTDSolver2D:
         dtype: torch.dtype = torch.float32,
         checkpoint_every: int = 64,
         check_eps: str = "warn",
+        e_store_gb: float | None = None,
     ):
         self.grid_shape = grid_shape
         self.dl = dl
@@ -63,6 +66,9 @@ class FDTDSolver2D:
         self.backward = backward
         self.dtype = dtype
         self.checkpoint_every = checkpoint_every
+        if e_store_gb is not None and not e_store_gb >= 0:
+            raise ValueError(f"e_store_gb must be None or >= 0, got {e_store_gb!r}")
+        self.e_store_gb = e_store_gb
         if check_eps not in EPS_CHECKS:
             raise ValueError(f"check_eps must be one of {EPS_CHECKS}, got {check_eps!r}")
         self.check_eps = check_eps
@@ -170,6 +176,8 @@ class FDTDSolver2D:
         plan = Plan(eng, self.n_steps, self._ckpt_every(), self._source_specs(src, self.n_steps),
                     mon_specs, out_order=[0], out_unsqueeze=True, probe=probe,
                     out_device=self._device)
+        if self.e_store_gb is not None:
+            plan.e_store_bytes = int(self.e_store_gb * 1e9)
         if torch.is_grad_enabled() and eps_r.requires_grad:
             outs = _TimeReversalFDTD.apply(eps_r, plan)
             field, mon_raw = outs[0], list(outs[1:])

@@ -135,7 +135,9 @@ enum {
                                   [DNX_FDTD2D_PER_STEP] */
   DNX_OPT_FDTD2D_SEPARATE_SIDE = 10, /* 1: 2D TM source and monitors as separate kernels
                                   instead of inside the step kernels [DNX_FDTD2D_SEPARATE] */
-  DNX_OPT_COUNT = 11
+  DNX_OPT_FDTD2D_TB_STEPS = 11,  /* 2D TM steps per launch of the temporally blocked kernels
+                                  (0: 8, the most; 1: the one-step kernels) [DNX_FDTD2D_TB] */
+  DNX_OPT_COUNT = 12
 };
 int32_t dnx_set_option(int32_t id, int32_t value);
 int32_t dnx_get_option(int32_t id, int32_t* value);

@@ -44,7 +44,9 @@ def test_storage_model_hand_values():
         == 90 * 3 * N * 4
     s1 = storage_bytes((4, 5, 6), 7, 1, elem_bytes=8, n_components=3)   # float64, 2D-style
     assert s1["checkpoints"] == 7 * 3 * 120 * 8 and s1["replay_store"] == 120 * 8
-    assert s1["final_segments"] == 0   # the 2D solvers keep no such store
+    assert s1["final_segments"] == 6 * 120 * 8   # E (one component) of steps 1..6
+    assert storage_bytes((4, 5, 6), 7, 1, elem_bytes=8, n_components=3,
+                         e_store_bytes=2 * 120 * 8)["final_segments"] == 2 * 120 * 8
     assert storage_bytes((4, 4, 4), 0, 8)["checkpoints"] == 0
     # "checkpoint" mode: K = ceil(T / segments); others: checkpoint_every (default 64)
     assert model_peak_mb((16, 16, 16), 100, {"use_checkpoint": True, "checkpoint_segments": 2}) \

@@ -260,19 +260,27 @@ def test_c0_full_case_vs_reference():
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("src", [{"type": "gaussian_pulse", "pos": [20, 13], "probe": (25, 17)},
-                                 {"type": "continuous", "row": 9, "amplitude": 0.7}])
-def test_fdtd2d_tm_fused_source_and_monitors_bitwise(src, dtype, lib_opts):
+@pytest.mark.parametrize("shape,src,mons,K", [
+    ((45, 38), {"type": "gaussian_pulse", "pos": [20, 13]},
+     [{"box": (30, 31, 5, 33), "wavelengths": [1300.0, 1550.0, 1700.0]},
+      {"box": (0, 8, 30, 38), "wavelengths": [1550.0]}], 32),
+    ((45, 38), {"type": "continuous", "row": 9, "amplitude": 0.7},
+     [{"box": (30, 31, 5, 33), "wavelengths": [1300.0, 1550.0, 1700.0]}], 13),
+    ((131, 100), {"type": "gaussian_pulse", "row": 60},
+     [{"box": (95, 96, 0, 100), "wavelengths": [1500.0, 1600.0]},
+      {"box": (40, 56, 44, 52), "wavelengths": [1550.0]}], 21),
+])
+def test_fdtd2d_tm_fused_and_blocked_kernels_bitwise(shape, src, mons, K, dtype, lib_opts):
     """The TM step kernels that apply the source and accumulate the DFT
-    monitors themselves give the bits of the separate source, monitor and
-    adjoint kernels: final field, two monitors (one at the grid's edge, three
-    wavelengths) and dL/d eps."""
-    s = _solver2d(grid_shape=(45, 38), dl=20.0, pml_layers=6, n_steps=150, dtype=dtype,
-                  checkpoint_every=32)
+    monitors themselves, and the temporally blocked kernels (8 and 3 steps
+    per launch, tiles of 48 x 48 cells with 8-cell halos), give the bits of
+    the separate source, monitor and adjoint kernels: final field, monitors
+    (at the grid's edge, several wavelengths, a 16-row box across tiles) and
+    dL/d eps, with checkpoint intervals that cut the launches short."""
+    s = _solver2d(grid_shape=shape, dl=20.0, pml_layers=6, n_steps=150, dtype=dtype,
+                  checkpoint_every=K)
     gen = torch.Generator().manual_seed(11)
-    eps0 = 1.0 + 3.0 * torch.rand(45, 38, generator=gen, dtype=torch.float64)
-    mons = [{"box": (30, 31, 5, 33), "wavelengths": [1300.0, 1550.0, 1700.0]},
-            {"box": (0, 8, 30, 38), "wavelengths": [1550.0]}]
+    eps0 = 1.0 + 3.0 * torch.rand(shape, generator=gen, dtype=torch.float64)
 
     def run():
         eps = eps0.clone().to(dtype).requires_grad_(True)
@@ -280,16 +288,28 @@ def test_fdtd2d_tm_fused_source_and_monitors_bitwise(src, dtype, lib_opts):
         Fs = list(res.metadata["monitors"])
         loss = (res.field ** 2).sum() + sum((F.abs() ** 2).sum() for F in Fs)
         loss.backward()
-        probe = [torch.stack(res.metadata["snapshots"])] if res.metadata["snapshots"] else []
-        return [res.field.detach()] + [F.detach() for F in Fs] + probe + [eps.grad]
+        return [res.field.detach()] + [F.detach() for F in Fs] + [eps.grad]
 
-    lib_opts.set("fdtd2d_separate", 0)
-    fused = run()
     lib_opts.set("fdtd2d_separate", 1)
-    separate = run()
-    for a, b in zip(fused, separate):
-        assert torch.equal(a, b)
-    assert float(fused[-1].abs().max()) > 0
+    ref = run()
+    lib_opts.set("fdtd2d_separate", 0)
+    for tb in (1, 0, 3):          # one step per launch, 8, 3
+        lib_opts.set("fdtd2d_tb_steps", tb)
+        got = run()
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b), tb
+    assert float(ref[-1].abs().max()) > 0
+
+
+def test_fdtd2d_probe_runs_the_step_kernels_and_matches(lib_opts):
+    """A probe (time series) keeps the one-step kernels; its field equals
+    the blocked kernels' run without the probe."""
+    s = _solver2d(grid_shape=(70, 90), dl=20.0, pml_layers=6, n_steps=120)
+    eps = torch.full((70, 90), 2.0, dtype=torch.float64)
+    a = s.forward(eps, source={"type": "gaussian_pulse", "pos": [30, 40], "probe": (35, 45)})
+    b = s.forward(eps, source={"type": "gaussian_pulse", "pos": [30, 40]})
+    assert len(a.metadata["snapshots"]) == 120
+    assert torch.equal(a.field, b.field)
 
 
 # ---------------------------------------------------------------------------
@@ -534,6 +554,36 @@ def test_e_store_budget_does_not_change_results(T, K, dtype):
     for f, F, g in out[1:]:
         assert torch.equal(f, out[0][0]) and torch.equal(F, out[0][1])
         assert torch.equal(g, out[0][2])
+
+
+@pytest.mark.parametrize("pol", ["TM", "TE"])
+@pytest.mark.parametrize("T,K", [(50, 8), (33, 6)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_e_store_budget_does_not_change_results_2d(pol, T, K, dtype):
+    """The same for the 2D solver: E (Ez, or Hz for TE) of the final
+    segments kept or replayed gives bitwise the same field, monitors and
+    gradient."""
+    shape = (40, 52)
+    gen = torch.Generator().manual_seed(6)
+    eps0 = 1.5 + 2.0 * torch.rand(shape, generator=gen, dtype=torch.float64)
+    src = {"type": "gaussian_pulse", "row": 8, "amplitude": 0.8}
+    mons = [{"box": (30, 31, 4, 48), "wavelengths": [1500.0, 1600.0]}]
+    ebytes = shape[0] * shape[1] * (4 if dtype == torch.float32 else 8)
+    out = []
+    for budget in (0.0, 2 * K * ebytes / 1e9, None):
+        s = _solver2d(grid_shape=shape, dl=20.0, pml_layers=5, n_steps=T, polarization=pol,
+                      checkpoint_every=K, dtype=dtype, e_store_gb=budget)
+        eps = eps0.clone().requires_grad_(True)
+        res = s.forward(eps, source=dict(src), monitors=[dict(m) for m in mons])
+        F = res.metadata["monitors"][0]
+        ((F.abs() ** 2).sum() + (res.field ** 2).sum()).backward()
+        out.append((res.field.detach().clone(), F.detach().clone(), eps.grad.clone()))
+    assert out[0][2].abs().max() > 0
+    for f, F, g in out[1:]:
+        assert torch.equal(f, out[0][0]) and torch.equal(F, out[0][1])
+        assert torch.equal(g, out[0][2])
+    with pytest.raises(ValueError):
+        _solver2d(grid_shape=shape, n_steps=T, e_store_gb=-1.0)
 
 
 def test_e_store_budget_keeps_the_final_segments():
