@@ -231,10 +231,10 @@ static Side2<T> make_side2(const dnx_source* srcs, int n_src, const dnx_monitor*
 
 // ---- TM, several steps per launch (temporal blocking) --------------------
 //
-// One 1024-thread work-group per tile of TBI x TBI cells.  It loads the tile
-// with a halo of TBS cells on each side (a region of TBH x TBW cells; Ez and
-// Hx in LDS, Hy, the coefficients and the source profile in registers, TBR
-// rows per thread) and advances the whole region by ks <= TBS steps.  A
+// One 1024-thread work-group per tile of TBW - 2 HS cells square.  It loads
+// the tile with a halo of HS cells on each side (a region of TBH x TBW cells;
+// Ez and Hx in LDS, Hy, the coefficients and the source profile in registers,
+// TBR rows per thread) and advances the whole region by ks <= HS steps.  A
 // region cell whose neighbour lies outside the region (but inside the grid)
 // takes its own value in the neighbour's place; the error that starts there
 // moves one cell per step (the H update reads the +1 neighbours, the E update
@@ -257,8 +257,7 @@ constexpr int TBW = 64;               // region columns, one thread column each
 constexpr int TBRB = 1024 / TBW;      // row blocks (16)
 constexpr int TBR = 4;                // rows per thread
 constexpr int TBH = TBRB * TBR;       // region rows (64)
-constexpr int TBS = 8;                // halo = most steps per launch
-constexpr int TBI = TBW - 2 * TBS;    // tile edge (48)
+constexpr int TBS = 16;               // largest halo (= most steps per launch)
 constexpr int TB_NF = 8;              // most frequencies per monitor on this path
 constexpr int TB_SLOTS = 1024;        // LDS slots for accumulators / gradients
 static_assert(TBH == TBW, "square regions");
@@ -268,15 +267,18 @@ struct TbTile {
   int or0, or1, oc0, oc1;  // the tile: rows [or0, or1), columns [oc0, oc1)
 };
 
+// The tile of this work-group for a halo of HS cells (tile edge TBW - 2 HS).
+template <int HS>
 __device__ __forceinline__ TbTile tb_tile(int ny, int nx) {
+  constexpr int TI = TBW - 2 * HS;
   TbTile t;
-  const int a = blockIdx.y * TBI, b = blockIdx.x * TBI;
-  t.ri0 = max(0, min(a - TBS, ny - TBH));
-  t.ci0 = max(0, min(b - TBS, nx - TBW));
+  const int a = blockIdx.y * TI, b = blockIdx.x * TI;
+  t.ri0 = max(0, min(a - HS, ny - TBH));
+  t.ci0 = max(0, min(b - HS, nx - TBW));
   t.or0 = a;
-  t.or1 = min(a + TBI, ny);
+  t.or1 = min(a + TI, ny);
   t.oc0 = b;
-  t.oc1 = min(b + TBI, nx);
+  t.oc1 = min(b + TI, nx);
   return t;
 }
 
@@ -284,12 +286,14 @@ __device__ __forceinline__ TbTile tb_tile(int ny, int nx) {
 // slot(k, r, c) = base + (k * (r1 - r0) + r - r0) * (c1 - c0) + c - c0.
 struct TbClip {
   int r0, r1, c0, c1, base;
+  int nf, y0, x0, bnx;  // the monitor's frequencies and box origin / width
 };
 
 template <typename T>
 __device__ __forceinline__ int tb_clip(const Side2<T>& f, int rr0, int rr1, int cc0, int cc1,
                                        TbClip* cl) {
   int n = 0;
+  for (int m = f.n_mon; m < kMaxMon2; ++m) cl[m] = TbClip{0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int m = 0; m < f.n_mon; ++m) {
     const Box& b = f.mb[m];
     TbClip k;
@@ -298,6 +302,10 @@ __device__ __forceinline__ int tb_clip(const Side2<T>& f, int rr0, int rr1, int 
     k.c0 = max(b.x0, cc0);
     k.c1 = max(k.c0, min(b.x0 + b.nx, cc1));
     k.base = n;
+    k.nf = f.nf[m];
+    k.y0 = b.y0;
+    k.x0 = b.x0;
+    k.bnx = b.nx;
     cl[m] = k;
     n += f.nf[m] * (k.r1 - k.r0) * (k.c1 - k.c0);
   }
@@ -321,20 +329,37 @@ __device__ __forceinline__ int64_t tb_slot_global(const Side2<T>& f, const TbCli
   return (int64_t(kf) * b.cells() + q) * 2;
 }
 
-template <typename T>
+// This thread's cells inside the monitors' clipped boxes (among the rows
+// set in `rows`): bit j * kMaxMon2 + m for row j and monitor m.
+__device__ __forceinline__ unsigned tb_mon_cells(const TbClip* clip, int n_mon, unsigned rows,
+                                                 int r0, int c) {
+  unsigned bits = 0;
+  for (int m = 0; m < n_mon; ++m) {
+    const TbClip k = clip[m];
+#pragma unroll
+    for (int j = 0; j < TBR; ++j) {
+      const int r = r0 + j;
+      const bool in = ((rows >> j) & 1u) && r >= k.r0 && r < k.r1 && c >= k.c0 && c < k.c1;
+      bits |= unsigned(in) << (j * kMaxMon2 + m);
+    }
+  }
+  return bits;
+}
+
+template <typename T, int HS>
 __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict__ sin,
                                                     T* __restrict__ sout, T* __restrict__ es,
                                                     Side2<T> f, int n0, int ks) {
   __shared__ T A[TBH * TBW];     // Ez
   __shared__ T B[TBH * TBW];     // Hx
   __shared__ T Ch[TBRB * TBW];   // Hy of each row block's last row
-  __shared__ double PH[kMaxMon2 * TBS * TB_NF * 2];
+  __shared__ double PH[kMaxMon2 * HS * TB_NF * 2];
   __shared__ double LA[TB_SLOTS * 2];
-  __shared__ T WV[TBS];
+  __shared__ T WV[HS];
   const int ny = g.ny, nx = g.nx;
   const int64_t N = g.n;
   const int tid = threadIdx.x;
-  const TbTile t = tb_tile(ny, nx);
+  const TbTile t = tb_tile<HS>(ny, nx);
   const int cl = tid % TBW, rb = tid / TBW;
   const int rl0 = rb * TBR;
   const int c = t.ci0 + cl;
@@ -349,7 +374,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
     const int nf = f.nf[m];
     for (int e = tid; e < ks * nf * 2; e += 1024) {
       const int sidx = e / (nf * 2), o = e - sidx * nf * 2;
-      PH[(m * TBS + sidx) * TB_NF * 2 + o] = f.ph[m][int64_t(n0 + sidx) * nf * 2 + o];
+      PH[(m * HS + sidx) * TB_NF * 2 + o] = f.ph[m][int64_t(n0 + sidx) * nf * 2 + o];
     }
   }
   __shared__ TbClip clip[kMaxMon2];
@@ -389,6 +414,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
       spr[j] = f.sprof[q];
     }
   }
+  const unsigned monb = tb_mon_cells(clip, f.n_mon, own, t.ri0 + rl0, c);
   const T* const Anext = A + min(rl0 + TBR, TBH - 1) * TBW + cl;
   // every load of the prologue lands here: the step loop then waits on no
   // memory counter (a wait there would also wait for the E-store writes)
@@ -426,27 +452,30 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
       if ((own >> j) & 1u) {
         const int64_t i = int64_t(r) * nx + c;
         if (es) es[int64_t(s) * N + i] = ez;
-        for (int m = 0; m < f.n_mon; ++m) {
-          const TbClip& k = clip[m];
-          if (r < k.r0 || r >= k.r1 || c < k.c0 || c >= k.c1) continue;
-          const double* ph = PH + (m * TBS + s) * TB_NF * 2;
+        if ((monb >> (j * kMaxMon2)) & ((1u << kMaxMon2) - 1u)) {
           const double v = double(ez);
-          const int w_ = k.c1 - k.c0, h_ = k.r1 - k.r0;
-          const Box& b = f.mb[m];
-          const int64_t cells = b.cells();
-          const int64_t q = int64_t(r - b.y0) * b.nx + (c - b.x0);
-          // two loops, so that the LDS one compiles to LDS (not flat) accesses
-          if (lds_acc) {
-            for (int kf = 0; kf < f.nf[m]; ++kf) {
-              double* a = LA + 2 * (k.base + (kf * h_ + r - k.r0) * w_ + c - k.c0);
-              a[0] += v * ph[2 * kf];
-              a[1] -= v * ph[2 * kf + 1];
-            }
-          } else {
-            for (int kf = 0; kf < f.nf[m]; ++kf) {
-              double* a = f.acc[m] + (int64_t(kf) * cells + q) * 2;
-              a[0] += v * ph[2 * kf];
-              a[1] -= v * ph[2 * kf + 1];
+#pragma unroll 1
+          for (int m = 0; m < f.n_mon; ++m) {
+            if (!((monb >> (j * kMaxMon2 + m)) & 1u)) continue;
+            const TbClip k = clip[m];
+            const double* ph = PH + (m * HS + s) * TB_NF * 2;
+            // two loops, so that the LDS one compiles to LDS (not flat) accesses
+            if (lds_acc) {
+              const int w_ = k.c1 - k.c0, hw = (k.r1 - k.r0) * w_;
+              const int sl = k.base + (r - k.r0) * w_ + (c - k.c0);
+              for (int kf = 0; kf < k.nf; ++kf) {
+                double* a = LA + 2 * (sl + kf * hw);
+                a[0] += v * ph[2 * kf];
+                a[1] -= v * ph[2 * kf + 1];
+              }
+            } else {
+              const int64_t cells = f.mb[m].cells();
+              const int64_t q = int64_t(r - k.y0) * k.bnx + (c - k.x0);
+              for (int kf = 0; kf < k.nf; ++kf) {
+                double* a = f.acc[m] + (int64_t(kf) * cells + q) * 2;
+                a[0] += v * ph[2 * kf];
+                a[1] -= v * ph[2 * kf + 1];
+              }
             }
           }
         }
@@ -475,7 +504,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
 
 // Adjoint steps m_top, m_top - 1, ..., m_top - ks + 1 (all >= 1).  E^m of
 // step m comes from e_lo (m == m_lo) or es slot m - m_lo - 1.
-template <typename T>
+template <typename T, int HS>
 __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict__ lin,
                                                     T* __restrict__ lout, T* __restrict__ S,
                                                     const T* __restrict__ e_lo,
@@ -484,13 +513,13 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
   __shared__ T U[TBH * TBW];     // ce * lambda_E
   __shared__ T X[TBH * TBW];     // mu_x
   __shared__ T Cm[TBRB * TBW];   // mu_y of each row block's last row
-  __shared__ double PH[kMaxMon2 * TBS * TB_NF * 2];
+  __shared__ double PH[kMaxMon2 * HS * TB_NF * 2];
   __shared__ double GR[TB_SLOTS * 2];
-  __shared__ T WV[TBS];
+  __shared__ T WV[HS];
   const int ny = g.ny, nx = g.nx;
   const int64_t N = g.n;
   const int tid = threadIdx.x;
-  const TbTile t = tb_tile(ny, nx);
+  const TbTile t = tb_tile<HS>(ny, nx);
   const int cl = tid % TBW, rb = tid / TBW;
   const int rl0 = rb * TBR;
   const int c = t.ci0 + cl;
@@ -505,7 +534,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
     const int nf = f.nf[m];
     for (int e = tid; e < ks * nf * 2; e += 1024) {
       const int sidx = e / (nf * 2), o = e - sidx * nf * 2;
-      PH[(m * TBS + sidx) * TB_NF * 2 + o] = f.ph[m][int64_t(m_top - sidx - 1) * nf * 2 + o];
+      PH[(m * HS + sidx) * TB_NF * 2 + o] = f.ph[m][int64_t(m_top - sidx - 1) * nf * 2 + o];
     }
   }
   __shared__ TbClip clip[kMaxMon2];
@@ -553,6 +582,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
       }
     }
   }
+  const unsigned monb = tb_mon_cells(clip, f.n_mon, valid, t.ri0 + rl0, c);
   const T* const Unext = U + min(rl0 + TBR, TBH - 1) * TBW + cl;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -591,23 +621,24 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
       const bool o = (own >> j) & 1u;
       const T emv = emr[j];
       if (o) Sr[j] += emv * tt;
-      if (!((valid >> j) & 1u)) continue;
+      if (!((monb >> (j * kMaxMon2)) & ((1u << kMaxMon2) - 1u))) continue;
+#pragma unroll 1
       for (int mm = 0; mm < f.n_mon; ++mm) {
-        const TbClip& k = clip[mm];
-        if (r < k.r0 || r >= k.r1 || c < k.c0 || c >= k.c1) continue;
-        const double* ph = PH + (mm * TBS + s) * TB_NF * 2;
-        const int w_ = k.c1 - k.c0, h_ = k.r1 - k.r0;
-        const Box& b = f.mb[mm];
-        const int64_t cells = b.cells();
-        const int64_t q = int64_t(r - b.y0) * b.nx + (c - b.x0);
+        if (!((monb >> (j * kMaxMon2 + mm)) & 1u)) continue;
+        const TbClip k = clip[mm];
+        const double* ph = PH + (mm * HS + s) * TB_NF * 2;
         double d = 0.0;
         if (lds_grd) {
-          for (int kf = 0; kf < f.nf[mm]; ++kf) {
-            const double* gg = GR + 2 * (k.base + (kf * h_ + r - k.r0) * w_ + c - k.c0);
+          const int w_ = k.c1 - k.c0, hw = (k.r1 - k.r0) * w_;
+          const int sl = k.base + (r - k.r0) * w_ + (c - k.c0);
+          for (int kf = 0; kf < k.nf; ++kf) {
+            const double* gg = GR + 2 * (sl + kf * hw);
             d += gg[0] * ph[2 * kf] - gg[1] * ph[2 * kf + 1];
           }
         } else {
-          for (int kf = 0; kf < f.nf[mm]; ++kf) {
+          const int64_t cells = f.mb[mm].cells();
+          const int64_t q = int64_t(r - k.y0) * k.bnx + (c - k.x0);
+          for (int kf = 0; kf < k.nf; ++kf) {
             const double* gg = f.grd[mm] + (int64_t(kf) * cells + q) * 2;
             d += gg[0] * ph[2 * kf] - gg[1] * ph[2 * kf + 1];
           }
@@ -644,10 +675,13 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
   }
 }
 
-// Steps per launch of the temporally blocked TM kernels: 0 when the step
-// kernels run instead (TE, more than one source or kMaxMon2 monitors, a
-// monitor of more than TB_NF frequencies, a probe, no second buffer, or the
-// option set to 1).
+// Steps per launch of the temporally blocked TM kernels (the halo is 8 for
+// up to 8 steps, 16 above), or 0 when the step kernels run instead (TE, more
+// than one source or kMaxMon2 monitors, a monitor of more than TB_NF
+// frequencies, a probe, no second buffer, or the option set to 1).  By
+// default 16 steps on grids of up to 512 tiles of 32 x 32 (where the launches
+// are latency-bound, so that halving their number pays for the wider halos),
+// 8 steps on larger grids.
 static int tb_steps(const dnx_grid2d* gd, int n_src, const dnx_monitor* mons, int n_mon,
                     int64_t probe, const void* a, const void* b) {
   const int opt = option(DNX_OPT_FDTD2D_TB_STEPS);
@@ -655,10 +689,13 @@ static int tb_steps(const dnx_grid2d* gd, int n_src, const dnx_monitor* mons, in
     return 0;
   for (int j = 0; j < n_mon; ++j)
     if (mons[j].n_freq > TB_NF) return 0;
-  return opt <= 0 ? TBS : std::min(opt, TBS);
+  if (opt > 1) return std::min(opt, TBS);
+  return cdiv(gd->nx, TBW - 32) * cdiv(gd->ny, TBW - 32) <= 512 ? 16 : 8;
 }
 
-static dim3 tb_grid(const dnx_grid2d* g) { return dim3(cdiv(g->nx, TBI), cdiv(g->ny, TBI)); }
+static dim3 tb_grid(const dnx_grid2d* g, int hs) {
+  return dim3(cdiv(g->nx, TBW - 2 * hs), cdiv(g->ny, TBW - 2 * hs));
+}
 
 // ---- TE forward ----------------------------------------------------------
 
@@ -803,8 +840,13 @@ static int run2(const dnx_grid2d* gd, T* st, T* work, int n0, int n_steps,
       }
       int ks = std::min(tb, n_steps - k);
       if (ckpt) ks = std::min(ks, ckpt_every - k % ckpt_every);
-      hipLaunchKernelGGL(k2tm_tb_fwd<T>, tb_grid(gd), dim3(1024), 0, s, g, cur, nxt,
-                         es ? es + int64_t(k) * N : nullptr, side, n0 + k, ks);
+      T* es_k = es ? es + int64_t(k) * N : nullptr;
+      if (tb > 8)
+        hipLaunchKernelGGL((k2tm_tb_fwd<T, 16>), tb_grid(gd, 16), dim3(1024), 0, s, g, cur, nxt,
+                           es_k, side, n0 + k, ks);
+      else
+        hipLaunchKernelGGL((k2tm_tb_fwd<T, 8>), tb_grid(gd, 8), dim3(1024), 0, s, g, cur, nxt,
+                           es_k, side, n0 + k, ks);
       DNX_CHECK_HIP(hipGetLastError());
       std::swap(cur, nxt);
       k += ks;
@@ -887,8 +929,12 @@ static int adj_run2(const dnx_grid2d* gd, T* lam, T* work, int m_hi, int m_lo, c
     const int m_stop = std::max(m_lo, 1);
     for (int m = m_hi - 1; m >= m_stop;) {
       const int ks = std::min(tb, m - m_stop + 1);
-      hipLaunchKernelGGL(k2tm_tb_adj<T>, tb_grid(gd), dim3(1024), 0, s, g, cur, nxt, S, e_lo,
-                         es, m_lo, side, m, ks);
+      if (tb > 8)
+        hipLaunchKernelGGL((k2tm_tb_adj<T, 16>), tb_grid(gd, 16), dim3(1024), 0, s, g, cur, nxt,
+                           S, e_lo, es, m_lo, side, m, ks);
+      else
+        hipLaunchKernelGGL((k2tm_tb_adj<T, 8>), tb_grid(gd, 8), dim3(1024), 0, s, g, cur, nxt,
+                           S, e_lo, es, m_lo, side, m, ks);
       DNX_CHECK_HIP(hipGetLastError());
       std::swap(cur, nxt);
       m -= ks;
