@@ -391,7 +391,9 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
       LA[2 * sl + 1] = f.acc[m][o + 1];
     }
   }
-  T hy[TBR], cer[TBR], byr[TBR], ayr[TBR], spr[TBR];
+  // this thread's cells in registers; A and B carry the values its
+  // neighbours read
+  T ezr[TBR], hxr[TBR], hy[TBR], cer[TBR], byr[TBR], ayr[TBR], spr[TBR];
   unsigned own = 0, srcm = 0;
   T* const Ar = A + rl0 * TBW + cl;
   T* const Br = B + rl0 * TBW + cl;
@@ -400,8 +402,9 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
     const int r = t.ri0 + rl0 + j;
     const bool v = cv && r < ny;
     const int64_t i = int64_t(r) * nx + c;
-    Ar[j * TBW] = v ? sin[i] : T(0);
-    Br[j * TBW] = v ? sin[N + i] : T(0);
+    ezr[j] = v ? sin[i] : T(0);
+    hxr[j] = v ? sin[N + i] : T(0);
+    Ar[j * TBW] = ezr[j];
     hy[j] = v ? sin[2 * N + i] : T(0);
     cer[j] = v ? g.ce[i] : T(0);
     byr[j] = r < ny ? g.by[r] : T(0);
@@ -425,12 +428,13 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
 #pragma unroll
     for (int j = 0; j < TBR; ++j) {
       const int r = t.ri0 + rl0 + j;
-      const T e = Ar[j * TBW];
-      const T er = Ar[j * TBW + dxr];
-      const T eu = j + 1 < TBR ? Ar[(j + 1) * TBW] : *Anext;
+      const T e = ezr[j];
+      const T er = dxr ? Ar[j * TBW + 1] : e;
+      const T eu = j + 1 < TBR ? ezr[j + 1] : *Anext;
       const T dc = (c + 1 < nx) ? er - e : T(0);
       const T dr = (r + 1 < ny) ? eu - e : T(0);
-      Br[j * TBW] = byr[j] * Br[j * TBW] - kh * (ayr[j] * dc);
+      hxr[j] = byr[j] * hxr[j] - kh * (ayr[j] * dc);
+      Br[j * TBW] = hxr[j];
       hy[j] = bxv * hy[j] + kh * (axv * dr);
     }
     Ch[rb * TBW + cl] = hy[TBR - 1];
@@ -442,12 +446,13 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
     for (int j = 0; j < TBR; ++j) {
       const int r = t.ri0 + rl0 + j;
       const T hb = j > 0 ? hy[j - 1] : hbelow;
-      const T hxo = Br[j * TBW];
-      const T hl = Br[j * TBW + dxl];
+      const T hxo = hxr[j];
+      const T hl = dxl ? Br[j * TBW - 1] : hxo;
       const T dHy = r > 0 ? hy[j] - hb : T(0);
       const T dHx = c > 0 ? hxo - hl : T(0);
-      T ez = (bxv * byr[j]) * Ar[j * TBW] + cer[j] * (axv * dHy - ayr[j] * dHx);
+      T ez = (bxv * byr[j]) * ezr[j] + cer[j] * (axv * dHy - ayr[j] * dHx);
       if ((srcm >> j) & 1u) ez += w * spr[j];
+      ezr[j] = ez;
       Ar[j * TBW] = ez;
       if ((own >> j) & 1u) {
         const int64_t i = int64_t(r) * nx + c;
@@ -488,8 +493,8 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
     if (!((own >> j) & 1u)) continue;
     const int r = t.ri0 + rl0 + j;
     const int64_t i = int64_t(r) * nx + c;
-    sout[i] = Ar[j * TBW];
-    sout[N + i] = Br[j * TBW];
+    sout[i] = ezr[j];
+    sout[N + i] = hxr[j];
     sout[2 * N + i] = hy[j];
   }
   if (lds_acc) {
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
       GR[2 * sl + 1] = f.grd[m][o + 1];
     }
   }
-  T L[TBR], my[TBR], Sr[TBR], cer[TBR], byr[TBR], ayr[TBR], spr[TBR], emr[TBR];
+  T L[TBR], mx[TBR], my[TBR], Sr[TBR], cer[TBR], byr[TBR], ayr[TBR], spr[TBR], emr[TBR];
   unsigned own = 0, srcm = 0, valid = 0;
   T* const Ur = U + rl0 * TBW + cl;
   T* const Xr = X + rl0 * TBW + cl;
@@ -564,7 +569,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
       const bool v = cv && r < ny;
       const int64_t i = int64_t(r) * nx + c;
       L[j] = v ? lin[i] : T(0);
-      Xr[j * TBW] = v ? lin[N + i] : T(0);
+      mx[j] = v ? lin[N + i] : T(0);
       my[j] = v ? lin[2 * N + i] : T(0);
       cer[j] = v ? g.ce[i] : T(0);
       byr[j] = r < ny ? g.by[r] : T(0);
@@ -590,21 +595,24 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
     const int m = m_top - s;
     // the source's adjoint term (tile cells), then ce * lambda for the H update
     const T w = WV[s];
+    T ur[TBR];
 #pragma unroll
     for (int j = 0; j < TBR; ++j) {
       if (((srcm & own) >> j) & 1u) Sr[j] -= w * spr[j] * L[j];
-      Ur[j * TBW] = cer[j] * L[j];
+      ur[j] = cer[j] * L[j];
+      Ur[j * TBW] = ur[j];
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < TBR; ++j) {
       const int r = t.ri0 + rl0 + j;
-      const T u = Ur[j * TBW];
-      const T uc = Ur[j * TBW + dxr];
-      const T uu = j + 1 < TBR ? Ur[(j + 1) * TBW] : *Unext;
+      const T u = ur[j];
+      const T uc = dxr ? Ur[j * TBW + 1] : u;
+      const T uu = j + 1 < TBR ? ur[j + 1] : *Unext;
       const T tc = (c >= 1 ? u : T(0)) - (c + 1 < nx ? uc : T(0));
       const T tr = (r >= 1 ? u : T(0)) - (r + 1 < ny ? uu : T(0));
-      Xr[j * TBW] = byr[j] * Xr[j * TBW] - ayr[j] * tc;
+      mx[j] = byr[j] * mx[j] - ayr[j] * tc;
+      Xr[j * TBW] = mx[j];
       my[j] = bxv * my[j] + axv * tr;
     }
     Cm[rb * TBW + cl] = my[TBR - 1];
@@ -615,7 +623,8 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
       const int r = t.ri0 + rl0 + j;
       const T mb = j > 0 ? my[j - 1] : mbelow;
       const T r_my = (r >= 1 ? mb : T(0)) - (r <= ny - 2 ? my[j] : T(0));
-      const T c_mx = (c >= 1 ? Xr[j * TBW + dxl] : T(0)) - (c <= nx - 2 ? Xr[j * TBW] : T(0));
+      const T xl = dxl ? Xr[j * TBW - 1] : mx[j];
+      const T c_mx = (c >= 1 ? xl : T(0)) - (c <= nx - 2 ? mx[j] : T(0));
       const T tt = kh * (axv * r_my - ayr[j] * c_mx);
       L[j] = (bxv * byr[j]) * L[j] + tt;
       const bool o = (own >> j) & 1u;
@@ -669,7 +678,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
     const int r = t.ri0 + rl0 + j;
     const int64_t i = int64_t(r) * nx + c;
     lout[i] = L[j];
-    lout[N + i] = Xr[j * TBW];
+    lout[N + i] = mx[j];
     lout[2 * N + i] = my[j];
     S[i] = Sr[j];
   }
