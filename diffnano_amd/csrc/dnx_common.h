@@ -67,7 +67,7 @@ inline Box make_box(const dnx_box& b) {
 struct TimedLaunch {
   hipEvent_t start = nullptr;
 };
-TimedLaunch timed_begin(hipStream_t s);
+TimedLaunch timed_begin(const char* name, hipStream_t s);
 void timed_end(const char* name, hipStream_t s, TimedLaunch& t);
 
 // Validate sources / monitors against a grid (2D callers pass nz = 1).

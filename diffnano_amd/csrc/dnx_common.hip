@@ -65,13 +65,20 @@ int option(int id) {
 // Per-launch timing registry
 // ---------------------------------------------------------------------------
 
-static std::atomic<bool> g_timing_on{false};
+// g_timing_stride: 0 off, k >= 1 one launch in every k of each name timed
+static std::atomic<int> g_timing_stride{0};
 static std::mutex g_timing_mu;
 static std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> g_timing;
+static std::map<std::string, int64_t> g_timing_seen;
 
-TimedLaunch timed_begin(hipStream_t s) {
+TimedLaunch timed_begin(const char* name, hipStream_t s) {
   TimedLaunch t;
-  if (!g_timing_on.load(std::memory_order_relaxed)) return t;
+  const int stride = g_timing_stride.load(std::memory_order_relaxed);
+  if (stride <= 0) return t;
+  if (stride > 1) {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    if (g_timing_seen[name]++ % stride != 0) return t;
+  }
   if (hipEventCreate(&t.start) != hipSuccess) {
     t.start = nullptr;
     return t;
@@ -432,7 +439,12 @@ extern "C" int32_t dnx_stream_probe(int32_t kind, const void* src, void* dst, in
 }
 
 extern "C" int32_t dnx_timing_enable(int32_t on) {
-  dnx::g_timing_on = on != 0;
+  DNX_REQUIRE(on >= 0, "timing_enable: negative stride %d", on);
+  {
+    std::lock_guard<std::mutex> lk(dnx::g_timing_mu);
+    dnx::g_timing_seen.clear();
+  }
+  dnx::g_timing_stride = on;
   return DNX_OK;
 }
 

@@ -461,9 +461,10 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
         f2.a.es = slot;
         f2.es2 = slot ? slot + 3 * N : nullptr;
         f2.n = n;
-        TimedLaunch tl = timed_begin(s);
+        const char* tname = slot ? "fdtd3d_fwd2_store" : "fdtd3d_fwd2";
+        TimedLaunch tl = timed_begin(tname, s);
         int rc = launch_fused2_fwd<T>(f2, ty2, s);
-        timed_end(slot ? "fdtd3d_fwd2_store" : "fdtd3d_fwd2", s, tl);
+        timed_end(tname, s, tl);
         if (rc) return rc;
         for (int j = monx ? 1 : 0; j < n_mon; ++j)
           if ((rc = launch_monitor_pair_fwd<T>(f2, mons[j], n, s))) return rc;
@@ -478,9 +479,10 @@ static int run3(const dnx_grid3d* gd, T* st, T* work, int n0, int n_steps,
       fa.out = out;
       fa.es = slot;
       if (fa.src_comp >= 0) fa.src_wave = static_cast<const T*>(srcs[0].wave) + n;
-      TimedLaunch tl = timed_begin(s);
+      const char* tname = slot ? "fdtd3d_fwd_store" : "fdtd3d_fwd";
+      TimedLaunch tl = timed_begin(tname, s);
       int rc = launch_fused3_fwd<T>(fa, ty, s);
-      timed_end(slot ? "fdtd3d_fwd_store" : "fdtd3d_fwd", s, tl);
+      timed_end(tname, s, tl);
       if (rc) return rc;
     } else {
       hipLaunchKernelGGL(k3_update_h<T>, grid, block, 0, s, g, cur);
@@ -580,7 +582,7 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
         int rc;
         for (int j = 0; j < n_mon; ++j)
           if ((rc = launch_monitor_d<T>(f2.side[j], mons[j], m - 1, s))) return rc;
-        TimedLaunch tl = timed_begin(s);
+        TimedLaunch tl = timed_begin("fdtd3d_adj2", s);
         rc = launch_fused2_adj<T>(f2, ty2, s);
         timed_end("fdtd3d_adj2", s, tl);
         if (rc) return rc;
@@ -605,7 +607,7 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
       fa.out = nxt;
       fa.em = em;
       if (fa.src_comp >= 0) fa.src_wave = static_cast<const T*>(srcs[0].wave) + m;
-      TimedLaunch tl = timed_begin(s);
+      TimedLaunch tl = timed_begin("fdtd3d_adj", s);
       int rc = launch_fused3_adj<T>(fa, ty, s);
       timed_end("fdtd3d_adj", s, tl);
       if (rc) return rc;

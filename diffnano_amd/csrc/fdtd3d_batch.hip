@@ -257,7 +257,7 @@ static int seq_forward(dnx_batch3d_plan* p, int steps, int store, hipStream_t s,
       Batch3Launch L = p->Lf2;
       L.store = store;
       for (; k + 1 < steps; k += 2) {
-        TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+        TimedLaunch tl = timed ? timed_begin(names2[store], s) : TimedLaunch{};
         rc = launch_batch3_pair<T>(L, false, p->ty_f2, p->max_w, p->grid_f2, s);
         timed_end(names2[store], s, tl);
         if (rc) return rc;
@@ -271,7 +271,7 @@ static int seq_forward(dnx_batch3d_plan* p, int steps, int store, hipStream_t s,
   Batch3Launch L = p->Lf;
   L.store = store;
   for (; k < steps; ++k) {
-    TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+    TimedLaunch tl = timed ? timed_begin(names1[store], s) : TimedLaunch{};
     rc = launch_batch3<T>(L, false, p->ty_f, p->max_w, p->grid_f, s);
     timed_end(names1[store], s, tl);
     if (rc) return rc;
@@ -296,7 +296,7 @@ static int seq_adjoint(dnx_batch3d_plan* p, int steps, hipStream_t s, bool timed
         if ((rc = launch_batch_monitor_pair<T>(p->d_adj2, p->n, p->max_mon_cells, p->d_ctr, true,
                                                s)))
           return rc;
-        TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+        TimedLaunch tl = timed ? timed_begin("batch3d_adj2", s) : TimedLaunch{};
         rc = launch_batch3_pair<T>(p->La2, true, p->ty_a2, p->max_w, p->grid_a2, s);
         timed_end("batch3d_adj2", s, tl);
         if (rc) return rc;
@@ -308,7 +308,7 @@ static int seq_adjoint(dnx_batch3d_plan* p, int steps, hipStream_t s, bool timed
     }
   }
   for (; k < steps; ++k) {
-    TimedLaunch tl = timed ? timed_begin(s) : TimedLaunch{};
+    TimedLaunch tl = timed ? timed_begin("batch3d_adj", s) : TimedLaunch{};
     rc = launch_batch3<T>(p->La, true, p->ty_a, p->max_w, p->grid_a, s);
     timed_end("batch3d_adj", s, tl);
     if (rc) return rc;

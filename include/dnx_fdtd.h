@@ -145,8 +145,11 @@ int32_t dnx_get_option(int32_t id, int32_t* value);
 /* Per-launch timing of the hot kernels with HIP events on the launch
  * stream (no reference counterpart; measurement only).  Names:
  * "fdtd3d_fwd", "fdtd3d_fwd_store" (replay, also stores E), "fdtd3d_adj".
- * dnx_timing_read waits for the recorded events, returns the summed
- * duration and the launch count, and clears that name. */
+ * dnx_timing_enable(k): 0 off, k >= 1 times one launch in every k of each
+ * name (counted from this call), so that the events' own cost stays out of
+ * the wall time of long runs.  dnx_timing_read waits for the recorded
+ * events, returns the summed duration and the count of timed launches, and
+ * clears that name. */
 int32_t dnx_timing_enable(int32_t on);
 /* Streaming-bandwidth probe on `stream` (no reference counterpart: the
  * measured ceiling the kernels are compared with).  kind 0: copy n floats
