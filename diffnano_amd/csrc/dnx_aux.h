@@ -41,6 +41,10 @@ int launch_monitor_pair(const T* side, const dnx_monitor& m, int n, hipStream_t 
 // launch_monitor_adj adds to lam), for the two-step adjoint kernel.
 template <typename T>
 int launch_monitor_d(T* d, const dnx_monitor& m, int sample_row, hipStream_t st);
+// launch_monitor_adj at adj_row and launch_monitor_d at d_row, one launch.
+template <typename T>
+int launch_monitor_adj_d(Comps<T> lam, Comps<T> E, T* S, int ny, int nx, const dnx_monitor& m,
+                         int adj_row, T* d, int d_row, hipStream_t st);
 
 // Fill n elements with zero (a kernel, so it is safe inside captured graphs).
 template <typename T>

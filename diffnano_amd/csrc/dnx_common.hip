@@ -212,13 +212,15 @@ __global__ void k_monitor(Comps<T> E, int ny, int nx, Box b, int comp_mask, int 
   }
 }
 
+// One cell (box-local index i < cells) of the monitor adjoint at one sample:
+// lam += dv, S += E . dv with dv = T(sum_f Re(g e^{i phase})).
 template <typename T>
-__global__ void k_monitor_adj(Comps<T> lam, Comps<T> E, T* __restrict__ S, int use_s, int ny,
-                              int nx, Box b, int comp_mask, int nf,
-                              const double* __restrict__ ph, const double* __restrict__ g) {
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void monitor_adj_cell(Comps<T> lam, Comps<T> E, T* __restrict__ S,
+                                                 int use_s, int ny, int nx, const Box& b,
+                                                 int comp_mask, int nf,
+                                                 const double* __restrict__ ph,
+                                                 const double* __restrict__ g, int64_t i) {
   int64_t cells = b.cells();
-  if (i >= cells) return;
   int64_t idx = box_index(b, i, ny, nx);
   int ncomp = ((comp_mask >> 0) & 1) + ((comp_mask >> 1) & 1) + ((comp_mask >> 2) & 1);
   int ci = 0;
@@ -236,6 +238,15 @@ __global__ void k_monitor_adj(Comps<T> lam, Comps<T> E, T* __restrict__ S, int u
     ++ci;
   }
   if (use_s) S[idx] += sacc;
+}
+
+template <typename T>
+__global__ void k_monitor_adj(Comps<T> lam, Comps<T> E, T* __restrict__ S, int use_s, int ny,
+                              int nx, Box b, int comp_mask, int nf,
+                              const double* __restrict__ ph, const double* __restrict__ g) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.cells()) return;
+  monitor_adj_cell<T>(lam, E, S, use_s, ny, nx, b, comp_mask, nf, ph, g, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -323,10 +334,9 @@ __global__ void k_monitor_pair(const T* __restrict__ side, int64_t cells, int co
 // d[ci][i] = T(sum_f Re(g e^{i phi})) at one sample (the adjoint source
 // k_monitor_adj adds), for the two-step adjoint kernel to apply.
 template <typename T>
-__global__ void k_monitor_d(T* __restrict__ d, int64_t cells, int comp_mask, int nf,
-                            const double* __restrict__ ph, const double* __restrict__ g) {
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= cells) return;
+__device__ __forceinline__ void monitor_d_cell(T* __restrict__ d, int64_t cells, int comp_mask,
+                                               int nf, const double* __restrict__ ph,
+                                               const double* __restrict__ g, int64_t i) {
   int ncomp = ((comp_mask >> 0) & 1) + ((comp_mask >> 1) & 1) + ((comp_mask >> 2) & 1);
   for (int ci = 0; ci < ncomp; ++ci) {
     double s = 0.0;
@@ -336,6 +346,28 @@ __global__ void k_monitor_d(T* __restrict__ d, int64_t cells, int comp_mask, int
     }
     d[int64_t(ci) * cells + i] = T(s);
   }
+}
+
+template <typename T>
+__global__ void k_monitor_d(T* __restrict__ d, int64_t cells, int comp_mask, int nf,
+                            const double* __restrict__ ph, const double* __restrict__ g) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  monitor_d_cell<T>(d, cells, comp_mask, nf, ph, g, i);
+}
+
+// k_monitor_adj at one sample and k_monitor_d at another in one launch (the
+// two touch different buffers): the monitor adjoint after one two-step
+// adjoint launch and the adjoint source the next one reads.
+template <typename T>
+__global__ void k_monitor_adj_d(Comps<T> lam, Comps<T> E, T* __restrict__ S, int use_s, int ny,
+                                int nx, Box b, int comp_mask, int nf,
+                                const double* __restrict__ ph_adj, const double* __restrict__ g,
+                                T* __restrict__ d, const double* __restrict__ ph_d) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.cells()) return;
+  monitor_adj_cell<T>(lam, E, S, use_s, ny, nx, b, comp_mask, nf, ph_adj, g, i);
+  monitor_d_cell<T>(d, b.cells(), comp_mask, nf, ph_d, g, i);
 }
 
 template <typename T>
@@ -358,7 +390,22 @@ int launch_monitor_d(T* d, const dnx_monitor& m, int sample_row, hipStream_t st)
   return DNX_OK;
 }
 
+template <typename T>
+int launch_monitor_adj_d(Comps<T> lam, Comps<T> E, T* S, int ny, int nx, const dnx_monitor& m,
+                         int adj_row, T* d, int d_row, hipStream_t st) {
+  Box b = make_box(m.box);
+  int use_s = (E.c[0] != nullptr || E.c[1] != nullptr || E.c[2] != nullptr) && S != nullptr;
+  hipLaunchKernelGGL(k_monitor_adj_d<T>, dim3(cdiv(b.cells(), 256)), dim3(256), 0, st, lam, E, S,
+                     use_s, ny, nx, b, m.comp_mask, m.n_freq,
+                     m.phase + int64_t(adj_row) * m.n_freq * 2, m.grad, d,
+                     m.phase + int64_t(d_row) * m.n_freq * 2);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
 #define DNX_INST(T)                                                                        \
+  template int launch_monitor_adj_d<T>(Comps<T>, Comps<T>, T*, int, int, const dnx_monitor&, \
+                                       int, T*, int, hipStream_t);                       \
   template int launch_monitor_pair<T>(const T*, const dnx_monitor&, int, hipStream_t);   \
   template int launch_monitor_d<T>(T*, const dnx_monitor&, int, hipStream_t);            \
   template int launch_source<T>(T*, int, int, const dnx_source&, int, hipStream_t);      \

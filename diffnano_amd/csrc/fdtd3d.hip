@@ -570,6 +570,20 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
   auto estep = [&](int mm) -> const T* {
     return (mm == m_lo) ? e_lo : es + int64_t(mm - m_lo - 1) * 3 * N;
   };
+  // The monitor adjoint of a pair's last sample waits for the next pair's
+  // adjoint-source launch and shares it (k_monitor_adj_d); it is applied
+  // before anything else reads lambda or S.
+  Comps<T> pend_l{}, pend_e{};
+  int pend_row = -1;
+  auto flush = [&]() -> int {
+    if (pend_row < 0) return DNX_OK;
+    for (int j = 0; j < n_mon; ++j) {
+      int rc = launch_monitor_adj<T>(pend_l, pend_e, S, g.ny, g.nx, mons[j], pend_row, s);
+      if (rc) return rc;
+    }
+    pend_row = -1;
+    return DNX_OK;
+  };
   for (int m = m_hi - 1; m >= m_lo && m >= 1; --m) {
     const T* em = estep(m);
     if (pairs && m - 1 >= m_lo && m - 1 >= 1) {
@@ -580,17 +594,21 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
         f2.em2 = estep(m - 1);
         f2.n = m;
         int rc;
-        for (int j = 0; j < n_mon; ++j)
-          if ((rc = launch_monitor_d<T>(f2.side[j], mons[j], m - 1, s))) return rc;
+        for (int j = 0; j < n_mon; ++j) {
+          rc = pend_row >= 0 ? launch_monitor_adj_d<T>(pend_l, pend_e, S, g.ny, g.nx, mons[j],
+                                                       pend_row, f2.side[j], m - 1, s)
+                             : launch_monitor_d<T>(f2.side[j], mons[j], m - 1, s);
+          if (rc) return rc;
+        }
+        pend_row = -1;
         TimedLaunch tl = timed_begin("fdtd3d_adj2", s);
         rc = launch_fused2_adj<T>(f2, ty2, s);
         timed_end("fdtd3d_adj2", s, tl);
         if (rc) return rc;
         T* e2 = const_cast<T*>(f2.em2);
-        Comps<T> L2{{nxt, nxt + N, nxt + 2 * N}};
-        Comps<T> E2{{e2, e2 + N, e2 + 2 * N}};
-        for (int j = 0; j < n_mon; ++j)
-          if ((rc = launch_monitor_adj<T>(L2, E2, S, g.ny, g.nx, mons[j], m - 2, s))) return rc;
+        pend_l = Comps<T>{{nxt, nxt + N, nxt + 2 * N}};
+        pend_e = Comps<T>{{e2, e2 + N, e2 + 2 * N}};
+        pend_row = n_mon > 0 ? m - 2 : -1;
       }
       T* t = cur;
       cur = nxt;
@@ -598,6 +616,7 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
       --m;  // the pair covered steps m and m-1
       continue;
     }
+    if (int rc = flush()) return rc;
     for (int j = j0; j < n_src; ++j) {
       int rc = launch_source_adj<T>(S, cur + srcs[j].comp * N, g.ny, g.nx, srcs[j], m, s);
       if (rc) return rc;
@@ -628,6 +647,7 @@ static int adj_run3(const dnx_grid3d* gd, T* lam, T* work, int m_hi, int m_lo, c
       nxt = t;
     }
   }
+  if (int rc = flush()) return rc;
   if (m_lo == 0) {
     for (int j = 0; j < n_src; ++j) {
       int rc = launch_source_adj<T>(S, cur + srcs[j].comp * N, g.ny, g.nx, srcs[j], 0, s);
