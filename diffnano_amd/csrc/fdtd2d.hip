@@ -1092,8 +1092,11 @@ constexpr int RES_NX = 128;              // LDS row pitch and largest nx
 constexpr int RES_R = 16;                // rows per thread
 constexpr int RES_RB = 1024 / RES_NX;    // row blocks (8): largest ny = 128
 constexpr int RES_NY = RES_RB * RES_R;
+constexpr int RES_MON_SLOTS = 2048;      // DFT accumulators (complex) staged in LDS
+constexpr int RES_SRC_CELLS = 512;       // source profile values staged in LDS
 constexpr size_t RES_SMEM =
-    sizeof(float) * (size_t(2) * RES_NY * RES_NX + RES_RB * RES_NX + 2 * RES_NY);
+    sizeof(float) * (size_t(2) * RES_NY * RES_NX + RES_RB * RES_NX + 2 * RES_NY +
+                     2 * RES_MON_SLOTS + RES_SRC_CELLS);
 
 __global__ __launch_bounds__(1024) void k2tm_batch_res(Tm2Batch a, int n_steps) {
   extern __shared__ float rsm[];
@@ -1101,6 +1104,8 @@ __global__ __launch_bounds__(1024) void k2tm_batch_res(Tm2Batch a, int n_steps) 
   float* Bh = A + RES_NY * RES_NX;           // Hx [row][RES_NX]
   float* Ch = Bh + RES_NY * RES_NX;          // Hy of each row block's last row
   float* Py = Ch + RES_RB * RES_NX;          // (by, ay) per row
+  float* MLA = Py + 2 * RES_NY;              // the design's DFT accumulators (re, im)
+  float* SPL = MLA + 2 * RES_MON_SLOTS;      // the source profile
   const int ny = a.ny, nx = a.nx;
   const int64_t n = a.n;
   const int b = blockIdx.x;
@@ -1148,6 +1153,23 @@ __global__ __launch_bounds__(1024) void k2tm_batch_res(Tm2Batch a, int n_steps) 
   const int dx = cr - c, dxl = cl - c;      // 1 / -1, or 0 at a clamped edge
   // row below the block (clamped into the array; only used where r + 1 < ny)
   const float* const Anext = A + min(r0 + RES_R, RES_NY - 1) * RES_NX + c;
+  // The DFT accumulators and the source profile live in LDS for the whole
+  // call when they fit (loaded here, accumulators stored after the last
+  // step), so that the step loop reads no global memory for them; per
+  // accumulator the additions are those of the global-memory path, in the
+  // same order.
+  const bool lds_mon = a.nf > 0 && int64_t(a.nf) * mcells <= RES_MON_SLOTS;
+  const bool lds_src = src && sb.cells() <= RES_SRC_CELLS;
+  if (lds_mon) {
+    for (int e = threadIdx.x; e < a.nf * mcells; e += 1024) {
+      const int f = e / int(mcells), q = e - f * int(mcells);
+      MLA[2 * e] = acc[(2 * f) * mcells + q];
+      MLA[2 * e + 1] = acc[(2 * f + 1) * mcells + q];
+    }
+  }
+  if (lds_src) {
+    for (int e = threadIdx.x; e < sb.cells(); e += 1024) SPL[e] = a.sprof[e];
+  }
   __syncthreads();
   for (int k = 0; k < n_steps; ++k) {
     // H from E (k2tm_batch_h)
@@ -1181,21 +1203,45 @@ __global__ __launch_bounds__(1024) void k2tm_batch_res(Tm2Batch a, int n_steps) 
     // the soft source, then the DFT of the new Ez (own cells only)
     if (sj0 < sj1) {
       const float w = a.wave[k];
-      for (int j = sj0; j < sj1; ++j)
-        Ar[j * RES_NX] += w * a.sprof[(r0 + j - sb.y0) * sb.nx + (c - sb.x0)];
+      if (lds_src) {
+        for (int j = sj0; j < sj1; ++j)
+          Ar[j * RES_NX] += w * SPL[(r0 + j - sb.y0) * sb.nx + (c - sb.x0)];
+      } else {
+        for (int j = sj0; j < sj1; ++j)
+          Ar[j * RES_NX] += w * a.sprof[(r0 + j - sb.y0) * sb.nx + (c - sb.x0)];
+      }
     }
     if (mj0 < mj1) {
       const float* ph = a.phase + int64_t(k) * a.nf * 2;
-      for (int j = mj0; j < mj1; ++j) {
-        const float e = Ar[j * RES_NX];
-        const int64_t q = int64_t(r0 + j - mb.y0) * mb.nx + (c - mb.x0);
-        for (int f = 0; f < a.nf; ++f) {
-          acc[(2 * f) * mcells + q] += e * ph[2 * f];
-          acc[(2 * f + 1) * mcells + q] -= e * ph[2 * f + 1];
+      // two loops, so that the LDS one compiles to LDS (not flat) accesses
+      if (lds_mon) {
+        for (int j = mj0; j < mj1; ++j) {
+          const float e = Ar[j * RES_NX];
+          const int q = (r0 + j - mb.y0) * mb.nx + (c - mb.x0);
+          for (int f = 0; f < a.nf; ++f) {
+            MLA[2 * (f * int(mcells) + q)] += e * ph[2 * f];
+            MLA[2 * (f * int(mcells) + q) + 1] -= e * ph[2 * f + 1];
+          }
+        }
+      } else {
+        for (int j = mj0; j < mj1; ++j) {
+          const float e = Ar[j * RES_NX];
+          const int64_t q = int64_t(r0 + j - mb.y0) * mb.nx + (c - mb.x0);
+          for (int f = 0; f < a.nf; ++f) {
+            acc[(2 * f) * mcells + q] += e * ph[2 * f];
+            acc[(2 * f + 1) * mcells + q] -= e * ph[2 * f + 1];
+          }
         }
       }
     }
     __syncthreads();
+  }
+  if (lds_mon) {
+    for (int e = threadIdx.x; e < a.nf * mcells; e += 1024) {
+      const int f = e / int(mcells), q = e - f * int(mcells);
+      acc[(2 * f) * mcells + q] = MLA[2 * e];
+      acc[(2 * f + 1) * mcells + q] = MLA[2 * e + 1];
+    }
   }
 #pragma unroll
   for (int j = 0; j < RES_R; ++j) {
