@@ -154,6 +154,12 @@ __global__ __launch_bounds__(512) void k_mode_wgrad(const float* __restrict__ X,
 // A work-group owns one split and a 32 x 32 (o, i) tile; per 64-column step
 // it stages dY[o][k] and X[i][k] (transposed to [k][i]) in LDS, and each of
 // 256 threads accumulates 4 outputs (one o, four consecutive i).  FP32.
+// The transposed X rows are XOR-swizzled in 4-element groups,
+// Xs[k][i ^ wsw(k)], so that the transposing stores of a wave (16 k values
+// x 4 i values) fall in 64 distinct banks; the float4 reads of four
+// consecutive i stay contiguous.
+__device__ __forceinline__ int wsw(int k) { return ((k >> 4) & 3) << 2; }
+
 __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY,
                                                const float* __restrict__ X,
                                                float* __restrict__ part,
@@ -191,17 +197,17 @@ __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY,
       const int idx = j * 256 + t;
       const int r = idx >> 4, c4 = (idx & 15) * 4;
       *reinterpret_cast<float4*>(&Ds[r][c4]) = dv[j];
-      Xs[c4][r] = xv[j].x;
-      Xs[c4 + 1][r] = xv[j].y;
-      Xs[c4 + 2][r] = xv[j].z;
-      Xs[c4 + 3][r] = xv[j].w;
+      Xs[c4][r ^ wsw(c4)] = xv[j].x;
+      Xs[c4 + 1][r ^ wsw(c4 + 1)] = xv[j].y;
+      Xs[c4 + 2][r ^ wsw(c4 + 2)] = xv[j].z;
+      Xs[c4 + 3][r ^ wsw(c4 + 3)] = xv[j].w;
     }
     __syncthreads();
     fetch(k0 + 64);
 #pragma unroll 8
     for (int k = 0; k < 64; ++k) {
       const float d = Ds[o][k];
-      const float4 x = *reinterpret_cast<const float4*>(&Xs[k][i4]);
+      const float4 x = *reinterpret_cast<const float4*>(&Xs[k][i4 ^ wsw(k)]);
       acc[0] += d * x.x;
       acc[1] += d * x.y;
       acc[2] += d * x.z;
