@@ -162,7 +162,7 @@ def lib():
 OPTIONS = {"simple": 0, "one_step": 1, "adj2": 2, "fwd2_pf": 3, "adj2_occ": 4,
            "gemm_generic": 5, "gemm_pw_valu": 6, "gemm_small": 7, "gemm_big_reg": 8,
            "fdtd2d_per_step": 9, "fdtd2d_separate": 10,
-           "fdtd2d_tb_steps": 11}
+           "fdtd2d_tb_steps": 11, "fdtd2d_tb_region": 12}
 
 
 def get_option(name):

@@ -54,6 +54,7 @@ static void init_options() {
   g_opt[DNX_OPT_FDTD2D_LAUNCH_PER_STEP] = std::min(2, std::max(0, env_int("DNX_FDTD2D_PER_STEP", 0)));
   g_opt[DNX_OPT_FDTD2D_SEPARATE_SIDE] = env_int("DNX_FDTD2D_SEPARATE", 0) != 0 ? 1 : 0;
   g_opt[DNX_OPT_FDTD2D_TB_STEPS] = std::max(0, env_int("DNX_FDTD2D_TB", 0));
+  g_opt[DNX_OPT_FDTD2D_TB_REGION] = env_int("DNX_FDTD2D_TB_REGION", 0);
 }
 
 int option(int id) {

@@ -231,10 +231,11 @@ static Side2<T> make_side2(const dnx_source* srcs, int n_src, const dnx_monitor*
 
 // ---- TM, several steps per launch (temporal blocking) --------------------
 //
-// One 1024-thread work-group per tile of TBW - 2 HS cells square.  It loads
-// the tile with a halo of HS cells on each side (a region of TBH x TBW cells;
-// Ez and Hx in LDS, Hy, the coefficients and the source profile in registers,
-// TBR rows per thread) and advances the whole region by ks <= HS steps.  A
+// One 1024-thread work-group per tile of RW - 2 HS cells square.  It loads
+// the tile with a halo of HS cells on each side (a region of RW x RW cells;
+// each thread's cells in registers, LDS for the values its neighbours read,
+// RW * RW / 1024 rows per thread) and advances the whole region by ks <= HS
+// steps.  A
 // region cell whose neighbour lies outside the region (but inside the grid)
 // takes its own value in the neighbour's place; the error that starts there
 // moves one cell per step (the H update reads the +1 neighbours, the E update
@@ -253,24 +254,24 @@ static Side2<T> make_side2(const dnx_source* srcs, int n_src, const dnx_monitor*
 // when they fit in TB_SLOTS complex values (otherwise those go to global
 // memory at every step, as in the step kernels).
 
-constexpr int TBW = 64;               // region columns, one thread column each
-constexpr int TBRB = 1024 / TBW;      // row blocks (16)
-constexpr int TBR = 4;                // rows per thread
-constexpr int TBH = TBRB * TBR;       // region rows (64)
+// Region shapes: RW x RW cells (RW = 64: four rows per thread; RW = 32:
+// one), RW threads per row block, 1024 / RW row blocks.
 constexpr int TBS = 16;               // largest halo (= most steps per launch)
 constexpr int TB_NF = 8;              // most frequencies per monitor on this path
 constexpr int TB_SLOTS = 1024;        // LDS slots for accumulators / gradients
-static_assert(TBH == TBW, "square regions");
 
 struct TbTile {
   int ri0, ci0;            // region origin (row, column)
   int or0, or1, oc0, oc1;  // the tile: rows [or0, or1), columns [oc0, oc1)
 };
 
-// The tile of this work-group for a halo of HS cells (tile edge TBW - 2 HS).
-template <int HS>
+// The tile of this work-group for a halo of HS cells in RW x RW regions
+// (tile edge RW - 2 HS).
+template <int HS, int RW>
 __device__ __forceinline__ TbTile tb_tile(int ny, int nx) {
+  constexpr int TBW = RW, TBH = RW;
   constexpr int TI = TBW - 2 * HS;
+  static_assert(TI > 0, "the halo must leave a tile");
   TbTile t;
   const int a = blockIdx.y * TI, b = blockIdx.x * TI;
   t.ri0 = max(0, min(a - HS, ny - TBH));
@@ -331,6 +332,7 @@ __device__ __forceinline__ int64_t tb_slot_global(const Side2<T>& f, const TbCli
 
 // This thread's cells inside the monitors' clipped boxes (among the rows
 // set in `rows`): bit j * kMaxMon2 + m for row j and monitor m.
+template <int TBR>
 __device__ __forceinline__ unsigned tb_mon_cells(const TbClip* clip, int n_mon, unsigned rows,
                                                  int r0, int c) {
   unsigned bits = 0;
@@ -346,10 +348,12 @@ __device__ __forceinline__ unsigned tb_mon_cells(const TbClip* clip, int n_mon, 
   return bits;
 }
 
-template <typename T, int HS>
+template <typename T, int HS, int RW>
 __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict__ sin,
                                                     T* __restrict__ sout, T* __restrict__ es,
                                                     Side2<T> f, int n0, int ks) {
+  constexpr int TBW = RW, TBH = RW, TBRB = 1024 / RW, TBR = RW * RW / 1024;
+  static_assert(TBR >= 1 && TBRB * TBR == TBH, "square regions of 32 or 64 cells");
   __shared__ T A[TBH * TBW];     // Ez
   __shared__ T B[TBH * TBW];     // Hx
   __shared__ T Ch[TBRB * TBW];   // Hy of each row block's last row
@@ -359,7 +363,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
   const int ny = g.ny, nx = g.nx;
   const int64_t N = g.n;
   const int tid = threadIdx.x;
-  const TbTile t = tb_tile<HS>(ny, nx);
+  const TbTile t = tb_tile<HS, RW>(ny, nx);
   const int cl = tid % TBW, rb = tid / TBW;
   const int rl0 = rb * TBR;
   const int c = t.ci0 + cl;
@@ -417,7 +421,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
       spr[j] = f.sprof[q];
     }
   }
-  const unsigned monb = tb_mon_cells(clip, f.n_mon, own, t.ri0 + rl0, c);
+  const unsigned monb = tb_mon_cells<TBR>(clip, f.n_mon, own, t.ri0 + rl0, c);
   const T* const Anext = A + min(rl0 + TBR, TBH - 1) * TBW + cl;
   // every load of the prologue lands here: the step loop then waits on no
   // memory counter (a wait there would also wait for the E-store writes)
@@ -509,12 +513,14 @@ __global__ __launch_bounds__(1024) void k2tm_tb_fwd(G2<T> g, const T* __restrict
 
 // Adjoint steps m_top, m_top - 1, ..., m_top - ks + 1 (all >= 1).  E^m of
 // step m comes from e_lo (m == m_lo) or es slot m - m_lo - 1.
-template <typename T, int HS>
+template <typename T, int HS, int RW>
 __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict__ lin,
                                                     T* __restrict__ lout, T* __restrict__ S,
                                                     const T* __restrict__ e_lo,
                                                     const T* __restrict__ es, int m_lo,
                                                     Side2<T> f, int m_top, int ks) {
+  constexpr int TBW = RW, TBH = RW, TBRB = 1024 / RW, TBR = RW * RW / 1024;
+  static_assert(TBR >= 1 && TBRB * TBR == TBH, "square regions of 32 or 64 cells");
   __shared__ T U[TBH * TBW];     // ce * lambda_E
   __shared__ T X[TBH * TBW];     // mu_x
   __shared__ T Cm[TBRB * TBW];   // mu_y of each row block's last row
@@ -524,7 +530,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
   const int ny = g.ny, nx = g.nx;
   const int64_t N = g.n;
   const int tid = threadIdx.x;
-  const TbTile t = tb_tile<HS>(ny, nx);
+  const TbTile t = tb_tile<HS, RW>(ny, nx);
   const int cl = tid % TBW, rb = tid / TBW;
   const int rl0 = rb * TBR;
   const int c = t.ci0 + cl;
@@ -587,7 +593,7 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
       }
     }
   }
-  const unsigned monb = tb_mon_cells(clip, f.n_mon, valid, t.ri0 + rl0, c);
+  const unsigned monb = tb_mon_cells<TBR>(clip, f.n_mon, valid, t.ri0 + rl0, c);
   const T* const Unext = U + min(rl0 + TBR, TBH - 1) * TBW + cl;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -684,26 +690,41 @@ __global__ __launch_bounds__(1024) void k2tm_tb_adj(G2<T> g, const T* __restrict
   }
 }
 
-// Steps per launch of the temporally blocked TM kernels (the halo is 8 for
-// up to 8 steps, 16 above), or 0 when the step kernels run instead (TE, more
-// than one source or kMaxMon2 monitors, a monitor of more than TB_NF
-// frequencies, a probe, no second buffer, or the option set to 1).  By
-// default 16 steps on grids of up to 512 tiles of 32 x 32 (where the launches
-// are latency-bound, so that halving their number pays for the wider halos),
-// 8 steps on larger grids.
-static int tb_steps(const dnx_grid2d* gd, int n_src, const dnx_monitor* mons, int n_mon,
-                    int64_t probe, const void* a, const void* b) {
+// Launch shape of the temporally blocked TM kernels: steps per launch, halo
+// and region width, or steps = 0 when the step kernels run instead (TE,
+// more than one source or kMaxMon2 monitors, a monitor of more than TB_NF
+// frequencies, a probe, no second buffer, or DNX_OPT_FDTD2D_TB_STEPS = 1).
+// The launches are latency-bound while the tiles fit the chip once, so by
+// default: 32 x 32 regions (16 x 16 tiles, 8 steps) when at most 256 tiles
+// cover the grid, else 64 x 64 regions with 16-cell halos (16 steps) on up
+// to 512 tiles of 32 x 32, else 64 x 64 regions with 8-cell halos (8 steps,
+// 48 x 48 tiles: 1.8x recompute instead of 4x).  DNX_OPT_FDTD2D_TB_STEPS = k
+// in 2..16 forces 64 x 64 regions and k steps (halo 8 up to 8 steps, 16
+// above); DNX_OPT_FDTD2D_TB_REGION = 32 or 64 forces the region width.
+struct TbShape {
+  int steps, hs, rw;
+};
+
+static TbShape tb_shape(const dnx_grid2d* gd, int n_src, const dnx_monitor* mons, int n_mon,
+                        int64_t probe, const void* a, const void* b) {
   const int opt = option(DNX_OPT_FDTD2D_TB_STEPS);
+  const int region = option(DNX_OPT_FDTD2D_TB_REGION);
   if (opt == 1 || probe >= 0 || b == nullptr || a == b || !side2_ok(gd, n_src, mons, n_mon))
-    return 0;
+    return TbShape{0, 0, 0};
   for (int j = 0; j < n_mon; ++j)
-    if (mons[j].n_freq > TB_NF) return 0;
-  if (opt > 1) return std::min(opt, TBS);
-  return cdiv(gd->nx, TBW - 32) * cdiv(gd->ny, TBW - 32) <= 512 ? 16 : 8;
+    if (mons[j].n_freq > TB_NF) return TbShape{0, 0, 0};
+  if (opt > 1) {
+    const int k = std::min(opt, TBS);
+    return TbShape{k, k > 8 ? 16 : 8, 64};
+  }
+  const int64_t t16 = int64_t(cdiv(gd->nx, 16)) * cdiv(gd->ny, 16);
+  const int64_t t32 = int64_t(cdiv(gd->nx, 32)) * cdiv(gd->ny, 32);
+  if (region == 32 || (region != 64 && t16 <= 256)) return TbShape{8, 8, 32};
+  return t32 <= 512 ? TbShape{16, 16, 64} : TbShape{8, 8, 64};
 }
 
-static dim3 tb_grid(const dnx_grid2d* g, int hs) {
-  return dim3(cdiv(g->nx, TBW - 2 * hs), cdiv(g->ny, TBW - 2 * hs));
+static dim3 tb_grid(const dnx_grid2d* g, int hs, int rw) {
+  return dim3(cdiv(g->nx, rw - 2 * hs), cdiv(g->ny, rw - 2 * hs));
 }
 
 // ---- TE forward ----------------------------------------------------------
@@ -837,7 +858,8 @@ static int run2(const dnx_grid2d* gd, T* st, T* work, int n0, int n_steps,
   Comps<T> F{{st, nullptr, nullptr}};
   const bool fuse = side2_ok(gd, n_src, mons, n_mon);
   const Side2<T> side = fuse ? make_side2<T>(srcs, n_src, mons, n_mon, false) : Side2<T>{};
-  if (const int tb = tb_steps(gd, n_src, mons, n_mon, probe, st, work)) {
+  const TbShape tbs = tb_shape(gd, n_src, mons, n_mon, probe, st, work);
+  if (const int tb = tbs.steps) {
     // temporally blocked: launches of up to tb steps, ping-ponging between
     // st and work, none crossing a checkpoint
     T* cur = st;
@@ -850,11 +872,15 @@ static int run2(const dnx_grid2d* gd, T* st, T* work, int n0, int n_steps,
       int ks = std::min(tb, n_steps - k);
       if (ckpt) ks = std::min(ks, ckpt_every - k % ckpt_every);
       T* es_k = es ? es + int64_t(k) * N : nullptr;
-      if (tb > 8)
-        hipLaunchKernelGGL((k2tm_tb_fwd<T, 16>), tb_grid(gd, 16), dim3(1024), 0, s, g, cur, nxt,
+      const dim3 grid_tb = tb_grid(gd, tbs.hs, tbs.rw);
+      if (tbs.rw == 32)
+        hipLaunchKernelGGL((k2tm_tb_fwd<T, 8, 32>), grid_tb, dim3(1024), 0, s, g, cur, nxt,
+                           es_k, side, n0 + k, ks);
+      else if (tbs.hs == 16)
+        hipLaunchKernelGGL((k2tm_tb_fwd<T, 16, 64>), grid_tb, dim3(1024), 0, s, g, cur, nxt,
                            es_k, side, n0 + k, ks);
       else
-        hipLaunchKernelGGL((k2tm_tb_fwd<T, 8>), tb_grid(gd, 8), dim3(1024), 0, s, g, cur, nxt,
+        hipLaunchKernelGGL((k2tm_tb_fwd<T, 8, 64>), grid_tb, dim3(1024), 0, s, g, cur, nxt,
                            es_k, side, n0 + k, ks);
       DNX_CHECK_HIP(hipGetLastError());
       std::swap(cur, nxt);
@@ -932,18 +958,23 @@ static int adj_run2(const dnx_grid2d* gd, T* lam, T* work, int m_hi, int m_lo, c
   const int64_t N = g.n;
   const bool fuse = side2_ok(gd, n_src, mons, n_mon);
   const Side2<T> side = fuse ? make_side2<T>(srcs, n_src, mons, n_mon, true) : Side2<T>{};
-  if (const int tb = tb_steps(gd, n_src, mons, n_mon, -1, lam, work)) {
+  const TbShape tbs = tb_shape(gd, n_src, mons, n_mon, -1, lam, work);
+  if (const int tb = tbs.steps) {
     T* cur = lam;
     T* nxt = work;
     const int m_stop = std::max(m_lo, 1);
     for (int m = m_hi - 1; m >= m_stop;) {
       const int ks = std::min(tb, m - m_stop + 1);
-      if (tb > 8)
-        hipLaunchKernelGGL((k2tm_tb_adj<T, 16>), tb_grid(gd, 16), dim3(1024), 0, s, g, cur, nxt,
-                           S, e_lo, es, m_lo, side, m, ks);
+      const dim3 grid_tb = tb_grid(gd, tbs.hs, tbs.rw);
+      if (tbs.rw == 32)
+        hipLaunchKernelGGL((k2tm_tb_adj<T, 8, 32>), grid_tb, dim3(1024), 0, s, g, cur, nxt, S,
+                           e_lo, es, m_lo, side, m, ks);
+      else if (tbs.hs == 16)
+        hipLaunchKernelGGL((k2tm_tb_adj<T, 16, 64>), grid_tb, dim3(1024), 0, s, g, cur, nxt, S,
+                           e_lo, es, m_lo, side, m, ks);
       else
-        hipLaunchKernelGGL((k2tm_tb_adj<T, 8>), tb_grid(gd, 8), dim3(1024), 0, s, g, cur, nxt,
-                           S, e_lo, es, m_lo, side, m, ks);
+        hipLaunchKernelGGL((k2tm_tb_adj<T, 8, 64>), grid_tb, dim3(1024), 0, s, g, cur, nxt, S,
+                           e_lo, es, m_lo, side, m, ks);
       DNX_CHECK_HIP(hipGetLastError());
       std::swap(cur, nxt);
       m -= ks;

@@ -137,7 +137,9 @@ enum {
                                   instead of inside the step kernels [DNX_FDTD2D_SEPARATE] */
   DNX_OPT_FDTD2D_TB_STEPS = 11,  /* 2D TM steps per launch of the temporally blocked kernels
                                   (0: 8, the most; 1: the one-step kernels) [DNX_FDTD2D_TB] */
-  DNX_OPT_COUNT = 12
+  DNX_OPT_FDTD2D_TB_REGION = 12, /* region width of the 2D blocked kernels: 0 picks, 32 or
+                                  64 forces it [DNX_FDTD2D_TB_REGION] */
+  DNX_OPT_COUNT = 13
 };
 int32_t dnx_set_option(int32_t id, int32_t value);
 int32_t dnx_get_option(int32_t id, int32_t* value);

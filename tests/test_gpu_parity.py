@@ -273,8 +273,8 @@ def test_c0_full_case_vs_reference():
 def test_fdtd2d_tm_fused_and_blocked_kernels_bitwise(shape, src, mons, K, dtype, lib_opts):
     """The TM step kernels that apply the source and accumulate the DFT
     monitors themselves, and the temporally blocked kernels (up to 8 steps
-    per launch on tiles of 48 x 48 cells with 8-cell halos, up to 16 on
-    32 x 32 tiles with 16-cell halos), give the bits of
+    per launch on tiles of 16 x 16 or 48 x 48 cells with 8-cell halos, up to
+    16 on 32 x 32 tiles with 16-cell halos), give the bits of
     the separate source, monitor and adjoint kernels: final field, monitors
     (at the grid's edge, several wavelengths, a 16-row box across tiles) and
     dL/d eps, with checkpoint intervals that cut the launches short."""
@@ -294,11 +294,14 @@ def test_fdtd2d_tm_fused_and_blocked_kernels_bitwise(shape, src, mons, K, dtype,
     lib_opts.set("fdtd2d_separate", 1)
     ref = run()
     lib_opts.set("fdtd2d_separate", 0)
-    for tb in (1, 0, 3, 8, 13):   # one step per launch, the default (16 here), 3, 8, 13
+    # one step per launch; the default (32 x 32 regions, 8 steps, here); 64 x 64
+    # regions with 16 steps; 3, 8 and 13 steps on 64 x 64 regions
+    for tb, region in ((1, 0), (0, 0), (0, 64), (3, 0), (8, 0), (13, 0)):
         lib_opts.set("fdtd2d_tb_steps", tb)
+        lib_opts.set("fdtd2d_tb_region", region)
         got = run()
         for a, b in zip(got, ref):
-            assert torch.equal(a, b), tb
+            assert torch.equal(a, b), (tb, region)
     assert float(ref[-1].abs().max()) > 0
 
 
