@@ -269,6 +269,11 @@ def test_c0_full_case_vs_reference():
     ((131, 100), {"type": "gaussian_pulse", "row": 60},
      [{"box": (95, 96, 0, 100), "wavelengths": [1500.0, 1600.0]},
       {"box": (40, 56, 44, 52), "wavelengths": [1550.0]}], 21),
+    # a full-grid monitor: more accumulators per tile than the LDS slots, so
+    # the blocked kernels update them in global memory
+    ((70, 90), {"type": "gaussian_pulse", "pos": [35, 20]},
+     [{"box": (0, 70, 0, 90), "wavelengths": [1300.0, 1550.0, 1700.0]},
+      {"box": (50, 51, 10, 80), "wavelengths": [1550.0]}], 16),
 ])
 def test_fdtd2d_tm_fused_and_blocked_kernels_bitwise(shape, src, mons, K, dtype, lib_opts):
     """The TM step kernels that apply the source and accumulate the DFT
