@@ -1191,6 +1191,103 @@ __global__ __launch_bounds__(256) void k_pw(GemmArgs g) {
   }
 }
 
+// The FNO's projection head in one pass, for inference:
+//   out[o][n] = b2[o] + sum_j W2[o][j] bf(gelu(b1[j] + sum_i W1[j][i] bf(h[i][n])))
+// with I = 32 inputs, J = 128 hidden channels and O <= 32 outputs.  The
+// first product is k_pw's (same operand layout and MFMA order, so the hidden
+// values are bitwise those k_pw<128, 32> writes); the hidden values never
+// leave the registers: rounded to BF16 (as k_cmix rounds its input), they
+// are the B operand of the second product, whose k index runs over the
+// hidden rows in the order the first product's accumulators hold them (W2's
+// columns are permuted to match).  Against k_pw + k_cmix, this saves
+// writing and reading J FP32 values per cell; the second product's sum runs
+// in the MFMA's order instead of k_cmix's sequential one (FP32 rounding).
+template <int O>
+__global__ __launch_bounds__(256) void k_proj(const float* __restrict__ h,
+                                              const float* __restrict__ W1,
+                                              const float* __restrict__ b1,
+                                              const float* __restrict__ W2,
+                                              const float* __restrict__ b2,
+                                              float* __restrict__ out, int N) {
+  constexpr int I = 32, J = 128, JT = J / 32;
+  static_assert(O >= 1 && O <= 32, "at most 32 outputs");
+  __shared__ float bs1[J];
+  __shared__ float bs2[32];
+  // MFMA A fragments in LDS, one 16-byte entry per (k-step, lane): W1 as k_pw
+  // holds it (row 32 jt + lane % 32, k = 16 ks + 8 (lane / 32) + 0..7); W2
+  // for k-step (jt, s): entry i of lane half hf is hidden row
+  // 32 jt + 8 (v / 4) + 4 hf + v % 4 with v = 8 s + i (the row the first
+  // product's accumulator element v holds), output row lane % 32 (zero past O)
+  __shared__ u16x8 f1[JT * 2][64];
+  __shared__ u16x8 f2[JT * 2][64];
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 31, half = lane >> 5, kq = half * 8;
+  if (threadIdx.x < J) bs1[threadIdx.x] = b1[threadIdx.x];
+  if (threadIdx.x < 32) bs2[threadIdx.x] = int(threadIdx.x) < O ? b2[threadIdx.x] : 0.f;
+  for (int e = threadIdx.x; e < JT * 2 * 64; e += 256) {
+    const int step = e / 64, l = e % 64;
+    const int jt = step / 2, ks = step % 2, r = l & 31, hf = l >> 5;
+    u16x8 u1, u2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      u1[i] = f2bf(W1[(jt * 32 + r) * I + ks * 16 + hf * 8 + i]);
+      const int v = 8 * ks + i;
+      const int jrow = jt * 32 + 8 * (v >> 2) + 4 * hf + (v & 3);
+      u2[i] = r < O ? f2bf(W2[r * J + jrow]) : (unsigned short)0;
+    }
+    f1[step][l] = u1;
+    f2[step][l] = u2;
+  }
+  __syncthreads();
+  const int groups = (N + 31) / 32;
+  const int stride = int(gridDim.x) * 4;
+  for (int gi = int(blockIdx.x) * 4 + int(threadIdx.x >> 6); gi < groups; gi += stride) {
+    const int n = gi * 32 + lr;
+    const bool nv = n < N;
+    const int nc = nv ? n : N - 1;
+    bf16x8 xb[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      u16x8 u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) u[j] = f2bf(h[int64_t(ks * 16 + kq + j) * N + nc]);
+      xb[ks] = __builtin_bit_cast(bf16x8, u);
+    }
+    f32x16 acc2;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc2[v] = 0.f;
+#pragma unroll 1
+    for (int jt = 0; jt < JT; ++jt) {
+      f32x16 acc;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            __builtin_bit_cast(bf16x8, f1[jt * 2 + ks][lane]), xb[ks], acc, 0, 0, 0);
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        u16x8 u;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int v = 8 * s2 + i;
+          const int jrow = jt * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+          u[i] = f2bf(gelu(1.0f * acc[v] + bs1[jrow]));
+        }
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            __builtin_bit_cast(bf16x8, f2[jt * 2 + s2][lane]), __builtin_bit_cast(bf16x8, u),
+            acc2, 0, 0, 0);
+      }
+    }
+    if (!nv) continue;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int o = 8 * (v >> 2) + 4 * half + (v & 3);
+      if (o < O) out[int64_t(o) * N + n] = 1.0f * acc2[v] + bs2[o];
+    }
+  }
+}
+
 template <int M, int K>
 static bool pw_mk(const GemmArgs& g, hipStream_t s) {
   if (g.M != M || g.K != K) return false;
@@ -1298,6 +1395,32 @@ extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, flo
   const int64_t threads = cdiv(n, 4);
   hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(threads, 256)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), part, s, n, out, beta);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_fno_proj(const float* h, const float* W1, const float* b1,
+                                const float* W2, const float* b2, float* out, int32_t I,
+                                int32_t J, int32_t O, int64_t n, void* stream) {
+  DNX_REQUIRE(h && W1 && b1 && W2 && b2 && out, "fno_proj: null operand");
+  DNX_REQUIRE(I == 32 && J == 128 && O >= 1 && O <= 32 && n >= 1 && n <= 0x7fffffff,
+              "fno_proj: needs 32 inputs, 128 hidden channels, 1-32 outputs (I=%d J=%d O=%d "
+              "n=%lld)", I, J, O, (long long)n);
+  const int64_t groups = (n + 31) / 32;
+  const int blocks = int(std::min<int64_t>((groups + 3) / 4, 4096));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (O) {
+#define DNX_PROJ_CASE(o)                                                                      \
+  case o:                                                                                  \
+    hipLaunchKernelGGL(k_proj<o>, dim3(blocks), dim3(256), 0, s, h, W1, b1, W2, b2, out,   \
+                       int(n));                                                            \
+    break;
+    DNX_PROJ_CASE(1) DNX_PROJ_CASE(2) DNX_PROJ_CASE(3) DNX_PROJ_CASE(4) DNX_PROJ_CASE(5)
+    DNX_PROJ_CASE(6) DNX_PROJ_CASE(8) DNX_PROJ_CASE(16) DNX_PROJ_CASE(32)
+#undef DNX_PROJ_CASE
+    default:
+      DNX_REQUIRE(false, "fno_proj: %d outputs (built for 1-6, 8, 16, 32)", O);
+  }
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }

@@ -214,6 +214,14 @@ class FNO2d:
     # by the kernels that produce it (bitwise the same products)
     bf16_copies = True
 
+    # inference runs the projection head as one kernel (dnx_fno_proj) where
+    # the shapes allow
+    fused_head = True
+
+    def _fused_head_ok(self):
+        return (self.fused_head and self.width == 32 and self.hidden == 128
+                and self.out_ch in (1, 2, 3, 4, 5, 6, 8, 16, 32))
+
     def _fast_mix_ok(self, B):
         return self.fast_mix and self.width == 32 and B % 8 == 0 and self.nmodes % 16 == 0
 
@@ -272,10 +280,18 @@ class FNO2d:
                 saved["Z"].append(Z)
                 saved["pre"].append(pre)
             h = hn
+        out = torch.empty(self.out_ch, n, device=dev)
+        if not save and self._fused_head_ok():
+            # inference: the projection 32 -> 128 -> GELU -> out_ch in one
+            # pass, the hidden activations never written
+            L.check(L.lib().dnx_fno_proj(
+                h.data_ptr(), p["w_p1"].data_ptr(), p["b_p1"].data_ptr(),
+                p["w_p2"].data_ptr(), p["b_p2"].data_ptr(), out.data_ptr(), c, self.hidden,
+                self.out_ch, n, L.stream_handle(dev)))
+            return out.view(self.out_ch, B, P), None
         q = torch.empty(self.hidden, n, device=dev)
         qpre = torch.empty(self.hidden, n, device=dev) if save else None
         self._pointwise(p["w_p1"], h, q, bias=p["b_p1"], act=1, pre=qpre)
-        out = torch.empty(self.out_ch, n, device=dev)
         self._pointwise(p["w_p2"], q, out, bias=p["b_p2"])
         saved.update({"hL": h, "q": q, "qpre": qpre, "B": B})
         return out.view(self.out_ch, B, P), (saved if save else None)
@@ -419,7 +435,10 @@ class FNO2d:
         B = x.shape[0]
         xt = x.to(self.dev, torch.float32).permute(1, 0, 2, 3).reshape(
             self.in_ch, B, self.H * self.W).contiguous()
-        out = _FNOFn.apply(self, xt, *[self.params[k] for k in self.names])
+        if torch.is_grad_enabled():
+            out = _FNOFn.apply(self, xt, *[self.params[k] for k in self.names])
+        else:  # inference: nothing saved for a backward
+            out, _ = self.forward_raw(xt, save=False)
         return out.view(self.out_ch, B, self.H, self.W).permute(1, 0, 2, 3)
 
 

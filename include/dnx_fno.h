@@ -58,6 +58,16 @@ int32_t dnx_gemm_bf16(const dnx_gemm_args* args, void* stream);
 int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out, float beta,
                          void* stream);
 
+/* The projection head of the FNO in one pass (inference; no reference
+ * counterpart beyond the surrogate's output layers, surrogate.py:23-75):
+ *   out[o][n] = b2[o] + sum_j W2[o][j] bf(gelu(b1[j] + sum_i W1[j][i] bf(h[i][n])))
+ * h [I][n], W1 [J][I], W2 [O][J], out [O][n], FP32 row-major; bf() rounds to
+ * BF16 (nearest even), products on the matrix cores with FP32 accumulation.
+ * I = 32, J = 128, O in 1-6, 8, 16 or 32. */
+int32_t dnx_fno_proj(const float* h, const float* W1, const float* b1, const float* W2,
+                     const float* b2, float* out, int32_t I, int32_t J, int32_t O, int64_t n,
+                     void* stream);
+
 /* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32; dx_bf16, if not NULL,
  * also receives dx rounded to BF16 (nearest even). */
 int32_t dnx_gelu_bwd(const float* dy, const float* pre, float* dx, unsigned short* dx_bf16,

@@ -347,6 +347,38 @@ def test_fno_bf16_copies_change_nothing():
         assert torch.equal(res[0][1][k], res[1][1][k]), k
 
 
+@pytest.mark.parametrize("out_ch,B,H", [(6, 8, 128), (3, 2, 40), (32, 1, 24)])
+def test_fused_projection_head_matches_two_kernels(out_ch, B, H):
+    """Inference with the projection head in one kernel (dnx_fno_proj) against
+    the two channel-mixing kernels: the hidden layer is bitwise the same
+    product, so the outputs differ only by the order of the last FP32 sum
+    (and cells past a multiple of 32 are handled)."""
+    from diffnano_amd.fno import FNO2d
+    m = FNO2d(H, H, in_ch=3, out_ch=out_ch, width=32, modes=(4, 4), layers=2, hidden=128,
+              device="cuda", seed=6)
+    torch.manual_seed(7)
+    x = torch.randn(B, 3, H, H, device="cuda")
+    with torch.no_grad():
+        m.fused_head = True
+        a = m(x).clone()
+        m.fused_head = False
+        b = m(x).clone()
+    m.fused_head = True
+    err = rel_l2(a.double().cpu().numpy(), b.double().cpu().numpy())
+    print("fused head rel L2", err)
+    assert err < 1e-5
+    assert float(b.abs().max()) > 0
+
+
+def test_fused_projection_head_rejects_bad_shapes():
+    import diffnano_amd._lib as L
+    t = torch.zeros(1024, device="cuda")
+    p = t.data_ptr()
+    assert L.lib().dnx_fno_proj(p, p, p, p, p, p, 32, 64, 6, 32, None) != 0
+    assert b"fno_proj" in L.lib().dnx_last_error()
+    assert L.lib().dnx_fno_proj(p, p, p, p, p, p, 32, 128, 7, 32, None) != 0
+
+
 @pytest.mark.parametrize("conj", [False, True])
 def test_mode_mix_kernels_match_fp64(conj):
     """Per-mode complex channel mixing and its weight gradient (FP32 kernels)
