@@ -1184,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_pw(GemmArgs g) {
         else if (g.bias_mode == 1) r += g.bias[n];
         if (g.pre) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rp, vo, so, 0);
         if (g.act == 1) r = gelu(r);
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rc, vo, so, 0);
+        if (g.C) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rc, vo, so, 0);
         if (g.c_bf16) __builtin_amdgcn_raw_buffer_store_b16(f2bf(r), r16, vo / 2, so / 2, 0);
       }
     }
@@ -1202,8 +1202,8 @@ __global__ __launch_bounds__(256) void k_pw(GemmArgs g) {
 // columns are permuted to match).  Against k_pw + k_cmix, this saves
 // writing and reading J FP32 values per cell; the second product's sum runs
 // in the MFMA's order instead of k_cmix's sequential one (FP32 rounding).
-template <int O>
-__global__ __launch_bounds__(256) void k_proj(const float* __restrict__ h,
+template <int O, bool XBF>
+__global__ __launch_bounds__(256) void k_proj(const void* __restrict__ hin,
                                               const float* __restrict__ W1,
                                               const float* __restrict__ b1,
                                               const float* __restrict__ W2,
@@ -1250,7 +1250,11 @@ __global__ __launch_bounds__(256) void k_proj(const float* __restrict__ h,
     for (int ks = 0; ks < 2; ++ks) {
       u16x8 u;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) u[j] = f2bf(h[int64_t(ks * 16 + kq + j) * N + nc]);
+      for (int j = 0; j < 8; ++j) {
+        const int64_t e = int64_t(ks * 16 + kq + j) * N + nc;
+        if constexpr (XBF) u[j] = static_cast<const unsigned short*>(hin)[e];
+        else u[j] = f2bf(static_cast<const float*>(hin)[e]);
+      }
       xb[ks] = __builtin_bit_cast(bf16x8, u);
     }
     f32x16 acc2;
@@ -1324,7 +1328,7 @@ static bool launch_pw(const GemmArgs& g, hipStream_t s) {
 using namespace dnx;
 
 extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
-  DNX_REQUIRE(g && g->A && g->B && g->C, "gemm: null operand");
+  DNX_REQUIRE(g && g->A && g->B && (g->C || g->c_bf16), "gemm: null operand");
   DNX_REQUIRE(g->M > 0 && g->N > 0 && g->K > 0 && g->batch > 0, "gemm: bad sizes");
   DNX_REQUIRE(g->beta == 0.f || g->Cin, "gemm: beta needs Cin");
   DNX_REQUIRE(g->bias_mode == 0 || g->bias, "gemm: bias missing");
@@ -1337,6 +1341,10 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
   bool done = false;
   if (!generic && option(DNX_OPT_GEMM_PW_VALU) == 0 && launch_pw(*g, s)) {
     done = true;
+  } else if (!g->C) {
+    DNX_REQUIRE(false, "gemm: C may be NULL (BF16 copy only) on the pointwise MFMA kernels "
+                       "only (M, K of 32 x 9, 32 x 32, 128 x 32 or 128 x 6, one batch, "
+                       "contiguous cells)");
   } else if (!generic && launch_cmix(*g, s)) {
     done = true;
   } else if (g->c_bf16) {
@@ -1399,9 +1407,9 @@ extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, flo
   return DNX_OK;
 }
 
-extern "C" int32_t dnx_fno_proj(const float* h, const float* W1, const float* b1,
-                                const float* W2, const float* b2, float* out, int32_t I,
-                                int32_t J, int32_t O, int64_t n, void* stream) {
+extern "C" int32_t dnx_fno_proj(const void* h, int32_t h_bf16, const float* W1,
+                                const float* b1, const float* W2, const float* b2, float* out,
+                                int32_t I, int32_t J, int32_t O, int64_t n, void* stream) {
   DNX_REQUIRE(h && W1 && b1 && W2 && b2 && out, "fno_proj: null operand");
   DNX_REQUIRE(I == 32 && J == 128 && O >= 1 && O <= 32 && n >= 1 && n <= 0x7fffffff,
               "fno_proj: needs 32 inputs, 128 hidden channels, 1-32 outputs (I=%d J=%d O=%d "
@@ -1412,8 +1420,12 @@ extern "C" int32_t dnx_fno_proj(const float* h, const float* W1, const float* b1
   switch (O) {
 #define DNX_PROJ_CASE(o)                                                                      \
   case o:                                                                                  \
-    hipLaunchKernelGGL(k_proj<o>, dim3(blocks), dim3(256), 0, s, h, W1, b1, W2, b2, out,   \
-                       int(n));                                                            \
+    if (h_bf16)                                                                            \
+      hipLaunchKernelGGL((k_proj<o, true>), dim3(blocks), dim3(256), 0, s, h, W1, b1, W2,  \
+                         b2, out, int(n));                                                 \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_proj<o, false>), dim3(blocks), dim3(256), 0, s, h, W1, b1, W2, \
+                         b2, out, int(n));                                                 \
     break;
     DNX_PROJ_CASE(1) DNX_PROJ_CASE(2) DNX_PROJ_CASE(3) DNX_PROJ_CASE(4) DNX_PROJ_CASE(5)
     DNX_PROJ_CASE(6) DNX_PROJ_CASE(8) DNX_PROJ_CASE(16) DNX_PROJ_CASE(32)

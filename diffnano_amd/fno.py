@@ -52,7 +52,8 @@ def gemm(M, N, K, A, sA, B, sB, Cout, sC, *, batch=1, a_off=0, b_off=0, c_off=0,
     g.M, g.N, g.K, g.batch = int(M), int(N), int(K), int(batch)
     g.A, (g.sAm, g.sAk, g.sAb) = _off(A, a_off), sA
     g.B, (g.sBk, g.sBn, g.sBb) = _off(B, b_off), sB
-    g.C, (g.sCm, g.sCn, g.sCb) = _off(Cout, c_off), sC
+    # Cout None: only the BF16 copy c16 is written (pointwise shapes)
+    g.C, (g.sCm, g.sCn, g.sCb) = (_off(Cout, c_off) if Cout is not None else None), sC
     if Cin is Cout and i_off == 0:
         i_off = c_off  # accumulating in place: read C where it is written
     if Cin is not None:
@@ -71,7 +72,8 @@ def gemm(M, N, K, A, sA, B, sB, Cout, sC, *, batch=1, a_off=0, b_off=0, c_off=0,
     for t in (A, B):
         if t.dtype not in (torch.float32, torch.bfloat16):
             raise TypeError("gemm operands must be float32 or bfloat16")
-    L.check(L.lib().dnx_gemm_bf16(C.byref(g), L.stream_handle(Cout.device)))
+    dev = (Cout if Cout is not None else c16).device
+    L.check(L.lib().dnx_gemm_bf16(C.byref(g), L.stream_handle(dev)))
     return Cout
 
 
@@ -244,8 +246,20 @@ class FNO2d:
         gemm(B, c, c, Z, sA, sr, sB, Y, sC, a_off=M, c_off=M, beta=1.0, Cin=Y, **kw)  # + Zi Wr
 
     # -- forward / backward ----------------------------------------------------
+    # inference keeps the hidden activations in BF16 only (every consumer
+    # rounds them to BF16 anyway: bitwise the same output, fewer bytes)
+    bf16_inference = True
+
+    def _bf16_inference_ok(self, n):
+        # the pointwise kernel's 32-bit row-block offsets: 33 rows of n cells
+        # of FP32 below 2^31 bytes (larger batches take the FP32 path)
+        return (self.bf16_inference and self.bf16_copies and self._fused_head_ok()
+                and 33 * 4 * n < (1 << 31))
+
     def forward_raw(self, x, save=True):
         """x: [in_ch][B][H*W] FP32 -> ([out_ch][B][H*W] FP32, saved tensors)."""
+        if not save and self._bf16_inference_ok(x.shape[1] * self.H * self.W):
+            return self._infer_bf16(x), None
         p = self.params
         c, B = self.width, x.shape[1]
         P = self.H * self.W
@@ -285,7 +299,7 @@ class FNO2d:
             # inference: the projection 32 -> 128 -> GELU -> out_ch in one
             # pass, the hidden activations never written
             L.check(L.lib().dnx_fno_proj(
-                h.data_ptr(), p["w_p1"].data_ptr(), p["b_p1"].data_ptr(),
+                h.data_ptr(), 0, p["w_p1"].data_ptr(), p["b_p1"].data_ptr(),
                 p["w_p2"].data_ptr(), p["b_p2"].data_ptr(), out.data_ptr(), c, self.hidden,
                 self.out_ch, n, L.stream_handle(dev)))
             return out.view(self.out_ch, B, P), None
@@ -295,6 +309,44 @@ class FNO2d:
         self._pointwise(p["w_p2"], q, out, bias=p["b_p2"])
         saved.update({"hL": h, "q": q, "qpre": qpre, "B": B})
         return out.view(self.out_ch, B, P), (saved if save else None)
+
+    def _infer_bf16(self, x):
+        """The forward without saving, with the layers' activations written
+        only as their BF16 copies: the forward DFT, the next pointwise layer
+        and the projection head all read h rounded to BF16, so the FP32 h is
+        never needed (the lifting layer still writes it where its shape is
+        not a pointwise-kernel shape)."""
+        p = self.params
+        c, B = self.width, x.shape[1]
+        P = self.H * self.W
+        n = B * P
+        M2 = 2 * self.nmodes
+        dev = self.dev
+        x2 = x.reshape(self.in_ch, n)
+        h16 = torch.empty(c, n, device=dev, dtype=torch.bfloat16)
+        nxt = torch.empty(c, n, device=dev, dtype=torch.bfloat16)
+        # the lifting layer on the pointwise kernel writes only the copy
+        h = None if (c, self.in_ch) in ((32, 9), (32, 32)) else torch.empty(c, n, device=dev)
+        self._pointwise(p["w_lift"], x2, h, bias=p["b_lift"], c16=h16)
+        del h
+        for l in range(self.layers):
+            Z = torch.empty(c * B, M2, device=dev)
+            gemm_nt(c * B, M2, P, h16, self.FT, Z)
+            Y = torch.empty(c * B, M2, device=dev,
+                            dtype=torch.bfloat16 if self._fast_mix_ok(B) else torch.float32)
+            self._mix(Z, p[f"sr{l}"], p[f"si{l}"], Y, B)
+            S = torch.empty(c, n, device=dev)
+            gemm(c * B, P, M2, Y, (M2, 1, 0), self.GT, (1, M2, 0), S, (P, 1, 0))
+            last = l == self.layers - 1
+            gemm(c, n, c, p[f"w{l}"], (c, 1, 0), h16, (n, 1, 0), None, (n, 1, 0),
+                 bias=p[f"b{l}"], bias_mode=2, act=0 if last else 1, Cin=S, beta=1.0, c16=nxt)
+            h16, nxt = nxt, h16
+        out = torch.empty(self.out_ch, n, device=dev)
+        L.check(L.lib().dnx_fno_proj(
+            h16.data_ptr(), 1, p["w_p1"].data_ptr(), p["b_p1"].data_ptr(),
+            p["w_p2"].data_ptr(), p["b_p2"].data_ptr(), out.data_ptr(), c, self.hidden,
+            self.out_ch, n, L.stream_handle(dev)))
+        return out.view(self.out_ch, B, P)
 
     SPLIT = 256  # split-K factor for reductions over the B*H*W axis
 

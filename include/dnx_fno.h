@@ -31,7 +31,10 @@ extern "C" {
  * c_bf16: if not NULL, also receives C rounded to BF16 (nearest even), at
  * C's element offsets -- the operand a later BF16 product reads instead of
  * rounding C itself (same values, half the bytes); channel-mixing
- * products only (M, K <= 128), others are rejected. */
+ * products only (M, K <= 128), others are rejected.  C may then be NULL
+ * (only the BF16 copy is written) on the pointwise MFMA shapes (M x K of
+ * 32 x 9, 32 x 32, 128 x 32 or 128 x 6, batch 1, contiguous cells); other
+ * products with a NULL C are rejected. */
 typedef struct dnx_gemm_args {
   int32_t M, N, K, batch;
   const void* A;
@@ -61,12 +64,13 @@ int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out, fl
 /* The projection head of the FNO in one pass (inference; no reference
  * counterpart beyond the surrogate's output layers, surrogate.py:23-75):
  *   out[o][n] = b2[o] + sum_j W2[o][j] bf(gelu(b1[j] + sum_i W1[j][i] bf(h[i][n])))
- * h [I][n], W1 [J][I], W2 [O][J], out [O][n], FP32 row-major; bf() rounds to
- * BF16 (nearest even), products on the matrix cores with FP32 accumulation.
- * I = 32, J = 128, O in 1-6, 8, 16 or 32. */
-int32_t dnx_fno_proj(const float* h, const float* W1, const float* b1, const float* W2,
-                     const float* b2, float* out, int32_t I, int32_t J, int32_t O, int64_t n,
-                     void* stream);
+ * h [I][n] FP32 (h_bf16 = 0) or BF16 (= 1, used as is), W1 [J][I],
+ * W2 [O][J], out [O][n], FP32 row-major; bf() rounds to BF16 (nearest
+ * even), products on the matrix cores with FP32 accumulation.  I = 32,
+ * J = 128, O in 1-6, 8, 16 or 32. */
+int32_t dnx_fno_proj(const void* h, int32_t h_bf16, const float* W1, const float* b1,
+                     const float* W2, const float* b2, float* out, int32_t I, int32_t J,
+                     int32_t O, int64_t n, void* stream);
 
 /* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32; dx_bf16, if not NULL,
  * also receives dx rounded to BF16 (nearest even). */

@@ -374,9 +374,48 @@ def test_fused_projection_head_rejects_bad_shapes():
     import diffnano_amd._lib as L
     t = torch.zeros(1024, device="cuda")
     p = t.data_ptr()
-    assert L.lib().dnx_fno_proj(p, p, p, p, p, p, 32, 64, 6, 32, None) != 0
+    assert L.lib().dnx_fno_proj(p, 0, p, p, p, p, p, 32, 64, 6, 32, None) != 0
     assert b"fno_proj" in L.lib().dnx_last_error()
-    assert L.lib().dnx_fno_proj(p, p, p, p, p, p, 32, 128, 7, 32, None) != 0
+    assert L.lib().dnx_fno_proj(p, 0, p, p, p, p, p, 32, 128, 7, 32, None) != 0
+
+
+@pytest.mark.parametrize("B,H", [(8, 128), (3, 40)])
+def test_bf16_only_inference_is_bitwise_the_fp32_activation_path(B, H):
+    """Inference writing the layers' activations only as BF16 copies (the
+    pointwise kernel's FP32 output skipped, the projection head reading BF16)
+    gives the bits of the path that writes and reads FP32 activations."""
+    from diffnano_amd.fno import FNO2d
+    m = FNO2d(H, H, in_ch=9, out_ch=6, width=32, modes=(4, 4), layers=3, hidden=128,
+              device="cuda", seed=8)
+    torch.manual_seed(9)
+    x = torch.randn(B, 9, H, H, device="cuda")
+    with torch.no_grad():
+        m.bf16_inference = True
+        a = m(x).clone()
+        m.bf16_inference = False
+        b = m(x).clone()
+    m.bf16_inference = True
+    assert torch.equal(a, b)
+    assert float(a.abs().max()) > 0
+
+
+def test_gemm_without_fp32_output_only_on_pointwise_shapes():
+    """C = NULL (only the BF16 copy written) is taken by the pointwise MFMA
+    kernel and rejected elsewhere."""
+    from diffnano_amd.fno import gemm
+    torch.manual_seed(10)
+    W = torch.randn(32, 32, device="cuda")
+    X = torch.randn(32, 4096, device="cuda")
+    ref = torch.empty(32, 4096, device="cuda")
+    r16 = torch.empty(32, 4096, device="cuda", dtype=torch.bfloat16)
+    gemm(32, 4096, 32, W, (32, 1, 0), X, (4096, 1, 0), ref, (4096, 1, 0), act=1, c16=r16)
+    o16 = torch.empty(32, 4096, device="cuda", dtype=torch.bfloat16)
+    gemm(32, 4096, 32, W, (32, 1, 0), X, (4096, 1, 0), None, (4096, 1, 0), act=1, c16=o16)
+    assert torch.equal(o16, r16)
+    W2 = torch.randn(48, 32, device="cuda")
+    with pytest.raises(RuntimeError, match="C may be NULL"):
+        gemm(48, 4096, 32, W2, (32, 1, 0), X, (4096, 1, 0), None, (4096, 1, 0),
+             c16=torch.empty(48, 4096, device="cuda", dtype=torch.bfloat16))
 
 
 @pytest.mark.parametrize("conj", [False, True])
