@@ -15,7 +15,7 @@ import threading
 import torch
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdnx_fdtd.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 DNX_F32 = 0
 DNX_F64 = 1

@@ -39,7 +39,7 @@ extern "C" {
 #pragma GCC visibility push(default)
 #endif
 
-#define DNX_ABI_VERSION 4
+#define DNX_ABI_VERSION 5
 
 enum { DNX_F32 = 0, DNX_F64 = 1 };
 enum {
