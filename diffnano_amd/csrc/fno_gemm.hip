@@ -1381,21 +1381,36 @@ extern "C" int32_t dnx_gemm_bf16(const GemmArgs* g, void* stream) {
 
 // Few outputs, many partials (weight-gradient reductions): one output per
 // thread, the partials of consecutive outputs read by consecutive lanes.
-__global__ void k_sum_partials1(const float* __restrict__ part, int s, int64_t n,
-                                float* __restrict__ out, float beta) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float a = beta != 0.f ? beta * out[i] : 0.f;
-#pragma unroll 8
-  for (int j = 0; j < s; ++j) a += part[int64_t(j) * n + i];
-  out[i] = a;
+// Few outputs (the weight and bias gradients), many partials: a work-group
+// of 1024 threads per 64 outputs, 16 threads per output each summing every
+// 16th partial, then the 16 sums added in a fixed order (deterministic; 16x
+// the loads in flight of one thread per output).
+__global__ __launch_bounds__(1024) void k_sum_partials1(const float* __restrict__ part, int s,
+                                                        int64_t n, float* __restrict__ out,
+                                                        float beta) {
+  __shared__ float red[16][64];
+  const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t i = int64_t(blockIdx.x) * 64 + t;
+  float a = 0.f;
+  if (i < n) {
+#pragma unroll 4
+    for (int j = g; j < s; j += 16) a += part[int64_t(j) * n + i];
+  }
+  red[g][t] = a;
+  __syncthreads();
+  if (g == 0 && i < n) {
+    float r = beta != 0.f ? beta * out[i] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r += red[k][t];
+    out[i] = r;
+  }
 }
 
 extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, float* out,
                                     float beta, void* stream) {
   DNX_REQUIRE(part && out && s > 0 && n > 0, "sum_partials: bad arguments");
   if (n < (int64_t(1) << 16)) {
-    hipLaunchKernelGGL(k_sum_partials1, dim3(cdiv(n, 64)), dim3(64), 0,
+    hipLaunchKernelGGL(k_sum_partials1, dim3(cdiv(n, 64)), dim3(1024), 0,
                        static_cast<hipStream_t>(stream), part, s, n, out, beta);
     DNX_CHECK_HIP(hipGetLastError());
     return DNX_OK;

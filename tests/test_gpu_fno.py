@@ -418,6 +418,20 @@ def test_gemm_without_fp32_output_only_on_pointwise_shapes():
              c16=torch.empty(48, 4096, device="cuda", dtype=torch.bfloat16))
 
 
+@pytest.mark.parametrize("s,n,beta", [(37, 100, 0.0), (1024, 1024, 0.5), (3, 70000, 1.0),
+                                      (1, 5, 2.0)])
+def test_sum_partials_matches_fp64(s, n, beta):
+    """The split-K reduction (16 threads per output for few outputs, one per
+    four outputs for many) against an FP64 sum."""
+    from diffnano_amd.fno import sum_partials
+    torch.manual_seed(11)
+    part = torch.randn(s, n, device="cuda")
+    out = torch.randn(n, device="cuda")
+    ref = part.double().sum(0) + beta * out.double()
+    sum_partials(part.reshape(-1), s, n, out, beta=beta)
+    assert rel_l2(out.double().cpu().numpy(), ref.cpu().numpy()) < 1e-6
+
+
 @pytest.mark.parametrize("conj", [False, True])
 def test_mode_mix_kernels_match_fp64(conj):
     """Per-mode complex channel mixing and its weight gradient (FP32 kernels)
