@@ -27,6 +27,7 @@
 #include "dnx_common.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace dnx {
@@ -990,6 +991,156 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lds(GemmArgs g, int tiles_n, in
   tile(acc[1][0], 1, 0); tile(acc[1][1], 1, 1); tile(acc[1][2], 1, 2); tile(acc[1][3], 1, 3);
 }
 
+// An FNO layer's inverse DFT and its pointwise part in one kernel:
+//   out[c][b P + p] = act(sum_k Y[c B + b][k] GT[p][k]
+//                         + sum_c' W[c][c'] h16[c'][b P + p] + bias[c])
+// with 32 channels c, B samples, P cells and K = 2 x modes; Y, GT, h16 BF16,
+// W rounded to BF16, FP32 accumulation.  The product runs as k_gemm_lds
+// does (256 x 256 x 64 tiles, 8 waves, operands straight into LDS) over
+// virtual rows v = 32 b + c, so that a tile's 256 rows are the 32 channels
+// of 8 whole samples; a wave's 32 x 32 accumulator sub-tile is then one
+// sample's 32 channels at 32 cells, which is also the shape of the
+// pointwise product (W times the 32 channels of h16 at those cells): two
+// more MFMAs add it into the same accumulators before the bias, the
+// activation and the stores.  Against the inverse-DFT product followed by
+// the pointwise kernel, the FP32 sum of the spectral branch is neither
+// written nor read back, and the two sums are added inside the
+// accumulators (FP32 rounding differs from the two-kernel path).
+struct InvPwArgs {
+  const unsigned short* Y;   // [32 B][K]
+  const unsigned short* GT;  // [P][K]
+  const unsigned short* h16; // [32][B P], the layer's input
+  const float* W;            // [32][32]
+  const float* bias;         // [32]
+  float* out;                // [32][B P] or null
+  float* pre;                // before the activation, or null
+  unsigned short* out16;     // out rounded to BF16, or null
+  int act, B, P, K;
+};
+
+__global__ __launch_bounds__(512, 1) void k_inv_pw(InvPwArgs a, int tiles_n, int tiles) {
+  constexpr int C = 32, IPO = 4;
+  extern __shared__ unsigned short lsm[];
+  const int nb = gridDim.x, bx = blockIdx.x;
+  const int logical = (bx & 7) * (nb >> 3) + (bx >> 3);
+  if (logical >= tiles) return;
+  const int m0 = (logical / tiles_n) * 256;  // virtual rows (32 b + c)
+  const int n0 = (logical % tiles_n) * 256;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 128;
+  // as k_gemm_lds: instruction j of this wave fills rows 8q .. 8q+7
+  // (q = wave + 8 j) of a stage, lane i the chunk (i%8) ^ swz(row); the
+  // virtual row 32 b + c reads Y's row c B + b (every tile is inside the
+  // matrix: 32 B and P are multiples of 256)
+  int64_t ga[IPO], gb[IPO];
+#pragma unroll
+  for (int j = 0; j < IPO; ++j) {
+    const int row = (wave + 8 * j) * 8 + (lane >> 3);
+    const int c = (lane & 7) ^ lds_swz(row);
+    const int v = m0 + row;
+    ga[j] = (int64_t(v % C) * a.B + v / C) * a.K + c * 8;
+    gb[j] = int64_t(n0 + row) * a.K + c * 8;
+  }
+  const int nk = a.K / LBK;
+  auto issue = [&](int kt) {
+    unsigned short* sa = lsm + (kt & 1) * 2 * LSTAGE;
+    unsigned short* sb = sa + LSTAGE;
+#pragma unroll
+    for (int j = 0; j < IPO; ++j) {
+      const int q = wave + 8 * j;
+      lds_dma16(a.Y + ga[j] + int64_t(kt) * LBK, sa + q * 512);
+      lds_dma16(a.GT + gb[j] + int64_t(kt) * LBK, sb + q * 512);
+    }
+  };
+  f32x16 acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+  const int lr = lane & 31, kh = lane >> 5;
+  issue(0);
+  for (int kt = 0; kt < nk; ++kt) {
+    __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
+    __builtin_amdgcn_s_barrier();
+    if (kt + 1 < nk) issue(kt + 1);
+    const unsigned short* Ac = lsm + (kt & 1) * 2 * LSTAGE;
+    const unsigned short* Bc = Ac + LSTAGE;
+#pragma unroll
+    for (int ks = 0; ks < LBK; ks += 16) {
+      const int c = ks / 8 + kh;
+      bf16x8 af[2], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int r = wm + i * 32 + lr;
+        af[i] = *reinterpret_cast<const bf16x8*>(&Ac[r * LBK + ((c ^ lds_swz(r)) << 3)]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = wn + j * 32 + lr;
+        bfr[j] = *reinterpret_cast<const bf16x8*>(&Bc[r * LBK + ((c ^ lds_swz(r)) << 3)]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[j], af[i], acc[i][j], 0, 0, 0);
+    }
+  }
+  // the pointwise product into the same accumulators: lane holds channel
+  // lr of W (k = c' in chunk kh) and cell lr of the sub-tile (h16 rows c')
+  bf16x8 wf[2];
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    u16x8 u;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) u[e] = f2bf(a.W[lr * C + ks * 16 + kh * 8 + e]);
+    wf[ks] = __builtin_bit_cast(bf16x8, u);
+  }
+  const int64_t BP = int64_t(a.B) * a.P;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int bb = (m0 + wm) / C + i;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t col = int64_t(bb) * a.P + n0 + wn + j * 32 + lr;
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        u16x8 u;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) u[e] = a.h16[int64_t(ks * 16 + kh * 8 + e) * BP + col];
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, u),
+                                                            wf[ks], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // accumulator element v of a sub-tile: channel lr, cell
+  // 8 (v / 4) + 4 kh + v % 4 -- four consecutive cells per v / 4
+  const float bc = a.bias[lr];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int bb = (m0 + wm) / C + i;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int qv = 0; qv < 4; ++qv) {
+        const int64_t idx =
+            int64_t(lr) * BP + int64_t(bb) * a.P + n0 + wn + j * 32 + 8 * qv + 4 * kh;
+        f32x4 x = f32x4{acc[i][j][4 * qv], acc[i][j][4 * qv + 1], acc[i][j][4 * qv + 2],
+                        acc[i][j][4 * qv + 3]} + bc;
+        if (a.pre) *reinterpret_cast<f32x4*>(a.pre + idx) = x;
+        if (a.act == 1) x = f32x4{gelu(x[0]), gelu(x[1]), gelu(x[2]), gelu(x[3])};
+        if (a.out) *reinterpret_cast<f32x4*>(a.out + idx) = x;
+        if (a.out16)
+          *reinterpret_cast<uint2*>(a.out16 + idx) =
+              uint2{uint32_t(f2bf(x[0])) | (uint32_t(f2bf(x[1])) << 16),
+                    uint32_t(f2bf(x[2])) | (uint32_t(f2bf(x[3])) << 16)};
+      }
+    }
+  }
+}
+
 static int launch_lds(const GemmArgs& g, hipStream_t s) {
   static std::atomic<bool> attr{false};
   if (!attr.load()) {
@@ -1080,7 +1231,6 @@ __global__ void k_gelu_bwd(const float* dy, const float* __restrict__ pre, float
 // cell l%32, so every load and store of X, Cin, pre and C covers 32
 // consecutive cells of a row (128-byte segments of FP32).  Each wave walks
 // groups of 32 cells.
-typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 template <int M, int K, bool BBF>
 __global__ __launch_bounds__(256) void k_pw(GemmArgs g) {
@@ -1418,6 +1568,36 @@ extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, flo
   const int64_t threads = cdiv(n, 4);
   hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(threads, 256)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), part, s, n, out, beta);
+  DNX_CHECK_HIP(hipGetLastError());
+  return DNX_OK;
+}
+
+extern "C" int32_t dnx_fno_inverse_pw(const void* Y, const void* GT, const void* h16,
+                                      const float* W, const float* bias, float* out,
+                                      float* pre, unsigned short* out16, int32_t act,
+                                      int32_t C, int32_t B, int32_t P, int32_t K,
+                                      void* stream) {
+  DNX_REQUIRE(Y && GT && h16 && W && bias && (out || out16), "fno_inverse_pw: null operand");
+  DNX_REQUIRE(C == 32 && B > 0 && B % 8 == 0 && P > 0 && P % 256 == 0 && K >= LBK &&
+                  K % LBK == 0 && int64_t(32) * B * P < (int64_t(1) << 31) * 4,
+              "fno_inverse_pw: needs 32 channels, B %% 8 == 0, P %% 256 == 0, K %% 64 == 0 "
+              "(C=%d B=%d P=%d K=%d)", C, B, P, K);
+  auto al = [](const void* p, uintptr_t m) { return (reinterpret_cast<uintptr_t>(p) & m) == 0; };
+  DNX_REQUIRE(al(Y, 15) && al(GT, 15) && al(h16, 1) && al(out, 15) && al(pre, 15) &&
+                  al(out16, 7),
+              "fno_inverse_pw: misaligned operand");
+  InvPwArgs a{static_cast<const unsigned short*>(Y), static_cast<const unsigned short*>(GT),
+              static_cast<const unsigned short*>(h16), W, bias, out, pre, out16, act, B, P, K};
+  static std::atomic<bool> attr{false};
+  if (!attr.load()) {
+    DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inv_pw),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(LSMEM)));
+    attr = true;
+  }
+  const int tn = P / 256, tiles = (B / 8) * tn;
+  const int nb = ((tiles + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_inv_pw, dim3(nb), dim3(512), LSMEM, static_cast<hipStream_t>(stream), a,
+                     tn, tiles);
   DNX_CHECK_HIP(hipGetLastError());
   return DNX_OK;
 }

@@ -221,6 +221,26 @@ class FNO2d:
     # 5.09 -> 4.72 ms per inference, 10.75 -> 10.15 ms per training step)
     dft_blocks = 256
 
+    # each layer's inverse DFT and pointwise part run as one kernel
+    # (dnx_fno_inverse_pw) where the shapes allow
+    fused_inverse = True
+
+    def _fused_inverse_ok(self, B):
+        P = self.H * self.W
+        return (self.fused_inverse and self.bf16_copies and self._fast_mix_ok(B)
+                and P % 256 == 0 and (2 * self.nmodes) % 64 == 0)
+
+    def _inverse_pw(self, Y, h16, l, out, pre, out16, act, B):
+        """out = act(Y GT + W_l h + b_l) per layer (FP32 out, pre and the
+        BF16 copy out16 each optional), the sum of the spectral branch kept
+        in the accumulators."""
+        p = self.params
+        ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        L.check(L.lib().dnx_fno_inverse_pw(
+            Y.data_ptr(), self.GT.data_ptr(), h16.data_ptr(), p[f"w{l}"].data_ptr(),
+            p[f"b{l}"].data_ptr(), ptr(out), ptr(pre), ptr(out16), int(act), self.width,
+            int(B), self.H * self.W, 2 * self.nmodes, L.stream_handle(self.dev)))
+
     # inference runs the projection head as one kernel (dnx_fno_proj) where
     # the shapes allow
     fused_head = True
@@ -279,6 +299,10 @@ class FNO2d:
         h16 = torch.empty(c, n, device=dev, dtype=torch.bfloat16) if self.bf16_copies else None
         self._pointwise(p["w_lift"], x2, h, bias=p["b_lift"], c16=h16)
         saved = {"x": x2, "h": [], "Z": [], "pre": []}
+        # the fused layer kernel reads the layer's input copy while it writes
+        # the output's: a second BF16 buffer
+        fuse = self._fused_inverse_ok(B)
+        h16b = torch.empty_like(h16) if fuse else None
         for l in range(self.layers):
             Z = torch.empty(c * B, M2, device=dev)
             gemm_nt(c * B, M2, P, h, self.FT, Z, A16=h16, target_blocks=self.dft_blocks)
@@ -287,13 +311,18 @@ class FNO2d:
             Y = torch.empty(c * B, M2, device=dev,
                             dtype=torch.bfloat16 if self._fast_mix_ok(B) else torch.float32)
             self._mix(Z, p[f"sr{l}"], p[f"si{l}"], Y, B)
-            S = torch.empty(c, n, device=dev)
-            gemm(c * B, P, M2, Y, (M2, 1, 0), self.GT, (1, M2, 0), S, (P, 1, 0))
             last = l == self.layers - 1
             hn = torch.empty(c, n, device=dev)
             pre = None if (last or not save) else torch.empty(c, n, device=dev)
-            self._pointwise(p[f"w{l}"], h, hn, bias=p[f"b{l}"], act=0 if last else 1, pre=pre,
-                            Cin=S, beta=1.0, c16=None if last else h16)
+            if fuse:
+                self._inverse_pw(Y, h16, l, hn, pre, None if last else h16b, 0 if last else 1, B)
+                if not last:
+                    h16, h16b = h16b, h16
+            else:
+                S = torch.empty(c, n, device=dev)
+                gemm(c * B, P, M2, Y, (M2, 1, 0), self.GT, (1, M2, 0), S, (P, 1, 0))
+                self._pointwise(p[f"w{l}"], h, hn, bias=p[f"b{l}"], act=0 if last else 1,
+                                pre=pre, Cin=S, beta=1.0, c16=None if last else h16)
             if save:
                 saved["h"].append(h)
                 saved["Z"].append(Z)
@@ -328,6 +357,7 @@ class FNO2d:
         M2 = 2 * self.nmodes
         dev = self.dev
         x2 = x.reshape(self.in_ch, n)
+        fuse = self._fused_inverse_ok(B)
         h16 = torch.empty(c, n, device=dev, dtype=torch.bfloat16)
         nxt = torch.empty(c, n, device=dev, dtype=torch.bfloat16)
         # the lifting layer on the pointwise kernel writes only the copy
@@ -340,11 +370,15 @@ class FNO2d:
             Y = torch.empty(c * B, M2, device=dev,
                             dtype=torch.bfloat16 if self._fast_mix_ok(B) else torch.float32)
             self._mix(Z, p[f"sr{l}"], p[f"si{l}"], Y, B)
-            S = torch.empty(c, n, device=dev)
-            gemm(c * B, P, M2, Y, (M2, 1, 0), self.GT, (1, M2, 0), S, (P, 1, 0))
             last = l == self.layers - 1
-            gemm(c, n, c, p[f"w{l}"], (c, 1, 0), h16, (n, 1, 0), None, (n, 1, 0),
-                 bias=p[f"b{l}"], bias_mode=2, act=0 if last else 1, Cin=S, beta=1.0, c16=nxt)
+            if fuse:
+                self._inverse_pw(Y, h16, l, None, None, nxt, 0 if last else 1, B)
+            else:
+                S = torch.empty(c, n, device=dev)
+                gemm(c * B, P, M2, Y, (M2, 1, 0), self.GT, (1, M2, 0), S, (P, 1, 0))
+                gemm(c, n, c, p[f"w{l}"], (c, 1, 0), h16, (n, 1, 0), None, (n, 1, 0),
+                     bias=p[f"b{l}"], bias_mode=2, act=0 if last else 1, Cin=S, beta=1.0,
+                     c16=nxt)
             h16, nxt = nxt, h16
         out = torch.empty(self.out_ch, n, device=dev)
         L.check(L.lib().dnx_fno_proj(

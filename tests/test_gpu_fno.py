@@ -329,6 +329,7 @@ def test_fno_bf16_copies_change_nothing():
     from diffnano_amd.fno import FNO2d
     m = FNO2d(128, 128, in_ch=3, out_ch=6, width=32, modes=(16, 16), layers=2, hidden=32,
               device="cuda", seed=4)
+    m.fused_inverse = False  # the layer kernel needs the copies; compare the two-kernel path
     torch.manual_seed(5)
     x = torch.randn(8, 3, 128, 128, device="cuda")
     w = torch.randn(8, 6, 128, 128, device="cuda")
@@ -397,6 +398,89 @@ def test_bf16_only_inference_is_bitwise_the_fp32_activation_path(B, H):
     m.bf16_inference = True
     assert torch.equal(a, b)
     assert float(a.abs().max()) > 0
+
+
+@pytest.mark.parametrize("B,H,train", [(8, 128, False), (16, 64, True), (8, 32, True)])
+def test_fused_inverse_and_pointwise_match_two_kernels(B, H, train):
+    """Each layer's inverse DFT and pointwise part in one kernel against the
+    inverse-DFT product followed by the pointwise kernel: the same sums,
+    added in the accumulators instead of after an FP32 round, so outputs
+    (and in training the parameter gradients) agree to FP32 rounding."""
+    from diffnano_amd.fno import FNO2d
+    m = FNO2d(H, H, in_ch=9, out_ch=6, width=32, modes=(8, 8), layers=3, hidden=128,
+              device="cuda", seed=12)
+    torch.manual_seed(13)
+    x = torch.randn(B, 9, H, H, device="cuda")
+    w = torch.randn(B, 6, H, H, device="cuda")
+    res = []
+    for fused in (True, False):
+        m.fused_inverse = fused
+        for v in m.params.values():
+            v.grad = None
+        if train:
+            out = m(x)
+            (out * w).sum().backward()
+            res.append((out.detach().clone(), {k: v.grad.clone() for k, v in m.params.items()}))
+        else:
+            with torch.no_grad():
+                res.append((m(x).clone(), {}))
+    m.fused_inverse = True
+    e = rel_l2(res[0][0].double().cpu().numpy(), res[1][0].double().cpu().numpy())
+    print("fused inverse: output rel L2", e)
+    # an FP32-rounding difference in one layer can flip the BF16 rounding of
+    # a few of the next layer's inputs (2^-9 relative each)
+    assert e < 1e-4 and float(res[1][0].abs().max()) > 0
+    # the gradients (BF16 products throughout, against the FP64 oracle within
+    # 2e-2) carry those flips back through every layer: 1.4e-4 seen on the
+    # lifting weights
+    for k in res[0][1]:
+        eg = rel_l2(res[0][1][k].double().cpu().numpy(), res[1][1][k].double().cpu().numpy())
+        assert eg < 1e-3, (k, eg)
+
+
+@pytest.mark.parametrize("B,P,K,act,outs", [(8, 256, 64, 1, "all"), (16, 512, 1024, 0, "out"),
+                                           (8, 1024, 256, 1, "bf16")])
+def test_fused_inverse_kernel_matches_fp64(B, P, K, act, outs):
+    """dnx_fno_inverse_pw on random operands against the formula in FP64 on
+    the same BF16-rounded values (out, pre and the BF16 copy)."""
+    import diffnano_amd._lib as L
+    torch.manual_seed(14)
+    C = 32
+    Y = torch.randn(C * B, K, device="cuda").to(torch.bfloat16)
+    GT = (torch.randn(P, K, device="cuda") / K ** 0.5).to(torch.bfloat16)
+    h16 = torch.randn(C, B * P, device="cuda").to(torch.bfloat16)
+    W = torch.randn(C, C, device="cuda") / C ** 0.5
+    b = torch.randn(C, device="cuda")
+    out = torch.full((C, B * P), float("nan"), device="cuda") if outs in ("all", "out") else None
+    pre = torch.full((C, B * P), float("nan"), device="cuda") if outs == "all" else None
+    o16 = (torch.zeros(C, B * P, device="cuda", dtype=torch.bfloat16)
+           if outs in ("all", "bf16") else None)
+    ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+    L.check(L.lib().dnx_fno_inverse_pw(Y.data_ptr(), GT.data_ptr(), h16.data_ptr(), W.data_ptr(),
+                                       b.data_ptr(), ptr(out), ptr(pre), ptr(o16), act, C, B, P,
+                                       K, None))
+    torch.cuda.synchronize()
+    S = (Y.double() @ GT.double().T).reshape(C, B * P)            # rows (c, b) -> [c][b P]
+    Wb = W.to(torch.bfloat16).double()
+    z = S + Wb @ h16.double() + b.double()[:, None]
+    ref = torch.nn.functional.gelu(z) if act else z
+    if pre is not None:
+        assert rel_l2(pre.double().cpu().numpy(), z.cpu().numpy()) < 1e-5
+    if out is not None:
+        assert rel_l2(out.double().cpu().numpy(), ref.cpu().numpy()) < 1e-5
+    if o16 is not None:
+        assert rel_l2(o16.double().cpu().numpy(), ref.cpu().numpy()) < 5e-3
+
+
+def test_fused_inverse_rejects_bad_shapes():
+    import diffnano_amd._lib as L
+    t = torch.zeros(4096, device="cuda")
+    p = t.data_ptr()
+    assert L.lib().dnx_fno_inverse_pw(p, p, p, p, p, p, None, None, 1, 32, 7, 256, 64,
+                                      None) != 0
+    assert b"fno_inverse_pw" in L.lib().dnx_last_error()
+    assert L.lib().dnx_fno_inverse_pw(p, p, p, p, p, p, None, None, 1, 32, 8, 100, 64,
+                                      None) != 0
 
 
 def test_gemm_without_fp32_output_only_on_pointwise_shapes():
