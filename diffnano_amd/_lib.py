@@ -105,7 +105,8 @@ _SIGS = {
     "dnx_gemm_bf16": ([C.POINTER(GemmArgs), _P], _I),
     "dnx_gelu_bwd": ([_P, _P, _P, _P, _L, _P], _I),
     "dnx_fno_proj": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _L, _P], _I),
-    "dnx_fno_inverse_pw": ([_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
+    "dnx_fno_inverse_pw": ([_P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _P],
+                           _I),
     "dnx_sum_partials": ([_P, _I, _L, _P, C.c_float, _P], _I),
     "dnx_fno_mode_mix": ([_P, _P, _P, _P, _I, _I, _I, _L, _I, _I, _P], _I),
     "dnx_fno_mode_wgrad": ([_P, _P, _P, _P, _I, _I, _L, _I, _I, _P], _I),

@@ -51,6 +51,14 @@ __device__ inline unsigned short load_bf(const void* p, int64_t i, int is_bf16) 
 
 __device__ inline float gelu(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f)); }
 
+// Elementwise helpers of the FNO (FP32).
+// GELU backward: dx = dy * gelu'(pre)
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   __shared__ unsigned short As[GBM * LDK];
   __shared__ unsigned short Bs[GBN * LDK];
@@ -1015,7 +1023,8 @@ struct InvPwArgs {
   float* out;                // [32][B P] or null
   float* pre;                // before the activation, or null
   unsigned short* out16;     // out rounded to BF16, or null
-  int act, B, P, K;
+  const float* gpre;         // backward: out = (sum) * GELU'(gpre), or null
+  int act, transW, B, P, K;  // transW: W read as [c'][c] (W^T)
 };
 
 __global__ __launch_bounds__(512, 1) void k_inv_pw(InvPwArgs a, int tiles_n, int tiles) {
@@ -1095,7 +1104,10 @@ __global__ __launch_bounds__(512, 1) void k_inv_pw(InvPwArgs a, int tiles_n, int
   for (int ks = 0; ks < 2; ++ks) {
     u16x8 u;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) u[e] = f2bf(a.W[lr * C + ks * 16 + kh * 8 + e]);
+    for (int e = 0; e < 8; ++e) {
+      const int k = ks * 16 + kh * 8 + e;
+      u[e] = f2bf(a.transW ? a.W[k * C + lr] : a.W[lr * C + k]);
+    }
     wf[ks] = __builtin_bit_cast(bf16x8, u);
   }
   const int64_t BP = int64_t(a.B) * a.P;
@@ -1117,7 +1129,7 @@ __global__ __launch_bounds__(512, 1) void k_inv_pw(InvPwArgs a, int tiles_n, int
   }
   // accumulator element v of a sub-tile: channel lr, cell
   // 8 (v / 4) + 4 kh + v % 4 -- four consecutive cells per v / 4
-  const float bc = a.bias[lr];
+  const float bc = a.bias ? a.bias[lr] : 0.f;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int bb = (m0 + wm) / C + i;
@@ -1131,6 +1143,11 @@ __global__ __launch_bounds__(512, 1) void k_inv_pw(InvPwArgs a, int tiles_n, int
                         acc[i][j][4 * qv + 3]} + bc;
         if (a.pre) *reinterpret_cast<f32x4*>(a.pre + idx) = x;
         if (a.act == 1) x = f32x4{gelu(x[0]), gelu(x[1]), gelu(x[2]), gelu(x[3])};
+        if (a.gpre) {  // as k_gelu_bwd: dy * GELU'(pre)
+          const f32x4 g = *reinterpret_cast<const f32x4*>(a.gpre + idx);
+          x = f32x4{x[0] * gelu_grad(g[0]), x[1] * gelu_grad(g[1]), x[2] * gelu_grad(g[2]),
+                    x[3] * gelu_grad(g[3])};
+        }
         if (a.out) *reinterpret_cast<f32x4*>(a.out + idx) = x;
         if (a.out16)
           *reinterpret_cast<uint2*>(a.out16 + idx) =
@@ -1179,13 +1196,6 @@ __global__ void k_sum_partials(const float* __restrict__ part, int s, int64_t n,
   }
 }
 
-// Elementwise helpers of the FNO (FP32).
-// GELU backward: dx = dy * gelu'(pre)
-__device__ __forceinline__ float gelu_grad(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-  return cdf + x * pdf;
-}
 
 // four elements per thread (16-byte accesses when all three arrays are
 // 16-byte aligned and n % 4 == 0; element by element otherwise)
@@ -1573,21 +1583,23 @@ extern "C" int32_t dnx_sum_partials(const float* part, int32_t s, int64_t n, flo
 }
 
 extern "C" int32_t dnx_fno_inverse_pw(const void* Y, const void* GT, const void* h16,
-                                      const float* W, const float* bias, float* out,
-                                      float* pre, unsigned short* out16, int32_t act,
-                                      int32_t C, int32_t B, int32_t P, int32_t K,
-                                      void* stream) {
-  DNX_REQUIRE(Y && GT && h16 && W && bias && (out || out16), "fno_inverse_pw: null operand");
+                                      const float* W, int32_t transW, const float* bias,
+                                      float* out, float* pre, unsigned short* out16,
+                                      int32_t act, const float* gelu_pre, int32_t C, int32_t B,
+                                      int32_t P, int32_t K, void* stream) {
+  DNX_REQUIRE(Y && GT && h16 && W && (out || out16), "fno_inverse_pw: null operand");
+  DNX_REQUIRE(!gelu_pre || (act == 0 && !pre), "fno_inverse_pw: gelu_pre excludes act and pre");
   DNX_REQUIRE(C == 32 && B > 0 && B % 8 == 0 && P > 0 && P % 256 == 0 && K >= LBK &&
                   K % LBK == 0 && int64_t(32) * B * P < (int64_t(1) << 31) * 4,
               "fno_inverse_pw: needs 32 channels, B %% 8 == 0, P %% 256 == 0, K %% 64 == 0 "
               "(C=%d B=%d P=%d K=%d)", C, B, P, K);
   auto al = [](const void* p, uintptr_t m) { return (reinterpret_cast<uintptr_t>(p) & m) == 0; };
   DNX_REQUIRE(al(Y, 15) && al(GT, 15) && al(h16, 1) && al(out, 15) && al(pre, 15) &&
-                  al(out16, 7),
+                  al(out16, 7) && al(gelu_pre, 15),
               "fno_inverse_pw: misaligned operand");
   InvPwArgs a{static_cast<const unsigned short*>(Y), static_cast<const unsigned short*>(GT),
-              static_cast<const unsigned short*>(h16), W, bias, out, pre, out16, act, B, P, K};
+              static_cast<const unsigned short*>(h16), W, bias, out, pre, out16, gelu_pre,
+              act, transW ? 1 : 0, B, P, K};
   static std::atomic<bool> attr{false};
   if (!attr.load()) {
     DNX_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inv_pw),

@@ -230,16 +230,20 @@ class FNO2d:
         return (self.fused_inverse and self.bf16_copies and self._fast_mix_ok(B)
                 and P % 256 == 0 and (2 * self.nmodes) % 64 == 0)
 
-    def _inverse_pw(self, Y, h16, l, out, pre, out16, act, B):
+    def _inverse_pw(self, Y, h16, l, out, pre, out16, act, B, *, backward=False,
+                    gelu_pre=None):
         """out = act(Y GT + W_l h + b_l) per layer (FP32 out, pre and the
         BF16 copy out16 each optional), the sum of the spectral branch kept
-        in the accumulators."""
+        in the accumulators.  backward: d h = dZ F^T + W_l^T d(pre), times
+        GELU'(gelu_pre) if given (the previous layer's d(pre))."""
         p = self.params
         ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
         L.check(L.lib().dnx_fno_inverse_pw(
-            Y.data_ptr(), self.GT.data_ptr(), h16.data_ptr(), p[f"w{l}"].data_ptr(),
-            p[f"b{l}"].data_ptr(), ptr(out), ptr(pre), ptr(out16), int(act), self.width,
-            int(B), self.H * self.W, 2 * self.nmodes, L.stream_handle(self.dev)))
+            Y.data_ptr(), (self.F if backward else self.GT).data_ptr(), h16.data_ptr(),
+            p[f"w{l}"].data_ptr(), 1 if backward else 0,
+            None if backward else p[f"b{l}"].data_ptr(), ptr(out), ptr(pre), ptr(out16),
+            int(act), ptr(gelu_pre), self.width, int(B), self.H * self.W, 2 * self.nmodes,
+            L.stream_handle(self.dev)))
 
     # inference runs the projection head as one kernel (dnx_fno_proj) where
     # the shapes allow
@@ -467,12 +471,21 @@ class FNO2d:
         d16 = torch.empty(c, n, device=dev, dtype=torch.bfloat16) if self.bf16_copies else None
         gemm(c, n, self.hidden, p["w_p1"], (1, c, 0), dq, (n, 1, 0), dh, (n, 1, 0), c16=d16)
         del dq
+        # the fused layer kernel computes each layer's input gradient and, with
+        # the previous layer's pre-activation, that layer's d(pre) and its copy
+        fuse = self._fused_inverse_ok(B)
+        d16b = torch.empty_like(d16) if fuse else None
+        dpre_next = None
         for l in range(self.layers - 1, -1, -1):
             h, Z, pre = saved["h"][l], saved["Z"][l], saved["pre"][l]
-            dpre = dh if pre is None else gelu_bwd(dh, pre, torch.empty_like(dh), d16)
+            if dpre_next is not None:
+                dpre = dpre_next
+            else:
+                dpre = dh if pre is None else gelu_bwd(dh, pre, torch.empty_like(dh), d16)
             self._wgrad_bias(dpre, h, g[f"w{l}"], g[f"b{l}"])
-            dhn = torch.empty(c, n, device=dev)
-            gemm(c, n, c, p[f"w{l}"], (1, c, 0), dpre, (n, 1, 0), dhn, (n, 1, 0))
+            if not fuse:
+                dhn = torch.empty(c, n, device=dev)
+                gemm(c, n, c, p[f"w{l}"], (1, c, 0), dpre, (n, 1, 0), dhn, (n, 1, 0))
             # spectral branch: dY = dS G^T
             dY = torch.empty(c * B, M2, device=dev)
             gemm_nt(c * B, M2, P, dpre, self.G, dY, A16=d16, target_blocks=self.dft_blocks)
@@ -493,6 +506,18 @@ class FNO2d:
                 mode_wgrad(Z, dY, pr, pi, B, M, ld, c, ns)
                 g[f"sr{l}"] = pr.sum(0).permute(2, 0, 1).contiguous()
                 g[f"si{l}"] = pi.sum(0).permute(2, 0, 1).contiguous()
+                if fuse:
+                    prev = saved["pre"][l - 1] if l > 0 else None
+                    if prev is not None:  # the previous layer's d(pre) and its copy
+                        dpre_next = torch.empty(c, n, device=dev)
+                        self._inverse_pw(dZ, d16, l, dpre_next, None, d16b, 0, B,
+                                         backward=True, gelu_pre=prev)
+                        d16, d16b = d16b, d16
+                    else:  # d h of the lifting layer's output
+                        dpre_next = None
+                        dh = torch.empty(c, n, device=dev)
+                        self._inverse_pw(dZ, d16, l, dh, None, None, 0, B, backward=True)
+                    continue
                 gemm(c * B, P, M2, dZ, (M2, 1, 0), self.F, (1, M2, 0), dhn, (P, 1, 0), beta=1.0,
                      Cin=dhn)
                 dh = dhn

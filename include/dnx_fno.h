@@ -74,18 +74,20 @@ int32_t dnx_fno_proj(const void* h, int32_t h_bf16, const float* W1, const float
 
 /* One FNO layer's inverse DFT and pointwise part in one pass (no reference
  * counterpart, as above):
- *   out[c][b P + p] = act(sum_k Y[c B + b][k] GT[p][k]
- *                         + sum_c' W[c][c'] h16[c'][b P + p] + bias[c])
+ *   z[c][b P + p] = sum_k Y[c B + b][k] GT[p][k] + sum_c' W[c][c'] h16[c'][b P + p]
+ *                   + bias[c]
+ *   out = act(z), or out = z * GELU'(gelu_pre) (the backward's form)
  * Y [C B][K], GT [P][K], h16 [C][B P] BF16 (the layer's input rounded);
- * W [C][C], bias [C] FP32 (W rounded to BF16); out, pre (before act) FP32
- * and out16 (out rounded to BF16) each optional, at least one of out and
- * out16; FP32 accumulation, the two sums added in the accumulators.
- * C = 32, B a multiple of 8, P of 256, K of 64; Y, GT, out, pre 16-byte and
+ * W [C][C] (read transposed if transW), bias [C] (optional) FP32, W rounded
+ * to BF16; out, pre (z) FP32 and out16 (out rounded to BF16) each
+ * optional, at least one of out and out16; gelu_pre excludes act and pre.
+ * FP32 accumulation, the two sums added in the accumulators.  C = 32, B a
+ * multiple of 8, P of 256, K of 64; Y, GT, out, pre, gelu_pre 16-byte and
  * out16 8-byte aligned. */
 int32_t dnx_fno_inverse_pw(const void* Y, const void* GT, const void* h16, const float* W,
-                           const float* bias, float* out, float* pre, unsigned short* out16,
-                           int32_t act, int32_t C, int32_t B, int32_t P, int32_t K,
-                           void* stream);
+                           int32_t transW, const float* bias, float* out, float* pre,
+                           unsigned short* out16, int32_t act, const float* gelu_pre,
+                           int32_t C, int32_t B, int32_t P, int32_t K, void* stream);
 
 /* dx[i] = dy[i] * GELU'(pre[i]) (erf form), FP32; dx_bf16, if not NULL,
  * also receives dx rounded to BF16 (nearest even). */

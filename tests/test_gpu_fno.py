@@ -2,6 +2,8 @@
 inputs (pins the MFMA lane layout, strides, transposes, epilogues), and the
 FNO forward / hand-written backward against the FP64 torch.fft oracle."""
 
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -439,10 +441,11 @@ def test_fused_inverse_and_pointwise_match_two_kernels(B, H, train):
 
 
 @pytest.mark.parametrize("B,P,K,act,outs", [(8, 256, 64, 1, "all"), (16, 512, 1024, 0, "out"),
-                                           (8, 1024, 256, 1, "bf16")])
+                                           (8, 1024, 256, 1, "bf16"), (8, 512, 128, 0, "bwd")])
 def test_fused_inverse_kernel_matches_fp64(B, P, K, act, outs):
     """dnx_fno_inverse_pw on random operands against the formula in FP64 on
-    the same BF16-rounded values (out, pre and the BF16 copy)."""
+    the same BF16-rounded values (out, pre and the BF16 copy; "bwd": W
+    transposed, no bias, times GELU' of a pre-activation)."""
     import diffnano_amd._lib as L
     torch.manual_seed(14)
     C = 32
@@ -451,19 +454,28 @@ def test_fused_inverse_kernel_matches_fp64(B, P, K, act, outs):
     h16 = torch.randn(C, B * P, device="cuda").to(torch.bfloat16)
     W = torch.randn(C, C, device="cuda") / C ** 0.5
     b = torch.randn(C, device="cuda")
-    out = torch.full((C, B * P), float("nan"), device="cuda") if outs in ("all", "out") else None
+    bwd = outs == "bwd"
+    out = (torch.full((C, B * P), float("nan"), device="cuda")
+           if outs in ("all", "out", "bwd") else None)
     pre = torch.full((C, B * P), float("nan"), device="cuda") if outs == "all" else None
     o16 = (torch.zeros(C, B * P, device="cuda", dtype=torch.bfloat16)
-           if outs in ("all", "bf16") else None)
+           if outs in ("all", "bf16", "bwd") else None)
+    gp = torch.randn(C, B * P, device="cuda") if bwd else None
     ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
     L.check(L.lib().dnx_fno_inverse_pw(Y.data_ptr(), GT.data_ptr(), h16.data_ptr(), W.data_ptr(),
-                                       b.data_ptr(), ptr(out), ptr(pre), ptr(o16), act, C, B, P,
-                                       K, None))
+                                       int(bwd), None if bwd else b.data_ptr(), ptr(out),
+                                       ptr(pre), ptr(o16), act, ptr(gp), C, B, P, K, None))
     torch.cuda.synchronize()
     S = (Y.double() @ GT.double().T).reshape(C, B * P)            # rows (c, b) -> [c][b P]
     Wb = W.to(torch.bfloat16).double()
-    z = S + Wb @ h16.double() + b.double()[:, None]
-    ref = torch.nn.functional.gelu(z) if act else z
+    if bwd:
+        z = S + Wb.T @ h16.double()
+        g = gp.double()
+        gd = 0.5 * (1 + torch.erf(g / 2 ** 0.5)) + g * torch.exp(-0.5 * g * g) / (2 * math.pi) ** 0.5
+        ref = z * gd
+    else:
+        z = S + Wb @ h16.double() + b.double()[:, None]
+        ref = torch.nn.functional.gelu(z) if act else z
     if pre is not None:
         assert rel_l2(pre.double().cpu().numpy(), z.cpu().numpy()) < 1e-5
     if out is not None:
@@ -476,11 +488,13 @@ def test_fused_inverse_rejects_bad_shapes():
     import diffnano_amd._lib as L
     t = torch.zeros(4096, device="cuda")
     p = t.data_ptr()
-    assert L.lib().dnx_fno_inverse_pw(p, p, p, p, p, p, None, None, 1, 32, 7, 256, 64,
+    assert L.lib().dnx_fno_inverse_pw(p, p, p, p, 0, p, p, None, None, 1, None, 32, 7, 256, 64,
                                       None) != 0
     assert b"fno_inverse_pw" in L.lib().dnx_last_error()
-    assert L.lib().dnx_fno_inverse_pw(p, p, p, p, p, p, None, None, 1, 32, 8, 100, 64,
+    assert L.lib().dnx_fno_inverse_pw(p, p, p, p, 0, p, p, None, None, 1, None, 32, 8, 100, 64,
                                       None) != 0
+    assert L.lib().dnx_fno_inverse_pw(p, p, p, p, 1, None, p, None, None, 1, p, 32, 8, 256, 64,
+                                      None) != 0   # GELU' and act together
 
 
 def test_gemm_without_fp32_output_only_on_pointwise_shapes():
