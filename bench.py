@@ -5,6 +5,9 @@ checkpoint every 64 steps; 128x128x16 random-rho design slab between input
 and output Si waveguides; Ey Gaussian sheet source with a broadband pulse
 at 1550 nm; 5-wavelength DFT of Ey on the output plane; loss = sum |DFT|^2.
 One bench "step" = forward + loss.backward() (dL/d eps over the full grid).
+The forward keeps E of as many final checkpoint segments as the device
+memory left after the checkpoints holds (``--e-store-gb``), and the backward
+replays the others.
 
 Prints one JSON line.  value = Mcell-updates/s of fwd+adj = N_cells * T /
 (t_fwd + t_adj).  roofline: the dominant kernel (the timed launch class
@@ -660,6 +663,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4,
                     help="CPU baseline: forward steps timed and T_cpu of the fwd+adj sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--e-store-gb", type=float, default=None,
+                    help="c1/c2: device GB for E of the final segments (default: the "
+                         "spare memory after checkpoints, one replay segment and 16 GiB)")
     ap.add_argument("--designs-sample", type=int, default=48,
                     help="c2: designs of the batched-vs-plain sub-record (0: skip)")
     ap.add_argument("--pmc-probe", action="store_true")
@@ -711,9 +717,20 @@ def main():
     if args.pmc_probe:
         _pmc_probe(torch, dev, solver_kw)
         return
-    solver = FDTDSolver3D(**solver_kw, device=dev)
-    eps0 = torch.tensor(eps_np, dtype=torch.float64, device=dev)
     D, H, W = solver_kw["grid_shape"]
+    e_store_gb = args.e_store_gb
+    if e_store_gb is None:
+        # The E store of the final segments may take what the checkpoints,
+        # one segment's replay store and 16 GiB leave of the device memory
+        # (the library's default takes half of it, leaving room for a
+        # caller's own tensors).
+        from diffnano_amd._common import spare_device_bytes
+        n_cells, n_t, k_ck = D * H * W, solver_kw["n_steps"], solver_kw["checkpoint_every"]
+        ckpt_b = -(-n_t // k_ck) * 6 * n_cells * 4
+        seg_b = k_ck * 3 * n_cells * 4
+        e_store_gb = max(0.0, 0.9 * (spare_device_bytes(dev) - ckpt_b - seg_b - (16 << 30)) / 1e9)
+    solver = FDTDSolver3D(**solver_kw, device=dev, e_store_gb=e_store_gb)
+    eps0 = torch.tensor(eps_np, dtype=torch.float64, device=dev)
     tiles = None
     if args.tune:  # per-grid tile shapes from the autotuner (default shapes are 256^3 sweeps)
         from diffnano_amd import autotune
@@ -764,8 +781,12 @@ def main():
     value = N * T / (wall / args.steps) / 1e6
 
     # Per-phase algorithmic bandwidth (phases include the small side kernels).
-    nseg = (T + K - 1) // K
-    updates = {"forward": N * T * args.steps, "replay": N * (T - nseg) * args.steps,
+    # the replay covers the segments that did not keep E, less each one's
+    # restart step
+    nseg = solver.last_plan.segments
+    n_rep = nseg - solver.last_plan.tail_segments
+    replayed = min(T, n_rep * K) - n_rep
+    updates = {"forward": N * T * args.steps, "replay": N * replayed * args.steps,
                "adjoint": N * (T - 1) * args.steps}
     gbs = {k: BYTES[k] * updates[k] / (phases[k] / 1e3) / 1e9 for k in phases if phases[k] > 0}
     total_bytes = sum(BYTES[k] * updates[k] for k in BYTES)
@@ -807,6 +828,8 @@ def main():
                                f"{solver_kw['pml_layers']}-cell CPML, fwd+time-reversal adjoint, "
                                f"checkpoint every {K}",
                    "grid": [D, H, W], "n_steps": T, "parallelism": "single GPU",
+                   "e_store_gb": round(e_store_gb, 1),
+                   "segments": nseg, "segments_replayed": n_rep,
                    "tiles": tiles or "default"},
         "roofline": {"bound": "hbm", "kernel": f"{dom_k['kernel']} ({TIMED[dom][0]} launches)",
                      "achieved": dom_k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

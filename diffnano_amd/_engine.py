@@ -268,6 +268,10 @@ class Plan:
         # device bytes the time-reversal forward may spend on E of the final
         # steps: None = half the spare device memory above 16 GiB
         self.e_store_bytes = None
+        # set by the time-reversal forward: checkpoint segments, and how many
+        # trailing ones kept E (their replay is skipped)
+        self.segments = 0
+        self.tail_segments = 0
 
 
 _KEEP_FREE = 16 << 30
@@ -350,6 +354,7 @@ class _TimeReversalFDTD(torch.autograd.Function):
         elem = torch.tensor([], dtype=eng.dtype).element_size()
         n_tail = _tail_segments(plan, nseg, K, K * ne * N * elem) if nseg else 0
         a_tail = (nseg - n_tail) * K  # first step whose segment keeps its E
+        plan.segments, plan.tail_segments = nseg, n_tail
         tail = (torch.empty((T - a_tail) * ne * N, dtype=eng.dtype, device=eng.dev)
                 if n_tail else None)
         _begin("forward")

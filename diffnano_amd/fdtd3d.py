@@ -81,6 +81,7 @@ class FDTDSolver3D:
         if e_store_gb is not None and not e_store_gb >= 0:
             raise ValueError(f"e_store_gb must be None or >= 0, got {e_store_gb!r}")
         self.e_store_gb = e_store_gb
+        self.last_plan = None  # the plan of the latest forward (segment counts)
         if check_eps not in EPS_CHECKS:
             raise ValueError(f"check_eps must be one of {EPS_CHECKS}, got {check_eps!r}")
         self.check_eps = check_eps
@@ -201,6 +202,7 @@ class FDTDSolver3D:
                     out_order=[2, 0, 1], out_unsqueeze=False, out_device=self._device)
         if self.e_store_gb is not None:
             plan.e_store_bytes = int(self.e_store_gb * 1e9)
+        self.last_plan = plan
         if torch.is_grad_enabled() and eps_r.requires_grad:
             outs = _TimeReversalFDTD.apply(eps_r, plan)
             field, mon_raw = outs[0], list(outs[1:])
