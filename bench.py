@@ -46,6 +46,7 @@ design loop against the exhaustive batched and the plain loops
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import re
@@ -402,7 +403,8 @@ def _bench_batched(args):
         print(msg, file=sys.stderr, flush=True)
 
     res_b = inverse_design.adam_loop(probs_b, iterations=args.iters, n_steps=T, mode="batched",
-                                     device=dev, tiles=tiles_batch, log=log)
+                                     device=dev, tiles=tiles_batch, log=log, group=args.group)
+    gc.collect()  # the batched plans and stores go back to torch's cache first
     probs_p = inverse_design.batched_problems(args.designs, seed=0)
     res_p = inverse_design.adam_loop(probs_p, iterations=args.iters, n_steps=T, mode="plain",
                                      device=dev, tiles=tiles_plain, log=log)
@@ -415,7 +417,7 @@ def _bench_batched(args):
         "plain_value": round(res_p["design_iters_per_s"], 4),
         "batched_over_plain": round(res_b["design_iters_per_s"] / res_p["design_iters_per_s"], 3),
         "designs": args.designs, "iterations": args.iters, "n_steps": T,
-        "total_cells": cells,
+        "group": args.group, "total_cells": cells,
         "batched_mcells_fwd_adj": round(cells * T * args.iters / res_b["wall_s"] / 1e6, 1),
         "plain_mcells_fwd_adj": round(cells * T * args.iters / res_p["wall_s"] / 1e6, 1),
         "final_fom_max_rel_diff": float(np.max(np.abs(fb - fp) / np.abs(fp))),
@@ -440,6 +442,7 @@ def _designs_sample(n_designs, n_steps, iters, dev):
 
     res = {}
     for mode in ("batched", "plain"):
+        gc.collect()  # the other loop's plans and stores go back to torch's cache first
         probs = inverse_design.batched_problems(n_designs, seed=0)
         inverse_design.adam_loop(probs[:2], iterations=1, n_steps=8, mode=mode, device=dev)
         torch.cuda.synchronize()
@@ -670,6 +673,8 @@ def main():
                     help="c2: designs of the batched-vs-plain sub-record (0: skip)")
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--designs", type=int, default=48, help="c3: number of designs")
+    ap.add_argument("--group", type=int, default=None,
+                    help="c3: designs per batched plan (plans run in turn; default one plan)")
     ap.add_argument("--iters", type=int, default=None,
                     help="c3: Adam iterations per mode (default 20); c4: design iterations of the "
                          "reference loop (default 10)")

@@ -1707,33 +1707,56 @@ int launch_batch_monitor_pair(const void* designs, int n_designs, int64_t max_ce
   return DNX_OK;
 }
 
-// Designs are packed onto the 8 XCDs (work-group b runs on XCD b % 8)
-// longest-processing-time first by their total tile cost; on each XCD the
-// designs run in decreasing tile cost, their tiles contiguous (neighbouring
-// tiles share halo rows in that XCD's L2).  Entry s * 8 + k is the s-th tile
-// of XCD k; shorter lists are padded with idle entries.
+// Each design's tiles are cut into runs of consecutive tiles, about 1/64 of
+// the batch's tile cost each (so a batch of a few designs still spreads over
+// all 8 XCDs, and a large batch balances within a run), and the runs are
+// packed onto the 8 XCDs (work-group b runs on XCD b % 8) longest-
+// processing-time first.  On each XCD the runs go in decreasing tile cost,
+// then design, then first tile, so the runs of one design that share an XCD
+// follow each other in tile order (neighbouring tiles share halo rows in
+// that XCD's L2).  Entry s * 8 + k is the s-th tile of XCD k; shorter lists
+// are padded with idle entries.
 void batch_block_map(const int* tiles, const int64_t* tile_cost, int n, std::vector<int2>& out) {
-  std::vector<int> order(n);
-  for (int d = 0; d < n; ++d) order[d] = d;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return tile_cost[a] * tiles[a] > tile_cost[b] * tiles[b];
-  });
-  std::vector<std::vector<int>> bin(8);
+  struct Run {
+    int d, t0, t1;
+    int64_t cost;
+  };
+  constexpr int64_t kRunsPerBatch = 64;
+  int64_t total = 0;
+  for (int d = 0; d < n; ++d) total += tile_cost[d] * tiles[d];
+  const int64_t unit = std::max<int64_t>(1, total / kRunsPerBatch);
+  std::vector<Run> runs;
+  for (int d = 0; d < n; ++d) {
+    if (tiles[d] <= 0) continue;
+    const int64_t c = tile_cost[d] * tiles[d];
+    const int64_t k = std::max<int64_t>(1, std::min<int64_t>(tiles[d], (c + unit - 1) / unit));
+    const int len = int((tiles[d] + k - 1) / k);
+    for (int t = 0; t < tiles[d]; t += len) {
+      const int e = std::min(tiles[d], t + len);
+      runs.push_back({d, t, e, tile_cost[d] * (e - t)});
+    }
+  }
+  std::stable_sort(runs.begin(), runs.end(),
+                   [](const Run& a, const Run& b) { return a.cost > b.cost; });
+  std::vector<std::vector<Run>> bin(8);
   int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int d : order) {
+  for (const Run& r : runs) {
     int k = 0;
     for (int j = 1; j < 8; ++j)
       if (load[j] < load[k]) k = j;
-    bin[k].push_back(d);
-    load[k] += tile_cost[d] * tiles[d];
+    bin[k].push_back(r);
+    load[k] += r.cost;
   }
   size_t rows = 0;
   std::vector<std::vector<int2>> lists(8);
   for (int k = 0; k < 8; ++k) {
-    std::stable_sort(bin[k].begin(), bin[k].end(),
-                     [&](int a, int b) { return tile_cost[a] > tile_cost[b]; });
-    for (int d : bin[k])
-      for (int t = 0; t < tiles[d]; ++t) lists[k].push_back(make_int2(d, t));
+    std::sort(bin[k].begin(), bin[k].end(), [&](const Run& a, const Run& b) {
+      if (tile_cost[a.d] != tile_cost[b.d]) return tile_cost[a.d] > tile_cost[b.d];
+      if (a.d != b.d) return a.d < b.d;
+      return a.t0 < b.t0;
+    });
+    for (const Run& r : bin[k])
+      for (int t = r.t0; t < r.t1; ++t) lists[k].push_back(make_int2(r.d, t));
     rows = std::max(rows, lists[k].size());
   }
   out.assign(rows * 8, make_int2(-1, -1));
