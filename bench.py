@@ -62,7 +62,9 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 BYTES = {"forward": 52, "replay": 64, "adjoint": 72}
 # library timing name -> (phase, kernel symbol, algorithmic bytes per cell-update)
-# (the two-step kernels advance two time steps per launch)
+# (the two-step kernels advance two time steps per launch; the "_store" names
+# time every launch with an E store: the replay's and the forward's final
+# segments, which also accumulate the monitor)
 TIMED = {"fdtd3d_fwd": ("forward", "k3_fused_fwd", 52),
          "fdtd3d_fwd_store": ("replay", "k3_fused_fwd", 64),
          "fdtd3d_adj": ("adjoint", "k3_fused_adj", 72),
